@@ -1,0 +1,293 @@
+#!/usr/bin/env python3
+"""Write the committed golden fixtures for the Delta-U path.
+
+Each fixture is produced by running the reference's own C++ hot path
+(``oracle/_ref/ref_harness``, compiled from the reference sources in place by
+``make -C oracle ref``) on a small seeded system.  A fixture holds the inputs
+(box, particles, groups, types, moves, RNG seed) and what the reference
+computed: energies of every Energybase entry point, and the per-trial energy
+change / accept sequence of a short Metropolis run.
+
+Run in the build container (where the reference tree exists):
+
+    make -C oracle ref && python tests/golden/make_golden.py
+
+The GPU box never needs the reference: tests read only the JSON written here.
+"""
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+
+
+# --------------------------------------------------------------------------
+# mt19937 text state for a seed (std::mt19937::seed, C++ [rand.eng.mers])
+# --------------------------------------------------------------------------
+def mt_state_text(seed):
+    x = [0] * 624
+    x[0] = seed & 0xFFFFFFFF
+    for i in range(1, 624):
+        x[i] = (1812433253 * (x[i - 1] ^ (x[i - 1] >> 30)) + i) & 0xFFFFFFFF
+    return " ".join(str(v) for v in x) + " 624"
+
+
+# --------------------------------------------------------------------------
+# seeded configurations
+# --------------------------------------------------------------------------
+def min_image(d, L):
+    return d - L * np.round(d / L)
+
+
+def rsa(rng, n, L, radii, placed_xyz=None, placed_r=None, max_tries=200000):
+    """Random sequential addition of n spheres in a periodic cube of side L,
+    rejecting any overlap (|r_ij| < r_i + r_j) with earlier spheres."""
+    xyz = [] if placed_xyz is None else list(placed_xyz)
+    rr = [] if placed_r is None else list(placed_r)
+    out = []
+    tries = 0
+    for k in range(n):
+        while True:
+            tries += 1
+            if tries > max_tries:
+                raise RuntimeError("RSA failed")
+            p = (rng.random(3) - 0.5) * L
+            if xyz:
+                d = min_image(np.asarray(xyz) - p, L)
+                r2 = (d * d).sum(1)
+                if np.any(r2 < (np.asarray(rr) + radii[k]) ** 2):
+                    continue
+            xyz.append(p)
+            rr.append(radii[k])
+            out.append(p)
+            break
+    return np.array(out)
+
+
+def fmt(v):
+    return repr(float(v))
+
+
+def write_state(path, L, xyz, charge, radius, mw, ids, groups, rng_text):
+    """Space::save text format (space.h:736-764, point.h:544-567,
+    group.h:392-398)."""
+    with open(path, "w") as f:
+        f.write(f"{fmt(L[0])} {fmt(L[1])} {fmt(L[2])}\n{len(xyz)}\n")
+        for i in range(len(xyz)):
+            f.write(" ".join(fmt(c) for c in xyz[i]) +
+                    f" {fmt(charge[i])} {fmt(radius[i])} {fmt(mw[i])} {int(ids[i])} 0\n")
+        f.write(f"{len(groups)}\n")
+        for g in groups:
+            f.write(f"{g['front']} {g['back']} {g['molid']} {g['molsize']} 0 0 0\n")
+        f.write("randomstate\n" + rng_text + "\n")
+
+
+def run_harness(args):
+    out = subprocess.run([HARNESS] + args, check=True, capture_output=True, text=True).stdout
+    return json.loads(out)
+
+
+def build_fixture(name, kind, cfg, L, parts, groups, seed, ncalls, moves_meta, energy=True):
+    with tempfile.TemporaryDirectory() as td:
+        cpath = os.path.join(td, "in.json")
+        spath = os.path.join(td, "state")
+        with open(cpath, "w") as f:
+            json.dump(cfg, f, indent=1)
+        write_state(spath, L, parts["xyz"], parts["charge"], parts["radius"], parts["mw"],
+                    parts["id"], groups, mt_state_text(seed))
+        fx = {
+            "name": name,
+            "kind": kind,
+            "box": list(map(float, L)),
+            "config": cfg,
+            "particles": {k: (np.asarray(v).reshape(-1).tolist()) for k, v in parts.items()},
+            "groups": groups,
+            "moves": moves_meta,
+            "rng_seed": seed,
+        }
+        if energy:
+            fx["energy"] = run_harness(["energy", kind, cpath, spath])
+        if ncalls > 0:
+            fx["mc"] = run_harness(["mc", kind, cpath, spath, str(ncalls)])
+            fx["mc"]["ncalls"] = ncalls
+    path = os.path.join(HERE, name + ".json")
+    with open(path, "w") as f:
+        json.dump(fx, f, separators=(",", ":"))
+    print(f"wrote {path} ({os.path.getsize(path) // 1024} KiB)")
+    return fx
+
+
+def lB_epsr(lB, T=298.15):
+    """relative permittivity giving Bjerrum length lB at T (physconst.h:91-94)"""
+    e, e0, kB, pi = 1.602177e-19, 8.85419e-12, 1.380658e-23, np.arccos(-1.0)
+    return e * e / (4 * pi * e0 * 1e-10 * kB * T) / lB
+
+
+def salt(name, kind, n, L, seed, ncalls, elec_cfg, dp=2.0, r=2.0):
+    rng = np.random.default_rng(seed)
+    radii = np.full(n, r)
+    xyz = rsa(rng, n, L, radii)
+    half = n // 2
+    # atom names sort as insertion order (nlohmann json objects are ordered
+    # maps), so ids are 1 = "T1p", 2 = "T2m"; id 0 is the dummy "unk"
+    ids = np.array([1] * half + [2] * (n - half))
+    charge = np.where(ids == 1, 1.0, -1.0)
+    cfg = {
+        "system": {"temperature": 298.15, "geometry": {"length": L}},
+        "atomlist": {"T1p": {"q": 1.0, "r": r, "dp": dp, "mw": 1.0},
+                     "T2m": {"q": -1.0, "r": r, "dp": dp, "mw": 1.0}},
+        "moleculelist": {"salt": {"atoms": "T1p T2m", "atomic": True, "Ninit": 0}},
+        "moves": {"atomtranslate": {"salt": {"peratom": True}}},
+        "energy": {"nonbonded": elec_cfg},
+    }
+    parts = {"xyz": xyz, "charge": charge, "radius": radii, "mw": np.ones(n), "id": ids}
+    groups = [{"front": 0, "back": n - 1, "molid": 0, "molsize": 1}]
+    moves = [{"type": "AT", "molid": 0, "prob": 1.0, "peratom": 1, "permol": 0}]
+    return build_fixture(name, kind, cfg, [L, L, L], parts, groups, seed, ncalls, moves)
+
+
+def lj_mix(name, n, L, seed, ncalls):
+    rng = np.random.default_rng(seed)
+    half = n // 2
+    # sigma 3 and 4.5 A, eps 0.15 / 0.25 kJ/mol (Lorentz-Berthelot mixing)
+    radii = np.array([1.5] * half + [2.25] * (n - half))
+    xyz = rsa(rng, n, L, radii * 0.9)
+    ids = np.array([1] * half + [2] * (n - half))
+    cfg = {
+        "system": {"temperature": 298.15, "geometry": {"length": L}},
+        "atomlist": {"T1a": {"eps": 0.15, "sigma": 3.0, "dp": 1.5, "mw": 1.0},
+                     "T2b": {"eps": 0.25, "sigma": 4.5, "dp": 1.0, "mw": 1.0}},
+        "moleculelist": {"fluid": {"atoms": "T1a T2b", "atomic": True, "Ninit": 0}},
+        "moves": {"atomtranslate": {"fluid": {"peratom": True}}},
+        "energy": {"nonbonded": {}},
+    }
+    parts = {"xyz": xyz, "charge": np.zeros(n), "radius": radii, "mw": np.ones(n), "id": ids}
+    groups = [{"front": 0, "back": n - 1, "molid": 0, "molsize": 1}]
+    moves = [{"type": "AT", "molid": 0, "prob": 1.0, "peratom": 1, "permol": 0}]
+    return build_fixture(name, "lj", cfg, [L, L, L], parts, groups, seed, ncalls, moves)
+
+
+def ewald(name, n, L, seed, ncalls, eps_surf, update_frequency, alpha=0.35, rc=7.0, kc=4.0):
+    rng = np.random.default_rng(seed)
+    half = n // 2
+    radii = np.full(n, 1.5)
+    xyz = rsa(rng, n, L, radii)
+    ids = np.array([1] * half + [2] * (n - half))
+    charge = np.where(ids == 1, 1.0, -1.0)
+    cfg = {
+        "system": {"temperature": 298.15, "geometry": {"length": L}},
+        "atomlist": {"T1p": {"q": 1.0, "sigma": 3.0, "eps": 0.2479, "dp": 1.0, "mw": 1.0},
+                     "T2m": {"q": -1.0, "sigma": 3.0, "eps": 0.2479, "dp": 1.0, "mw": 1.0}},
+        "moleculelist": {"salt": {"atoms": "T1p T2m", "atomic": True, "Ninit": 0}},
+        "moves": {"atomtranslate": {"salt": {"peratom": True}}},
+        "energy": {"nonbonded": {"epsr": lB_epsr(7.1), "ewald": {
+            "alpha": alpha, "cutoff": rc, "cutoffK": kc, "debyelength": 1e12,
+            "eps_surf": eps_surf, "spherical_sum": True, "update_frequency": update_frequency}}},
+    }
+    parts = {"xyz": xyz, "charge": charge, "radius": radii, "mw": np.ones(n), "id": ids}
+    groups = [{"front": 0, "back": n - 1, "molid": 0, "molsize": 1}]
+    moves = [{"type": "AT", "molid": 0, "prob": 1.0, "peratom": 1, "permol": 0}]
+    return build_fixture(name, "ewaldlj", cfg, [L, L, L], parts, groups, seed, ncalls, moves)
+
+
+def molecules(name, kind, nmol, nsite, nion, L, seed, ncalls, center_rotation=True,
+              cutoff_g2g=None, R=9.0):
+    """Rigid molecules (sites packed in a sphere of radius R, charges from
+    {-1,0,+1} with p = 0.2/0.6/0.2, radii U[1.5,3]) plus monovalent ions."""
+    rng = np.random.default_rng(seed)
+    # molecule centres: RSA of exclusion spheres of radius R + 3
+    centres = rsa(rng, nmol, L, np.full(nmol, R + 3.0))
+    xyz, charge, radius, ids = [], [], [], []
+    tnames = {}
+    atomlist = {}
+    groups = []
+    for m in range(nmol):
+        front = len(xyz)
+        local, lr = [], []
+        for s in range(nsite):
+            r = 1.5 + 1.5 * rng.random()
+            while True:   # sites do not overlap inside a molecule
+                v = (rng.random(3) - 0.5) * 2 * R
+                if (v * v).sum() > R * R:
+                    continue
+                if local and np.any(((np.asarray(local) - v) ** 2).sum(1) < (np.asarray(lr) + r) ** 2):
+                    continue
+                break
+            local.append(v)
+            lr.append(r)
+            q = rng.choice([-1.0, 0.0, 1.0], p=[0.2, 0.6, 0.2])
+            p = min_image(centres[m] + v, L)
+            xyz.append(p)
+            charge.append(q)
+            radius.append(r)
+        groups.append({"front": front, "back": len(xyz) - 1, "molid": 0, "molsize": nsite})
+    # one atom type per site charge class (ids 1..3 sorted names), ions 4,5
+    for q, nm in ((-1.0, "M1n"), (0.0, "M2z"), (1.0, "M3p")):
+        atomlist[nm] = {"q": q, "r": 2.0, "mw": 1.0}
+    atomlist["M4ip"] = {"q": 1.0, "r": 2.0, "dp": 2.0, "mw": 1.0}
+    atomlist["M5im"] = {"q": -1.0, "r": 2.0, "dp": 2.0, "mw": 1.0}
+    ids = [1 if q < 0 else (2 if q == 0 else 3) for q in charge]
+    ion_r = np.full(nion, 2.0)
+    ion_xyz = rsa(rng, nion, L, ion_r, placed_xyz=[np.asarray(x) for x in xyz], placed_r=radius)
+    gfront = len(xyz)
+    for k in range(nion):
+        xyz.append(ion_xyz[k])
+        charge.append(1.0 if k % 2 == 0 else -1.0)
+        radius.append(2.0)
+        ids.append(4 if k % 2 == 0 else 5)
+    if nion:
+        groups.append({"front": gfront, "back": len(xyz) - 1, "molid": 1, "molsize": 1})
+    nb = {"epsr": lB_epsr(7.1), "debyelength": 15.0}
+    if cutoff_g2g is not None:
+        nb["cutoff_g2g"] = cutoff_g2g
+    # molecule ids follow moleculelist key order from 0: "amol" = 0, "bions" = 1
+    cfg = {
+        "system": {"temperature": 298.15, "geometry": {"length": L}},
+        "atomlist": atomlist,
+        "moleculelist": {"amol": {"atoms": " ".join(["M2z"] * nsite), "Ninit": 0},
+                         "bions": {"atoms": "M4ip M5im", "atomic": True, "Ninit": 0}},
+        "moves": {"moltransrot": {"center_rotation": center_rotation,
+                                  "amol": {"dp": 5.0, "dprot": 0.5, "permol": True}}},
+        "energy": {"nonbonded": nb},
+    }
+    parts = {"xyz": np.array(xyz), "charge": np.array(charge), "radius": np.array(radius),
+             "mw": np.ones(len(xyz)), "id": np.array(ids)}
+    moves = [{"type": "TR", "molid": 0, "prob": 1.0, "peratom": 0, "permol": 1,
+              "dp": 5.0, "dprot": 0.5, "center_rotation": int(center_rotation)}]
+    return build_fixture(name, kind, cfg, [L, L, L], parts, groups, seed, ncalls, moves)
+
+
+def rng_fixture():
+    fx = {"seed_default": run_harness(["rng", "0", "2000"]),
+          "seed_7": run_harness(["rng", "7", "500"])}
+    path = os.path.join(HERE, "rng.json")
+    with open(path, "w") as f:
+        json.dump(fx, f, separators=(",", ":"))
+    print(f"wrote {path}")
+
+
+def main():
+    if not os.path.exists(HARNESS):
+        sys.exit("build oracle/_ref/ref_harness first: make -C oracle ref")
+    rng_fixture()
+    dh = {"epsr": lB_epsr(7.1), "debyelength": 10.0}
+    salt("salt_dhhs", "dhhs", 80, 30.0, 11, 25, dh)
+    salt("salt_dhhs_vec", "dhhs_vec", 40, 25.0, 12, 5, dh)
+    salt("salt_coulhs", "coulhs", 60, 30.0, 13, 10, {"epsr": lB_epsr(7.1)})
+    salt("salt_dh", "dh", 40, 20.0, 14, 8, dh)
+    lj_mix("lj_mix", 50, 25.0, 15, 10)
+    ewald("ewald_tinfoil", 40, 20.0, 16, 6, eps_surf=0.5, update_frequency=7)
+    ewald("ewald_surface", 30, 18.0, 17, 4, eps_surf=1.0, update_frequency=-1, kc=3.0)
+    molecules("mol_dhhs", "dhhs", 4, 12, 16, 60.0, 18, 12)
+    molecules("mol_dhhs_offcenter", "dhhs", 3, 10, 10, 50.0, 19, 10, center_rotation=False)
+    molecules("mol_dhhs_cut", "dhhs_cut", 5, 8, 10, 70.0, 20, 10, cutoff_g2g=25.0)
+
+
+if __name__ == "__main__":
+    main()
