@@ -1,0 +1,116 @@
+// fgx_internal.hpp -- host-side definition of an fgx_system handle and the
+// helpers shared by the translation units of libfgx.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/fgx.h"
+#include "fgx_types.hpp"
+
+namespace fgx {
+
+void set_error(const std::string &msg);
+
+#define FGX_HIP(call)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            ::fgx::set_error(std::string("HIP error ") + hipGetErrorString(e_) + " at " + \
+                             __FILE__ + ":" + std::to_string(__LINE__) + " in " + #call); \
+            return FGX_EHIP;                                                            \
+        }                                                                               \
+    } while (0)
+
+#define FGX_CHECK(cond, code, msg)          \
+    do {                                    \
+        if (!(cond)) {                      \
+            ::fgx::set_error(msg);          \
+            return (code);                  \
+        }                                   \
+    } while (0)
+
+// Device view of a system's reciprocal-space state (NonbondedEwald,
+// ewald.h:229-760).  k-vectors are stored both as FP64 vectors (the
+// reference's kVectors) and as integer triples for the factorised
+// exp(i k.r) = prod_axis exp(i k_a r_a) used by the device sweeps.
+struct EwaldDev {
+    int K, kcc;
+    double *kv;     // 3K
+    double *aks;    // K
+    int *kn;        // 3K integer triples
+    double *Q;      // 2K (re, im) current
+    double *Qt;     // 2K trial
+    double *eff;    // per type
+    double V;
+    double const_inf, eps_surf, alpha, kappa;
+    int update_frequency, cnt_accepted;
+    double surf, surf_trial, reci, reci_trial, update_drift;
+    int *cnt_dev = nullptr;   // device copy of cnt_accepted for the sweeps
+};
+
+// device pointers of one particle vector (FP64 SoA)
+struct Vec64 {
+    double *x, *y, *z;
+};
+
+}  // namespace fgx
+
+struct fgx_system {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    fgx::Box box{};
+    fgx::Pot pot{};
+    int n = 0, ng = 0, ntypes = 0;
+    // host mirrors
+    std::vector<fgx::GroupDesc> groups;
+    std::vector<double> cm, cm_trial;      // 3 * ng
+    std::vector<double> h_charge, h_radius, h_mw;
+    std::vector<int> h_type, h_gid;
+    std::vector<double> type_dp, type_charge, type_radius;
+    std::vector<double> tab;               // TAB_COUNT * ntypes^2
+    // device particle data
+    fgx::Vec64 v[2]{};                     // 0 = p, 1 = trial
+    double *d_q = nullptr, *d_r = nullptr, *d_mw = nullptr;
+    int *d_type = nullptr, *d_gid = nullptr;
+    int32_t *d_fx = nullptr, *d_fy = nullptr, *d_fz = nullptr;  // fixed-point p
+    float *d_qf = nullptr, *d_rf = nullptr;
+    fgx::GroupDesc *d_groups = nullptr;
+    double *d_cm = nullptr;                // 6 * ng: cm, cm_trial
+    double *d_tab = nullptr;
+    float *d_tabf = nullptr;
+    double *d_type_dp = nullptr;
+    unsigned char *d_mask = nullptr;       // 2 * ng keep masks (new, old)
+    double *d_partial = nullptr;           // reduction scratch
+    int partial_cap = 0;
+    double *d_out = nullptr;               // 8 result slots
+    double *h_out = nullptr;               // pinned mirror
+    int *d_idx = nullptr;                  // moved-index scratch
+    int idx_cap = 0;
+    // Ewald
+    bool has_ewald = false;
+    fgx::EwaldDev ew{};
+    std::vector<double> h_eff;
+    // moves and RNG
+    std::vector<fgx::MoveDesc> moves;
+    std::vector<long> move_cnt;
+    uint32_t rng_global[625]{};            // 624 words + position
+    uint32_t rng_internal[625]{};
+    uint32_t *d_rng = nullptr;             // 2 x 625 words (global, internal)
+    bool propagator_started = false;
+    double uinit = 0, dusum = 0;
+    long *d_stats_i = nullptr;
+    double *d_stats_d = nullptr;
+    size_t device_bytes = 0;
+};
+
+namespace fgx {
+// implemented in fgx_system.hip, used by fgx_sweep.hip
+int ensure_partials(fgx_system *s, int nblocks);
+int ewald_full_recompute(fgx_system *s, int which, double *Qdst);
+int ewald_reciprocal(fgx_system *s, const double *Q, double V, double *out_host);
+void mt_seed_host(uint32_t *state625, uint32_t seed);
+}  // namespace fgx

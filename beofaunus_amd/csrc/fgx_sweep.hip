@@ -1,0 +1,1112 @@
+// fgx_sweep.hip -- device-resident Propagator::move (move.h:7877-7891) and
+// Movebase::move (move.h:515-555) for AtomicTranslation (move.h:818-901)
+// and TranslateRotate (move.h:1620-1713).
+//
+// One workgroup owns one Markov chain for the whole launch: it replays the
+// reference's two mt19937 engines (global `slump` and the move-internal
+// `_slump()`) in LDS, generates each trial with the reference's FP64
+// arithmetic, sums the energy change over all partners, takes the Metropolis
+// decision and commits -- no host round trip per trial.  Many chains (batch,
+// replicas, jobs) run as many workgroups of one launch.
+//
+// Pair math is FP64, or (mixed) FP32 on fixed-point coordinates whose
+// integer difference is the exact minimum image, with FP64 accumulation.
+// Ewald S(k) stays FP64 in registers of the thread that owns each k-vector.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "fgx_device.hpp"
+#include "fgx_internal.hpp"
+
+namespace fgx {
+
+static constexpr int MAXK_AXIS = 64;     // |n| per axis for the Ewald tables
+static constexpr int MAX_GROUP_SITES = 512;
+static constexpr int MAX_GROUPS_LDS = 4096;
+
+struct ChainDev {
+    Box box;
+    Pot pot;
+    int n, ng;
+    double *x, *y, *z;                 // accepted vector (FP64)
+    const double *q, *r;
+    const int *t, *gid;
+    int32_t *fx, *fy, *fz;             // fixed-point accepted vector
+    const float *qf, *rf;
+    const double *tab;
+    const float *tabf;
+    const GroupDesc *groups;
+    double *cm;                        // 6 * ng (cm, cm_trial)
+    const double *type_dp;
+    MoveDesc mv;
+    int nmoves, repeat;
+    const int *gvec;
+    int ngvec;
+    uint32_t *rng;                     // 2 x 625: global, internal
+    // Ewald
+    int K, kcc;
+    const int *kn;
+    const double *aks;
+    double *Q;
+    const double *eff;
+    double V, recip_pref, surf_pref;   // (2pi/V) lB ; const_inf 2pi/((2eps+1)V) lB
+    int update_frequency;
+    int *ew_cnt;                       // cnt_accepted (device)
+    // results
+    long *stats_i;                     // trials, accepted, inf-rejects, refreshes
+    double *stats_d;                   // utot, dusum
+    double *du_rec;
+    int *acc_rec;
+    long cap;
+};
+
+// ------------------------------------------------------------------------
+// pair-loop helpers
+// ------------------------------------------------------------------------
+template <bool MIX>
+struct Row;
+
+template <>
+struct Row<false> {
+    double nx, ny, nz, ox, oy, oz, q, r;
+    int t;
+};
+template <>
+struct Row<true> {
+    int32_t nx, ny, nz, ox, oy, oz;
+    float q, r;
+    int t;
+};
+
+// u(new, j) - u(old, j) for one partner j, FP64
+__device__ __forceinline__ double du_pair64(const ChainDev &C, double nx, double ny, double nz,
+                                            double ox, double oy, double oz, double aq, double ar,
+                                            int at, int j)
+{
+    double bx = C.x[j], by = C.y[j], bz = C.z[j];
+    double bq = C.q[j], br = C.r[j];
+    int bt = C.t[j];
+    double r2n, r2o;
+    if (C.pot.nbkind == NB_VECTOR || C.pot.elec == ELEC_EWALD) {
+        double rx, ry, rz;
+        vdist_d(C.box, nx, ny, nz, bx, by, bz, rx, ry, rz);
+        r2n = rx * rx + (ry * ry + rz * rz);
+        vdist_d(C.box, ox, oy, oz, bx, by, bz, rx, ry, rz);
+        r2o = rx * rx + (ry * ry + rz * rz);
+    } else {
+        r2n = sqdist_d(C.box, nx, ny, nz, bx, by, bz);
+        r2o = sqdist_d(C.box, ox, oy, oz, bx, by, bz);
+    }
+    return pair_u64(C.pot, C.tab, r2n, aq, bq, ar, br, at, bt) -
+           pair_u64(C.pot, C.tab, r2o, aq, bq, ar, br, at, bt);
+}
+
+// mixed: FP32 pair energies from fixed-point minimum-image differences
+__device__ __forceinline__ float du_pair32(const ChainDev &C, int32_t nx, int32_t ny, int32_t nz,
+                                           int32_t ox, int32_t oy, int32_t oz, float aq, float ar,
+                                           int at, int j)
+{
+    int32_t bx = C.fx[j], by = C.fy[j], bz = C.fz[j];
+    float bq = C.qf[j], br = C.rf[j];
+    int bt = C.t[j];
+    const float sx = C.box.fscale[0], sy = C.box.fscale[1], sz = C.box.fscale[2];
+    float dx = (float)(int32_t)(nx - bx) * sx, dy = (float)(int32_t)(ny - by) * sy,
+          dz = (float)(int32_t)(nz - bz) * sz;
+    float r2n = dx * dx + dy * dy + dz * dz;
+    dx = (float)(int32_t)(ox - bx) * sx; dy = (float)(int32_t)(oy - by) * sy;
+    dz = (float)(int32_t)(oz - bz) * sz;
+    float r2o = dx * dx + dy * dy + dz * dz;
+    return pair_u32(C.pot, C.tabf, r2n, aq, bq, ar, br, at, bt) -
+           pair_u32(C.pot, C.tabf, r2o, aq, bq, ar, br, at, bt);
+}
+
+// ------------------------------------------------------------------------
+// Ewald per-trial helpers: exp(i 2 pi n x / L) for n = 0..kcc per axis
+// ------------------------------------------------------------------------
+__device__ __forceinline__ void ew_factor(const double2 *tab, int kcc1, int axis, int n, double &re,
+                                          double &im)
+{
+    double2 e = tab[axis * kcc1 + (n < 0 ? -n : n)];
+    re = e.x;
+    im = n < 0 ? -e.y : e.y;
+}
+
+__device__ __forceinline__ void cmul(double ar, double ai, double br, double bi, double &cr,
+                                     double &ci)
+{
+    cr = ar * br - ai * bi;
+    ci = ar * bi + ai * br;
+}
+
+// exp(i k.r) from the per-axis tables
+__device__ __forceinline__ void ew_phase(const double2 *tab, int kcc1, int nx, int ny, int nz,
+                                         double &re, double &im)
+{
+    double ar, ai, br, bi, cr, ci;
+    ew_factor(tab, kcc1, 0, nx, ar, ai);
+    ew_factor(tab, kcc1, 1, ny, br, bi);
+    cmul(ar, ai, br, bi, cr, ci);
+    ew_factor(tab, kcc1, 2, nz, ar, ai);
+    cmul(cr, ci, ar, ai, re, im);
+}
+
+// ------------------------------------------------------------------------
+// AtomicTranslation sweep
+// ------------------------------------------------------------------------
+struct TrialAT {
+    int i;
+    int go;
+    int acc;
+    int collide;
+    double ox, oy, oz, nx, ny, nz;
+};
+
+template <bool MIX, int KPT>
+__global__ __launch_bounds__(1024) void k_sweep_at(const ChainDev *__restrict__ chains, int ncalls)
+{
+    const ChainDev &C = chains[blockIdx.x];
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const bool wave0 = tid < 64;
+    __shared__ uint32_t mtg[624], mti[624];
+    __shared__ double red[16];
+    __shared__ TrialAT T[2];
+    __shared__ double2 etab[2][3 * MAXK_AXIS];   // old / new per-axis phase tables
+
+    for (int k = tid; k < 624; k += nthr) {
+        mtg[k] = C.rng[k];
+        mti[k] = C.rng[625 + k];
+    }
+    WaveMT G{mtg, (int)C.rng[624]}, I{mti, (int)C.rng[625 + 624]};
+
+    // Ewald: each thread owns k = tid + m * nthr, m < KPT
+    const bool ew = (C.pot.elec == ELEC_EWALD);
+    const int kcc1 = C.kcc + 1;
+    double S_re[KPT], S_im[KPT], A[KPT], dS_re[KPT], dS_im[KPT];
+    int kx[KPT], ky[KPT], kz[KPT];
+#pragma unroll
+    for (int m = 0; m < KPT; ++m) {
+        int k = tid + m * nthr;
+        bool own = ew && k < C.K;
+        S_re[m] = own ? C.Q[2 * k] : 0.0;
+        S_im[m] = own ? C.Q[2 * k + 1] : 0.0;
+        A[m] = own ? C.aks[k] : 0.0;
+        kx[m] = own ? C.kn[3 * k] : 0;
+        ky[m] = own ? C.kn[3 * k + 1] : 0;
+        kz[m] = own ? C.kn[3 * k + 2] : 0;
+        dS_re[m] = dS_im[m] = 0.0;
+    }
+    // surface dipole M = sum q eff r (wave 0 keeps it; ewald.h:278-292)
+    double Mx = 0, My = 0, Mz = 0;
+    int ew_cnt = ew ? *C.ew_cnt : 0;
+    long n_trials = 0, n_acc = 0, n_inf = 0, n_ref = 0;
+    double utot = 0.0, dusum = 0.0;
+    long rec = 0;
+    if (ew && C.surf_pref != 0.0) {
+        for (int j = tid; j < C.n; j += nthr) {
+            double w = C.q[j] * C.eff[C.t[j]];
+            Mx += C.q[j] * C.x[j] * C.eff[C.t[j]];
+            My += C.q[j] * C.y[j] * C.eff[C.t[j]];
+            Mz += C.q[j] * C.z[j] * C.eff[C.t[j]];
+            (void)w;
+        }
+        Mx = block_sum(Mx, red);
+        My = block_sum(My, red);
+        Mz = block_sum(Mz, red);
+    }
+    __syncthreads();
+
+    int par = 0;
+    for (int call = 0; call < ncalls; ++call) {
+        // ---- Propagator::move (move.h:7887) + Movebase::move prologue
+        if (wave0) {
+            (void)mt_range(I, 0, C.nmoves - 1);       // pick the move
+            (void)mt_range(I, 0, 0);                  // randomMolId: mollist element
+            bool run = mt_canonical(I) < C.mv.prob;   // Movebase::run (move.h:575-581)
+            if (tid == 0) T[par].go = run ? 1 : 0;
+        }
+        __syncthreads();
+        if (!T[par].go) { par ^= 1; continue; }
+        double utot_call = 0.0;
+        for (int rep = 0; rep < C.repeat; ++rep) {
+            TrialAT &tr = T[par];
+            // ---- AtomicTranslation::_trialMove (move.h:818-864), wave 0
+            if (wave0) {
+                int g = C.gvec[mt_range(G, 0, C.ngvec - 1)];
+                GroupDesc gd = C.groups[g];
+                int i = gd.front + mt_range(G, 0, gd.back - gd.front);
+                double dp = C.type_dp[C.t[i]];
+                double ox = C.x[i], oy = C.y[i], oz = C.z[i];
+                double nx, ny, nz;
+                {
+#pragma clang fp contract(off)
+                    double tx = C.mv.dir[0] * dp, ty = C.mv.dir[1] * dp, tz = C.mv.dir[2] * dp;
+                    tx *= mt_canonical(G) - 0.5;
+                    ty *= mt_canonical(G) - 0.5;
+                    tz *= mt_canonical(G) - 0.5;
+                    nx = ox + tx; ny = oy + ty; nz = oz + tz;
+                    boundary_d(C.box, nx, ny, nz);
+                }
+                if (tid == 0) {
+                    tr.i = i;
+                    tr.ox = ox; tr.oy = oy; tr.oz = oz;
+                    tr.nx = nx; tr.ny = ny; tr.nz = nz;
+                    tr.collide = collision_d(C.box, nx, ny, nz) ? 1 : 0;
+                }
+            }
+            __syncthreads();
+            const int i = tr.i;
+            double acc = 0.0;
+            // ---- pair part: sum_{j != i} u(i', j) - u(i, j)
+            if (!MIX) {
+                const double aq = C.q[i], ar = C.r[i];
+                const int at = C.t[i];
+                const double nx = tr.nx, ny = tr.ny, nz = tr.nz, ox = tr.ox, oy = tr.oy,
+                             oz = tr.oz;
+                for (int j = tid; j < C.n; j += nthr) {
+                    if (j == i) continue;
+                    acc += du_pair64(C, nx, ny, nz, ox, oy, oz, aq, ar, at, j);
+                }
+            } else {
+                const float aq = C.qf[i], ar = C.rf[i];
+                const int at = C.t[i];
+                const int32_t nx = to_fixed(tr.nx, C.box.inv[0]), ny = to_fixed(tr.ny, C.box.inv[1]),
+                              nz = to_fixed(tr.nz, C.box.inv[2]);
+                const int32_t ox = C.fx[i], oy = C.fy[i], oz = C.fz[i];
+                for (int j = tid; j < C.n; j += nthr) {
+                    if (j == i) continue;
+                    acc += (double)du_pair32(C, nx, ny, nz, ox, oy, oz, aq, ar, at, j);
+                }
+            }
+            // ---- reciprocal part: dS_k and A_k (|S+dS|^2 - |S|^2)
+            if (ew) {
+                // per-axis phase tables for old (0) and new (1) position
+                const int ntab = 2 * 3 * kcc1;
+                for (int e = tid; e < ntab; e += nthr) {
+                    int which = e / (3 * kcc1), rem = e % (3 * kcc1);
+                    int axis = rem / kcc1, nn = rem % kcc1;
+                    double coord = which ? (axis == 0 ? tr.nx : axis == 1 ? tr.ny : tr.nz)
+                                         : (axis == 0 ? tr.ox : axis == 1 ? tr.oy : tr.oz);
+                    double ang = 2.0 * M_PI * (double)nn * coord * C.box.inv[axis];
+                    double s, c;
+                    sincos(ang, &s, &c);
+                    etab[which][axis * kcc1 + nn] = make_double2(c, s);
+                }
+                __syncthreads();
+                const double w = C.q[i] * C.eff[C.t[i]];
+                double de = 0.0;
+#pragma unroll
+                for (int m = 0; m < KPT; ++m) {
+                    double er, ei, orr, oi;
+                    ew_phase(etab[1], kcc1, kx[m], ky[m], kz[m], er, ei);
+                    ew_phase(etab[0], kcc1, kx[m], ky[m], kz[m], orr, oi);
+                    dS_re[m] = w * (er - orr);
+                    dS_im[m] = w * (ei - oi);
+                    de += A[m] * (2.0 * (S_re[m] * dS_re[m] + S_im[m] * dS_im[m]) +
+                                  (dS_re[m] * dS_re[m] + dS_im[m] * dS_im[m]));
+                }
+                acc += C.recip_pref * de;
+            }
+            double du = block_sum(acc, red);
+            if (ew && C.surf_pref != 0.0) {
+                const double w = C.q[i] * C.eff[C.t[i]];
+                double dMx = w * (tr.nx - tr.ox), dMy = w * (tr.ny - tr.oy), dMz = w * (tr.nz - tr.oz);
+                double Nx = Mx + dMx, Ny = My + dMy, Nz = Mz + dMz;
+                du += C.surf_pref * ((Nx * Nx + (Ny * Ny + Nz * Nz)) - (Mx * Mx + (My * My + Mz * Mz)));
+            }
+            if (tr.collide) du = __builtin_inf();   // energy.h:3597-3607
+            // ---- Movebase::metropolis (move.h:567-573), wave 0
+            if (wave0) {
+                bool accept = !(mt_canonical(G) > exp(-du));
+                if (tid == 0) tr.acc = accept ? 1 : 0;
+            }
+            __syncthreads();
+            const bool accept = tr.acc;
+            n_trials++;
+            if (tid == 0 && C.du_rec && rec < C.cap) {
+                C.du_rec[rec] = du;
+                C.acc_rec[rec] = accept;
+            }
+            rec++;
+            if (accept) {
+                n_acc++;
+                utot_call += du;
+                dusum += du;
+                if (tid == 0) {   // AtomicTranslation::_acceptMove (move.h:866-877)
+                    C.x[i] = tr.nx; C.y[i] = tr.ny; C.z[i] = tr.nz;
+                    C.fx[i] = to_fixed(tr.nx, C.box.inv[0]);
+                    C.fy[i] = to_fixed(tr.ny, C.box.inv[1]);
+                    C.fz[i] = to_fixed(tr.nz, C.box.inv[2]);
+                }
+                if (ew) {
+                    const double w = C.q[i] * C.eff[C.t[i]];
+                    Mx += w * (tr.nx - tr.ox); My += w * (tr.ny - tr.oy); Mz += w * (tr.nz - tr.oz);
+#pragma unroll
+                    for (int m = 0; m < KPT; ++m) { S_re[m] += dS_re[m]; S_im[m] += dS_im[m]; }
+                    // NonbondedEwald::update (ewald.h:626-643): full refresh
+                    if (++ew_cnt > C.update_frequency - 1) {
+                        __syncthreads();   // commit of x[i] visible
+                        double eb = 0.0;
+#pragma unroll
+                        for (int m = 0; m < KPT; ++m) eb += A[m] * (S_re[m] * S_re[m] + S_im[m] * S_im[m]);
+                        eb = block_sum(eb, red);
+#pragma unroll
+                        for (int m = 0; m < KPT; ++m) { S_re[m] = 0.0; S_im[m] = 0.0; }
+                        double mx = 0, my = 0, mz = 0;
+                        for (int j0 = 0; j0 < C.n; j0 += 32) {
+                            int nj = min(32, C.n - j0);
+                            // tables for 32 particles: reuse etab as scratch is too small,
+                            // so evaluate the phase directly from the FP64 k-vector
+                            for (int jj = 0; jj < nj; ++jj) {
+                                int j = j0 + jj;
+                                double wj = C.q[j] * C.eff[C.t[j]];
+                                double xj = C.x[j], yj = C.y[j], zj = C.z[j];
+#pragma unroll
+                                for (int m = 0; m < KPT; ++m) {
+                                    if (A[m] == 0.0 && kx[m] == 0 && ky[m] == 0 && kz[m] == 0) continue;
+                                    double ph = 2.0 * M_PI *
+                                                ((double)kx[m] * xj * C.box.inv[0] +
+                                                 (double)ky[m] * yj * C.box.inv[1] +
+                                                 (double)kz[m] * zj * C.box.inv[2]);
+                                    double s, c;
+                                    sincos(ph, &s, &c);
+                                    S_re[m] += wj * c;
+                                    S_im[m] += wj * s;
+                                }
+                                if (tid == 0) { mx += wj * xj; my += wj * yj; mz += wj * zj; }
+                            }
+                        }
+                        double ea = 0.0;
+#pragma unroll
+                        for (int m = 0; m < KPT; ++m) ea += A[m] * (S_re[m] * S_re[m] + S_im[m] * S_im[m]);
+                        ea = block_sum(ea, red);
+                        if (C.surf_pref != 0.0) {
+                            // keep M consistent with the refreshed sums
+                            mx = __shfl(mx, 0, 64);
+                            my = __shfl(my, 0, 64);
+                            mz = __shfl(mz, 0, 64);
+                            if (wave0) { Mx = mx; My = my; Mz = mz; }
+                        }
+                        double d = C.recip_pref * (ea - eb);
+                        utot_call += d;
+                        ew_cnt = 0;
+                        n_ref++;
+                    }
+                }
+            } else if (du == __builtin_inf()) {
+                n_inf++;
+            }
+            par ^= 1;
+        }
+        utot += utot_call;
+    }
+    // ---- write back chain state
+    __syncthreads();
+    for (int k = tid; k < 624; k += nthr) {
+        C.rng[k] = mtg[k];
+        C.rng[625 + k] = mti[k];
+    }
+    if (tid == 0) {
+        C.rng[624] = (uint32_t)G.p;
+        C.rng[625 + 624] = (uint32_t)I.p;
+        C.stats_i[0] += n_trials;
+        C.stats_i[1] += n_acc;
+        C.stats_i[2] += n_inf;
+        C.stats_i[3] += n_ref;
+        C.stats_d[0] += utot;
+        C.stats_d[1] += dusum;
+        if (ew) *C.ew_cnt = ew_cnt;
+    }
+    if (ew) {
+#pragma unroll
+        for (int m = 0; m < KPT; ++m) {
+            int k = tid + m * nthr;
+            if (k < C.K) { C.Q[2 * k] = S_re[m]; C.Q[2 * k + 1] = S_im[m]; }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// TranslateRotate sweep (one workgroup per chain; moved group staged in LDS)
+// ------------------------------------------------------------------------
+struct TrialTR {
+    int g, go, acc, collide, rot, trans;
+    double qw, qx, qy, qz;           // quaternion (Eigen AngleAxis -> Quaternion)
+    double ox, oy, oz;               // rotation origin
+    double px, py, pz;               // translation
+    double cmx, cmy, cmz;            // cm_trial after the move
+};
+
+__device__ __forceinline__ void cross3(double a0, double a1, double a2, double b0, double b1,
+                                       double b2, double &c0, double &c1, double &c2)
+{
+#pragma clang fp contract(off)
+    c0 = a1 * b2 - a2 * b1;
+    c1 = a2 * b0 - a0 * b2;
+    c2 = a0 * b1 - a1 * b0;
+}
+
+// QuaternionRotate::operator() (geometry.h:130-139) with Eigen 3.2.9's
+// _transformVector: uv = 2 vec x v; v' = v + w uv + vec x uv
+__device__ __forceinline__ void qrotate(const Box &b, const TrialTR &T, double &x, double &y,
+                                        double &z)
+{
+#pragma clang fp contract(off)
+    double ax = x - T.ox, ay = y - T.oy, az = z - T.oz;
+    boundary_d(b, ax, ay, az);
+    double u0, u1, u2, c0, c1, c2;
+    cross3(T.qx, T.qy, T.qz, ax, ay, az, u0, u1, u2);
+    u0 += u0; u1 += u1; u2 += u2;
+    cross3(T.qx, T.qy, T.qz, u0, u1, u2, c0, c1, c2);
+    x = ((ax + T.qw * u0) + c0) + T.ox;
+    y = ((ay + T.qw * u1) + c1) + T.oy;
+    z = ((az + T.qw * u2) + c2) + T.oz;
+    boundary_d(b, x, y, z);
+}
+
+__device__ __forceinline__ void translate_pt(const Box &b, const TrialTR &T, double &x, double &y,
+                                             double &z)
+{
+#pragma clang fp contract(off)
+    x += T.px; y += T.py; z += T.pz;
+    boundary_d(b, x, y, z);
+}
+
+template <bool MIX>
+__global__ __launch_bounds__(1024) void k_sweep_tr(const ChainDev *__restrict__ chains, int ncalls)
+{
+    const ChainDev &C = chains[blockIdx.x];
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const bool wave0 = tid < 64;
+    __shared__ uint32_t mtg[624], mti[624];
+    __shared__ double red[16];
+    __shared__ TrialTR T[2];
+    __shared__ double sn[3][MAX_GROUP_SITES], so[3][MAX_GROUP_SITES];
+    __shared__ int32_t snf[3][MAX_GROUP_SITES], sof[3][MAX_GROUP_SITES];
+    __shared__ double sq[MAX_GROUP_SITES], sr[MAX_GROUP_SITES];
+    __shared__ int st[MAX_GROUP_SITES];
+    __shared__ unsigned char keep_n[MAX_GROUPS_LDS], keep_o[MAX_GROUPS_LDS];
+    __shared__ int s_collide;
+
+    for (int k = tid; k < 624; k += nthr) {
+        mtg[k] = C.rng[k];
+        mti[k] = C.rng[625 + k];
+    }
+    WaveMT G{mtg, (int)C.rng[624]}, I{mti, (int)C.rng[625 + 624]};
+    long n_trials = 0, n_acc = 0, n_inf = 0;
+    double utot = 0.0, dusum = 0.0;
+    long rec = 0;
+    __syncthreads();
+    int par = 0;
+    for (int call = 0; call < ncalls; ++call) {
+        if (wave0) {
+            (void)mt_range(I, 0, C.nmoves - 1);
+            (void)mt_range(I, 0, 0);
+            bool run = mt_canonical(I) < C.mv.prob;
+            if (tid == 0) T[par].go = run ? 1 : 0;
+        }
+        __syncthreads();
+        if (!T[par].go) { par ^= 1; continue; }
+        for (int rep = 0; rep < C.repeat; ++rep) {
+            TrialTR &tr = T[par];
+            // ---- TranslateRotate::_trialMove (move.h:1620-1686), wave 0
+            if (wave0) {
+#pragma clang fp contract(off)
+                int g = C.gvec[mt_range(G, 0, C.ngvec - 1)];
+                GroupDesc gd = C.groups[g];
+                double cmx = C.cm[6 * g + 3], cmy = C.cm[6 * g + 4], cmz = C.cm[6 * g + 5];
+                double qw = 1, qx = 0, qy = 0, qz = 0, orx = 0, ory = 0, orz = 0;
+                int rot = 0, trans = 0;
+                double px = 0, py = 0, pz = 0;
+                if (C.mv.dprot > 1e-6) {
+                    // PointBase::ranunit (point.h:74-89)
+                    double v0, v1, v2, r2;
+                    do {
+                        v0 = (2 * mt_canonical(G) - 1) * C.mv.dir2[0];
+                        v1 = (2 * mt_canonical(G) - 1) * C.mv.dir2[1];
+                        v2 = (2 * mt_canonical(G) - 1) * C.mv.dir2[2];
+                        r2 = v0 * v0 + (v1 * v1 + v2 * v2);
+                    } while (r2 > 1);
+                    double nrm = sqrt(r2);
+                    v0 = v0 / nrm; v1 = v1 / nrm; v2 = v2 / nrm;
+                    double cx = C.cm[6 * g], cy = C.cm[6 * g + 1], cz = C.cm[6 * g + 2];
+                    if (!C.mv.center_rotation) {
+                        int site = (mt_canonical(G) < 0.5) ? gd.front : gd.back;
+                        cx = C.x[site]; cy = C.y[site]; cz = C.z[site];
+                    }
+                    double ex = cx + v0, ey = cy + v1, ez = cz + v2;
+                    double angle = C.mv.dprot * (mt_canonical(G) - 0.5);
+                    // QuaternionRotate::setAxis (src/geometry.cpp:753-766)
+                    double ux = ex - cx, uy = ey - cy, uz = ez - cz;
+                    boundary_d(C.box, ux, uy, uz);
+                    double un = sqrt(ux * ux + (uy * uy + uz * uz));
+                    ux /= un; uy /= un; uz /= un;
+                    double ha = 0.5 * angle;
+                    qw = cos(ha);
+                    double sh = sin(ha);
+                    qx = sh * ux; qy = sh * uy; qz = sh * uz;
+                    orx = cx; ory = cy; orz = cz;
+                    rot = 1;
+                }
+                if (C.mv.dp > 1e-6) {
+                    px = C.mv.dir[0] * C.mv.dp * (mt_canonical(G) - 0.5);
+                    py = C.mv.dir[1] * C.mv.dp * (mt_canonical(G) - 0.5);
+                    pz = C.mv.dir[2] * C.mv.dp * (mt_canonical(G) - 0.5);
+                    trans = 1;
+                }
+                if (tid == 0) {
+                    tr.g = g; tr.rot = rot; tr.trans = trans;
+                    tr.qw = qw; tr.qx = qx; tr.qy = qy; tr.qz = qz;
+                    tr.ox = orx; tr.oy = ory; tr.oz = orz;
+                    tr.px = px; tr.py = py; tr.pz = pz;
+                    // Group::newrotate rotates cm_trial; Group::translate moves it
+                    if (rot) qrotate(C.box, tr, cmx, cmy, cmz);
+                    if (trans) translate_pt(C.box, tr, cmx, cmy, cmz);
+                    tr.cmx = cmx; tr.cmy = cmy; tr.cmz = cmz;
+                    s_collide = 0;
+                }
+            }
+            __syncthreads();
+            const int g = tr.g;
+            const GroupDesc gd = C.groups[g];
+            const int ns = gd.back - gd.front + 1;
+            // ---- move the sites (Group::newrotate / Group::translate)
+            for (int s = tid; s < ns; s += nthr) {
+                int i = gd.front + s;
+                double x = C.x[i], y = C.y[i], z = C.z[i];
+                so[0][s] = x; so[1][s] = y; so[2][s] = z;
+                sof[0][s] = C.fx[i]; sof[1][s] = C.fy[i]; sof[2][s] = C.fz[i];
+                sq[s] = C.q[i]; sr[s] = C.r[i]; st[s] = C.t[i];
+                if (tr.rot) qrotate(C.box, tr, x, y, z);
+                if (tr.trans) translate_pt(C.box, tr, x, y, z);
+                sn[0][s] = x; sn[1][s] = y; sn[2][s] = z;
+                snf[0][s] = to_fixed(x, C.box.inv[0]);
+                snf[1][s] = to_fixed(y, C.box.inv[1]);
+                snf[2][s] = to_fixed(z, C.box.inv[2]);
+                if (collision_d(C.box, x, y, z)) s_collide = 1;
+            }
+            // ---- NonbondedCutg2g keep flags per other group (energy.h:1343-1358)
+            for (int h = tid; h < C.ng; h += nthr) {
+                unsigned char kn = 1, ko = 1;
+                if (C.pot.nbkind == NB_CUTG2G) {
+                    const GroupDesc hd = C.groups[h];
+                    bool mol_h = (hd.molsize == hd.back - hd.front + 1);
+                    bool mol_g = (gd.molsize == ns);
+                    if (mol_h && mol_g) {
+                        double hx = C.cm[6 * h], hy = C.cm[6 * h + 1], hz = C.cm[6 * h + 2];
+                        kn = sqdist_d(C.box, tr.cmx, tr.cmy, tr.cmz, hx, hy, hz) > C.pot.g2g_rc2 ? 0 : 1;
+                        ko = sqdist_d(C.box, C.cm[6 * g], C.cm[6 * g + 1], C.cm[6 * g + 2], hx, hy,
+                                      hz) > C.pot.g2g_rc2 ? 0 : 1;
+                    }
+                }
+                keep_n[h] = kn;
+                keep_o[h] = ko;
+            }
+            __syncthreads();
+            double acc = 0.0;
+            for (int j = tid; j < C.n; j += nthr) {
+                if (j >= gd.front && j <= gd.back) continue;
+                int h = C.gid[j];
+                bool kn = keep_n[h], ko = keep_o[h];
+                if (!kn && !ko) continue;
+                if (!MIX) {
+                    double bx = C.x[j], by = C.y[j], bz = C.z[j], bq = C.q[j], br = C.r[j];
+                    int bt = C.t[j];
+                    const bool vec = (C.pot.nbkind == NB_VECTOR);
+                    for (int s = 0; s < ns; ++s) {
+                        double un = 0.0, uo = 0.0;
+                        if (kn) {
+                            double r2;
+                            if (vec) {
+                                double rx, ry, rz;
+                                vdist_d(C.box, sn[0][s], sn[1][s], sn[2][s], bx, by, bz, rx, ry, rz);
+                                r2 = rx * rx + (ry * ry + rz * rz);
+                            } else {
+                                r2 = sqdist_d(C.box, sn[0][s], sn[1][s], sn[2][s], bx, by, bz);
+                            }
+                            un = pair_u64(C.pot, C.tab, r2, sq[s], bq, sr[s], br, st[s], bt);
+                        }
+                        if (ko) {
+                            double r2;
+                            if (vec) {
+                                double rx, ry, rz;
+                                vdist_d(C.box, so[0][s], so[1][s], so[2][s], bx, by, bz, rx, ry, rz);
+                                r2 = rx * rx + (ry * ry + rz * rz);
+                            } else {
+                                r2 = sqdist_d(C.box, so[0][s], so[1][s], so[2][s], bx, by, bz);
+                            }
+                            uo = pair_u64(C.pot, C.tab, r2, sq[s], bq, sr[s], br, st[s], bt);
+                        }
+                        acc += un - uo;
+                    }
+                } else {
+                    int32_t bx = C.fx[j], by = C.fy[j], bz = C.fz[j];
+                    float bq = C.qf[j], br = C.rf[j];
+                    int bt = C.t[j];
+                    const float fsx = C.box.fscale[0], fsy = C.box.fscale[1], fsz = C.box.fscale[2];
+                    for (int s = 0; s < ns; ++s) {
+                        float aq = (float)sq[s], ar = (float)sr[s];
+                        int at = st[s];
+                        float un = 0.f, uo = 0.f;
+                        if (kn) {
+                            float dx = (float)(int32_t)(snf[0][s] - bx) * fsx;
+                            float dy = (float)(int32_t)(snf[1][s] - by) * fsy;
+                            float dz = (float)(int32_t)(snf[2][s] - bz) * fsz;
+                            un = pair_u32(C.pot, C.tabf, dx * dx + dy * dy + dz * dz, aq, bq, ar, br, at, bt);
+                        }
+                        if (ko) {
+                            float dx = (float)(int32_t)(sof[0][s] - bx) * fsx;
+                            float dy = (float)(int32_t)(sof[1][s] - by) * fsy;
+                            float dz = (float)(int32_t)(sof[2][s] - bz) * fsz;
+                            uo = pair_u32(C.pot, C.tabf, dx * dx + dy * dy + dz * dz, aq, bq, ar, br, at, bt);
+                        }
+                        acc += (double)(un - uo);
+                    }
+                }
+            }
+            double du = block_sum(acc, red);
+            if (s_collide) du = __builtin_inf();
+            if (!tr.rot && !tr.trans) du = 0.0;   // move.h:1709-1710
+            if (wave0) {
+                bool accept = !(mt_canonical(G) > exp(-du));
+                if (tid == 0) tr.acc = accept ? 1 : 0;
+            }
+            __syncthreads();
+            const bool accept = tr.acc;
+            n_trials++;
+            if (tid == 0 && C.du_rec && rec < C.cap) {
+                C.du_rec[rec] = du;
+                C.acc_rec[rec] = accept;
+            }
+            rec++;
+            if (accept) {   // Group::accept (group.h:381-389)
+                n_acc++;
+                utot += du;
+                dusum += du;
+                for (int s = tid; s < ns; s += nthr) {
+                    int i = gd.front + s;
+                    C.x[i] = sn[0][s]; C.y[i] = sn[1][s]; C.z[i] = sn[2][s];
+                    C.fx[i] = to_fixed(sn[0][s], C.box.inv[0]);
+                    C.fy[i] = to_fixed(sn[1][s], C.box.inv[1]);
+                    C.fz[i] = to_fixed(sn[2][s], C.box.inv[2]);
+                }
+                if (tid == 0) {
+                    C.cm[6 * g] = tr.cmx; C.cm[6 * g + 1] = tr.cmy; C.cm[6 * g + 2] = tr.cmz;
+                    C.cm[6 * g + 3] = tr.cmx; C.cm[6 * g + 4] = tr.cmy; C.cm[6 * g + 5] = tr.cmz;
+                }
+            } else if (du == __builtin_inf()) {
+                n_inf++;
+            }
+            __syncthreads();   // committed sites / cm visible to the next trial
+            par ^= 1;
+        }
+    }
+    __syncthreads();
+    for (int k = tid; k < 624; k += nthr) {
+        C.rng[k] = mtg[k];
+        C.rng[625 + k] = mti[k];
+    }
+    if (tid == 0) {
+        C.rng[624] = (uint32_t)G.p;
+        C.rng[625 + 624] = (uint32_t)I.p;
+        C.stats_i[0] += n_trials;
+        C.stats_i[1] += n_acc;
+        C.stats_i[2] += n_inf;
+        C.stats_d[0] += utot;
+        C.stats_d[1] += dusum;
+    }
+}
+
+// ------------------------------------------------------------------------
+// host side
+// ------------------------------------------------------------------------
+
+static int chain_block(const fgx_system *s)
+{
+    if (s->moves.empty()) return 256;
+    if (s->n >= 8192) return 1024;
+    if (s->n >= 2048) return 512;
+    return 256;
+}
+
+static int kpt_for(const fgx_system *s, int nthr)
+{
+    if (!s->has_ewald) return 1;
+    int need = (s->ew.K + nthr - 1) / nthr;
+    if (need <= 1) return 1;
+    if (need <= 2) return 2;
+    if (need <= 4) return 4;
+    if (need <= 8) return 8;
+    return 16;
+}
+
+struct Prepared {
+    std::vector<int> gvec;
+    int *d_gvec = nullptr;
+};
+
+static int prepare_chain(fgx_system *s, ChainDev &C, int **d_gvec, double *du_rec, int *acc_rec,
+                         long cap)
+{
+    FGX_CHECK(s->moves.size() == 1, FGX_EUNSUPPORTED,
+              "device Propagator: exactly one registered move is supported");
+    const MoveDesc &mv = s->moves[0];
+    std::vector<int> gvec;
+    for (int g = 0; g < s->ng; ++g)
+        if (s->groups[g].molid == mv.molid) gvec.push_back(g);
+    FGX_CHECK(!gvec.empty(), FGX_ESTATE, "device Propagator: no group carries the move's molid");
+    int repeat = 1;
+    if (mv.permol) repeat *= (int)gvec.size();
+    if (mv.peratom) repeat *= s->groups[gvec[0]].back - s->groups[gvec[0]].front + 1;
+    if (mv.type == MOVE_AT) {
+        for (int g : gvec)
+            FGX_CHECK(s->groups[g].molsize == 1, FGX_EUNSUPPORTED,
+                      "device AtomicTranslation: moved groups must be atomic");
+        FGX_CHECK(!s->has_ewald || s->ew.kcc < MAXK_AXIS, FGX_EUNSUPPORTED,
+                  "device Ewald: cutoffK too large");
+    } else {
+        FGX_CHECK(!s->has_ewald, FGX_EUNSUPPORTED, "device TranslateRotate with Ewald");
+        FGX_CHECK(s->ng <= MAX_GROUPS_LDS, FGX_EUNSUPPORTED, "device TranslateRotate: too many groups");
+        for (int g : gvec)
+            FGX_CHECK(s->groups[g].back - s->groups[g].front + 1 <= MAX_GROUP_SITES,
+                      FGX_EUNSUPPORTED, "device TranslateRotate: group larger than LDS stage");
+    }
+    FGX_HIP(hipMalloc(d_gvec, sizeof(int) * gvec.size()));
+    FGX_HIP(hipMemcpyAsync(*d_gvec, gvec.data(), sizeof(int) * gvec.size(), hipMemcpyHostToDevice,
+                           s->stream));
+    C.box = s->box;
+    C.pot = s->pot;
+    C.n = s->n;
+    C.ng = s->ng;
+    C.x = s->v[0].x; C.y = s->v[0].y; C.z = s->v[0].z;
+    C.q = s->d_q; C.r = s->d_r; C.t = s->d_type; C.gid = s->d_gid;
+    C.fx = s->d_fx; C.fy = s->d_fy; C.fz = s->d_fz;
+    C.qf = s->d_qf; C.rf = s->d_rf;
+    C.tab = s->d_tab; C.tabf = s->d_tabf;
+    C.groups = s->d_groups;
+    C.cm = s->d_cm;
+    C.type_dp = s->d_type_dp;
+    C.mv = mv;
+    C.nmoves = (int)s->moves.size();
+    C.repeat = repeat;
+    C.gvec = *d_gvec;
+    C.ngvec = (int)gvec.size();
+    C.rng = s->d_rng;
+    if (s->has_ewald) {
+        const double pi = std::acos(-1.);
+        C.K = s->ew.K;
+        C.kcc = s->ew.kcc;
+        C.kn = s->ew.kn;
+        C.aks = s->ew.aks;
+        C.Q = s->ew.Q;
+        C.eff = s->ew.eff;
+        C.V = s->ew.V;
+        C.recip_pref = (2 * pi / s->ew.V) * s->pot.lB;
+        C.surf_pref = s->ew.const_inf >= 0.5
+                          ? s->ew.const_inf * (2 * pi / ((2 * s->ew.eps_surf + 1) * s->ew.V)) * s->pot.lB
+                          : 0.0;
+        C.update_frequency = s->ew.update_frequency;
+    } else {
+        C.K = 0;
+        C.kcc = 0;
+    }
+    C.stats_i = s->d_stats_i;
+    C.stats_d = s->d_stats_d;
+    C.du_rec = du_rec;
+    C.acc_rec = acc_rec;
+    C.cap = cap;
+    return FGX_OK;
+}
+
+// Movebase::trialMove at cnt == 0 (move.h:466-475): all mass centres are
+// recomputed from the accepted vector before the first trial of a move.
+static int init_mass_centres(fgx_system *s)
+{
+    std::vector<double> xyz(3 * (size_t)s->n);
+    int rc = fgx_get_positions(s, 0, xyz.data());
+    if (rc) return rc;
+    for (int g = 0; g < s->ng; ++g) {
+        double cm[3];
+        const GroupDesc &G = s->groups[g];
+        cm[0] = cm[1] = cm[2] = 0;
+        if (G.back >= G.front) {
+            int io = (int)(G.front + (G.back - G.front) * 0.5);
+            double o[3] = {xyz[3 * io], xyz[3 * io + 1], xyz[3 * io + 2]};
+            double sum = 0;
+            for (int i = G.front; i <= G.back; ++i) {
+                double t[3] = {xyz[3 * i] - o[0], xyz[3 * i + 1] - o[1], xyz[3 * i + 2] - o[2]};
+                for (int k = 0; k < 3; ++k)
+                    if (std::fabs(t[k]) > s->box.half[k]) {
+                        double a = t[k] * s->box.inv[k];
+                        t[k] -= s->box.len[k] * (int)(a > 0. ? a + .5 : a - .5);
+                    }
+                for (int k = 0; k < 3; ++k) cm[k] += t[k] * s->h_mw[i];
+                sum += s->h_mw[i];
+            }
+            if (std::fabs(sum) < 1e-6) sum = 1;
+            for (int k = 0; k < 3; ++k) {
+                cm[k] = cm[k] / sum + o[k];
+                if (std::fabs(cm[k]) > s->box.half[k]) {
+                    double a = cm[k] * s->box.inv[k];
+                    cm[k] -= s->box.len[k] * (int)(a > 0. ? a + .5 : a - .5);
+                }
+            }
+        }
+        for (int k = 0; k < 3; ++k) { s->cm[3 * g + k] = cm[k]; s->cm_trial[3 * g + k] = cm[k]; }
+    }
+    std::vector<double> cm6(6 * (size_t)s->ng);
+    for (int g = 0; g < s->ng; ++g)
+        for (int k = 0; k < 3; ++k) {
+            cm6[6 * g + k] = s->cm[3 * g + k];
+            cm6[6 * g + 3 + k] = s->cm_trial[3 * g + k];
+        }
+    FGX_HIP(hipMemcpyAsync(s->d_cm, cm6.data(), sizeof(double) * cm6.size(), hipMemcpyHostToDevice,
+                           s->stream));
+    FGX_HIP(hipStreamSynchronize(s->stream));
+    return FGX_OK;
+}
+
+template <bool MIX>
+static int launch_at(hipStream_t st, const ChainDev *d_chains, int nchains, int nthr, int kpt,
+                     int ncalls)
+{
+    switch (kpt) {
+    case 1: hipLaunchKernelGGL((k_sweep_at<MIX, 1>), dim3(nchains), dim3(nthr), 0, st, d_chains, ncalls); break;
+    case 2: hipLaunchKernelGGL((k_sweep_at<MIX, 2>), dim3(nchains), dim3(nthr), 0, st, d_chains, ncalls); break;
+    case 4: hipLaunchKernelGGL((k_sweep_at<MIX, 4>), dim3(nchains), dim3(nthr), 0, st, d_chains, ncalls); break;
+    case 8: hipLaunchKernelGGL((k_sweep_at<MIX, 8>), dim3(nchains), dim3(nthr), 0, st, d_chains, ncalls); break;
+    default: hipLaunchKernelGGL((k_sweep_at<MIX, 16>), dim3(nchains), dim3(nthr), 0, st, d_chains, ncalls); break;
+    }
+    FGX_HIP(hipGetLastError());
+    return FGX_OK;
+}
+
+static int run_chains(std::vector<fgx_system *> &sys, int ncalls, double *du_rec, int *acc_rec,
+                      long cap, double *seconds)
+{
+    fgx_system *s0 = sys[0];
+    const int nc = (int)sys.size();
+    std::vector<ChainDev> chains(nc);
+    std::vector<int *> gvecs(nc, nullptr);
+    int rc = FGX_OK;
+    int nthr = 0, kpt = 1;
+    for (int c = 0; c < nc && !rc; ++c) {
+        fgx_system *s = sys[c];
+        FGX_CHECK(s->moves.size() == s0->moves.size() && !s->moves.empty() &&
+                      s->moves[0].type == s0->moves[0].type &&
+                      s->pot.precision == s0->pot.precision,
+                  FGX_EUNSUPPORTED, "batched chains must share move type and precision");
+        if (!s->propagator_started) {
+            // Propagator: uinit = systemEnergy before the first move (move.h:7884-7885)
+            if ((rc = fgx_system_energy(s, 0, &s->uinit))) break;
+            s->propagator_started = true;
+        }
+        if (s->move_cnt.empty()) s->move_cnt.assign(s->moves.size(), 0);
+        if (s->move_cnt[0] == 0 && (rc = init_mass_centres(s))) break;
+        FGX_HIP(hipMemcpyAsync(s->d_rng, s->rng_global, sizeof(uint32_t) * 625, hipMemcpyHostToDevice, s->stream));
+        FGX_HIP(hipMemcpyAsync(s->d_rng + 625, s->rng_internal, sizeof(uint32_t) * 625, hipMemcpyHostToDevice, s->stream));
+        FGX_HIP(hipMemsetAsync(s->d_stats_i, 0, sizeof(long) * 16, s->stream));
+        FGX_HIP(hipMemsetAsync(s->d_stats_d, 0, sizeof(double) * 16, s->stream));
+        if (s->has_ewald) {
+            if (!s->ew.cnt_dev) {
+                FGX_HIP(hipMalloc(&s->ew.cnt_dev, sizeof(int)));
+            }
+            FGX_HIP(hipMemcpyAsync(s->ew.cnt_dev, &s->ew.cnt_accepted, sizeof(int), hipMemcpyHostToDevice, s->stream));
+        }
+        rc = prepare_chain(s, chains[c], &gvecs[c], c == 0 ? du_rec : nullptr,
+                           c == 0 ? acc_rec : nullptr, c == 0 ? cap : 0);
+        if (!rc && s->has_ewald) chains[c].ew_cnt = s->ew.cnt_dev;
+        FGX_HIP(hipStreamSynchronize(s->stream));
+        nthr = std::max(nthr, chain_block(s));
+        kpt = std::max(kpt, kpt_for(s, chain_block(s)));
+    }
+    ChainDev *d_chains = nullptr;
+    if (!rc) {
+        // recompute kpt for the common block size
+        kpt = 1;
+        for (auto *s : sys) kpt = std::max(kpt, kpt_for(s, nthr));
+        FGX_HIP(hipMalloc(&d_chains, sizeof(ChainDev) * nc));
+        FGX_HIP(hipMemcpyAsync(d_chains, chains.data(), sizeof(ChainDev) * nc, hipMemcpyHostToDevice,
+                               s0->stream));
+        FGX_HIP(hipStreamSynchronize(s0->stream));
+        hipEvent_t e0, e1;
+        FGX_HIP(hipEventCreate(&e0));
+        FGX_HIP(hipEventCreate(&e1));
+        FGX_HIP(hipEventRecord(e0, s0->stream));
+        if (s0->moves[0].type == MOVE_AT) {
+            rc = s0->pot.precision == FGX_MIXED ? launch_at<true>(s0->stream, d_chains, nc, nthr, kpt, ncalls)
+                                                : launch_at<false>(s0->stream, d_chains, nc, nthr, kpt, ncalls);
+        } else {
+            if (s0->pot.precision == FGX_MIXED)
+                hipLaunchKernelGGL(k_sweep_tr<true>, dim3(nc), dim3(nthr), 0, s0->stream, d_chains, ncalls);
+            else
+                hipLaunchKernelGGL(k_sweep_tr<false>, dim3(nc), dim3(nthr), 0, s0->stream, d_chains, ncalls);
+            if (hipGetLastError() != hipSuccess) { set_error("k_sweep_tr launch failed"); rc = FGX_EHIP; }
+        }
+        FGX_HIP(hipEventRecord(e1, s0->stream));
+        FGX_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        FGX_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (seconds) *seconds = ms * 1e-3;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        hipError_t err = hipStreamSynchronize(s0->stream);
+        if (err != hipSuccess && !rc) {
+            set_error(std::string("sweep kernel failed: ") + hipGetErrorString(err));
+            rc = FGX_EHIP;
+        }
+    }
+    // pull back RNG state and bookkeeping
+    for (int c = 0; c < nc && !rc; ++c) {
+        fgx_system *s = sys[c];
+        FGX_HIP(hipMemcpyAsync(s->rng_global, s->d_rng, sizeof(uint32_t) * 625, hipMemcpyDeviceToHost, s->stream));
+        FGX_HIP(hipMemcpyAsync(s->rng_internal, s->d_rng + 625, sizeof(uint32_t) * 625, hipMemcpyDeviceToHost, s->stream));
+        if (s->has_ewald) {
+            FGX_HIP(hipMemcpyAsync(&s->ew.cnt_accepted, s->ew.cnt_dev, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+        }
+        FGX_HIP(hipStreamSynchronize(s->stream));
+        if (s->has_ewald) {
+            // trial S(k) mirrors the current one between trials (undo/accept)
+            FGX_HIP(hipMemcpyAsync(s->ew.Qt, s->ew.Q, sizeof(double) * 2 * s->ew.K, hipMemcpyDeviceToDevice, s->stream));
+        }
+        // trial vector and mass centres mirror the accepted state
+        for (int k = 0; k < 3; ++k) {
+            double *src = k == 0 ? s->v[0].x : k == 1 ? s->v[0].y : s->v[0].z;
+            double *dst = k == 0 ? s->v[1].x : k == 1 ? s->v[1].y : s->v[1].z;
+            FGX_HIP(hipMemcpyAsync(dst, src, sizeof(double) * s->n, hipMemcpyDeviceToDevice, s->stream));
+        }
+        FGX_HIP(hipStreamSynchronize(s->stream));
+    }
+    for (auto *p : gvecs)
+        if (p) (void)hipFree(p);
+    if (d_chains) (void)hipFree(d_chains);
+    return rc;
+}
+
+}  // namespace fgx
+
+using namespace fgx;
+
+extern "C" int fgx_add_move(fgx_system *s, int type, int molid, double prob, int peratom,
+                            int permol, const double dir[3], const double dir2[3], double dp,
+                            double dprot, int center_rotation)
+{
+    FGX_CHECK(s && (type == MOVE_AT || type == MOVE_TR), FGX_EARG, "fgx_add_move: bad move type");
+    MoveDesc m{};
+    m.type = type;
+    m.molid = molid;
+    m.prob = prob;
+    m.peratom = peratom;
+    m.permol = permol;
+    for (int k = 0; k < 3; ++k) {
+        m.dir[k] = dir ? dir[k] : 1.0;
+        m.dir2[k] = dir2 ? dir2[k] : 1.0;
+    }
+    m.dp = dp;
+    m.dprot = dprot;
+    if (m.dprot > 4 * std::acos(-1.)) m.dprot = 4 * std::acos(-1.);   // move.h:1598-1599
+    m.center_rotation = center_rotation;
+    s->moves.push_back(m);
+    s->move_cnt.push_back(0);
+    return FGX_OK;
+}
+
+extern "C" int fgx_rng_seed(fgx_system *s, uint32_t seed)
+{
+    FGX_CHECK(s, FGX_EARG, "fgx_rng_seed: null system");
+    mt_seed_host(s->rng_global, seed);
+    return FGX_OK;
+}
+
+extern "C" int fgx_rng_set_state(fgx_system *s, const uint32_t words[624], int pos)
+{
+    FGX_CHECK(s && words && pos >= 0 && pos <= 624, FGX_EARG, "fgx_rng_set_state: bad arguments");
+    std::memcpy(s->rng_global, words, sizeof(uint32_t) * 624);
+    s->rng_global[624] = (uint32_t)pos;
+    return FGX_OK;
+}
+
+extern "C" int fgx_rng_get_state(fgx_system *s, uint32_t words[624], int *pos)
+{
+    FGX_CHECK(s && words && pos, FGX_EARG, "fgx_rng_get_state: bad arguments");
+    std::memcpy(words, s->rng_global, sizeof(uint32_t) * 624);
+    *pos = (int)s->rng_global[624];
+    return FGX_OK;
+}
+
+extern "C" int fgx_propagator_init(fgx_system *s)
+{
+    FGX_CHECK(s, FGX_EARG, "fgx_propagator_init: null system");
+    std::memcpy(s->rng_internal, s->rng_global, sizeof(s->rng_global));   // move.h:7781
+    s->propagator_started = false;
+    s->dusum = 0;
+    return FGX_OK;
+}
+
+static int collect_stats(fgx_system *s, fgx_stats *st, double seconds)
+{
+    long si[16];
+    double sd[16];
+    FGX_HIP(hipMemcpyAsync(si, s->d_stats_i, sizeof(si), hipMemcpyDeviceToHost, s->stream));
+    FGX_HIP(hipMemcpyAsync(sd, s->d_stats_d, sizeof(sd), hipMemcpyDeviceToHost, s->stream));
+    FGX_HIP(hipStreamSynchronize(s->stream));
+    if (si[0] > 0) s->move_cnt[0] += si[0];
+    s->dusum += sd[1];
+    if (st) {
+        st->trials += si[0];
+        st->accepted += si[1];
+        st->rejected_inf += si[2];
+        st->ewald_refresh += si[3];
+        st->utot += sd[0];
+        st->dusum += sd[1];
+        st->seconds = seconds;
+    }
+    return FGX_OK;
+}
+
+extern "C" int fgx_propagate(fgx_system *s, int ncalls, fgx_stats *st, double *du_out,
+                             int *acc_out, long cap)
+{
+    FGX_CHECK(s && ncalls >= 0, FGX_EARG, "fgx_propagate: bad arguments");
+    if (st) std::memset(st, 0, sizeof(*st));
+    double *d_du = nullptr;
+    int *d_acc = nullptr;
+    if (cap > 0 && du_out && acc_out) {
+        FGX_HIP(hipMalloc(&d_du, sizeof(double) * cap));
+        FGX_HIP(hipMalloc(&d_acc, sizeof(int) * cap));
+    } else {
+        cap = 0;
+    }
+    std::vector<fgx_system *> sys{s};
+    double secs = 0;
+    int rc = run_chains(sys, ncalls, d_du, d_acc, cap, &secs);
+    if (!rc) rc = collect_stats(s, st, secs);
+    if (!rc && cap > 0) {
+        long ntr = st ? st->trials : cap;
+        long ncopy = std::min(cap, ntr);
+        if (ncopy > 0) {
+            FGX_HIP(hipMemcpy(du_out, d_du, sizeof(double) * ncopy, hipMemcpyDeviceToHost));
+            FGX_HIP(hipMemcpy(acc_out, d_acc, sizeof(int) * ncopy, hipMemcpyDeviceToHost));
+        }
+    }
+    if (d_du) (void)hipFree(d_du);
+    if (d_acc) (void)hipFree(d_acc);
+    return rc;
+}
+
+extern "C" int fgx_propagate_batch(fgx_system **systems, int nsys, int ncalls, fgx_stats *st)
+{
+    FGX_CHECK(systems && nsys > 0 && ncalls >= 0, FGX_EARG, "fgx_propagate_batch: bad arguments");
+    std::vector<fgx_system *> sys(systems, systems + nsys);
+    for (auto *s : sys) FGX_CHECK(s, FGX_EARG, "fgx_propagate_batch: null system");
+    if (st) std::memset(st, 0, sizeof(*st));
+    double secs = 0;
+    int rc = run_chains(sys, ncalls, nullptr, nullptr, 0, &secs);
+    for (int c = 0; c < nsys && !rc; ++c) rc = collect_stats(sys[c], st, secs);
+    return rc;
+}

@@ -1,0 +1,296 @@
+"""ctypes binding of libfgx.so, the MI355X Metropolis Delta-U engine.
+
+This is the thin host-side mirror of the C ABI declared in include/fgx.h.
+It never computes energies itself: every number comes from the HIP library,
+and importing it on a machine without the built library raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libfgx.so")
+
+ELEC_NONE, ELEC_COULOMB, ELEC_DH, ELEC_EWALD = 0, 1, 2, 3
+SR_NONE, SR_HS, SR_LJ = 0, 1, 2
+NB_PLAIN, NB_VECTOR, NB_CUTG2G = 0, 1, 2
+FP64, MIXED = 0, 1
+MOVE_AT, MOVE_TR = 1, 2
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class FgxError(RuntimeError):
+    pass
+
+
+class Hamiltonian(C.Structure):
+    _fields_ = [
+        ("elec", C.c_int), ("sr", C.c_int), ("nbkind", C.c_int), ("precision", C.c_int),
+        ("lB", C.c_double), ("kappa", C.c_double), ("g2g_cut", C.c_double),
+        ("ntypes", C.c_int),
+        ("type_sigma", _dp), ("type_eps", _dp), ("type_charge", _dp),
+        ("type_radius", _dp), ("type_dp", _dp),
+        ("ewald_alpha", C.c_double), ("ewald_rc", C.c_double), ("ewald_kc", C.c_double),
+        ("ewald_debyelength", C.c_double), ("ewald_eps_surf", C.c_double),
+        ("ewald_spherical_sum", C.c_int), ("ewald_update_frequency", C.c_int),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("trials", C.c_long), ("accepted", C.c_long), ("utot", C.c_double),
+        ("dusum", C.c_double), ("rejected_inf", C.c_long), ("ewald_refresh", C.c_long),
+        ("seconds", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Change(C.Structure):
+    _fields_ = [("group", C.c_int), ("count", C.c_int), ("index", _ip)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libfgx.so; raise if it has not been built (no CPU fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FgxError("libfgx.so is not built (run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    P, I, D = C.c_void_p, C.c_int, C.c_double
+    pp = C.POINTER(C.c_void_p)
+    sig = {
+        "fgx_last_error": (C.c_char_p, []),
+        "fgx_device_info": (I, [C.c_char_p, I, _ip, C.POINTER(C.c_size_t)]),
+        "fgx_create": (I, [C.POINTER(Hamiltonian), _dp, I, _dp, _dp, _dp, _dp, _ip, I, _ip, _ip,
+                           _ip, _ip, pp]),
+        "fgx_destroy": (I, [P]),
+        "fgx_device_bytes": (C.c_size_t, [P]),
+        "fgx_set_positions": (I, [P, I, I, I, _dp]),
+        "fgx_get_positions": (I, [P, I, _dp]),
+        "fgx_get_group_cm": (I, [P, I, _dp, _dp]),
+        "fgx_set_group_cm": (I, [P, I, _dp, _dp]),
+        "fgx_i2i": (I, [P, I, I, I, _dp]),
+        "fgx_i2g": (I, [P, I, I, I, _dp]),
+        "fgx_i2all": (I, [P, I, I, _dp]),
+        "fgx_g2g": (I, [P, I, I, I, _dp]),
+        "fgx_g_internal": (I, [P, I, I, _dp]),
+        "fgx_g_external": (I, [P, I, I, _dp]),
+        "fgx_external": (I, [P, I, _dp]),
+        "fgx_system_energy": (I, [P, I, _dp]),
+        "fgx_update_change": (I, [P, C.POINTER(Change)]),
+        "fgx_energy_change": (I, [P, C.POINTER(Change), _dp]),
+        "fgx_commit": (I, [P, C.POINTER(Change), I, _dp]),
+        "fgx_add_move": (I, [P, I, I, D, I, I, _dp, _dp, D, D, I]),
+        "fgx_rng_seed": (I, [P, C.c_uint32]),
+        "fgx_rng_set_state": (I, [P, C.POINTER(C.c_uint32), I]),
+        "fgx_rng_get_state": (I, [P, C.POINTER(C.c_uint32), _ip]),
+        "fgx_propagator_init": (I, [P]),
+        "fgx_propagate": (I, [P, I, C.POINTER(Stats), _dp, _ip, C.c_long]),
+        "fgx_propagate_batch": (I, [pp, I, I, C.POINTER(Stats)]),
+        "fgx_temper_create": (I, [pp, I, C.c_uint32, C.c_uint32, pp]),
+        "fgx_temper_exchange": (I, [P, _ip, _dp]),
+        "fgx_temper_slots": (I, [P, _ip]),
+        "fgx_temper_destroy": (I, [P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        msg = lib().fgx_last_error()
+        raise FgxError("fgx error %d: %s" % (rc, msg.decode() if msg else ""))
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def device_info():
+    name = C.create_string_buffer(128)
+    cu = C.c_int()
+    mem = C.c_size_t()
+    _check(lib().fgx_device_info(name, 128, C.byref(cu), C.byref(mem)))
+    return {"arch": name.value.decode(), "cus": cu.value, "hbm_bytes": mem.value}
+
+
+class System:
+    """One Space<Cuboid,PointParticle> + non-bonded Hamiltonian on the GPU."""
+
+    def __init__(self, box, xyz, charge, radius, mw, types, groups, *, elec, sr,
+                 nbkind=NB_PLAIN, precision=FP64, lB=7.0, kappa=0.0, g2g_cut=0.0,
+                 type_sigma=None, type_eps=None, type_charge=None, type_radius=None,
+                 type_dp=None, ewald=None):
+        L = lib()
+        n = len(charge)
+        ntypes = len(type_sigma) if type_sigma is not None else int(np.max(types)) + 1
+        self._keep = []
+
+        def arr(v, default_len):
+            a = np.ascontiguousarray(
+                np.asarray(v if v is not None else np.zeros(default_len), dtype=np.float64))
+            self._keep.append(a)
+            return a
+
+        h = Hamiltonian()
+        h.elec, h.sr, h.nbkind, h.precision = elec, sr, nbkind, precision
+        h.lB, h.kappa, h.g2g_cut = lB, kappa, g2g_cut
+        h.ntypes = ntypes
+        h.type_sigma = _d(arr(type_sigma, ntypes))
+        h.type_eps = _d(arr(type_eps, ntypes))
+        h.type_charge = _d(arr(type_charge, ntypes))
+        h.type_radius = _d(arr(type_radius, ntypes))
+        h.type_dp = _d(arr(type_dp, ntypes))
+        if ewald:
+            h.ewald_alpha = ewald["alpha"]
+            h.ewald_rc = ewald["cutoff"]
+            h.ewald_kc = ewald["cutoffK"]
+            h.ewald_debyelength = ewald["debyelength"]
+            h.ewald_eps_surf = ewald["eps_surf"]
+            h.ewald_spherical_sum = int(ewald.get("spherical_sum", True))
+            h.ewald_update_frequency = int(ewald.get("update_frequency", -1))
+        self.ham = h
+        self.n = n
+        self.groups = [dict(g) for g in groups]
+        self.ng = len(groups)
+        b = arr(box, 3)
+        x = arr(np.asarray(xyz, dtype=np.float64).reshape(-1), 3 * n)
+        q, r, m = arr(charge, n), arr(radius, n), arr(mw, n)
+        t = np.ascontiguousarray(np.asarray(types, dtype=np.int32))
+        gf = np.array([g["front"] for g in groups], dtype=np.int32)
+        gb = np.array([g["back"] for g in groups], dtype=np.int32)
+        gs = np.array([g["molsize"] for g in groups], dtype=np.int32)
+        gm = np.array([g["molid"] for g in groups], dtype=np.int32)
+        self._keep += [t, gf, gb, gs, gm]
+        hp = C.c_void_p()
+        _check(L.fgx_create(C.byref(h), _d(b), n, _d(x), _d(q), _d(r), _d(m), _i(t), self.ng,
+                            _i(gf), _i(gb), _i(gs), _i(gm), C.byref(hp)))
+        self.h = hp
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fgx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- Energybase entry points
+    def _e(self, fn, *args):
+        u = C.c_double()
+        _check(getattr(lib(), fn)(self.h, *args, C.byref(u)))
+        return u.value
+
+    def i2i(self, i, j, which=0): return self._e("fgx_i2i", which, i, j)
+    def i2g(self, g, j, which=0): return self._e("fgx_i2g", which, g, j)
+    def i2all(self, i, which=0): return self._e("fgx_i2all", which, i)
+    def g2g(self, a, b, which=0): return self._e("fgx_g2g", which, a, b)
+    def g_internal(self, g, which=0): return self._e("fgx_g_internal", which, g)
+    def g_external(self, g, which=0): return self._e("fgx_g_external", which, g)
+    def external(self, which=0): return self._e("fgx_external", which)
+    def system_energy(self, which=0): return self._e("fgx_system_energy", which)
+
+    @property
+    def device_bytes(self):
+        return lib().fgx_device_bytes(self.h)
+
+    # ---- transfers
+    def positions(self, which=0):
+        out = np.zeros(3 * self.n)
+        _check(lib().fgx_get_positions(self.h, which, _d(out)))
+        return out.reshape(-1, 3)
+
+    def set_positions(self, first, xyz, which=1):
+        a = np.ascontiguousarray(np.asarray(xyz, dtype=np.float64).reshape(-1))
+        _check(lib().fgx_set_positions(self.h, which, first, len(a) // 3, _d(a)))
+
+    def group_cm(self, g):
+        cm, ct = np.zeros(3), np.zeros(3)
+        _check(lib().fgx_get_group_cm(self.h, g, _d(cm), _d(ct)))
+        return cm, ct
+
+    def set_group_cm(self, g, cm=None, cm_trial=None):
+        a = np.ascontiguousarray(np.asarray(cm, dtype=np.float64)) if cm is not None else None
+        b = np.ascontiguousarray(np.asarray(cm_trial, dtype=np.float64)) if cm_trial is not None else None
+        _check(lib().fgx_set_group_cm(self.h, g, _d(a) if a is not None else None,
+                                      _d(b) if b is not None else None))
+
+    # ---- Delta U of one change
+    def _change(self, group, index):
+        idx = np.ascontiguousarray(np.asarray(index if index is not None else [], dtype=np.int32))
+        ch = Change(group, len(idx), _i(idx) if len(idx) else None)
+        return ch, idx
+
+    def update_change(self, group, index=None):
+        ch, idx = self._change(group, index)
+        _check(lib().fgx_update_change(self.h, C.byref(ch)))
+
+    def energy_change(self, group, index=None):
+        ch, idx = self._change(group, index)
+        du = C.c_double()
+        _check(lib().fgx_energy_change(self.h, C.byref(ch), C.byref(du)))
+        return du.value
+
+    def commit(self, group, index, accepted):
+        ch, idx = self._change(group, index)
+        d = C.c_double()
+        _check(lib().fgx_commit(self.h, C.byref(ch), int(accepted), C.byref(d)))
+        return d.value
+
+    # ---- device Propagator
+    def add_move(self, kind, molid, prob=1.0, peratom=False, permol=False, dir=None, dir2=None,
+                 dp=0.0, dprot=0.0, center_rotation=True):
+        a = np.ascontiguousarray(np.asarray(dir if dir is not None else [1, 1, 1], dtype=np.float64))
+        b = np.ascontiguousarray(np.asarray(dir2 if dir2 is not None else [1, 1, 1], dtype=np.float64))
+        _check(lib().fgx_add_move(self.h, kind, molid, prob, int(peratom), int(permol), _d(a),
+                                  _d(b), dp, dprot, int(center_rotation)))
+
+    def rng_seed(self, seed):
+        _check(lib().fgx_rng_seed(self.h, seed))
+
+    def rng_state(self):
+        w = (C.c_uint32 * 624)()
+        p = C.c_int()
+        _check(lib().fgx_rng_get_state(self.h, w, C.byref(p)))
+        return list(w), p.value
+
+    def propagator_init(self):
+        _check(lib().fgx_propagator_init(self.h))
+
+    def propagate(self, ncalls, record=0):
+        st = Stats()
+        if record:
+            du = np.zeros(record)
+            acc = np.zeros(record, dtype=np.int32)
+            _check(lib().fgx_propagate(self.h, ncalls, C.byref(st), _d(du), _i(acc), record))
+            n = min(record, st.trials)
+            return st.as_dict(), du[:n], acc[:n]
+        _check(lib().fgx_propagate(self.h, ncalls, C.byref(st), None, None, 0))
+        return st.as_dict(), None, None
+
+
+def propagate_batch(systems, ncalls):
+    arr = (C.c_void_p * len(systems))(*[s.h for s in systems])
+    st = Stats()
+    _check(lib().fgx_propagate_batch(arr, len(systems), ncalls, C.byref(st)))
+    return st.as_dict()

@@ -1,0 +1,183 @@
+/*
+ * fgx.h -- C ABI of the MI355X (gfx950) Metropolis Delta-U engine.
+ *
+ * One `fgx_system` is one Faunus `Space<Cuboid,PointParticle>` plus its
+ * non-bonded Hamiltonian and registered moves, resident in HBM.  The entry
+ * points below are the operations the reference exposes through C++ virtual
+ * dispatch on `Energy::Energybase<Tspace>` and `Move::Movebase<Tspace>`; each
+ * declaration names the reference interface it replaces (paths relative to
+ * the reference tree).  Plain pointers and sizes only; no exceptions cross
+ * this boundary.  Every call returns FGX_OK (0) or a negative FGX_E* code,
+ * with a message retrievable through fgx_last_error().
+ *
+ * Units follow the reference: lengths in Angstrom, energies in kT, charges
+ * in e.  `which` selects the particle vector: 0 = accepted (Space::p),
+ * 1 = trial (Space::trial).
+ *
+ * Threading: a handle is used by one host thread at a time; calls on one
+ * handle are ordered on that handle's HIP stream.
+ */
+#ifndef FGX_H
+#define FGX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FGX_OK 0
+#define FGX_EARG (-1)        /* invalid argument */
+#define FGX_EHIP (-2)        /* HIP runtime error */
+#define FGX_ESTATE (-3)      /* call not valid in the current state */
+#define FGX_ENOMEM (-4)      /* device allocation or HBM budget exceeded */
+#define FGX_EUNSUPPORTED (-5)/* combination not implemented on the device */
+#define FGX_EDEVICE (-6)     /* device-side check failed (e.g. bounded wait) */
+
+/* electrostatic term: Potential::Coulomb (potentials.h:2515-2568),
+ * Potential::DebyeHuckel (potentials.h:3303-3375),
+ * Potential::EwaldReal + Energy::NonbondedEwald (ewald.h:35-130, 229-760) */
+enum { FGX_ELEC_NONE = 0, FGX_ELEC_COULOMB = 1, FGX_ELEC_DH = 2, FGX_ELEC_EWALD = 3 };
+/* short-range term: Potential::HardSphere (potentials.h:455-475),
+ * Potential::LennardJonesLB (potentials.h:2003-2116, 1816-1831, 3891) */
+enum { FGX_SR_NONE = 0, FGX_SR_HS = 1, FGX_SR_LJ = 2 };
+/* energy class: Energy::Nonbonded (energy.h:781-951, sqdist),
+ * Energy::NonbondedVector (energy.h:1062-1260, vdist),
+ * Energy::NonbondedCutg2g (energy.h:1328-1444, mass-centre cutoff) */
+enum { FGX_NB_PLAIN = 0, FGX_NB_VECTOR = 1, FGX_NB_CUTG2G = 2 };
+/* pair arithmetic: FP64 throughout, or FP32 pair geometry/potentials with
+ * FP64 accumulation, FP64 S(k) and FP64 Metropolis */
+enum { FGX_FP64 = 0, FGX_MIXED = 1 };
+/* moves: Move::AtomicTranslation (move.h:730-967),
+ * Move::TranslateRotate (move.h:1505-1801) */
+enum { FGX_MOVE_AT = 1, FGX_MOVE_TR = 2 };
+
+typedef struct fgx_system fgx_system;
+
+/* Hamiltonian parameters, as the reference derives them from its JSON
+ * input (energy.nonbonded.*, atomlist.*); per-type arrays include the dummy
+ * type 0 (species.h:494-504). */
+typedef struct {
+    int elec, sr, nbkind, precision;
+    double lB;        /* Bjerrum length, pc::lB(epsr) (physconst.h:91-94) */
+    double kappa;     /* DH inverse Debye length (src/potentials.cpp:770-782) */
+    double g2g_cut;   /* NonbondedCutg2g cutoff_g2g (A); <= 0 means infinity */
+    int ntypes;
+    const double *type_sigma, *type_eps, *type_charge, *type_radius, *type_dp;
+    /* NonbondedEwald (ewald.h:507-527) */
+    double ewald_alpha, ewald_rc, ewald_kc, ewald_debyelength, ewald_eps_surf;
+    int ewald_spherical_sum, ewald_update_frequency;
+} fgx_hamiltonian;
+
+typedef struct {
+    long trials, accepted;
+    double utot;        /* sum of accepted dU plus Ewald drift corrections */
+    double dusum;       /* Propagator dusum */
+    long rejected_inf;  /* trials rejected with dU = +inf (overlap) */
+    long ewald_refresh; /* full S(k) recomputes */
+    double seconds;     /* device time of the call */
+} fgx_stats;
+
+const char *fgx_last_error(void);
+int fgx_device_info(char *name, int cap, int *cu_count, size_t *hbm_bytes);
+
+/* Space construction + Space::load (space.h:807-880): particles as SoA,
+ * groups as [front, back] ranges with molsize (1 = atomic) and molId. */
+int fgx_create(const fgx_hamiltonian *h, const double box[3], int n,
+               const double *xyz, const double *charge, const double *radius,
+               const double *mw, const int *type,
+               int ng, const int *front, const int *back,
+               const int *molsize, const int *molid, fgx_system **out);
+int fgx_destroy(fgx_system *s);
+size_t fgx_device_bytes(const fgx_system *s);
+
+/* explicit transfers (the handle never retains host pointers) */
+int fgx_set_positions(fgx_system *s, int which, int first, int count, const double *xyz);
+int fgx_get_positions(fgx_system *s, int which, double *xyz);
+int fgx_get_group_cm(fgx_system *s, int g, double cm[3], double cm_trial[3]);
+/* Group::cm / Group::cm_trial (group.h:55-56), e.g. after a host-side
+ * Group::translate / newrotate of the trial vector */
+int fgx_set_group_cm(fgx_system *s, int g, const double cm[3], const double cm_trial[3]);
+
+/* Energy::Energybase entry points (energy.h:54-224) */
+int fgx_i2i(fgx_system *s, int which, int i, int j, double *u);        /* energy.h:842 */
+int fgx_i2g(fgx_system *s, int which, int g, int j, double *u);        /* energy.h:847 */
+int fgx_i2all(fgx_system *s, int which, int i, double *u);             /* energy.h:867 */
+int fgx_g2g(fgx_system *s, int which, int g1, int g2, double *u);      /* energy.h:879 */
+int fgx_g_internal(fgx_system *s, int which, int g, double *u);        /* energy.h:930 */
+int fgx_g_external(fgx_system *s, int which, int g, double *u);        /* ewald.h:712 */
+int fgx_external(fgx_system *s, int which, double *u);                 /* ewald.h:716 */
+/* Energy::systemEnergy (energy.h:3486-3491 -> 169-180) */
+int fgx_system_energy(fgx_system *s, int which, double *u);
+
+/* A Space::Change (space.h:287-318) restricted to one moved group:
+ * `count` moved indices of group g, or count = 0 meaning the whole group
+ * (TranslateRotate).  Trial coordinates must already be uploaded with
+ * fgx_set_positions(which = 1). */
+typedef struct {
+    int group;
+    int count;
+    const int *index;
+} fgx_change;
+
+/* Energybase::updateChange (ewald.h:651-705): incremental trial S(k) */
+int fgx_update_change(fgx_system *s, const fgx_change *c);
+/* Energy::energyChange (energy.h:3577-3621) */
+int fgx_energy_change(fgx_system *s, const fgx_change *c, double *du);
+/* accept/reject bookkeeping of the moves (move.h:866-889, group.h:371-389)
+ * followed by Energybase::update(accepted) (ewald.h:613-648); *drift gets
+ * the value update() returns (Ewald refresh correction, else 0). */
+int fgx_commit(fgx_system *s, const fgx_change *c, int accepted, double *drift);
+
+/* ------------------------------------------------------------------------
+ * Device-resident Propagator (move.h:7703-7926) and Movebase::move
+ * (move.h:515-555).  Moves are registered in Propagator key order.  The
+ * RNG is libstdc++'s mt19937 with generate_canonical<double,53> and
+ * uniform_int_distribution (slump.h:24-126), replayed on the device so the
+ * accept/reject sequence is the reference's.
+ * ------------------------------------------------------------------------ */
+int fgx_add_move(fgx_system *s, int type, int molid, double prob, int peratom, int permol,
+                 const double dir[3], const double dir2[3], double dp, double dprot,
+                 int center_rotation);
+/* global slump engine (slump.h:126): seed, or full state (624 words + pos) */
+int fgx_rng_seed(fgx_system *s, uint32_t seed);
+int fgx_rng_set_state(fgx_system *s, const uint32_t words[624], int pos);
+int fgx_rng_get_state(fgx_system *s, uint32_t words[624], int *pos);
+/* Propagator constructor: copies the global engine into the move engine
+ * (move.h:7781) */
+int fgx_propagator_init(fgx_system *s);
+/* run `ncalls` Propagator::move() calls on the device.  Optional per-trial
+ * record (dU and accept flag) up to `cap` trials. */
+int fgx_propagate(fgx_system *s, int ncalls, fgx_stats *st, double *du_out, int *acc_out,
+                  long cap);
+
+/* ------------------------------------------------------------------------
+ * Batched chains: many independent systems advanced by one persistent
+ * launch (one workgroup per chain).  Systems keep their own state.
+ * ------------------------------------------------------------------------ */
+int fgx_propagate_batch(fgx_system **systems, int nsys, int ncalls, fgx_stats *st);
+
+/* ------------------------------------------------------------------------
+ * ParallelTempering (move.h:6520-6772) for R replicas on one device.  The
+ * replicas differ in Bjerrum length (temperature enters through lB,
+ * physconst.h:91-94).  An exchange draws the partner direction from a
+ * shared controller engine (MPIController::random, mpi.h:48) and one
+ * Metropolis uniform from the shared global engine; pair energies are
+ * evaluated on the device (Energy::systemEnergy).  Configurations are
+ * swapped by exchanging replica labels, not coordinates.
+ * ------------------------------------------------------------------------ */
+typedef struct fgx_temper fgx_temper;
+int fgx_temper_create(fgx_system **replicas, int R, uint32_t ctl_seed, uint32_t glob_seed,
+                      fgx_temper **out);
+/* one exchange attempt; acc[r] = 1 if replica r swapped; du[r] = pair dU */
+int fgx_temper_exchange(fgx_temper *t, int *acc, double *du);
+/* slot[r] = index of the configuration (system) replica r's Hamiltonian
+ * currently holds */
+int fgx_temper_slots(fgx_temper *t, int *slot);
+int fgx_temper_destroy(fgx_temper *t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

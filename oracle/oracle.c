@@ -948,6 +948,11 @@ void or_set_pos(or_sys *s, int which, int i, const double *xyz)
     for (int k = 0; k < 3; ++k) p[i].r[k] = xyz[k];
 }
 
+void or_set_cm_trial(or_sys *s, int g, const double *cm_trial)
+{
+    memcpy(s->g[g].cm_trial, cm_trial, 3 * sizeof(double));
+}
+
 void or_get_cm(or_sys *s, int g, double *cm, double *cm_trial)
 {
     if (cm) memcpy(cm, s->g[g].cm, 3 * sizeof(double));

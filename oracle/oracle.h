@@ -80,6 +80,7 @@ void    or_destroy(or_sys *s);
 void   or_get_xyz(or_sys *s, int which, double *xyz);
 void   or_set_pos(or_sys *s, int which, int i, const double *xyz);
 void   or_get_cm(or_sys *s, int g, double *cm, double *cm_trial);
+void   or_set_cm_trial(or_sys *s, int g, const double *cm_trial);
 
 /* ---------------- energies (Energy::Nonbonded family) ---------------- */
 double or_pair(or_sys *s, int which, int i, int j);
