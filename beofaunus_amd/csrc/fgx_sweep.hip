@@ -901,11 +901,11 @@ static int run_chains(std::vector<fgx_system *> &sys, int ncalls, double *du_rec
                       s->moves[0].type == s0->moves[0].type &&
                       s->pot.precision == s0->pot.precision,
                   FGX_EUNSUPPORTED, "batched chains must share move type and precision");
-        if (!s->propagator_started) {
-            // Propagator: uinit = systemEnergy before the first move (move.h:7884-7885)
-            if ((rc = fgx_system_energy(s, 0, &s->uinit))) break;
-            s->propagator_started = true;
-        }
+        // Propagator::move evaluates uinit = systemEnergy before its first move
+        // (move.h:7884-7885) only for drift reporting; S(k) is already the full
+        // sum over the accepted vector, as setSpace would leave it, so the
+        // O(N^2) energy is left to callers that ask for it (fgx_system_energy).
+        s->propagator_started = true;
         if (s->move_cnt.empty()) s->move_cnt.assign(s->moves.size(), 0);
         if (s->move_cnt[0] == 0 && (rc = init_mass_centres(s))) break;
         FGX_HIP(hipMemcpyAsync(s->d_rng, s->rng_global, sizeof(uint32_t) * 625, hipMemcpyHostToDevice, s->stream));

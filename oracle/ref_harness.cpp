@@ -11,12 +11,14 @@
 //   ref_harness rng <seed> <n>
 //   ref_harness energy <kind> <config.json> <statefile>
 //   ref_harness mc <kind> <config.json> <statefile> <ncalls>
+//   ref_harness mctime <kind> <config.json> <statefile> <ncalls>   (CPU baseline)
 // kind: dhhs | coulhs | lj | dh | ewaldlj | dhhs_cut | dhhs_vec
 // Output: one JSON object on stdout, doubles printed with %.17g.
 
 #include <faunus/faunus.h>
 #include <faunus/ewald.h>
 
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 
@@ -210,6 +212,44 @@ static int run_mc(Tmjson &in, const char *state, int ncalls)
     return 0;
 }
 
+// Timed Metropolis loop for the CPU baseline: same construction as run_mc,
+// but only the loop of `ncalls` move() calls is timed and no O(N^2) system
+// energies are evaluated.  Prints trials, seconds and trials/s.
+template<class Tenergy>
+static int run_mctime(Tmjson &in, const char *state, int ncalls)
+{
+    Tspace spc(in);
+    Tenergy pot(in);
+    spc.load(state, Tspace::RESIZE);
+    Recorder rec;
+    std::unique_ptr<Move::Movebase<Tspace>> mv;
+    auto m = in.at("moves");
+    for (auto i = m.begin(); i != m.end(); ++i) {
+        RecAT<Tspace>::internal().eng = slump.eng;
+        if (i.key() == "atomtranslate") mv.reset(new RecAT<Tspace>(pot, spc, i.value(), &rec));
+        if (i.key() == "moltransrot") mv.reset(new RecTR<Tspace>(pot, spc, i.value(), &rec));
+    }
+    if (!mv) return 2;
+    auto t0 = std::chrono::steady_clock::now();
+    double utot = 0;
+    for (int c = 0; c < ncalls; ++c) {
+        (void)RecAT<Tspace>::internal().range(0, 0);
+        utot += mv->move();
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    double secs = std::chrono::duration<double>(t1 - t0).count();
+    long acc = 0;
+    for (int a : rec.acc) acc += a;
+    std::printf("{\n");
+    jnum("seconds", secs);
+    jnum("trials", (double)rec.du.size());
+    jnum("accepted", (double)acc);
+    jnum("utot", utot);
+    jnum("trials_per_s", rec.du.size() / secs, false);
+    std::printf("\n}\n");
+    return 0;
+}
+
 typedef CombinedPairPotential<DebyeHuckel, HardSphere> TDHHS;
 typedef CombinedPairPotential<Coulomb, HardSphere> TCoulHS;
 
@@ -245,7 +285,8 @@ int main(int argc, char **argv)
     Tmjson in = openjson(argv[3]);
     const char *state = argv[4];
     int ncalls = argc > 5 ? std::atoi(argv[5]) : 1;
-#define DISPATCH(E) return mode == "energy" ? run_energy<E>(in, state) : run_mc<E>(in, state, ncalls)
+#define DISPATCH(E) return mode == "energy" ? run_energy<E>(in, state) : \
+        mode == "mctime" ? run_mctime<E>(in, state, ncalls) : run_mc<E>(in, state, ncalls)
     typedef Energy::Nonbonded<Tspace, TDHHS> E_dhhs;
     typedef Energy::NonbondedVector<Tspace, TDHHS> E_dhhs_vec;
     typedef Energy::NonbondedCutg2g<Tspace, TDHHS> E_dhhs_cut;
