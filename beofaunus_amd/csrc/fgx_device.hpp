@@ -136,6 +136,81 @@ __device__ __forceinline__ float pair_u32(const Pot &P, const float *__restrict_
     return u1 + u2;
 }
 
+
+// Compile-time specialisations for the sweep kernels: only the terms of
+// the chosen Hamiltonian are generated, which keeps register pressure down.
+template <int ELEC, int SR>
+__device__ __forceinline__ double pair_u64_t(const Pot &P, const double *__restrict__ tab, double r2,
+                                             double qa, double qb, double ra, double rb, int ta,
+                                             int tb)
+{
+    double u1 = 0.0, u2 = 0.0;
+    if constexpr (ELEC == ELEC_COULOMB) {
+        u1 = P.lB * qa * qb / sqrt(r2);
+    } else if constexpr (ELEC == ELEC_DH) {
+        double r = sqrt(r2);
+        u1 = P.lB * qa * qb / r * exp(-P.kappa * r);
+    } else if constexpr (ELEC == ELEC_EWALD) {
+        if (r2 <= P.ew_rc2) {
+            double r1 = sqrt(r2);
+            double s = P.ew_kappa / 2.0 / P.ew_alpha;
+            u1 = qa * qb *
+                 (erfc(P.ew_alpha * r1 + s) * exp(P.ew_kappa * r1) +
+                  erfc(P.ew_alpha * r1 - s) * exp(-P.ew_kappa * r1)) /
+                 (2.0 * r1) * tab[TAB_QEQE * P.ntypes2 + ta * P.ntypes + tb] * P.lB;
+        }
+    }
+    if constexpr (SR == SR_HS) {
+        double m = ra + rb;
+        u2 = (r2 < m * m) ? __builtin_inf() : 0.0;
+    } else if constexpr (SR == SR_LJ) {
+        int k = ta * P.ntypes + tb;
+        double x = tab[TAB_S2 * P.ntypes2 + k] / r2;
+        x = x * x * x;
+        u2 = tab[TAB_EPS4 * P.ntypes2 + k] * (x * x - x);
+    }
+    return u1 + u2;
+}
+
+template <int ELEC, int SR>
+__device__ __forceinline__ float pair_u32_t(const Pot &P, const float *__restrict__ tabf, float r2,
+                                            float qa, float qb, float ra, float rb, int ta, int tb)
+{
+    float u1 = 0.f, u2 = 0.f;
+    if constexpr (ELEC == ELEC_COULOMB) {
+        u1 = P.lBf * qa * qb * __builtin_amdgcn_rsqf(r2);
+    } else if constexpr (ELEC == ELEC_DH) {
+        float rinv = __builtin_amdgcn_rsqf(r2);
+        float r = r2 * rinv;
+        u1 = P.lBf * qa * qb * rinv * __builtin_amdgcn_exp2f(-P.kappaf * 1.44269504088896341f * r);
+    } else if constexpr (ELEC == ELEC_EWALD) {
+        if (r2 <= P.ew_rc2f) {
+            float rinv = __builtin_amdgcn_rsqf(r2);
+            float r1 = r2 * rinv;
+            float e;
+            if (P.ew_plain)
+                e = erfcf(P.ew_alphaf * r1) * rinv;
+            else
+                e = (erfcf(P.ew_alphaf * r1 + P.ew_sf) * __expf(P.ew_kappaf * r1) +
+                     erfcf(P.ew_alphaf * r1 - P.ew_sf) * __expf(-P.ew_kappaf * r1)) *
+                    (0.5f * rinv);
+            u1 = qa * qb * e * tabf[TAB_QEQE * P.ntypes2 + ta * P.ntypes + tb] * P.lBf;
+        }
+    }
+    if constexpr (SR == SR_HS) {
+        // conservative contact test: anything within FP32 rounding of contact
+        // reports +inf so the caller settles it in FP64
+        float m = ra + rb;
+        u2 = (r2 < m * m * 1.000001f) ? __builtin_inff() : 0.f;
+    } else if constexpr (SR == SR_LJ) {
+        int k = ta * P.ntypes + tb;
+        float x = tabf[TAB_S2 * P.ntypes2 + k] * __builtin_amdgcn_rcpf(r2);
+        x = x * x * x;
+        u2 = tabf[TAB_EPS4 * P.ntypes2 + k] * (x * x - x);
+    }
+    return u1 + u2;
+}
+
 // fixed-point coordinates: x -> round(x / L * 2^32) as a wrapping int32, so
 // the difference of two coordinates is the minimum image exactly.
 __host__ __device__ __forceinline__ int32_t to_fixed(double x, double inv_len)

@@ -69,22 +69,8 @@ struct ChainDev {
 // ------------------------------------------------------------------------
 // pair-loop helpers
 // ------------------------------------------------------------------------
-template <bool MIX>
-struct Row;
-
-template <>
-struct Row<false> {
-    double nx, ny, nz, ox, oy, oz, q, r;
-    int t;
-};
-template <>
-struct Row<true> {
-    int32_t nx, ny, nz, ox, oy, oz;
-    float q, r;
-    int t;
-};
-
 // u(new, j) - u(old, j) for one partner j, FP64
+template <int ELEC, int SR>
 __device__ __forceinline__ double du_pair64(const ChainDev &C, double nx, double ny, double nz,
                                             double ox, double oy, double oz, double aq, double ar,
                                             int at, int j)
@@ -93,7 +79,7 @@ __device__ __forceinline__ double du_pair64(const ChainDev &C, double nx, double
     double bq = C.q[j], br = C.r[j];
     int bt = C.t[j];
     double r2n, r2o;
-    if (C.pot.nbkind == NB_VECTOR || C.pot.elec == ELEC_EWALD) {
+    if (ELEC == ELEC_EWALD || C.pot.nbkind == NB_VECTOR) {
         double rx, ry, rz;
         vdist_d(C.box, nx, ny, nz, bx, by, bz, rx, ry, rz);
         r2n = rx * rx + (ry * ry + rz * rz);
@@ -103,18 +89,26 @@ __device__ __forceinline__ double du_pair64(const ChainDev &C, double nx, double
         r2n = sqdist_d(C.box, nx, ny, nz, bx, by, bz);
         r2o = sqdist_d(C.box, ox, oy, oz, bx, by, bz);
     }
-    return pair_u64(C.pot, C.tab, r2n, aq, bq, ar, br, at, bt) -
-           pair_u64(C.pot, C.tab, r2o, aq, bq, ar, br, at, bt);
+    return pair_u64_t<ELEC, SR>(C.pot, C.tab, r2n, aq, bq, ar, br, at, bt) -
+           pair_u64_t<ELEC, SR>(C.pot, C.tab, r2o, aq, bq, ar, br, at, bt);
 }
 
-// mixed: FP32 pair energies from fixed-point minimum-image differences
-__device__ __forceinline__ float du_pair32(const ChainDev &C, int32_t nx, int32_t ny, int32_t nz,
-                                           int32_t ox, int32_t oy, int32_t oz, float aq, float ar,
-                                           int at, int j)
+// mixed: FP32 pair energies from fixed-point minimum-image differences.
+// Pairs whose FP32 energy exceeds Pot::exact_f in magnitude (close contacts,
+// where a relative FP32 error would exceed ~1e-7 kT) and every hard-sphere
+// contact decision are redone in FP64 from the FP64 coordinates, so the
+// mixed path keeps |dU - dU_FP64| at the 1e-7 kT level and takes the same
+// overlap decisions as the FP64 path.
+template <int ELEC, int SR>
+__device__ __forceinline__ double du_pair32(const ChainDev &C, int32_t nx, int32_t ny, int32_t nz,
+                                            int32_t ox, int32_t oy, int32_t oz, float aq, float ar,
+                                            int at, int j, double nx64, double ny64, double nz64,
+                                            double ox64, double oy64, double oz64, double aq64,
+                                            double ar64)
 {
     int32_t bx = C.fx[j], by = C.fy[j], bz = C.fz[j];
-    float bq = C.qf[j], br = C.rf[j];
-    int bt = C.t[j];
+    float bq = C.qf[j], br = (SR == SR_HS) ? C.rf[j] : 0.f;
+    int bt = (SR == SR_LJ || ELEC == ELEC_EWALD) ? C.t[j] : 0;
     const float sx = C.box.fscale[0], sy = C.box.fscale[1], sz = C.box.fscale[2];
     float dx = (float)(int32_t)(nx - bx) * sx, dy = (float)(int32_t)(ny - by) * sy,
           dz = (float)(int32_t)(nz - bz) * sz;
@@ -122,8 +116,11 @@ __device__ __forceinline__ float du_pair32(const ChainDev &C, int32_t nx, int32_
     dx = (float)(int32_t)(ox - bx) * sx; dy = (float)(int32_t)(oy - by) * sy;
     dz = (float)(int32_t)(oz - bz) * sz;
     float r2o = dx * dx + dy * dy + dz * dz;
-    return pair_u32(C.pot, C.tabf, r2n, aq, bq, ar, br, at, bt) -
-           pair_u32(C.pot, C.tabf, r2o, aq, bq, ar, br, at, bt);
+    float un = pair_u32_t<ELEC, SR>(C.pot, C.tabf, r2n, aq, bq, ar, br, at, bt);
+    float uo = pair_u32_t<ELEC, SR>(C.pot, C.tabf, r2o, aq, bq, ar, br, at, bt);
+    if (!(fabsf(un) <= C.pot.exact_f) || !(fabsf(uo) <= C.pot.exact_f))
+        return du_pair64<ELEC, SR>(C, nx64, ny64, nz64, ox64, oy64, oz64, aq64, ar64, at, j);
+    return (double)(un - uo);
 }
 
 // ------------------------------------------------------------------------
@@ -167,53 +164,67 @@ struct TrialAT {
     double ox, oy, oz, nx, ny, nz;
 };
 
-template <bool MIX, int KPT>
-__global__ __launch_bounds__(1024) void k_sweep_at(const ChainDev *__restrict__ chains, int ncalls)
+static constexpr int AT_THREADS = 512;
+static constexpr int REFRESH_CHUNK = 16;   // particles per LDS table chunk
+static constexpr int REFRESH_KAX = 16;     // kcc + 1 limit for the table refresh
+
+__device__ __forceinline__ int kpack(int nx, int ny, int nz)
 {
+    return (nx & 0xff) | ((ny + 128) << 8) | ((nz + 128) << 16);
+}
+__device__ __forceinline__ void kunpack(int v, int &nx, int &ny, int &nz)
+{
+    nx = v & 0xff;
+    ny = ((v >> 8) & 0xff) - 128;
+    nz = ((v >> 16) & 0xff) - 128;
+}
+
+// bytes of dynamic LDS a chain needs: S(k) (double2), A_k (double), packed n (int)
+__host__ __device__ inline size_t at_dyn_lds(int K) { return (size_t)K * (16 + 8 + 4); }
+
+template <int ELEC, int SR, bool MIX>
+__global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restrict__ chains,
+                                                         int ncalls)
+{
+    constexpr bool EW = (ELEC == ELEC_EWALD);
     const ChainDev &C = chains[blockIdx.x];
     const int tid = threadIdx.x, nthr = blockDim.x;
     const bool wave0 = tid < 64;
     __shared__ uint32_t mtg[624], mti[624];
-    __shared__ double red[16];
+    __shared__ double red[AT_THREADS / 64];
     __shared__ TrialAT T[2];
     __shared__ double2 etab[2][3 * MAXK_AXIS];   // old / new per-axis phase tables
+    __shared__ double2 rtab[EW ? REFRESH_CHUNK * 3 * REFRESH_KAX : 1];
+    __shared__ double rw[EW ? REFRESH_CHUNK : 1];
+    // Ewald state for the whole launch, LDS-resident: S(k) FP64, A_k, n-triples
+    extern __shared__ double2 dyn[];
+    double2 *S = dyn;
+    double *Ak = (double *)(S + (EW ? C.K : 0));
+    int *kid = (int *)(Ak + (EW ? C.K : 0));
 
     for (int k = tid; k < 624; k += nthr) {
         mtg[k] = C.rng[k];
         mti[k] = C.rng[625 + k];
     }
     WaveMT G{mtg, (int)C.rng[624]}, I{mti, (int)C.rng[625 + 624]};
-
-    // Ewald: each thread owns k = tid + m * nthr, m < KPT
-    const bool ew = (C.pot.elec == ELEC_EWALD);
     const int kcc1 = C.kcc + 1;
-    double S_re[KPT], S_im[KPT], A[KPT], dS_re[KPT], dS_im[KPT];
-    int kx[KPT], ky[KPT], kz[KPT];
-#pragma unroll
-    for (int m = 0; m < KPT; ++m) {
-        int k = tid + m * nthr;
-        bool own = ew && k < C.K;
-        S_re[m] = own ? C.Q[2 * k] : 0.0;
-        S_im[m] = own ? C.Q[2 * k + 1] : 0.0;
-        A[m] = own ? C.aks[k] : 0.0;
-        kx[m] = own ? C.kn[3 * k] : 0;
-        ky[m] = own ? C.kn[3 * k + 1] : 0;
-        kz[m] = own ? C.kn[3 * k + 2] : 0;
-        dS_re[m] = dS_im[m] = 0.0;
+    if constexpr (EW) {
+        for (int k = tid; k < C.K; k += nthr) {
+            S[k] = make_double2(C.Q[2 * k], C.Q[2 * k + 1]);
+            Ak[k] = C.aks[k];
+            kid[k] = kpack(C.kn[3 * k], C.kn[3 * k + 1], C.kn[3 * k + 2]);
+        }
     }
-    // surface dipole M = sum q eff r (wave 0 keeps it; ewald.h:278-292)
+    // surface dipole M = sum q eff r (getSurfaceEnergy, ewald.h:278-292)
     double Mx = 0, My = 0, Mz = 0;
-    int ew_cnt = ew ? *C.ew_cnt : 0;
+    int ew_cnt = EW ? *C.ew_cnt : 0;
     long n_trials = 0, n_acc = 0, n_inf = 0, n_ref = 0;
     double utot = 0.0, dusum = 0.0;
     long rec = 0;
-    if (ew && C.surf_pref != 0.0) {
+    if (EW && C.surf_pref != 0.0) {
         for (int j = tid; j < C.n; j += nthr) {
             double w = C.q[j] * C.eff[C.t[j]];
-            Mx += C.q[j] * C.x[j] * C.eff[C.t[j]];
-            My += C.q[j] * C.y[j] * C.eff[C.t[j]];
-            Mz += C.q[j] * C.z[j] * C.eff[C.t[j]];
-            (void)w;
+            Mx += w * C.x[j]; My += w * C.y[j]; Mz += w * C.z[j];
         }
         Mx = block_sum(Mx, red);
         My = block_sum(My, red);
@@ -232,7 +243,6 @@ __global__ __launch_bounds__(1024) void k_sweep_at(const ChainDev *__restrict__ 
         }
         __syncthreads();
         if (!T[par].go) { par ^= 1; continue; }
-        double utot_call = 0.0;
         for (int rep = 0; rep < C.repeat; ++rep) {
             TrialAT &tr = T[par];
             // ---- AtomicTranslation::_trialMove (move.h:818-864), wave 0
@@ -262,60 +272,64 @@ __global__ __launch_bounds__(1024) void k_sweep_at(const ChainDev *__restrict__ 
             __syncthreads();
             const int i = tr.i;
             double acc = 0.0;
-            // ---- pair part: sum_{j != i} u(i', j) - u(i, j)
-            if (!MIX) {
-                const double aq = C.q[i], ar = C.r[i];
-                const int at = C.t[i];
-                const double nx = tr.nx, ny = tr.ny, nz = tr.nz, ox = tr.ox, oy = tr.oy,
-                             oz = tr.oz;
-                for (int j = tid; j < C.n; j += nthr) {
-                    if (j == i) continue;
-                    acc += du_pair64(C, nx, ny, nz, ox, oy, oz, aq, ar, at, j);
-                }
-            } else {
-                const float aq = C.qf[i], ar = C.rf[i];
-                const int at = C.t[i];
-                const int32_t nx = to_fixed(tr.nx, C.box.inv[0]), ny = to_fixed(tr.ny, C.box.inv[1]),
-                              nz = to_fixed(tr.nz, C.box.inv[2]);
-                const int32_t ox = C.fx[i], oy = C.fy[i], oz = C.fz[i];
-                for (int j = tid; j < C.n; j += nthr) {
-                    if (j == i) continue;
-                    acc += (double)du_pair32(C, nx, ny, nz, ox, oy, oz, aq, ar, at, j);
-                }
-            }
-            // ---- reciprocal part: dS_k and A_k (|S+dS|^2 - |S|^2)
-            if (ew) {
-                // per-axis phase tables for old (0) and new (1) position
+            // ---- Ewald phase tables exp(i 2 pi n x / L) for old and new position
+            if constexpr (EW) {
                 const int ntab = 2 * 3 * kcc1;
                 for (int e = tid; e < ntab; e += nthr) {
                     int which = e / (3 * kcc1), rem = e % (3 * kcc1);
                     int axis = rem / kcc1, nn = rem % kcc1;
                     double coord = which ? (axis == 0 ? tr.nx : axis == 1 ? tr.ny : tr.nz)
                                          : (axis == 0 ? tr.ox : axis == 1 ? tr.oy : tr.oz);
-                    double ang = 2.0 * M_PI * (double)nn * coord * C.box.inv[axis];
-                    double s, c;
-                    sincos(ang, &s, &c);
-                    etab[which][axis * kcc1 + nn] = make_double2(c, s);
+                    double sn, cs;
+                    sincospi(2.0 * (double)nn * coord * C.box.inv[axis], &sn, &cs);
+                    etab[which][axis * kcc1 + nn] = make_double2(cs, sn);
                 }
-                __syncthreads();
-                const double w = C.q[i] * C.eff[C.t[i]];
+            }
+            // ---- pair part: sum_{j != i} u(i', j) - u(i, j)
+            if constexpr (!MIX) {
+                const double aq = C.q[i], ar = C.r[i];
+                const int at = C.t[i];
+                const double nx = tr.nx, ny = tr.ny, nz = tr.nz, ox = tr.ox, oy = tr.oy,
+                             oz = tr.oz;
+                for (int j = tid; j < C.n; j += nthr) {
+                    if (j == i) continue;
+                    acc += du_pair64<ELEC, SR>(C, nx, ny, nz, ox, oy, oz, aq, ar, at, j);
+                }
+            } else {
+                const float aq = C.qf[i], ar = C.rf[i];
+                const double aq64 = C.q[i], ar64 = C.r[i];
+                const int at = C.t[i];
+                const int32_t nx = to_fixed(tr.nx, C.box.inv[0]), ny = to_fixed(tr.ny, C.box.inv[1]),
+                              nz = to_fixed(tr.nz, C.box.inv[2]);
+                const int32_t ox = C.fx[i], oy = C.fy[i], oz = C.fz[i];
+                const double nx64 = tr.nx, ny64 = tr.ny, nz64 = tr.nz, ox64 = tr.ox, oy64 = tr.oy,
+                             oz64 = tr.oz;
+                for (int j = tid; j < C.n; j += nthr) {
+                    if (j == i) continue;
+                    acc += du_pair32<ELEC, SR>(C, nx, ny, nz, ox, oy, oz, aq, ar, at, j, nx64, ny64,
+                                               nz64, ox64, oy64, oz64, aq64, ar64);
+                }
+            }
+            // ---- reciprocal part: A_k (|S+dS|^2 - |S|^2) per k, FP64
+            const double wq = EW ? C.q[i] * C.eff[C.t[i]] : 0.0;
+            if constexpr (EW) {
+                __syncthreads();   // etab complete
                 double de = 0.0;
-#pragma unroll
-                for (int m = 0; m < KPT; ++m) {
+                for (int k = tid; k < C.K; k += nthr) {
+                    int nx, ny, nz;
+                    kunpack(kid[k], nx, ny, nz);
                     double er, ei, orr, oi;
-                    ew_phase(etab[1], kcc1, kx[m], ky[m], kz[m], er, ei);
-                    ew_phase(etab[0], kcc1, kx[m], ky[m], kz[m], orr, oi);
-                    dS_re[m] = w * (er - orr);
-                    dS_im[m] = w * (ei - oi);
-                    de += A[m] * (2.0 * (S_re[m] * dS_re[m] + S_im[m] * dS_im[m]) +
-                                  (dS_re[m] * dS_re[m] + dS_im[m] * dS_im[m]));
+                    ew_phase(etab[1], kcc1, nx, ny, nz, er, ei);
+                    ew_phase(etab[0], kcc1, nx, ny, nz, orr, oi);
+                    double dr = wq * (er - orr), di = wq * (ei - oi);
+                    double2 sk = S[k];
+                    de += Ak[k] * (2.0 * (sk.x * dr + sk.y * di) + (dr * dr + di * di));
                 }
                 acc += C.recip_pref * de;
             }
             double du = block_sum(acc, red);
-            if (ew && C.surf_pref != 0.0) {
-                const double w = C.q[i] * C.eff[C.t[i]];
-                double dMx = w * (tr.nx - tr.ox), dMy = w * (tr.ny - tr.oy), dMz = w * (tr.nz - tr.oz);
+            if (EW && C.surf_pref != 0.0) {
+                double dMx = wq * (tr.nx - tr.ox), dMy = wq * (tr.ny - tr.oy), dMz = wq * (tr.nz - tr.oz);
                 double Nx = Mx + dMx, Ny = My + dMy, Nz = Mz + dMz;
                 du += C.surf_pref * ((Nx * Nx + (Ny * Ny + Nz * Nz)) - (Mx * Mx + (My * My + Mz * Mz)));
             }
@@ -335,65 +349,92 @@ __global__ __launch_bounds__(1024) void k_sweep_at(const ChainDev *__restrict__ 
             rec++;
             if (accept) {
                 n_acc++;
-                utot_call += du;
                 dusum += du;
+                utot += du;
                 if (tid == 0) {   // AtomicTranslation::_acceptMove (move.h:866-877)
                     C.x[i] = tr.nx; C.y[i] = tr.ny; C.z[i] = tr.nz;
                     C.fx[i] = to_fixed(tr.nx, C.box.inv[0]);
                     C.fy[i] = to_fixed(tr.ny, C.box.inv[1]);
                     C.fz[i] = to_fixed(tr.nz, C.box.inv[2]);
                 }
-                if (ew) {
-                    const double w = C.q[i] * C.eff[C.t[i]];
-                    Mx += w * (tr.nx - tr.ox); My += w * (tr.ny - tr.oy); Mz += w * (tr.nz - tr.oz);
-#pragma unroll
-                    for (int m = 0; m < KPT; ++m) { S_re[m] += dS_re[m]; S_im[m] += dS_im[m]; }
-                    // NonbondedEwald::update (ewald.h:626-643): full refresh
+                if constexpr (EW) {
+                    Mx += wq * (tr.nx - tr.ox); My += wq * (tr.ny - tr.oy); Mz += wq * (tr.nz - tr.oz);
+                    // S(k) += dS(k): same k ownership as above, so no race
+                    for (int k = tid; k < C.K; k += nthr) {
+                        int nx, ny, nz;
+                        kunpack(kid[k], nx, ny, nz);
+                        double er, ei, orr, oi;
+                        ew_phase(etab[1], kcc1, nx, ny, nz, er, ei);
+                        ew_phase(etab[0], kcc1, nx, ny, nz, orr, oi);
+                        double2 sk = S[k];
+                        S[k] = make_double2(sk.x + wq * (er - orr), sk.y + wq * (ei - oi));
+                    }
+                    // NonbondedEwald::update (ewald.h:626-643): full recompute of
+                    // S(k) every update_frequency accepted moves; the returned
+                    // reciprocal-energy difference enters utot
                     if (++ew_cnt > C.update_frequency - 1) {
-                        __syncthreads();   // commit of x[i] visible
+                        __syncthreads();   // committed x[i] visible
                         double eb = 0.0;
-#pragma unroll
-                        for (int m = 0; m < KPT; ++m) eb += A[m] * (S_re[m] * S_re[m] + S_im[m] * S_im[m]);
+                        for (int k = tid; k < C.K; k += nthr)
+                            eb += Ak[k] * (S[k].x * S[k].x + S[k].y * S[k].y);
                         eb = block_sum(eb, red);
-#pragma unroll
-                        for (int m = 0; m < KPT; ++m) { S_re[m] = 0.0; S_im[m] = 0.0; }
-                        double mx = 0, my = 0, mz = 0;
-                        for (int j0 = 0; j0 < C.n; j0 += 32) {
-                            int nj = min(32, C.n - j0);
-                            // tables for 32 particles: reuse etab as scratch is too small,
-                            // so evaluate the phase directly from the FP64 k-vector
-                            for (int jj = 0; jj < nj; ++jj) {
-                                int j = j0 + jj;
-                                double wj = C.q[j] * C.eff[C.t[j]];
-                                double xj = C.x[j], yj = C.y[j], zj = C.z[j];
-#pragma unroll
-                                for (int m = 0; m < KPT; ++m) {
-                                    if (A[m] == 0.0 && kx[m] == 0 && ky[m] == 0 && kz[m] == 0) continue;
-                                    double ph = 2.0 * M_PI *
-                                                ((double)kx[m] * xj * C.box.inv[0] +
-                                                 (double)ky[m] * yj * C.box.inv[1] +
-                                                 (double)kz[m] * zj * C.box.inv[2]);
-                                    double s, c;
-                                    sincos(ph, &s, &c);
-                                    S_re[m] += wj * c;
-                                    S_im[m] += wj * s;
+                        for (int k = tid; k < C.K; k += nthr) S[k] = make_double2(0.0, 0.0);
+                        const bool tabled = kcc1 <= REFRESH_KAX;
+                        const int per = 3 * kcc1;
+                        for (int j0 = 0; j0 < C.n; j0 += REFRESH_CHUNK) {
+                            const int nj = min(REFRESH_CHUNK, C.n - j0);
+                            __syncthreads();
+                            if (tabled) {
+                                for (int e = tid; e < nj * per; e += nthr) {
+                                    int jj = e / per, rem = e % per;
+                                    int axis = rem / kcc1, nn = rem % kcc1;
+                                    int j = j0 + jj;
+                                    double coord = axis == 0 ? C.x[j] : axis == 1 ? C.y[j] : C.z[j];
+                                    double sn, cs;
+                                    sincospi(2.0 * (double)nn * coord * C.box.inv[axis], &sn, &cs);
+                                    rtab[jj * per + axis * kcc1 + nn] = make_double2(cs, sn);
                                 }
-                                if (tid == 0) { mx += wj * xj; my += wj * yj; mz += wj * zj; }
+                            }
+                            for (int jj = tid; jj < nj; jj += nthr)
+                                rw[jj] = C.q[j0 + jj] * C.eff[C.t[j0 + jj]];
+                            __syncthreads();
+                            for (int k = tid; k < C.K; k += nthr) {
+                                int nx, ny, nz;
+                                kunpack(kid[k], nx, ny, nz);
+                                double sr = 0.0, si = 0.0;
+                                for (int jj = 0; jj < nj; ++jj) {
+                                    double er, ei;
+                                    if (tabled) {
+                                        ew_phase(&rtab[jj * per], kcc1, nx, ny, nz, er, ei);
+                                    } else {
+                                        int j = j0 + jj;
+                                        double ph = 2.0 * ((double)nx * C.x[j] * C.box.inv[0] +
+                                                           (double)ny * C.y[j] * C.box.inv[1] +
+                                                           (double)nz * C.z[j] * C.box.inv[2]);
+                                        sincospi(ph, &ei, &er);
+                                    }
+                                    sr += rw[jj] * er;
+                                    si += rw[jj] * ei;
+                                }
+                                double2 sk = S[k];
+                                S[k] = make_double2(sk.x + sr, sk.y + si);
                             }
                         }
                         double ea = 0.0;
-#pragma unroll
-                        for (int m = 0; m < KPT; ++m) ea += A[m] * (S_re[m] * S_re[m] + S_im[m] * S_im[m]);
+                        for (int k = tid; k < C.K; k += nthr)
+                            ea += Ak[k] * (S[k].x * S[k].x + S[k].y * S[k].y);
                         ea = block_sum(ea, red);
                         if (C.surf_pref != 0.0) {
-                            // keep M consistent with the refreshed sums
-                            mx = __shfl(mx, 0, 64);
-                            my = __shfl(my, 0, 64);
-                            mz = __shfl(mz, 0, 64);
-                            if (wave0) { Mx = mx; My = my; Mz = mz; }
+                            double mx = 0, my = 0, mz = 0;
+                            for (int j = tid; j < C.n; j += nthr) {
+                                double w2 = C.q[j] * C.eff[C.t[j]];
+                                mx += w2 * C.x[j]; my += w2 * C.y[j]; mz += w2 * C.z[j];
+                            }
+                            Mx = block_sum(mx, red);
+                            My = block_sum(my, red);
+                            Mz = block_sum(mz, red);
                         }
-                        double d = C.recip_pref * (ea - eb);
-                        utot_call += d;
+                        utot += C.recip_pref * (ea - eb);
                         ew_cnt = 0;
                         n_ref++;
                     }
@@ -403,7 +444,6 @@ __global__ __launch_bounds__(1024) void k_sweep_at(const ChainDev *__restrict__ 
             }
             par ^= 1;
         }
-        utot += utot_call;
     }
     // ---- write back chain state
     __syncthreads();
@@ -420,14 +460,10 @@ __global__ __launch_bounds__(1024) void k_sweep_at(const ChainDev *__restrict__ 
         C.stats_i[3] += n_ref;
         C.stats_d[0] += utot;
         C.stats_d[1] += dusum;
-        if (ew) *C.ew_cnt = ew_cnt;
+        if (EW) *C.ew_cnt = ew_cnt;
     }
-    if (ew) {
-#pragma unroll
-        for (int m = 0; m < KPT; ++m) {
-            int k = tid + m * nthr;
-            if (k < C.K) { C.Q[2 * k] = S_re[m]; C.Q[2 * k + 1] = S_im[m]; }
-        }
+    if constexpr (EW) {
+        for (int k = tid; k < C.K; k += nthr) { C.Q[2 * k] = S[k].x; C.Q[2 * k + 1] = S[k].y; }
     }
 }
 
@@ -477,8 +513,8 @@ __device__ __forceinline__ void translate_pt(const Box &b, const TrialTR &T, dou
     boundary_d(b, x, y, z);
 }
 
-template <bool MIX>
-__global__ __launch_bounds__(1024) void k_sweep_tr(const ChainDev *__restrict__ chains, int ncalls)
+template <int ELEC, int SR, bool MIX>
+__global__ __launch_bounds__(AT_THREADS) void k_sweep_tr(const ChainDev *__restrict__ chains, int ncalls)
 {
     const ChainDev &C = chains[blockIdx.x];
     const int tid = threadIdx.x, nthr = blockDim.x;
@@ -629,7 +665,7 @@ __global__ __launch_bounds__(1024) void k_sweep_tr(const ChainDev *__restrict__ 
                             } else {
                                 r2 = sqdist_d(C.box, sn[0][s], sn[1][s], sn[2][s], bx, by, bz);
                             }
-                            un = pair_u64(C.pot, C.tab, r2, sq[s], bq, sr[s], br, st[s], bt);
+                            un = pair_u64_t<ELEC, SR>(C.pot, C.tab, r2, sq[s], bq, sr[s], br, st[s], bt);
                         }
                         if (ko) {
                             double r2;
@@ -640,7 +676,7 @@ __global__ __launch_bounds__(1024) void k_sweep_tr(const ChainDev *__restrict__ 
                             } else {
                                 r2 = sqdist_d(C.box, so[0][s], so[1][s], so[2][s], bx, by, bz);
                             }
-                            uo = pair_u64(C.pot, C.tab, r2, sq[s], bq, sr[s], br, st[s], bt);
+                            uo = pair_u64_t<ELEC, SR>(C.pot, C.tab, r2, sq[s], bq, sr[s], br, st[s], bt);
                         }
                         acc += un - uo;
                     }
@@ -657,15 +693,24 @@ __global__ __launch_bounds__(1024) void k_sweep_tr(const ChainDev *__restrict__ 
                             float dx = (float)(int32_t)(snf[0][s] - bx) * fsx;
                             float dy = (float)(int32_t)(snf[1][s] - by) * fsy;
                             float dz = (float)(int32_t)(snf[2][s] - bz) * fsz;
-                            un = pair_u32(C.pot, C.tabf, dx * dx + dy * dy + dz * dz, aq, bq, ar, br, at, bt);
+                            un = pair_u32_t<ELEC, SR>(C.pot, C.tabf, dx * dx + dy * dy + dz * dz, aq, bq, ar, br, at, bt);
                         }
                         if (ko) {
                             float dx = (float)(int32_t)(sof[0][s] - bx) * fsx;
                             float dy = (float)(int32_t)(sof[1][s] - by) * fsy;
                             float dz = (float)(int32_t)(sof[2][s] - bz) * fsz;
-                            uo = pair_u32(C.pot, C.tabf, dx * dx + dy * dy + dz * dz, aq, bq, ar, br, at, bt);
+                            uo = pair_u32_t<ELEC, SR>(C.pot, C.tabf, dx * dx + dy * dy + dz * dz, aq, bq, ar, br, at, bt);
                         }
-                        acc += (double)(un - uo);
+                        if (!(fabsf(un) <= C.pot.exact_f) || !(fabsf(uo) <= C.pot.exact_f)) {
+                            // strong pair or contact decision: FP64 (same as the FP64 path)
+                            double jx = C.x[j], jy = C.y[j], jz = C.z[j], jq = C.q[j], jr = C.r[j];
+                            double u64n = 0.0, u64o = 0.0;
+                            if (kn) u64n = pair_u64_t<ELEC, SR>(C.pot, C.tab, sqdist_d(C.box, sn[0][s], sn[1][s], sn[2][s], jx, jy, jz), sq[s], jq, sr[s], jr, at, bt);
+                            if (ko) u64o = pair_u64_t<ELEC, SR>(C.pot, C.tab, sqdist_d(C.box, so[0][s], so[1][s], so[2][s], jx, jy, jz), sq[s], jq, sr[s], jr, at, bt);
+                            acc += u64n - u64o;
+                        } else {
+                            acc += (double)(un - uo);
+                        }
                     }
                 }
             }
@@ -728,21 +773,8 @@ __global__ __launch_bounds__(1024) void k_sweep_tr(const ChainDev *__restrict__ 
 
 static int chain_block(const fgx_system *s)
 {
-    if (s->moves.empty()) return 256;
-    if (s->n >= 8192) return 1024;
-    if (s->n >= 2048) return 512;
+    if (s->n >= 2048) return AT_THREADS;
     return 256;
-}
-
-static int kpt_for(const fgx_system *s, int nthr)
-{
-    if (!s->has_ewald) return 1;
-    int need = (s->ew.K + nthr - 1) / nthr;
-    if (need <= 1) return 1;
-    if (need <= 2) return 2;
-    if (need <= 4) return 4;
-    if (need <= 8) return 8;
-    return 16;
 }
 
 struct Prepared {
@@ -769,6 +801,8 @@ static int prepare_chain(fgx_system *s, ChainDev &C, int **d_gvec, double *du_re
                       "device AtomicTranslation: moved groups must be atomic");
         FGX_CHECK(!s->has_ewald || s->ew.kcc < MAXK_AXIS, FGX_EUNSUPPORTED,
                   "device Ewald: cutoffK too large");
+        FGX_CHECK(!s->has_ewald || at_dyn_lds(s->ew.K) <= 120 * 1024, FGX_EUNSUPPORTED,
+                  "device Ewald: too many k-vectors for the LDS-resident S(k)");
     } else {
         FGX_CHECK(!s->has_ewald, FGX_EUNSUPPORTED, "device TranslateRotate with Ewald");
         FGX_CHECK(s->ng <= MAX_GROUPS_LDS, FGX_EUNSUPPORTED, "device TranslateRotate: too many groups");
@@ -871,17 +905,48 @@ static int init_mass_centres(fgx_system *s)
     return FGX_OK;
 }
 
+template <int ELEC, int SR, bool MIX>
+static void launch_at_k(hipStream_t st, const ChainDev *d_chains, int nchains, int nthr,
+                        size_t dyn, int ncalls)
+{
+    if (dyn > 0)
+        (void)hipFuncSetAttribute((const void *)k_sweep_at<ELEC, SR, MIX>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+    hipLaunchKernelGGL((k_sweep_at<ELEC, SR, MIX>), dim3(nchains), dim3(nthr), dyn, st, d_chains,
+                       ncalls);
+}
+
 template <bool MIX>
-static int launch_at(hipStream_t st, const ChainDev *d_chains, int nchains, int nthr, int kpt,
+static int launch_at(const Pot &P, hipStream_t st, const ChainDev *d_chains, int nchains, int nthr,
+                     size_t dyn, int ncalls)
+{
+#define FGX_AT(E, S) \
+    if (P.elec == E && P.sr == S) { launch_at_k<E, S, MIX>(st, d_chains, nchains, nthr, dyn, ncalls); }
+    FGX_AT(ELEC_NONE, SR_HS) else FGX_AT(ELEC_NONE, SR_LJ) else
+    FGX_AT(ELEC_COULOMB, SR_NONE) else FGX_AT(ELEC_COULOMB, SR_HS) else FGX_AT(ELEC_COULOMB, SR_LJ) else
+    FGX_AT(ELEC_DH, SR_NONE) else FGX_AT(ELEC_DH, SR_HS) else FGX_AT(ELEC_DH, SR_LJ) else
+    FGX_AT(ELEC_EWALD, SR_NONE) else FGX_AT(ELEC_EWALD, SR_HS) else FGX_AT(ELEC_EWALD, SR_LJ) else {
+        set_error("device sweep: no kernel for this Hamiltonian");
+        return FGX_EUNSUPPORTED;
+    }
+#undef FGX_AT
+    FGX_HIP(hipGetLastError());
+    return FGX_OK;
+}
+
+template <bool MIX>
+static int launch_tr(const Pot &P, hipStream_t st, const ChainDev *d_chains, int nchains, int nthr,
                      int ncalls)
 {
-    switch (kpt) {
-    case 1: hipLaunchKernelGGL((k_sweep_at<MIX, 1>), dim3(nchains), dim3(nthr), 0, st, d_chains, ncalls); break;
-    case 2: hipLaunchKernelGGL((k_sweep_at<MIX, 2>), dim3(nchains), dim3(nthr), 0, st, d_chains, ncalls); break;
-    case 4: hipLaunchKernelGGL((k_sweep_at<MIX, 4>), dim3(nchains), dim3(nthr), 0, st, d_chains, ncalls); break;
-    case 8: hipLaunchKernelGGL((k_sweep_at<MIX, 8>), dim3(nchains), dim3(nthr), 0, st, d_chains, ncalls); break;
-    default: hipLaunchKernelGGL((k_sweep_at<MIX, 16>), dim3(nchains), dim3(nthr), 0, st, d_chains, ncalls); break;
+#define FGX_TR(E, S) \
+    if (P.elec == E && P.sr == S) { hipLaunchKernelGGL((k_sweep_tr<E, S, MIX>), dim3(nchains), dim3(nthr), 0, st, d_chains, ncalls); }
+    FGX_TR(ELEC_NONE, SR_HS) else FGX_TR(ELEC_NONE, SR_LJ) else
+    FGX_TR(ELEC_COULOMB, SR_NONE) else FGX_TR(ELEC_COULOMB, SR_HS) else FGX_TR(ELEC_COULOMB, SR_LJ) else
+    FGX_TR(ELEC_DH, SR_NONE) else FGX_TR(ELEC_DH, SR_HS) else FGX_TR(ELEC_DH, SR_LJ) else {
+        set_error("device TranslateRotate: no kernel for this Hamiltonian");
+        return FGX_EUNSUPPORTED;
     }
+#undef FGX_TR
     FGX_HIP(hipGetLastError());
     return FGX_OK;
 }
@@ -894,12 +959,14 @@ static int run_chains(std::vector<fgx_system *> &sys, int ncalls, double *du_rec
     std::vector<ChainDev> chains(nc);
     std::vector<int *> gvecs(nc, nullptr);
     int rc = FGX_OK;
-    int nthr = 0, kpt = 1;
+    int nthr = 0;
+    size_t dyn = 0;
     for (int c = 0; c < nc && !rc; ++c) {
         fgx_system *s = sys[c];
         FGX_CHECK(s->moves.size() == s0->moves.size() && !s->moves.empty() &&
                       s->moves[0].type == s0->moves[0].type &&
-                      s->pot.precision == s0->pot.precision,
+                      s->pot.precision == s0->pot.precision && s->pot.elec == s0->pot.elec &&
+                      s->pot.sr == s0->pot.sr,
                   FGX_EUNSUPPORTED, "batched chains must share move type and precision");
         // Propagator::move evaluates uinit = systemEnergy before its first move
         // (move.h:7884-7885) only for drift reporting; S(k) is already the full
@@ -923,13 +990,10 @@ static int run_chains(std::vector<fgx_system *> &sys, int ncalls, double *du_rec
         if (!rc && s->has_ewald) chains[c].ew_cnt = s->ew.cnt_dev;
         FGX_HIP(hipStreamSynchronize(s->stream));
         nthr = std::max(nthr, chain_block(s));
-        kpt = std::max(kpt, kpt_for(s, chain_block(s)));
+        if (s->has_ewald) dyn = std::max(dyn, at_dyn_lds(s->ew.K));
     }
     ChainDev *d_chains = nullptr;
     if (!rc) {
-        // recompute kpt for the common block size
-        kpt = 1;
-        for (auto *s : sys) kpt = std::max(kpt, kpt_for(s, nthr));
         FGX_HIP(hipMalloc(&d_chains, sizeof(ChainDev) * nc));
         FGX_HIP(hipMemcpyAsync(d_chains, chains.data(), sizeof(ChainDev) * nc, hipMemcpyHostToDevice,
                                s0->stream));
@@ -939,14 +1003,13 @@ static int run_chains(std::vector<fgx_system *> &sys, int ncalls, double *du_rec
         FGX_HIP(hipEventCreate(&e1));
         FGX_HIP(hipEventRecord(e0, s0->stream));
         if (s0->moves[0].type == MOVE_AT) {
-            rc = s0->pot.precision == FGX_MIXED ? launch_at<true>(s0->stream, d_chains, nc, nthr, kpt, ncalls)
-                                                : launch_at<false>(s0->stream, d_chains, nc, nthr, kpt, ncalls);
+            rc = s0->pot.precision == FGX_MIXED
+                     ? launch_at<true>(s0->pot, s0->stream, d_chains, nc, nthr, dyn, ncalls)
+                     : launch_at<false>(s0->pot, s0->stream, d_chains, nc, nthr, dyn, ncalls);
         } else {
-            if (s0->pot.precision == FGX_MIXED)
-                hipLaunchKernelGGL(k_sweep_tr<true>, dim3(nc), dim3(nthr), 0, s0->stream, d_chains, ncalls);
-            else
-                hipLaunchKernelGGL(k_sweep_tr<false>, dim3(nc), dim3(nthr), 0, s0->stream, d_chains, ncalls);
-            if (hipGetLastError() != hipSuccess) { set_error("k_sweep_tr launch failed"); rc = FGX_EHIP; }
+            rc = s0->pot.precision == FGX_MIXED
+                     ? launch_tr<true>(s0->pot, s0->stream, d_chains, nc, nthr, ncalls)
+                     : launch_tr<false>(s0->pot, s0->stream, d_chains, nc, nthr, ncalls);
         }
         FGX_HIP(hipEventRecord(e1, s0->stream));
         FGX_HIP(hipEventSynchronize(e1));

@@ -661,6 +661,7 @@ extern "C" int fgx_create(const fgx_hamiltonian *h, const double box[3], int n, 
     P.lB = h->lB; P.kappa = h->kappa;
     P.g2g_rc2 = h->g2g_cut > 0 ? h->g2g_cut * h->g2g_cut : HUGE_VAL;
     P.lBf = (float)h->lB; P.kappaf = (float)h->kappa;
+    P.exact_f = 0.5f;
     // tables: LennardJonesMixed<LorentzBerthelot>::init (potentials.h:2010-2019)
     const int T = h->ntypes;
     s->tab.assign((size_t)TAB_COUNT * T * T, 0.0);

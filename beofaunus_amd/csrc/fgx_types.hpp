@@ -28,6 +28,7 @@ struct Pot {
     double ew_alpha, ew_kappa, ew_rc2;
     float lBf, kappaf, ew_alphaf, ew_kappaf, ew_rc2f, ew_sf;
     int ew_plain;  // kappa*rc below 1e-12: screened real space reduces to erfc(ar)/r
+    float exact_f; // mixed path: pairs with |u| above this (kT) are redone in FP64
 };
 
 // One registered move (Movebase::MolListData, move.h:305-330, plus the

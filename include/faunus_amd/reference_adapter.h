@@ -1,0 +1,416 @@
+/*
+ * reference_adapter.h -- the binding a Faunus maintainer adds to route the
+ * Metropolis Delta-U hot path through libfgx (include/fgx.h).
+ *
+ * Header-only C++11.  It is compiled inside a Faunus code base: it includes
+ * the Faunus headers (faunus/faunus.h, faunus/ewald.h) and plugs into their
+ * own extension points, with no change to Faunus itself:
+ *
+ *   Energy::GPUNonbonded<Tspace>     -- an Energy::Energybase<Tspace>
+ *       (energy.h:54-224) whose i2i / i2g / i2all / g2g / g_internal /
+ *       g_external / external / systemEnergy / updateChange / update run on
+ *       the GPU.  It stands in for Energy::Nonbonded<Tspace,DebyeHuckelHS>,
+ *       <CoulombHS>, <LennardJonesLB>, NonbondedVector, NonbondedCutg2g and
+ *       NonbondedEwald<Tspace,LennardJonesLB>.
+ *   Move::GPUAtomicTranslation<Tspace>, Move::GPUTranslateRotate<Tspace>
+ *       -- the reference moves (move.h:730-967, 1505-1801) with
+ *       _energyChange() replaced by one device call per trial (instead of
+ *       2(N-1) virtual i2i calls) and accept/reject mirrored on the device.
+ *   Move::GPUPropagator<Tspace> -- Propagator::move (move.h:7877-7891) run
+ *       entirely on the device: reads the global and move-internal engines
+ *       (slump.h:126, move.h:356-360), writes the coordinates and engines
+ *       back, so the host state afterwards is what the CPU Propagator leaves.
+ *
+ * Errors from libfgx become std::runtime_error, as the reference reports
+ * configuration errors.
+ */
+#ifndef FAUNUS_AMD_REFERENCE_ADAPTER_H
+#define FAUNUS_AMD_REFERENCE_ADAPTER_H
+
+#include <faunus/faunus.h>
+#include <faunus/ewald.h>
+
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../fgx.h"
+
+namespace Faunus {
+namespace AMD {
+
+inline void check(int rc)
+{
+    if (rc != FGX_OK) throw std::runtime_error(std::string("libfgx: ") + fgx_last_error());
+}
+
+/* The Hamiltonian as the reference builds it from its JSON input. */
+struct GPUParams {
+    int elec = FGX_ELEC_DH, sr = FGX_SR_HS, nbkind = FGX_NB_PLAIN, precision = FGX_FP64;
+    double lB = 7.0, kappa = 0.0, g2g_cut = 0.0;
+    double alpha = 0, rc = 0, kc = 0, debyelength = 0, eps_surf = 0;
+    int spherical_sum = 1, update_frequency = -1;
+
+    /* energy.nonbonded section as read by Potential::Coulomb / DebyeHuckel
+     * (src/potentials.cpp:357-363, 770-782), NonbondedCutg2g
+     * (energy.h:1363-1369) and NonbondedEwald (ewald.h:507-527) */
+    static GPUParams fromJson(Tmjson &j, int elec, int sr, int nbkind, int precision = FGX_FP64)
+    {
+        GPUParams p;
+        p.elec = elec; p.sr = sr; p.nbkind = nbkind; p.precision = precision;
+        auto nb = j["energy"]["nonbonded"];
+        p.lB = pc::lB(nb.value("epsr", 80.0));
+        if (elec == FGX_ELEC_DH) {
+            double I = nb.value("ionicstrength", 0.0);
+            double c = 8 * p.lB * pc::pi * pc::Nav / 1e27;
+            double k = std::sqrt(I * c);
+            p.kappa = (k < 1e-10) ? 1 / nb.at("debyelength").get<double>() : k;
+        }
+        if (nbkind == FGX_NB_CUTG2G) {
+            double rc2 = std::pow(nb["cutoff_g2g"] | pc::infty, 2);
+            p.g2g_cut = std::isinf(rc2) ? 0.0 : std::sqrt(rc2);
+        }
+        if (elec == FGX_ELEC_EWALD) {
+            auto e = nb["ewald"];
+            p.eps_surf = e.at("eps_surf");
+            p.spherical_sum = e.value("spherical_sum", true);
+            p.update_frequency = e.value("update_frequency", -1);
+            p.alpha = e.at("alpha");
+            p.rc = e.at("cutoff");
+            p.kc = e.at("cutoffK");
+            p.debyelength = e.at("debyelength");
+        }
+        return p;
+    }
+};
+
+/* Upload a Space (p and groups) and the global atom table. */
+template <class Tspace>
+fgx_system *createSystem(Tspace &spc, const GPUParams &prm)
+{
+    const int n = (int)spc.p.size();
+    std::vector<double> xyz(3 * n), q(n), r(n), mw(n);
+    std::vector<int> t(n);
+    for (int i = 0; i < n; ++i) {
+        xyz[3 * i] = spc.p[i].x(); xyz[3 * i + 1] = spc.p[i].y(); xyz[3 * i + 2] = spc.p[i].z();
+        q[i] = spc.p[i].charge; r[i] = spc.p[i].radius; mw[i] = spc.p[i].mw; t[i] = spc.p[i].id;
+    }
+    std::vector<double> sig, eps, tq, tr, tdp;
+    for (auto &a : atom) {
+        sig.push_back(a.sigma); eps.push_back(a.eps); tq.push_back(a.charge);
+        tr.push_back(a.radius); tdp.push_back(a.dp);
+    }
+    std::vector<int> fr, bk, ms, mi;
+    for (auto g : spc.groupList()) {
+        fr.push_back(g->front()); bk.push_back(g->back());
+        ms.push_back(g->isAtomic() ? 1 : (int)g->size());
+        mi.push_back(g->molId);
+    }
+    fgx_hamiltonian h{};
+    h.elec = prm.elec; h.sr = prm.sr; h.nbkind = prm.nbkind; h.precision = prm.precision;
+    h.lB = prm.lB; h.kappa = prm.kappa; h.g2g_cut = prm.g2g_cut;
+    h.ntypes = (int)sig.size();
+    h.type_sigma = sig.data(); h.type_eps = eps.data(); h.type_charge = tq.data();
+    h.type_radius = tr.data(); h.type_dp = tdp.data();
+    h.ewald_alpha = prm.alpha; h.ewald_rc = prm.rc; h.ewald_kc = prm.kc;
+    h.ewald_debyelength = prm.debyelength; h.ewald_eps_surf = prm.eps_surf;
+    h.ewald_spherical_sum = prm.spherical_sum; h.ewald_update_frequency = prm.update_frequency;
+    double box[3] = {spc.geo.len.x(), spc.geo.len.y(), spc.geo.len.z()};
+    fgx_system *s = nullptr;
+    check(fgx_create(&h, box, n, xyz.data(), q.data(), r.data(), mw.data(), t.data(),
+                     (int)fr.size(), fr.data(), bk.data(), ms.data(), mi.data(), &s));
+    return s;
+}
+
+}  // namespace AMD
+
+namespace Energy {
+
+template <class Tspace>
+class GPUNonbonded : public Energybase<Tspace> {
+    typedef Energybase<Tspace> base;
+    typedef typename base::Tpvec Tpvec;
+    AMD::GPUParams prm;
+    fgx_system *h = nullptr;
+    typename Tspace::Change pending;    // change staged by updateChange
+    std::string _info() override { return "  libfgx on " + device() + "\n"; }
+
+    int which(const Tpvec &p) const { return base::isTrial(p) ? 1 : 0; }
+
+public:
+    explicit GPUNonbonded(const AMD::GPUParams &p) : prm(p) { base::name = "Nonbonded (libfgx, MI355X)"; }
+    ~GPUNonbonded() { if (h) fgx_destroy(h); }
+
+    fgx_system *handle() { return h; }
+
+    static std::string device()
+    {
+        char name[64] = {0};
+        int cu = 0;
+        size_t mem = 0;
+        if (fgx_device_info(name, 64, &cu, &mem) != FGX_OK) return "unknown";
+        std::ostringstream o;
+        o << name << " (" << cu << " CUs)";
+        return o.str();
+    }
+
+    void setSpace(Tspace &s) override
+    {
+        base::setSpace(s);
+        if (!h) h = AMD::createSystem(s, prm);
+        else resync();
+    }
+
+    /* full upload of both vectors and mass centres, after host code outside
+     * the GPU moves changed the Space */
+    void resync()
+    {
+        upload(base::spc->p);
+        upload(base::spc->trial);
+        int gi = 0;
+        for (auto g : base::spc->groupList()) {
+            double cm[3] = {g->cm.x(), g->cm.y(), g->cm.z()};
+            double ct[3] = {g->cm_trial.x(), g->cm_trial.y(), g->cm_trial.z()};
+            AMD::check(fgx_set_group_cm(h, gi++, cm, ct));
+        }
+    }
+
+    void upload(const Tpvec &p)
+    {
+        std::vector<double> xyz(3 * p.size());
+        for (size_t i = 0; i < p.size(); ++i) {
+            xyz[3 * i] = p[i].x(); xyz[3 * i + 1] = p[i].y(); xyz[3 * i + 2] = p[i].z();
+        }
+        AMD::check(fgx_set_positions(h, which(p), 0, (int)p.size(), xyz.data()));
+    }
+
+    double i2i(const Tpvec &p, int i, int j) override
+    {
+        double u; upload(p); AMD::check(fgx_i2i(h, which(p), i, j, &u)); return u;
+    }
+    double i2g(const Tpvec &p, Group &g, int j) override
+    {
+        double u; upload(p); AMD::check(fgx_i2g(h, which(p), index(g), j, &u)); return u;
+    }
+    double i2all(Tpvec &p, int i) override
+    {
+        double u; upload(p); AMD::check(fgx_i2all(h, which(p), i, &u)); return u;
+    }
+    double g2g(const Tpvec &p, Group &a, Group &b) override
+    {
+        double u; upload(p); AMD::check(fgx_g2g(h, which(p), index(a), index(b), &u)); return u;
+    }
+    double g_internal(const Tpvec &p, Group &g) override
+    {
+        double u; upload(p); AMD::check(fgx_g_internal(h, which(p), index(g), &u)); return u;
+    }
+    double g_external(const Tpvec &p, Group &g) override
+    {
+        double u; upload(p); AMD::check(fgx_g_external(h, which(p), index(g), &u)); return u;
+    }
+    double external(const Tpvec &p) override
+    {
+        double u; upload(p); AMD::check(fgx_external(h, which(p), &u)); return u;
+    }
+    double systemEnergy(const Tpvec &p) override
+    {
+        double u; upload(p); AMD::check(fgx_system_energy(h, which(p), &u)); return u;
+    }
+
+    int index(Group &g) { return base::spc->findIndex(&g); }
+
+    /* Energybase::updateChange (ewald.h:651-705): stage the trial
+     * coordinates and mass centre of the moved group and the trial S(k) */
+    double updateChange(const typename Tspace::Change &c) override
+    {
+        pending = c;
+        if (c.mvGroup.size() != 1)
+            throw std::runtime_error("libfgx: a change must move exactly one group");
+        auto &m = *c.mvGroup.begin();
+        Group *g = base::spc->groupList()[m.first];
+        auto &t = base::spc->trial;
+        if (m.second.empty()) {
+            std::vector<double> xyz;
+            for (int i = g->front(); i <= g->back(); ++i) {
+                xyz.push_back(t[i].x()); xyz.push_back(t[i].y()); xyz.push_back(t[i].z());
+            }
+            AMD::check(fgx_set_positions(h, 1, g->front(), (int)g->size(), xyz.data()));
+            double ct[3] = {g->cm_trial.x(), g->cm_trial.y(), g->cm_trial.z()};
+            AMD::check(fgx_set_group_cm(h, m.first, nullptr, ct));
+        } else {
+            for (int i : m.second) {
+                double xyz[3] = {t[i].x(), t[i].y(), t[i].z()};
+                AMD::check(fgx_set_positions(h, 1, i, 1, xyz));
+            }
+            if (g->isMolecular()) {
+                double ct[3] = {g->cm_trial.x(), g->cm_trial.y(), g->cm_trial.z()};
+                AMD::check(fgx_set_group_cm(h, m.first, nullptr, ct));
+            }
+        }
+        fgx_change fc = change(c);
+        AMD::check(fgx_update_change(h, &fc));
+        return 0.0;
+    }
+
+    /* Energy::energyChange (energy.h:3577-3621) of the staged change */
+    double deltaU()
+    {
+        fgx_change fc = change(pending);
+        double du;
+        AMD::check(fgx_energy_change(h, &fc, &du));
+        return du;
+    }
+
+    /* Energybase::update (ewald.h:613-648): mirror accept/reject */
+    double update(bool accepted) override
+    {
+        if (pending.mvGroup.empty()) return 0.0;
+        fgx_change fc = change(pending);
+        double drift = 0.0;
+        AMD::check(fgx_commit(h, &fc, accepted ? 1 : 0, &drift));
+        pending.clear();
+        return drift;
+    }
+
+private:
+    fgx_change change(const typename Tspace::Change &c)
+    {
+        auto &m = *c.mvGroup.begin();
+        fgx_change fc;
+        fc.group = m.first;
+        fc.count = (int)m.second.size();
+        fc.index = m.second.empty() ? nullptr : m.second.data();
+        return fc;
+    }
+};
+
+}  // namespace Energy
+
+namespace Move {
+
+/* AtomicTranslation with the energy change computed by one device call */
+template <class Tspace>
+class GPUAtomicTranslation : public AtomicTranslation<Tspace> {
+    typedef AtomicTranslation<Tspace> base;
+    Energy::GPUNonbonded<Tspace> &gpu;
+
+public:
+    GPUAtomicTranslation(Energy::GPUNonbonded<Tspace> &e, Tspace &s, Tmjson &j)
+        : base(e, s, j), gpu(e) {}
+
+protected:
+    double _energyChange() override
+    {
+        if (this->iparticle < 0) return 0;
+        return gpu.deltaU();
+    }
+};
+
+/* TranslateRotate with the energy change computed by one device call */
+template <class Tspace>
+class GPUTranslateRotate : public TranslateRotate<Tspace> {
+    typedef TranslateRotate<Tspace> base;
+    Energy::GPUNonbonded<Tspace> &gpu;
+
+public:
+    GPUTranslateRotate(Energy::GPUNonbonded<Tspace> &e, Tspace &s, Tmjson &j)
+        : base(e, s, j), gpu(e) {}
+
+protected:
+    double _energyChange() override
+    {
+        if (this->dp_rot < 1e-6 && this->dp_trans < 1e-6) return 0;
+        return gpu.deltaU();
+    }
+};
+
+/* Propagator::move run on the device.  Registers the moves named in the
+ * JSON "moves" section (atomtranslate / moltransrot, one move), seeds the
+ * move engine from the global one as Propagator's constructor does
+ * (move.h:7781) and keeps the host Space and engines in step. */
+template <class Tspace>
+class GPUPropagator {
+    Energy::GPUNonbonded<Tspace> &gpu;
+    Tspace &spc;
+
+public:
+    fgx_stats last{};
+
+    GPUPropagator(Tmjson &in, Energy::GPUNonbonded<Tspace> &e, Tspace &s) : gpu(e), spc(s)
+    {
+        gpu.setSpace(s);
+        auto m = in.at("moves");
+        for (auto i = m.begin(); i != m.end(); ++i) {
+            bool at = i.key() == "atomtranslate", tr = i.key() == "moltransrot";
+            if (!at && !tr) continue;
+            auto js = i.value();
+            bool center = tr ? (bool)js["center_rotation"] : true;
+            for (auto it = js.begin(); it != js.end(); ++it) {
+                auto mol = spc.molList().find(it.key());
+                if (mol == spc.molList().end()) continue;
+                auto v = it.value();
+                Point dir, dir2;
+                dir << (v["dir"] | std::string("1 1 1"));
+                dir2 << (v["dir2"] | std::string("1 1 1"));
+                double d[3] = {dir.x(), dir.y(), dir.z()}, d2[3] = {dir2.x(), dir2.y(), dir2.z()};
+                AMD::check(fgx_add_move(gpu.handle(), at ? FGX_MOVE_AT : FGX_MOVE_TR, mol->id,
+                                        v.value("prob", 1.0), v.value("peratom", false),
+                                        v.value("permol", false), d, d2, v["dp"] | 0.0,
+                                        v["dprot"] | 0.0, center));
+            }
+        }
+        pushGlobalEngine();
+        AMD::check(fgx_propagator_init(gpu.handle()));
+    }
+
+    void pushGlobalEngine()
+    {
+        std::ostringstream o;
+        o << slump.eng;
+        std::istringstream is(o.str());
+        uint32_t w[624];
+        for (int k = 0; k < 624; ++k) { unsigned long v; is >> v; w[k] = (uint32_t)v; }
+        int pos;
+        is >> pos;
+        AMD::check(fgx_rng_set_state(gpu.handle(), w, pos));
+    }
+
+    void pullGlobalEngine()
+    {
+        uint32_t w[624];
+        int pos;
+        AMD::check(fgx_rng_get_state(gpu.handle(), w, &pos));
+        std::ostringstream o;
+        for (int k = 0; k < 624; ++k) o << w[k] << " ";
+        o << pos;
+        std::istringstream is(o.str());
+        is >> slump.eng;
+    }
+
+    /* n Propagator::move() calls; returns the summed energy change */
+    double move(int n = 1)
+    {
+        AMD::check(fgx_propagate(gpu.handle(), n, &last, nullptr, nullptr, 0));
+        std::vector<double> xyz(3 * spc.p.size());
+        AMD::check(fgx_get_positions(gpu.handle(), 0, xyz.data()));
+        for (size_t i = 0; i < spc.p.size(); ++i) {
+            spc.p[i].x() = xyz[3 * i]; spc.p[i].y() = xyz[3 * i + 1]; spc.p[i].z() = xyz[3 * i + 2];
+            spc.trial[i] = spc.p[i];
+        }
+        int gi = 0;
+        for (auto g : spc.groupList()) {
+            double cm[3], ct[3];
+            AMD::check(fgx_get_group_cm(gpu.handle(), gi++, cm, ct));
+            g->cm = Point(cm[0], cm[1], cm[2]);
+            g->cm_trial = g->cm;
+        }
+        pullGlobalEngine();
+        return last.utot;
+    }
+};
+
+}  // namespace Move
+}  // namespace Faunus
+
+#endif

@@ -1,0 +1,174 @@
+// adapter_test.cpp -- TEST INFRASTRUCTURE ONLY.
+//
+// Runs the reference's own Metropolis machinery three ways on the same
+// input (reference JSON + Space::save state file) and prints the results:
+//   A  reference CPU: Energy::Nonbonded / NonbondedEwald + AtomicTranslation
+//      or TranslateRotate (the recorded trajectory)
+//   B  the reference moves with include/faunus_amd/reference_adapter.h's
+//      GPUNonbonded energy and GPU move subclasses (host-driven, one device
+//      call per trial)
+//   C  GPUPropagator: the whole Propagator::move on the device
+// Built from the reference sources in place by oracle/Makefile (target
+// adapter), output in oracle/_ref/, run by tests/test_gpu_adapter.py.
+//
+// Usage: adapter_test <kind> <config.json> <statefile> <ncalls>
+
+#include <faunus/faunus.h>
+#include <faunus/ewald.h>
+
+#include <cstdio>
+#include <memory>
+
+#include "faunus_amd/reference_adapter.h"
+
+using namespace Faunus;
+using namespace Faunus::Potential;
+
+typedef Space<Geometry::Cuboid, PointParticle> Tspace;
+
+struct Rec {
+    std::vector<double> du;
+    std::vector<int> acc;
+};
+
+template <class Base>
+struct Recording : public Base {
+    Rec *rec;
+    template <class E>
+    Recording(E &e, Tspace &s, Tmjson &j, Rec *r) : Base(e, s, j), rec(r) {}
+    static RandomTwister<> &internal() { return Move::Movebase<Tspace>::_slump(); }
+
+protected:
+    double _energyChange() override
+    {
+        double du = Base::_energyChange();
+        rec->du.push_back(du);
+        return du;
+    }
+    void _acceptMove() override { rec->acc.push_back(1); Base::_acceptMove(); }
+    void _rejectMove() override { rec->acc.push_back(0); Base::_rejectMove(); }
+};
+
+static void pvec(const char *k, const std::vector<double> &v, bool comma = true)
+{
+    std::printf("\"%s\": [", k);
+    for (size_t i = 0; i < v.size(); ++i) {
+        if (std::isinf(v[i])) std::printf("\"%s\"", v[i] > 0 ? "inf" : "-inf");
+        else if (std::isnan(v[i])) std::printf("\"nan\"");
+        else std::printf("%.17g", v[i]);
+        std::printf(i + 1 < v.size() ? "," : "");
+    }
+    std::printf("]%s\n", comma ? "," : "");
+}
+
+static void pivec(const char *k, const std::vector<int> &v)
+{
+    std::printf("\"%s\": [", k);
+    for (size_t i = 0; i < v.size(); ++i) std::printf("%d%s", v[i], i + 1 < v.size() ? "," : "");
+    std::printf("],\n");
+}
+
+static std::vector<double> coords(Tspace &s)
+{
+    std::vector<double> x;
+    for (auto &p : s.p) { x.push_back(p.x()); x.push_back(p.y()); x.push_back(p.z()); }
+    return x;
+}
+
+static std::string engine_text()
+{
+    std::ostringstream o;
+    o << slump.eng;
+    return o.str();
+}
+
+template <class Tmove, class Tenergy>
+static std::unique_ptr<Move::Movebase<Tspace>> make_move(Tmjson &in, Tenergy &pot, Tspace &spc, Rec *rec)
+{
+    std::unique_ptr<Move::Movebase<Tspace>> mv;
+    auto m = in.at("moves");
+    for (auto i = m.begin(); i != m.end(); ++i) {
+        Recording<Move::AtomicTranslation<Tspace>>::internal().eng = slump.eng;
+        mv.reset(new Tmove(pot, spc, i.value(), rec));
+    }
+    return mv;
+}
+
+template <class Eref>
+static int run(const std::string &kind, Tmjson &in, const char *state, int ncalls, int elec, int sr,
+               int nbkind)
+{
+    bool tr = in.at("moves").count("moltransrot") > 0;
+    std::printf("{\n");
+    // ---- A: reference CPU
+    {
+        Tspace spc(in);
+        Eref pot(in);
+        spc.load(state, Tspace::RESIZE);
+        Rec rec;
+        std::unique_ptr<Move::Movebase<Tspace>> mv =
+            tr ? make_move<Recording<Move::TranslateRotate<Tspace>>>(in, pot, spc, &rec)
+               : make_move<Recording<Move::AtomicTranslation<Tspace>>>(in, pot, spc, &rec);
+        for (int c = 0; c < ncalls; ++c) {
+            (void)Recording<Move::AtomicTranslation<Tspace>>::internal().range(0, 0);
+            mv->move();
+        }
+        pvec("A_du", rec.du);
+        pivec("A_acc", rec.acc);
+        pvec("A_xyz", coords(spc));
+        std::printf("\"A_rng\": \"%s\",\n", engine_text().c_str());
+    }
+    // ---- B: reference moves, GPU energy
+    {
+        Tspace spc(in);
+        spc.load(state, Tspace::RESIZE);
+        auto prm = AMD::GPUParams::fromJson(in, elec, sr, nbkind);
+        Energy::GPUNonbonded<Tspace> pot(prm);
+        Rec rec;
+        std::unique_ptr<Move::Movebase<Tspace>> mv =
+            tr ? make_move<Recording<Move::GPUTranslateRotate<Tspace>>>(in, pot, spc, &rec)
+               : make_move<Recording<Move::GPUAtomicTranslation<Tspace>>>(in, pot, spc, &rec);
+        for (int c = 0; c < ncalls; ++c) {
+            (void)Recording<Move::AtomicTranslation<Tspace>>::internal().range(0, 0);
+            mv->move();
+        }
+        pvec("B_du", rec.du);
+        pivec("B_acc", rec.acc);
+        pvec("B_xyz", coords(spc));
+        std::printf("\"B_rng\": \"%s\",\n", engine_text().c_str());
+    }
+    // ---- C: device-resident Propagator
+    {
+        Tspace spc(in);
+        spc.load(state, Tspace::RESIZE);
+        auto prm = AMD::GPUParams::fromJson(in, elec, sr, nbkind);
+        Energy::GPUNonbonded<Tspace> pot(prm);
+        Move::GPUPropagator<Tspace> prop(in, pot, spc);
+        prop.move(ncalls);
+        std::printf("\"C_trials\": %ld,\n\"C_accepted\": %ld,\n", prop.last.trials, prop.last.accepted);
+        pvec("C_xyz", coords(spc));
+        std::printf("\"C_rng\": \"%s\"\n}\n", engine_text().c_str());
+    }
+    return 0;
+}
+
+int main(int argc, char **argv)
+{
+    std::cout.setstate(std::ios_base::failbit);
+    if (argc < 5) { std::fprintf(stderr, "usage: adapter_test kind config state ncalls\n"); return 2; }
+    std::string kind = argv[1];
+    Tmjson in = openjson(argv[2]);
+    int ncalls = std::atoi(argv[4]);
+    typedef Energy::Nonbonded<Tspace, CombinedPairPotential<DebyeHuckel, HardSphere>> E_dhhs;
+    typedef Energy::NonbondedCutg2g<Tspace, CombinedPairPotential<DebyeHuckel, HardSphere>> E_dhhs_cut;
+    typedef Energy::Nonbonded<Tspace, CombinedPairPotential<Coulomb, HardSphere>> E_coulhs;
+    typedef Energy::Nonbonded<Tspace, LennardJonesLB> E_lj;
+    typedef Energy::NonbondedEwald<Tspace, LennardJonesLB> E_ewaldlj;
+    if (kind == "dhhs") return run<E_dhhs>(kind, in, argv[3], ncalls, FGX_ELEC_DH, FGX_SR_HS, FGX_NB_PLAIN);
+    if (kind == "dhhs_cut") return run<E_dhhs_cut>(kind, in, argv[3], ncalls, FGX_ELEC_DH, FGX_SR_HS, FGX_NB_CUTG2G);
+    if (kind == "coulhs") return run<E_coulhs>(kind, in, argv[3], ncalls, FGX_ELEC_COULOMB, FGX_SR_HS, FGX_NB_PLAIN);
+    if (kind == "lj") return run<E_lj>(kind, in, argv[3], ncalls, FGX_ELEC_NONE, FGX_SR_LJ, FGX_NB_PLAIN);
+    if (kind == "ewaldlj") return run<E_ewaldlj>(kind, in, argv[3], ncalls, FGX_ELEC_EWALD, FGX_SR_LJ, FGX_NB_VECTOR);
+    std::fprintf(stderr, "unknown kind\n");
+    return 2;
+}

@@ -1,0 +1,70 @@
+"""The reference's own Move / Propagator machinery driving libfgx through
+include/faunus_amd/reference_adapter.h (the drop-in binding).
+
+oracle/_ref/adapter_test is the reference compiled from its sources with the
+adapter header and linked against libfgx (oracle/Makefile target `adapter`).
+For each golden input it runs (A) the reference on the CPU, (B) the
+reference moves with the GPU energy class, and (C) the device-resident
+GPUPropagator, and this test requires B and C to reproduce A: identical
+accept/reject sequences and global-engine state, energy changes within
+1e-10 relative, coordinates within 1e-9 A.
+"""
+import json
+import math
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+import pytest
+
+from oracle_py import GOLDEN, farr, load_fixture
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "oracle", "_ref", "adapter_test")
+sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+
+
+def write_inputs(fx, d):
+    from make_golden import mt_state_text, write_state
+    cpath, spath = os.path.join(d, "in.json"), os.path.join(d, "state")
+    with open(cpath, "w") as f:
+        json.dump(fx["config"], f)
+    p = fx["particles"]
+    xyz = np.array(p["xyz"]).reshape(-1, 3)
+    write_state(spath, fx["box"], xyz, p["charge"], p["radius"], p["mw"], p["id"], fx["groups"],
+                mt_state_text(fx["rng_seed"]))
+    return cpath, spath
+
+
+@pytest.mark.parametrize("name", ["salt_dhhs", "salt_coulhs", "lj_mix", "ewald_tinfoil",
+                                  "ewald_surface", "mol_dhhs", "mol_dhhs_offcenter", "mol_dhhs_cut"])
+def test_reference_moves_on_gpu_energy(name):
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    if not os.path.exists(BIN):
+        pytest.skip("oracle/_ref/adapter_test not built (needs the reference tree at build time)")
+    print("[adapter] %s" % name, flush=True)
+    fx = load_fixture(name)
+    ncalls = fx["mc"]["ncalls"]
+    with tempfile.TemporaryDirectory() as d:
+        c, s = write_inputs(fx, d)
+        out = subprocess.run([BIN, fx["kind"], c, s, str(ncalls)], capture_output=True, text=True,
+                             timeout=300)
+        assert out.returncode == 0, out.stderr[-2000:]
+        r = json.loads(out.stdout)
+    A_du, B_du = farr(r["A_du"]), farr(r["B_du"])
+    assert np.array_equal(A_du, farr(fx["mc"]["du"]))          # A is the reference itself
+    assert r["A_acc"] == r["B_acc"]
+    fin = np.isfinite(A_du)
+    assert np.array_equal(np.isinf(A_du), np.isinf(B_du))
+    scale = max(1.0, float(np.max(np.abs(A_du[fin])))) if fin.any() else 1.0
+    assert np.all(np.abs(A_du[fin] - B_du[fin]) <= 1e-10 * np.maximum(np.abs(A_du[fin]), scale))
+    assert r["A_rng"] == r["B_rng"] == r["C_rng"]
+    assert r["C_trials"] == len(A_du) and r["C_accepted"] == sum(r["A_acc"])
+    a = np.array(r["A_xyz"])
+    assert np.allclose(np.array(r["B_xyz"]), a, rtol=0, atol=1e-9)
+    assert np.allclose(np.array(r["C_xyz"]), a, rtol=0, atol=1e-9)
