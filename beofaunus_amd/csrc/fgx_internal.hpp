@@ -76,7 +76,7 @@ struct fgx_system {
     fgx::Vec64 v[2]{};                     // 0 = p, 1 = trial
     double *d_q = nullptr, *d_r = nullptr, *d_mw = nullptr;
     int *d_type = nullptr, *d_gid = nullptr;
-    int32_t *d_fx = nullptr, *d_fy = nullptr, *d_fz = nullptr;  // fixed-point p
+    int4 *d_pk = nullptr;                  // packed accepted vector: fixed-point x, y, z + FP32 charge
     float *d_qf = nullptr, *d_rf = nullptr;
     fgx::GroupDesc *d_groups = nullptr;
     double *d_cm = nullptr;                // 6 * ng: cm, cm_trial

@@ -37,7 +37,8 @@ struct ChainDev {
     double *x, *y, *z;                 // accepted vector (FP64)
     const double *q, *r;
     const int *t, *gid;
-    int32_t *fx, *fy, *fz;             // fixed-point accepted vector
+    const int4 *pk;                    // packed accepted vector (fixed-point xyz, FP32 charge)
+    int4 *pk_w;
     const float *qf, *rf;
     const double *tab;
     const float *tabf;
@@ -106,8 +107,9 @@ __device__ __forceinline__ double du_pair32(const ChainDev &C, int32_t nx, int32
                                             double ox64, double oy64, double oz64, double aq64,
                                             double ar64)
 {
-    int32_t bx = C.fx[j], by = C.fy[j], bz = C.fz[j];
-    float bq = C.qf[j], br = (SR == SR_HS) ? C.rf[j] : 0.f;
+    const int4 pj = C.pk[j];
+    int32_t bx = pj.x, by = pj.y, bz = pj.z;
+    float bq = __int_as_float(pj.w), br = (SR == SR_HS) ? C.rf[j] : 0.f;
     int bt = (SR == SR_LJ || ELEC == ELEC_EWALD) ? C.t[j] : 0;
     const float sx = C.box.fscale[0], sy = C.box.fscale[1], sz = C.box.fscale[2];
     float dx = (float)(int32_t)(nx - bx) * sx, dy = (float)(int32_t)(ny - by) * sy,
@@ -121,6 +123,81 @@ __device__ __forceinline__ double du_pair32(const ChainDev &C, int32_t nx, int32
     if (!(fabsf(un) <= C.pot.exact_f) || !(fabsf(uo) <= C.pot.exact_f))
         return du_pair64<ELEC, SR>(C, nx64, ny64, nz64, ox64, oy64, oz64, aq64, ar64, at, j);
     return (double)(un - uo);
+}
+
+static constexpr int DCAP = 2048;   // deferred-pair list capacity (LDS)
+
+// Mixed-path pair loop of one AtomicTranslation trial.  Partners are read as
+// packed 16-byte records (fixed-point x, y, z and FP32 charge), four per
+// thread in flight.  The bulk of the pairs are evaluated in FP32 and summed in
+// FP64.  Partners inside the Ewald real-space cutoff, and pairs whose FP32
+// energy is strong (|u| > Pot::exact_f) or a hard-sphere contact, are appended
+// to an LDS list and evaluated afterwards in FP64 by all threads, so the
+// erfc/exp work is compacted instead of idling divergent lanes.
+template <int ELEC, int SR>
+__device__ __forceinline__ double pair_loop_mixed(const ChainDev &C, int i, double tnx, double tny,
+                                                  double tnz, double tox, double toy, double toz,
+                                                  int *dlist, int *dcount)
+{
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int4 me = C.pk[i];
+    const float aq = __int_as_float(me.w);
+    const float ar = (SR == SR_HS) ? C.rf[i] : 0.f;
+    const int at = C.t[i];
+    const int32_t nx = to_fixed(tnx, C.box.inv[0]), ny = to_fixed(tny, C.box.inv[1]),
+                  nz = to_fixed(tnz, C.box.inv[2]);
+    const float sx = C.box.fscale[0], sy = C.box.fscale[1], sz = C.box.fscale[2];
+    const float exact = C.pot.exact_f;
+    double acc = 0.0;
+    constexpr int U = 4;
+    for (int j0 = tid; j0 < C.n; j0 += U * nthr) {
+        int4 P[U];
+        float R[U];
+        int TT[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            int j = j0 + u * nthr;
+            bool ok = j < C.n;
+            P[u] = ok ? C.pk[j] : make_int4(0, 0, 0, 0);
+            R[u] = (SR == SR_HS && ok) ? C.rf[j] : 0.f;
+            TT[u] = ((SR == SR_LJ || ELEC == ELEC_EWALD) && ok) ? C.t[j] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = j0 + u * nthr;
+            if (j >= C.n || j == i) continue;
+            float dx = (float)(int32_t)(nx - P[u].x) * sx, dy = (float)(int32_t)(ny - P[u].y) * sy,
+                  dz = (float)(int32_t)(nz - P[u].z) * sz;
+            const float r2n = dx * dx + dy * dy + dz * dz;
+            dx = (float)(int32_t)(me.x - P[u].x) * sx; dy = (float)(int32_t)(me.y - P[u].y) * sy;
+            dz = (float)(int32_t)(me.z - P[u].z) * sz;
+            const float r2o = dx * dx + dy * dy + dz * dz;
+            const float bq = __int_as_float(P[u].w);
+            bool defer = false;
+            float un = 0.f, uo = 0.f;
+            if constexpr (ELEC == ELEC_EWALD) {
+                defer = (r2n <= C.pot.ew_rc2f) || (r2o <= C.pot.ew_rc2f);
+                if (!defer) {   // beyond rc the real-space term is zero
+                    un = pair_u32_t<ELEC_NONE, SR>(C.pot, C.tabf, r2n, aq, bq, ar, R[u], at, TT[u]);
+                    uo = pair_u32_t<ELEC_NONE, SR>(C.pot, C.tabf, r2o, aq, bq, ar, R[u], at, TT[u]);
+                }
+            } else {
+                un = pair_u32_t<ELEC, SR>(C.pot, C.tabf, r2n, aq, bq, ar, R[u], at, TT[u]);
+                uo = pair_u32_t<ELEC, SR>(C.pot, C.tabf, r2o, aq, bq, ar, R[u], at, TT[u]);
+            }
+            defer = defer || !(fabsf(un) <= exact) || !(fabsf(uo) <= exact);
+            if (defer) {
+                int slot = atomicAdd(dcount, 1);
+                if (slot < DCAP)
+                    dlist[slot] = j;
+                else
+                    acc += du_pair64<ELEC, SR>(C, tnx, tny, tnz, tox, toy, toz, C.q[i], C.r[i], at, j);
+            } else {
+                acc += (double)(un - uo);
+            }
+        }
+    }
+    return acc;
 }
 
 // ------------------------------------------------------------------------
@@ -196,6 +273,8 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
     __shared__ double2 etab[2][3 * MAXK_AXIS];   // old / new per-axis phase tables
     __shared__ double2 rtab[EW ? REFRESH_CHUNK * 3 * REFRESH_KAX : 1];
     __shared__ double rw[EW ? REFRESH_CHUNK : 1];
+    __shared__ int dlist[MIX ? DCAP : 1];
+    __shared__ int dcount;
     // Ewald state for the whole launch, LDS-resident: S(k) FP64, A_k, n-triples
     extern __shared__ double2 dyn[];
     double2 *S = dyn;
@@ -267,6 +346,7 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
                     tr.ox = ox; tr.oy = oy; tr.oz = oz;
                     tr.nx = nx; tr.ny = ny; tr.nz = nz;
                     tr.collide = collision_d(C.box, nx, ny, nz) ? 1 : 0;
+                    dcount = 0;
                 }
             }
             __syncthreads();
@@ -296,19 +376,15 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
                     acc += du_pair64<ELEC, SR>(C, nx, ny, nz, ox, oy, oz, aq, ar, at, j);
                 }
             } else {
-                const float aq = C.qf[i], ar = C.rf[i];
+                acc = pair_loop_mixed<ELEC, SR>(C, i, tr.nx, tr.ny, tr.nz, tr.ox, tr.oy, tr.oz,
+                                                dlist, &dcount);
+                __syncthreads();   // deferred list complete
+                const int nd = min(dcount, DCAP);
                 const double aq64 = C.q[i], ar64 = C.r[i];
                 const int at = C.t[i];
-                const int32_t nx = to_fixed(tr.nx, C.box.inv[0]), ny = to_fixed(tr.ny, C.box.inv[1]),
-                              nz = to_fixed(tr.nz, C.box.inv[2]);
-                const int32_t ox = C.fx[i], oy = C.fy[i], oz = C.fz[i];
-                const double nx64 = tr.nx, ny64 = tr.ny, nz64 = tr.nz, ox64 = tr.ox, oy64 = tr.oy,
-                             oz64 = tr.oz;
-                for (int j = tid; j < C.n; j += nthr) {
-                    if (j == i) continue;
-                    acc += du_pair32<ELEC, SR>(C, nx, ny, nz, ox, oy, oz, aq, ar, at, j, nx64, ny64,
-                                               nz64, ox64, oy64, oz64, aq64, ar64);
-                }
+                for (int k = tid; k < nd; k += nthr)
+                    acc += du_pair64<ELEC, SR>(C, tr.nx, tr.ny, tr.nz, tr.ox, tr.oy, tr.oz, aq64,
+                                               ar64, at, dlist[k]);
             }
             // ---- reciprocal part: A_k (|S+dS|^2 - |S|^2) per k, FP64
             const double wq = EW ? C.q[i] * C.eff[C.t[i]] : 0.0;
@@ -353,9 +429,8 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
                 utot += du;
                 if (tid == 0) {   // AtomicTranslation::_acceptMove (move.h:866-877)
                     C.x[i] = tr.nx; C.y[i] = tr.ny; C.z[i] = tr.nz;
-                    C.fx[i] = to_fixed(tr.nx, C.box.inv[0]);
-                    C.fy[i] = to_fixed(tr.ny, C.box.inv[1]);
-                    C.fz[i] = to_fixed(tr.nz, C.box.inv[2]);
+                    C.pk_w[i] = make_int4(to_fixed(tr.nx, C.box.inv[0]), to_fixed(tr.ny, C.box.inv[1]),
+                                          to_fixed(tr.nz, C.box.inv[2]), C.pk[i].w);
                 }
                 if constexpr (EW) {
                     Mx += wq * (tr.nx - tr.ox); My += wq * (tr.ny - tr.oy); Mz += wq * (tr.nz - tr.oz);
@@ -616,7 +691,8 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_tr(const ChainDev *__restr
                 int i = gd.front + s;
                 double x = C.x[i], y = C.y[i], z = C.z[i];
                 so[0][s] = x; so[1][s] = y; so[2][s] = z;
-                sof[0][s] = C.fx[i]; sof[1][s] = C.fy[i]; sof[2][s] = C.fz[i];
+                const int4 pi4 = C.pk[i];
+                sof[0][s] = pi4.x; sof[1][s] = pi4.y; sof[2][s] = pi4.z;
                 sq[s] = C.q[i]; sr[s] = C.r[i]; st[s] = C.t[i];
                 if (tr.rot) qrotate(C.box, tr, x, y, z);
                 if (tr.trans) translate_pt(C.box, tr, x, y, z);
@@ -681,8 +757,9 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_tr(const ChainDev *__restr
                         acc += un - uo;
                     }
                 } else {
-                    int32_t bx = C.fx[j], by = C.fy[j], bz = C.fz[j];
-                    float bq = C.qf[j], br = C.rf[j];
+                    const int4 pj = C.pk[j];
+                    int32_t bx = pj.x, by = pj.y, bz = pj.z;
+                    float bq = __int_as_float(pj.w), br = C.rf[j];
                     int bt = C.t[j];
                     const float fsx = C.box.fscale[0], fsy = C.box.fscale[1], fsz = C.box.fscale[2];
                     for (int s = 0; s < ns; ++s) {
@@ -736,9 +813,7 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_tr(const ChainDev *__restr
                 for (int s = tid; s < ns; s += nthr) {
                     int i = gd.front + s;
                     C.x[i] = sn[0][s]; C.y[i] = sn[1][s]; C.z[i] = sn[2][s];
-                    C.fx[i] = to_fixed(sn[0][s], C.box.inv[0]);
-                    C.fy[i] = to_fixed(sn[1][s], C.box.inv[1]);
-                    C.fz[i] = to_fixed(sn[2][s], C.box.inv[2]);
+                    C.pk_w[i] = make_int4(snf[0][s], snf[1][s], snf[2][s], C.pk[i].w);
                 }
                 if (tid == 0) {
                     C.cm[6 * g] = tr.cmx; C.cm[6 * g + 1] = tr.cmy; C.cm[6 * g + 2] = tr.cmz;
@@ -819,7 +894,8 @@ static int prepare_chain(fgx_system *s, ChainDev &C, int **d_gvec, double *du_re
     C.ng = s->ng;
     C.x = s->v[0].x; C.y = s->v[0].y; C.z = s->v[0].z;
     C.q = s->d_q; C.r = s->d_r; C.t = s->d_type; C.gid = s->d_gid;
-    C.fx = s->d_fx; C.fy = s->d_fy; C.fz = s->d_fz;
+    C.pk = s->d_pk;
+    C.pk_w = s->d_pk;
     C.qf = s->d_qf; C.rf = s->d_rf;
     C.tab = s->d_tab; C.tabf = s->d_tabf;
     C.groups = s->d_groups;

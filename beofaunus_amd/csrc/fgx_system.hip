@@ -197,28 +197,28 @@ __global__ __launch_bounds__(RB) void k_reduce(const double *partial, int n, dou
 }
 
 // copy moved particles between vectors (accept: trial -> p, reject: p -> trial)
-__global__ void k_copy_moved(Vec64 dst, Vec64 src, const int *idx, int nm, int front,
-                             int32_t *fx, int32_t *fy, int32_t *fz, Box b, int update_fixed)
+__device__ __forceinline__ int4 pack_particle(double x, double y, double z, float q, const Box &b)
+{
+    return make_int4(to_fixed(x, b.inv[0]), to_fixed(y, b.inv[1]), to_fixed(z, b.inv[2]),
+                     __float_as_int(q));
+}
+
+__global__ void k_copy_moved(Vec64 dst, Vec64 src, const int *idx, int nm, int front, int4 *pk,
+                             const float *qf, Box b, int update_fixed)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nm) return;
     int m = idx ? idx[k] : front + k;
     dst.x[m] = src.x[m]; dst.y[m] = src.y[m]; dst.z[m] = src.z[m];
-    if (update_fixed) {
-        fx[m] = to_fixed(src.x[m], b.inv[0]);
-        fy[m] = to_fixed(src.y[m], b.inv[1]);
-        fz[m] = to_fixed(src.z[m], b.inv[2]);
-    }
+    if (update_fixed) pk[m] = pack_particle(src.x[m], src.y[m], src.z[m], qf[m], b);
 }
 
-__global__ void k_fixed_all(const double *x, const double *y, const double *z, int n, Box b,
-                            int32_t *fx, int32_t *fy, int32_t *fz)
+__global__ void k_fixed_all(const double *x, const double *y, const double *z, const float *qf,
+                            int n, Box b, int4 *pk)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    fx[i] = to_fixed(x[i], b.inv[0]);
-    fy[i] = to_fixed(y[i], b.inv[1]);
-    fz[i] = to_fixed(z[i], b.inv[2]);
+    pk[i] = pack_particle(x[i], y[i], z[i], qf[i], b);
 }
 
 // ------------------------------------------------------------------------
@@ -736,8 +736,7 @@ extern "C" int fgx_create(const fgx_hamiltonian *h, const double box[3], int n, 
     }
     if ((rc = dalloc(s, &s->d_q, n)) || (rc = dalloc(s, &s->d_r, n)) ||
         (rc = dalloc(s, &s->d_mw, n)) || (rc = dalloc(s, &s->d_type, n)) ||
-        (rc = dalloc(s, &s->d_gid, n)) || (rc = dalloc(s, &s->d_fx, n)) ||
-        (rc = dalloc(s, &s->d_fy, n)) || (rc = dalloc(s, &s->d_fz, n)) ||
+        (rc = dalloc(s, &s->d_gid, n)) || (rc = dalloc(s, &s->d_pk, n)) ||
         (rc = dalloc(s, &s->d_qf, n)) || (rc = dalloc(s, &s->d_rf, n)) ||
         (rc = dalloc(s, &s->d_groups, ng)) || (rc = dalloc(s, &s->d_cm, 6 * (size_t)ng)) ||
         (rc = dalloc(s, &s->d_tab, s->tab.size())) || (rc = dalloc(s, &s->d_tabf, s->tab.size())) ||
@@ -774,7 +773,7 @@ extern "C" int fgx_create(const fgx_hamiltonian *h, const double box[3], int n, 
         return fail(rc);
     if (n > 0) {
         hipLaunchKernelGGL(k_fixed_all, dim3((n + 255) / 256), dim3(256), 0, s->stream,
-                           s->v[0].x, s->v[0].y, s->v[0].z, n, s->box, s->d_fx, s->d_fy, s->d_fz);
+                           s->v[0].x, s->v[0].y, s->v[0].z, s->d_qf, n, s->box, s->d_pk);
     }
     // mass centres (Space::load -> Group::setMassCenter, space.h:850)
     std::vector<double> hx(xyz, xyz + 3 * (size_t)n);
@@ -804,7 +803,7 @@ extern "C" int fgx_destroy(fgx_system *s)
     if (!s) return FGX_OK;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     void *ptrs[] = {s->v[0].x, s->v[0].y, s->v[0].z, s->v[1].x, s->v[1].y, s->v[1].z, s->d_q,
-                    s->d_r, s->d_mw, s->d_type, s->d_gid, s->d_fx, s->d_fy, s->d_fz, s->d_qf,
+                    s->d_r, s->d_mw, s->d_type, s->d_gid, s->d_pk, s->d_qf,
                     s->d_rf, s->d_groups, s->d_cm, s->d_tab, s->d_tabf, s->d_type_dp, s->d_mask,
                     s->d_partial, s->d_out, s->d_idx, s->d_rng, s->d_stats_i, s->d_stats_d,
                     s->ew.kv, s->ew.aks, s->ew.kn, s->ew.Q, s->ew.Qt, s->ew.eff,
@@ -837,8 +836,7 @@ extern "C" int fgx_set_positions(fgx_system *s, int which, int first, int count,
     }
     if (which != 1 && count > 0) {
         hipLaunchKernelGGL(k_fixed_all, dim3((s->n + 255) / 256), dim3(256), 0, s->stream,
-                           s->v[0].x, s->v[0].y, s->v[0].z, s->n, s->box, s->d_fx, s->d_fy,
-                           s->d_fz);
+                           s->v[0].x, s->v[0].y, s->v[0].z, s->d_qf, s->n, s->box, s->d_pk);
         FGX_HIP(hipGetLastError());
     }
     FGX_HIP(hipStreamSynchronize(s->stream));
@@ -1243,8 +1241,8 @@ extern "C" int fgx_commit(fgx_system *s, const fgx_change *c, int accepted, doub
         Vec64 dst = accepted ? s->v[0] : s->v[1];
         Vec64 src = accepted ? s->v[1] : s->v[0];
         hipLaunchKernelGGL(k_copy_moved, dim3((nm + 255) / 256), dim3(256), 0, s->stream, dst, src,
-                           list ? s->d_idx : (const int *)nullptr, nm, G.front, s->d_fx, s->d_fy,
-                           s->d_fz, s->box, accepted ? 1 : 0);
+                           list ? s->d_idx : (const int *)nullptr, nm, G.front, s->d_pk, s->d_qf,
+                           s->box, accepted ? 1 : 0);
         FGX_HIP(hipGetLastError());
     }
     // mass centres: whole-group moves and molecular groups carry cm/cm_trial

@@ -294,3 +294,37 @@ def propagate_batch(systems, ncalls):
     st = Stats()
     _check(lib().fgx_propagate_batch(arr, len(systems), ncalls, C.byref(st)))
     return st.as_dict()
+
+
+class Temper:
+    """ParallelTempering over replicas resident on one GPU (fgx_temper_*)."""
+
+    def __init__(self, replicas, ctl_seed=5489, glob_seed=5489):
+        self.replicas = list(replicas)
+        arr = (C.c_void_p * len(self.replicas))(*[r.h for r in self.replicas])
+        h = C.c_void_p()
+        _check(lib().fgx_temper_create(arr, len(self.replicas), ctl_seed, glob_seed, C.byref(h)))
+        self.h = h
+
+    def exchange(self):
+        R = len(self.replicas)
+        acc = np.zeros(R, dtype=np.int32)
+        du = np.zeros(R)
+        _check(lib().fgx_temper_exchange(self.h, _i(acc), _d(du)))
+        return acc, du
+
+    def slots(self):
+        s = np.zeros(len(self.replicas), dtype=np.int32)
+        _check(lib().fgx_temper_slots(self.h, _i(s)))
+        return s
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fgx_temper_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
