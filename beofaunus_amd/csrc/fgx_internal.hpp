@@ -105,6 +105,8 @@ struct fgx_system {
     long *d_stats_i = nullptr;
     double *d_stats_d = nullptr;
     size_t device_bytes = 0;
+    int launch_threads = 0;                // 0 = default; set by fgx_set_launch / fgx_autotune
+    int launch_unroll = 0;
 };
 
 namespace fgx {

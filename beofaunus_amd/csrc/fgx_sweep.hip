@@ -72,9 +72,9 @@ struct ChainDev {
 // ------------------------------------------------------------------------
 // u(new, j) - u(old, j) for one partner j, FP64
 template <int ELEC, int SR>
-__device__ __forceinline__ double du_pair64(const ChainDev &C, double nx, double ny, double nz,
-                                            double ox, double oy, double oz, double aq, double ar,
-                                            int at, int j)
+__device__ __forceinline__ double du_pair64(const ChainDev &C, const double *__restrict__ tab,
+                                            double nx, double ny, double nz, double ox, double oy,
+                                            double oz, double aq, double ar, int at, int j)
 {
     double bx = C.x[j], by = C.y[j], bz = C.z[j];
     double bq = C.q[j], br = C.r[j];
@@ -90,42 +90,12 @@ __device__ __forceinline__ double du_pair64(const ChainDev &C, double nx, double
         r2n = sqdist_d(C.box, nx, ny, nz, bx, by, bz);
         r2o = sqdist_d(C.box, ox, oy, oz, bx, by, bz);
     }
-    return pair_u64_t<ELEC, SR>(C.pot, C.tab, r2n, aq, bq, ar, br, at, bt) -
-           pair_u64_t<ELEC, SR>(C.pot, C.tab, r2o, aq, bq, ar, br, at, bt);
-}
-
-// mixed: FP32 pair energies from fixed-point minimum-image differences.
-// Pairs whose FP32 energy exceeds Pot::exact_f in magnitude (close contacts,
-// where a relative FP32 error would exceed ~1e-7 kT) and every hard-sphere
-// contact decision are redone in FP64 from the FP64 coordinates, so the
-// mixed path keeps |dU - dU_FP64| at the 1e-7 kT level and takes the same
-// overlap decisions as the FP64 path.
-template <int ELEC, int SR>
-__device__ __forceinline__ double du_pair32(const ChainDev &C, int32_t nx, int32_t ny, int32_t nz,
-                                            int32_t ox, int32_t oy, int32_t oz, float aq, float ar,
-                                            int at, int j, double nx64, double ny64, double nz64,
-                                            double ox64, double oy64, double oz64, double aq64,
-                                            double ar64)
-{
-    const int4 pj = C.pk[j];
-    int32_t bx = pj.x, by = pj.y, bz = pj.z;
-    float bq = __int_as_float(pj.w), br = (SR == SR_HS) ? C.rf[j] : 0.f;
-    int bt = (SR == SR_LJ || ELEC == ELEC_EWALD) ? C.t[j] : 0;
-    const float sx = C.box.fscale[0], sy = C.box.fscale[1], sz = C.box.fscale[2];
-    float dx = (float)(int32_t)(nx - bx) * sx, dy = (float)(int32_t)(ny - by) * sy,
-          dz = (float)(int32_t)(nz - bz) * sz;
-    float r2n = dx * dx + dy * dy + dz * dz;
-    dx = (float)(int32_t)(ox - bx) * sx; dy = (float)(int32_t)(oy - by) * sy;
-    dz = (float)(int32_t)(oz - bz) * sz;
-    float r2o = dx * dx + dy * dy + dz * dz;
-    float un = pair_u32_t<ELEC, SR>(C.pot, C.tabf, r2n, aq, bq, ar, br, at, bt);
-    float uo = pair_u32_t<ELEC, SR>(C.pot, C.tabf, r2o, aq, bq, ar, br, at, bt);
-    if (!(fabsf(un) <= C.pot.exact_f) || !(fabsf(uo) <= C.pot.exact_f))
-        return du_pair64<ELEC, SR>(C, nx64, ny64, nz64, ox64, oy64, oz64, aq64, ar64, at, j);
-    return (double)(un - uo);
+    return pair_u64_t<ELEC, SR>(C.pot, tab, r2n, aq, bq, ar, br, at, bt) -
+           pair_u64_t<ELEC, SR>(C.pot, tab, r2o, aq, bq, ar, br, at, bt);
 }
 
 static constexpr int DCAP = 2048;   // deferred-pair list capacity (LDS)
+static constexpr int LDS_TYPES2 = 256;  // type tables staged in LDS up to 16 types
 
 // Mixed-path pair loop of one AtomicTranslation trial.  Partners are read as
 // packed 16-byte records (fixed-point x, y, z and FP32 charge), four per
@@ -134,10 +104,11 @@ static constexpr int DCAP = 2048;   // deferred-pair list capacity (LDS)
 // energy is strong (|u| > Pot::exact_f) or a hard-sphere contact, are appended
 // to an LDS list and evaluated afterwards in FP64 by all threads, so the
 // erfc/exp work is compacted instead of idling divergent lanes.
-template <int ELEC, int SR>
-__device__ __forceinline__ double pair_loop_mixed(const ChainDev &C, int i, double tnx, double tny,
-                                                  double tnz, double tox, double toy, double toz,
-                                                  int *dlist, int *dcount)
+template <int ELEC, int SR, int U>
+__device__ __forceinline__ double pair_loop_mixed(const ChainDev &C, const float *__restrict__ tabf,
+                                                  const double *__restrict__ tab, int i, double tnx,
+                                                  double tny, double tnz, double tox, double toy,
+                                                  double toz, int *dlist, int *dcount)
 {
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int4 me = C.pk[i];
@@ -149,7 +120,6 @@ __device__ __forceinline__ double pair_loop_mixed(const ChainDev &C, int i, doub
     const float sx = C.box.fscale[0], sy = C.box.fscale[1], sz = C.box.fscale[2];
     const float exact = C.pot.exact_f;
     double acc = 0.0;
-    constexpr int U = 4;
     for (int j0 = tid; j0 < C.n; j0 += U * nthr) {
         int4 P[U];
         float R[U];
@@ -178,12 +148,12 @@ __device__ __forceinline__ double pair_loop_mixed(const ChainDev &C, int i, doub
             if constexpr (ELEC == ELEC_EWALD) {
                 defer = (r2n <= C.pot.ew_rc2f) || (r2o <= C.pot.ew_rc2f);
                 if (!defer) {   // beyond rc the real-space term is zero
-                    un = pair_u32_t<ELEC_NONE, SR>(C.pot, C.tabf, r2n, aq, bq, ar, R[u], at, TT[u]);
-                    uo = pair_u32_t<ELEC_NONE, SR>(C.pot, C.tabf, r2o, aq, bq, ar, R[u], at, TT[u]);
+                    un = pair_u32_t<ELEC_NONE, SR>(C.pot, tabf, r2n, aq, bq, ar, R[u], at, TT[u]);
+                    uo = pair_u32_t<ELEC_NONE, SR>(C.pot, tabf, r2o, aq, bq, ar, R[u], at, TT[u]);
                 }
             } else {
-                un = pair_u32_t<ELEC, SR>(C.pot, C.tabf, r2n, aq, bq, ar, R[u], at, TT[u]);
-                uo = pair_u32_t<ELEC, SR>(C.pot, C.tabf, r2o, aq, bq, ar, R[u], at, TT[u]);
+                un = pair_u32_t<ELEC, SR>(C.pot, tabf, r2n, aq, bq, ar, R[u], at, TT[u]);
+                uo = pair_u32_t<ELEC, SR>(C.pot, tabf, r2o, aq, bq, ar, R[u], at, TT[u]);
             }
             defer = defer || !(fabsf(un) <= exact) || !(fabsf(uo) <= exact);
             if (defer) {
@@ -191,7 +161,7 @@ __device__ __forceinline__ double pair_loop_mixed(const ChainDev &C, int i, doub
                 if (slot < DCAP)
                     dlist[slot] = j;
                 else
-                    acc += du_pair64<ELEC, SR>(C, tnx, tny, tnz, tox, toy, toz, C.q[i], C.r[i], at, j);
+                    acc += du_pair64<ELEC, SR>(C, tab, tnx, tny, tnz, tox, toy, toz, C.q[i], C.r[i], at, j);
             } else {
                 acc += (double)(un - uo);
             }
@@ -259,7 +229,7 @@ __device__ __forceinline__ void kunpack(int v, int &nx, int &ny, int &nz)
 // bytes of dynamic LDS a chain needs: S(k) (double2), A_k (double), packed n (int)
 __host__ __device__ inline size_t at_dyn_lds(int K) { return (size_t)K * (16 + 8 + 4); }
 
-template <int ELEC, int SR, bool MIX>
+template <int ELEC, int SR, bool MIX, int U>
 __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restrict__ chains,
                                                          int ncalls)
 {
@@ -275,6 +245,9 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
     __shared__ double rw[EW ? REFRESH_CHUNK : 1];
     __shared__ int dlist[MIX ? DCAP : 1];
     __shared__ int dcount;
+    // per-type pair tables in LDS (the partner loop indexes them per lane)
+    __shared__ double ltab[TAB_COUNT * LDS_TYPES2];
+    __shared__ float ltabf[MIX ? TAB_COUNT * LDS_TYPES2 : 1];
     // Ewald state for the whole launch, LDS-resident: S(k) FP64, A_k, n-triples
     extern __shared__ double2 dyn[];
     double2 *S = dyn;
@@ -286,6 +259,16 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
         mti[k] = C.rng[625 + k];
     }
     WaveMT G{mtg, (int)C.rng[624]}, I{mti, (int)C.rng[625 + 624]};
+    const bool lds_tab = C.pot.ntypes2 <= LDS_TYPES2;
+    if (lds_tab) {
+        for (int k = tid; k < TAB_COUNT * C.pot.ntypes2; k += nthr) {
+            int t = k / C.pot.ntypes2, e = k % C.pot.ntypes2;
+            ltab[t * C.pot.ntypes2 + e] = C.tab[k];
+            if (MIX) ltabf[t * C.pot.ntypes2 + e] = C.tabf[k];
+        }
+    }
+    const double *tab = lds_tab ? ltab : C.tab;
+    const float *tabf = (lds_tab && MIX) ? ltabf : C.tabf;
     const int kcc1 = C.kcc + 1;
     if constexpr (EW) {
         for (int k = tid; k < C.K; k += nthr) {
@@ -373,17 +356,17 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
                              oz = tr.oz;
                 for (int j = tid; j < C.n; j += nthr) {
                     if (j == i) continue;
-                    acc += du_pair64<ELEC, SR>(C, nx, ny, nz, ox, oy, oz, aq, ar, at, j);
+                    acc += du_pair64<ELEC, SR>(C, tab, nx, ny, nz, ox, oy, oz, aq, ar, at, j);
                 }
             } else {
-                acc = pair_loop_mixed<ELEC, SR>(C, i, tr.nx, tr.ny, tr.nz, tr.ox, tr.oy, tr.oz,
-                                                dlist, &dcount);
+                acc = pair_loop_mixed<ELEC, SR, U>(C, tabf, tab, i, tr.nx, tr.ny, tr.nz, tr.ox, tr.oy,
+                                                   tr.oz, dlist, &dcount);
                 __syncthreads();   // deferred list complete
                 const int nd = min(dcount, DCAP);
                 const double aq64 = C.q[i], ar64 = C.r[i];
                 const int at = C.t[i];
                 for (int k = tid; k < nd; k += nthr)
-                    acc += du_pair64<ELEC, SR>(C, tr.nx, tr.ny, tr.nz, tr.ox, tr.oy, tr.oz, aq64,
+                    acc += du_pair64<ELEC, SR>(C, tab, tr.nx, tr.ny, tr.nz, tr.ox, tr.oy, tr.oz, aq64,
                                                ar64, at, dlist[k]);
             }
             // ---- reciprocal part: A_k (|S+dS|^2 - |S|^2) per k, FP64
@@ -981,23 +964,34 @@ static int init_mass_centres(fgx_system *s)
     return FGX_OK;
 }
 
-template <int ELEC, int SR, bool MIX>
-static void launch_at_k(hipStream_t st, const ChainDev *d_chains, int nchains, int nthr,
+template <int ELEC, int SR, bool MIX, int U>
+static void launch_at_u(hipStream_t st, const ChainDev *d_chains, int nchains, int nthr,
                         size_t dyn, int ncalls)
 {
     if (dyn > 0)
-        (void)hipFuncSetAttribute((const void *)k_sweep_at<ELEC, SR, MIX>,
+        (void)hipFuncSetAttribute((const void *)k_sweep_at<ELEC, SR, MIX, U>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
-    hipLaunchKernelGGL((k_sweep_at<ELEC, SR, MIX>), dim3(nchains), dim3(nthr), dyn, st, d_chains,
-                       ncalls);
+    hipLaunchKernelGGL((k_sweep_at<ELEC, SR, MIX, U>), dim3(nchains), dim3(nthr), dyn, st,
+                       d_chains, ncalls);
+}
+
+// partner-loop unroll depth: the autotuned knob of the mixed path (the FP64
+// path keeps a plain strided loop)
+template <int ELEC, int SR, bool MIX>
+static void launch_at_k(hipStream_t st, const ChainDev *d_chains, int nchains, int nthr,
+                        size_t dyn, int ncalls, int unroll)
+{
+    if (!MIX || unroll <= 4) launch_at_u<ELEC, SR, MIX, 4>(st, d_chains, nchains, nthr, dyn, ncalls);
+    else if (unroll <= 8) launch_at_u<ELEC, SR, MIX, 8>(st, d_chains, nchains, nthr, dyn, ncalls);
+    else launch_at_u<ELEC, SR, MIX, 16>(st, d_chains, nchains, nthr, dyn, ncalls);
 }
 
 template <bool MIX>
 static int launch_at(const Pot &P, hipStream_t st, const ChainDev *d_chains, int nchains, int nthr,
-                     size_t dyn, int ncalls)
+                     size_t dyn, int ncalls, int unroll)
 {
 #define FGX_AT(E, S) \
-    if (P.elec == E && P.sr == S) { launch_at_k<E, S, MIX>(st, d_chains, nchains, nthr, dyn, ncalls); }
+    if (P.elec == E && P.sr == S) { launch_at_k<E, S, MIX>(st, d_chains, nchains, nthr, dyn, ncalls, unroll); }
     FGX_AT(ELEC_NONE, SR_HS) else FGX_AT(ELEC_NONE, SR_LJ) else
     FGX_AT(ELEC_COULOMB, SR_NONE) else FGX_AT(ELEC_COULOMB, SR_HS) else FGX_AT(ELEC_COULOMB, SR_LJ) else
     FGX_AT(ELEC_DH, SR_NONE) else FGX_AT(ELEC_DH, SR_HS) else FGX_AT(ELEC_DH, SR_LJ) else
@@ -1037,6 +1031,7 @@ static int run_chains(std::vector<fgx_system *> &sys, int ncalls, double *du_rec
     int rc = FGX_OK;
     int nthr = 0;
     size_t dyn = 0;
+    const int unroll = s0->launch_unroll > 0 ? s0->launch_unroll : 4;
     for (int c = 0; c < nc && !rc; ++c) {
         fgx_system *s = sys[c];
         FGX_CHECK(s->moves.size() == s0->moves.size() && !s->moves.empty() &&
@@ -1065,7 +1060,7 @@ static int run_chains(std::vector<fgx_system *> &sys, int ncalls, double *du_rec
                            c == 0 ? acc_rec : nullptr, c == 0 ? cap : 0);
         if (!rc && s->has_ewald) chains[c].ew_cnt = s->ew.cnt_dev;
         FGX_HIP(hipStreamSynchronize(s->stream));
-        nthr = std::max(nthr, chain_block(s));
+        nthr = std::max(nthr, s->launch_threads > 0 ? s->launch_threads : chain_block(s));
         if (s->has_ewald) dyn = std::max(dyn, at_dyn_lds(s->ew.K));
     }
     ChainDev *d_chains = nullptr;
@@ -1080,8 +1075,8 @@ static int run_chains(std::vector<fgx_system *> &sys, int ncalls, double *du_rec
         FGX_HIP(hipEventRecord(e0, s0->stream));
         if (s0->moves[0].type == MOVE_AT) {
             rc = s0->pot.precision == FGX_MIXED
-                     ? launch_at<true>(s0->pot, s0->stream, d_chains, nc, nthr, dyn, ncalls)
-                     : launch_at<false>(s0->pot, s0->stream, d_chains, nc, nthr, dyn, ncalls);
+                     ? launch_at<true>(s0->pot, s0->stream, d_chains, nc, nthr, dyn, ncalls, unroll)
+                     : launch_at<false>(s0->pot, s0->stream, d_chains, nc, nthr, dyn, ncalls, unroll);
         } else {
             rc = s0->pot.precision == FGX_MIXED
                      ? launch_tr<true>(s0->pot, s0->stream, d_chains, nc, nthr, ncalls)
@@ -1248,4 +1243,65 @@ extern "C" int fgx_propagate_batch(fgx_system **systems, int nsys, int ncalls, f
     int rc = run_chains(sys, ncalls, nullptr, nullptr, 0, &secs);
     for (int c = 0; c < nsys && !rc; ++c) rc = collect_stats(sys[c], st, secs);
     return rc;
+}
+
+// ------------------------------------------------------------------------
+// launch configuration and autotuner
+// ------------------------------------------------------------------------
+
+extern "C" int fgx_set_launch(fgx_system *s, int threads, int unroll)
+{
+    FGX_CHECK(s && (threads == 0 || threads == 256 || threads == 512) &&
+                  (unroll == 0 || unroll == 4 || unroll == 8 || unroll == 16),
+              FGX_EARG, "fgx_set_launch: threads in {0,256,512}, unroll in {0,4,8,16}");
+    s->launch_threads = threads;
+    s->launch_unroll = unroll;
+    return FGX_OK;
+}
+
+extern "C" int fgx_get_launch(fgx_system *s, int *threads, int *unroll)
+{
+    FGX_CHECK(s && threads && unroll, FGX_EARG, "fgx_get_launch: bad arguments");
+    *threads = s->launch_threads > 0 ? s->launch_threads : chain_block(s);
+    *unroll = s->launch_unroll > 0 ? s->launch_unroll : 4;
+    return FGX_OK;
+}
+
+// Time `ncalls` Propagator::move() calls of a batch of chains under each
+// schedule (block size x partner-loop unroll) and keep the fastest on all of
+// them.  The chains advance (use it as the warm-up of a production batch).
+extern "C" int fgx_autotune_batch(fgx_system **systems, int nsys, int ncalls, int *threads,
+                                  int *unroll, double *best_seconds)
+{
+    FGX_CHECK(systems && nsys > 0 && ncalls > 0, FGX_EARG, "fgx_autotune_batch: bad arguments");
+    const int cand_t[2] = {256, 512};
+    const int cand_u[3] = {4, 8, 16};
+    double best = 1e300;
+    int bt = 0, bu = 0;
+    for (int t : cand_t)
+        for (int u : cand_u) {
+            if (systems[0]->pot.precision != FGX_MIXED && u != 4) continue;
+            for (int c = 0; c < nsys; ++c) {
+                systems[c]->launch_threads = t;
+                systems[c]->launch_unroll = u;
+            }
+            fgx_stats st;
+            int rc = fgx_propagate_batch(systems, nsys, ncalls, &st);
+            if (rc) return rc;
+            if (st.seconds < best) { best = st.seconds; bt = t; bu = u; }
+        }
+    for (int c = 0; c < nsys; ++c) {
+        systems[c]->launch_threads = bt;
+        systems[c]->launch_unroll = bu;
+    }
+    if (threads) *threads = bt;
+    if (unroll) *unroll = bu;
+    if (best_seconds) *best_seconds = best;
+    return FGX_OK;
+}
+
+extern "C" int fgx_autotune(fgx_system *probe, int ncalls, int *threads, int *unroll,
+                            double *best_seconds)
+{
+    return fgx_autotune_batch(&probe, 1, ncalls, threads, unroll, best_seconds);
 }

@@ -96,6 +96,10 @@ def lib():
         "fgx_propagator_init": (I, [P]),
         "fgx_propagate": (I, [P, I, C.POINTER(Stats), _dp, _ip, C.c_long]),
         "fgx_propagate_batch": (I, [pp, I, I, C.POINTER(Stats)]),
+        "fgx_set_launch": (I, [P, I, I]),
+        "fgx_get_launch": (I, [P, _ip, _ip]),
+        "fgx_autotune": (I, [P, I, _ip, _ip, _dp]),
+        "fgx_autotune_batch": (I, [pp, I, I, _ip, _ip, _dp]),
         "fgx_temper_create": (I, [pp, I, C.c_uint32, C.c_uint32, pp]),
         "fgx_temper_exchange": (I, [P, _ip, _dp]),
         "fgx_temper_slots": (I, [P, _ip]),
@@ -265,6 +269,19 @@ class System:
         _check(lib().fgx_add_move(self.h, kind, molid, prob, int(peratom), int(permol), _d(a),
                                   _d(b), dp, dprot, int(center_rotation)))
 
+    def set_launch(self, threads=0, unroll=0):
+        _check(lib().fgx_set_launch(self.h, threads, unroll))
+
+    def launch(self):
+        t, u = C.c_int(), C.c_int()
+        _check(lib().fgx_get_launch(self.h, C.byref(t), C.byref(u)))
+        return t.value, u.value
+
+    def autotune(self, ncalls=1):
+        t, u, sec = C.c_int(), C.c_int(), C.c_double()
+        _check(lib().fgx_autotune(self.h, ncalls, C.byref(t), C.byref(u), C.byref(sec)))
+        return {"threads": t.value, "unroll": u.value, "seconds": sec.value}
+
     def rng_seed(self, seed):
         _check(lib().fgx_rng_seed(self.h, seed))
 
@@ -287,6 +304,14 @@ class System:
             return st.as_dict(), du[:n], acc[:n]
         _check(lib().fgx_propagate(self.h, ncalls, C.byref(st), None, None, 0))
         return st.as_dict(), None, None
+
+
+def autotune_batch(systems, ncalls=1):
+    arr = (C.c_void_p * len(systems))(*[s.h for s in systems])
+    t, u, sec = C.c_int(), C.c_int(), C.c_double()
+    _check(lib().fgx_autotune_batch(arr, len(systems), ncalls, C.byref(t), C.byref(u),
+                                    C.byref(sec)))
+    return {"threads": t.value, "unroll": u.value, "seconds_per_call": sec.value / ncalls}
 
 
 def propagate_batch(systems, ncalls):

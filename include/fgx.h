@@ -152,6 +152,19 @@ int fgx_propagator_init(fgx_system *s);
 int fgx_propagate(fgx_system *s, int ncalls, fgx_stats *st, double *du_out, int *acc_out,
                   long cap);
 
+/* Launch schedule of the device sweeps: workgroup size (256 or 512 threads
+ * per chain) and partner-loop unroll depth (4, 8, 16) of the mixed path; 0
+ * picks the defaults.  No counterpart in the reference (a GPU schedule). */
+int fgx_set_launch(fgx_system *s, int threads, int unroll);
+int fgx_get_launch(fgx_system *s, int *threads, int *unroll);
+/* Autotuner: times `ncalls` Propagator::move() calls of a probe chain under
+ * every schedule and keeps the fastest on the probe (whose state advances). */
+int fgx_autotune(fgx_system *probe, int ncalls, int *threads, int *unroll, double *best_seconds);
+/* the same over a batch of chains advanced together (fgx_propagate_batch);
+ * the winning schedule is set on every chain */
+int fgx_autotune_batch(fgx_system **systems, int nsys, int ncalls, int *threads, int *unroll,
+                       double *best_seconds);
+
 /* ------------------------------------------------------------------------
  * Batched chains: many independent systems advanced by one persistent
  * launch (one workgroup per chain).  Systems keep their own state.
