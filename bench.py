@@ -35,38 +35,47 @@ PEAK = {"fp32": 157.3, "fp64": 78.6}           # MI355X dense vector TFLOP/s
 HBM_PEAK_GBS = 8000.0
 FLOPS_PAIR_EWALD_LJ = 31                       # EwaldReal(kappa->0) + LJ-LB per pair
 FLOPS_K_PER_TRIAL = 32                         # exp(i k.r) update per k-vector
+FLOPS_PAIR_DHHS = 27                           # Debye-Hueckel + hard sphere per pair
 
 
 def cpu_baseline(w, seconds_target=12.0):
     """Time the reference (or its C restatement) on a bounded sample: single
-    trials (peratom off) of the same system, one thread."""
+    trials (one trial per move() call) of the same system, one thread."""
     from beofaunus_amd import workloads
     harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    npart = len(w["charge"])
+    what = "single-group TranslateRotate" if w["move"]["kind"] == 2 else "single-particle"
     with tempfile.TemporaryDirectory() as td:
         cfg, state = workloads.write_reference_inputs(w, td, seed=7, peratom=False)
         if os.path.exists(harness):
-            # calibrate: a short run, then a sample sized for ~seconds_target
-            probe = json.loads(subprocess.run([harness, "mctime", w["ref_kind"], cfg, state, "200"],
-                                              check=True, capture_output=True, text=True).stdout)
+            # calibrate on a short run, then a sample sized for ~seconds_target
+            probe_n = 200 if what == "single-particle" else 2
+            probe = json.loads(subprocess.run([harness, "mctime", w["ref_kind"], cfg, state,
+                                               str(probe_n)], check=True, capture_output=True,
+                                              text=True).stdout)
             rate = probe["trials_per_s"]
-            ncalls = int(max(200, min(200000, rate * seconds_target)))
+            ncalls = int(max(probe_n, min(200000, rate * seconds_target)))
             out = json.loads(subprocess.run([harness, "mctime", w["ref_kind"], cfg, state,
                                              str(ncalls)], check=True, capture_output=True,
                                             text=True).stdout)
             return {"value": out["trials_per_s"], "unit": "trial dU evals/s", "cores": 1,
                     "kind": "reference",
-                    "sample": "%d single-particle trials of the same N=%d system "
+                    "sample": "%d %s trials of the same N=%d system "
                               "(reference C++ hot path, g++ -O2, 1 thread; timed loop only)"
-                              % (int(out["trials"]), len(w["charge"]))}
+                              % (int(out["trials"]), what, npart)}
+    if w["move"]["kind"] != 1:
+        return {"error": "reference binary absent; the C restatement baseline covers "
+                         "AtomicTranslation workloads only"}
     # C restatement fallback
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_py import OracleSystem, lib as olib
-    from fixture_systems import ELEC, SR, NB  # noqa: F401
-    o = OracleSystem("ewaldlj", w["box"], w["xyz"], w["charge"], w["radius"], w["mw"], w["types"],
-                     w["groups"], w["ham"]["lB"], type_sigma=w["ham"]["type_sigma"],
-                     type_eps=w["ham"]["type_eps"], type_charge=w["ham"]["type_charge"],
-                     type_radius=w["ham"]["type_radius"], type_dp=w["ham"]["type_dp"],
-                     ewald=w["ham"]["ewald"])
+    kind = {"ewaldlj": "ewaldlj", "dhhs": "dhhs"}[w["ref_kind"]]
+    h = w["ham"]
+    o = OracleSystem(kind, w["box"], w["xyz"], w["charge"], w["radius"], w["mw"], w["types"],
+                     w["groups"], h["lB"], kappa=h.get("kappa", 0.0),
+                     type_sigma=h["type_sigma"], type_eps=h["type_eps"],
+                     type_charge=h["type_charge"], type_radius=h["type_radius"],
+                     type_dp=h["type_dp"], ewald=h.get("ewald"))
     olib().or_add_move(o.h, 1, 0, 1.0, 0, 0, None, None, 0.0, 0.0, 1)
     o.seed(7)
     olib().or_propagator_init(o.h)
@@ -80,7 +89,7 @@ def cpu_baseline(w, seconds_target=12.0):
             "sample": "%d single-particle trials of the same system (C restatement, 1 thread)" % n}
 
 
-def du_error_vs_fp64(w, seed=11, ntrials=4000):
+def du_error_vs_fp64(w, seed=11, ntrials=4000, ncalls=1):
     """Mixed-path Delta-U error against the FP64 device path on the same RNG
     stream: compare trial by trial while the two chains agree."""
     from beofaunus_amd import fgx, workloads
@@ -90,7 +99,7 @@ def du_error_vs_fp64(w, seed=11, ntrials=4000):
         s = workloads.make_system(w, precision=prec)
         s.rng_seed(seed)
         s.propagator_init()
-        st, du, acc = s.propagate(1, record=ntrials)
+        st, du, acc = s.propagate(ncalls, record=ntrials)
         rec[prec] = (du[:ntrials], acc[:ntrials])
         s.close()
     d64, a64 = rec[fgx.FP64]
@@ -107,13 +116,186 @@ def du_error_vs_fp64(w, seed=11, ntrials=4000):
     return out
 
 
+METRIC = "trial-move ΔU evals/s at stated ΔU error vs FP64; p50/p99 sweep latency"
+
+
+def timed_batch(fgx, systems, steps):
+    """`steps` launches of one Propagator::move() per chain; wall clock around
+    all of them (synchronised), device time of each from HIP events."""
+    import torch
+    torch.cuda.synchronize()
+    dev, trials, acc = [], 0, 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = fgx.propagate_batch(systems, 1)
+        dev.append(st["seconds"])
+        trials += st["trials"]
+        acc += st["accepted"]
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return {"wall": wall, "dev": dev, "trials": trials, "accepted": acc,
+            "value": trials / wall,
+            "sweep_latency_ms": {"p50": 1e3 * float(np.percentile(dev, 50)),
+                                 "p99": 1e3 * float(np.percentile(dev, 99))}}
+
+
+def base_line(args, value, wall, dtype, config):
+    return {
+        "metric": METRIC,
+        "value": value,
+        "unit": "trial dU evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic (seeded configuration)",
+        "config": config,
+    }
+
+
+def bench_ewald(args, fgx, workloads):
+    """Ewald LJ+Coulomb electrolyte, AtomicTranslation sweeps, R replicas."""
+    prec = fgx.MIXED if args.precision == "mixed" else fgx.FP64
+    n = args.n or 20000
+    L = 120.0 * (n / 20000.0) ** (1.0 / 3.0)
+    w = workloads.ewald_electrolyte(n=n, L=L, precision=prec)
+    R = args.replicas or 256
+    t_build = time.perf_counter()
+    systems = []
+    for r in range(R):
+        s = workloads.make_system(w)
+        s.rng_seed(1000 + r)
+        s.propagator_init()
+        systems.append(s)
+    t_build = time.perf_counter() - t_build
+    # warm-up doubles as the schedule autotuner: every (block size, unroll)
+    # candidate advances the replicas by one sweep; the fastest is kept
+    tuned = fgx.autotune_batch(systems, 1) if args.warmup > 0 else None
+    for _ in range(max(0, args.warmup - 1)):
+        fgx.propagate_batch(systems, 1)
+    t = timed_batch(fgx, systems, args.steps)
+    for s in systems:
+        s.close()
+    # algorithmic work per trial: 2 (N-1) pair evaluations (old and new, as the
+    # reference counts them) + K k-vector updates
+    nk = 2242 if n == 20000 else None
+    flops_trial = 2 * (n - 1) * FLOPS_PAIR_EWALD_LJ + (nk or 0) * FLOPS_K_PER_TRIAL
+    trials_per_launch = t["trials"] / args.steps
+    achieved = flops_trial * trials_per_launch / float(np.mean(t["dev"])) / 1e12
+    peak = PEAK["fp32" if prec == fgx.MIXED else "fp64"]
+    line = base_line(args, t["value"], t["wall"],
+                     "f32 pairs + f64 accumulate" if prec == fgx.MIXED else "f64",
+                     {"workload": "Ewald LJ+Coulomb electrolyte: N=%d, L=%.1f A, +-1e 50/50, LJ-LB "
+                                  "sigma 4 A eps 0.1 kT uncut, Ewald alpha 0.25/A rc 12 A |n|<=10 "
+                                  "spherical, tinfoil, AtomicTranslation dp 1 A, S(k) refresh every "
+                                  "20000 accepts" % (n, L),
+                      "replicas": R, "trials_per_step": int(trials_per_launch),
+                      "precision": args.precision})
+    line.update({
+        "pairs_per_s": t["value"] * 2 * (n - 1),
+        "acceptance": t["accepted"] / max(1, t["trials"]),
+        "sweep_latency_ms": t["sweep_latency_ms"],
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None,
+                     "kernel": "k_sweep_at<%s>" % ("mixed" if prec == fgx.MIXED else "fp64"),
+                     "flops_per_trial": flops_trial},
+        "build_seconds": t_build,
+        "schedule": tuned,
+    })
+    if not args.no_error_check:
+        try:
+            line["du_error_vs_fp64"] = du_error_vs_fp64(w)
+        except Exception as e:  # reported, never hidden
+            line["du_error_vs_fp64"] = {"error": str(e)}
+    if not args.no_cpu_baseline:
+        try:
+            line["cpu_baseline"] = cpu_baseline(w)
+        except Exception as e:
+            line["cpu_baseline"] = {"error": str(e)}
+    return line
+
+
+def bench_groups(args, fgx, workloads):
+    """Rigid-group g2g: 64 molecules x 400 sites + 8000 ions, TranslateRotate.
+    The FP64 device path (the reference's two-pass dU) and the mixed path
+    (FP32 pairs, FP64 sums, group-pair energy cache) run the same replicas."""
+    R = args.replicas or 256
+    w = workloads.rigid_molecules()
+    n = len(w["charge"])
+    nsite = w["groups"][0]["back"] - w["groups"][0]["front"] + 1
+    pairs_ref = 2 * nsite * (n - nsite)          # old + new, as the reference evaluates
+    res = {}
+    for key, prec, cached in (("fp64", fgx.FP64, False), ("mixed", fgx.MIXED, True)):
+        systems = []
+        for r in range(R):
+            s = workloads.make_system(w, precision=prec)
+            s.set_energy_matrix(cached)
+            s.rng_seed(1000 + r)
+            s.propagator_init()
+            systems.append(s)
+        for _ in range(max(1, args.warmup)):
+            fgx.propagate_batch(systems, 1)        # includes the cache build
+        res[key] = timed_batch(fgx, systems, args.steps)
+        if key == "mixed" and not args.no_error_check:
+            # the same chains without the cache (FP32 pairs for old and new)
+            for s in systems:
+                s.set_energy_matrix(False)
+            res["mixed_uncached"] = timed_batch(fgx, systems, 1)
+        for s in systems:
+            s.close()
+    m, f = res["mixed"], res["fp64"]
+    trials_per_launch = m["trials"] / args.steps
+    pairs_exec = nsite * (n - nsite)              # cached path: trial state only
+    achieved = pairs_exec * FLOPS_PAIR_DHHS * trials_per_launch / float(np.mean(m["dev"])) / 1e12
+    line = base_line(args, m["value"], m["wall"], "f32 pairs + f64 accumulate",
+                     {"workload": "rigid g2g: 64 molecules x 400 sites (q -1/0/+1 p 0.2/0.6/0.2, "
+                                  "radii U[1.5,3] A, 20 A spheres) + 8000 ions, L=250 A, "
+                                  "Debye-Hueckel 15 A + hard spheres, TranslateRotate dp 5 A "
+                                  "dprot 0.5, sweep = 64 trials",
+                      "replicas": R, "trials_per_step": int(trials_per_launch),
+                      "precision": "mixed", "energy_matrix": True})
+    line.update({
+        "pairs_per_s": m["value"] * pairs_ref,
+        "acceptance": m["accepted"] / max(1, m["trials"]),
+        "sweep_latency_ms": m["sweep_latency_ms"],
+        "fp64": {"value": f["value"], "sweep_latency_ms": f["sweep_latency_ms"],
+                 "pairs_per_s": f["value"] * pairs_ref},
+        "speedup_vs_fp64": m["value"] / f["value"],
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK["fp32"],
+                     "unit": "TFLOP/s", "frac": achieved / PEAK["fp32"], "traffic": None,
+                     "kernel": "k_sweep_tr<DH,HS,mixed> (cached)",
+                     "flops_per_trial": pairs_exec * FLOPS_PAIR_DHHS},
+    })
+    if "mixed_uncached" in res:
+        u = res["mixed_uncached"]
+        line["mixed_uncached"] = {"value": u["value"], "sweep_latency_ms": u["sweep_latency_ms"]}
+    if not args.no_error_check:
+        try:
+            line["du_error_vs_fp64"] = du_error_vs_fp64(w, ntrials=512, ncalls=8)
+        except Exception as e:
+            line["du_error_vs_fp64"] = {"error": str(e)}
+    if not args.no_cpu_baseline:
+        try:
+            line["cpu_baseline"] = cpu_baseline(w)
+        except Exception as e:
+            line["cpu_baseline"] = {"error": str(e)}
+    return line
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--replicas", type=int, default=256)
-    ap.add_argument("--n", type=int, default=20000)
+    ap.add_argument("--config", choices=["ewald", "groups"], default="ewald",
+                    help="ewald: N=20000 Ewald electrolyte (headline); groups: 64 x 400-site "
+                         "rigid molecules + 8000 ions")
+    ap.add_argument("--replicas", type=int, default=0, help="chains per launch (default 256)")
+    ap.add_argument("--n", type=int, default=0, help="particles (ewald workload)")
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error-check", action="store_true")
@@ -129,92 +311,10 @@ def main():
         sys.exit(1)
     torch.cuda.set_device(0)
     from beofaunus_amd import fgx, workloads
-
-    prec = fgx.MIXED if args.precision == "mixed" else fgx.FP64
-    L = 120.0 * (args.n / 20000.0) ** (1.0 / 3.0)
-    w = workloads.ewald_electrolyte(n=args.n, L=L, precision=prec)
-    R = args.replicas
-    t_build = time.perf_counter()
-    systems = []
-    for r in range(R):
-        s = workloads.make_system(w)
-        s.rng_seed(1000 + r)
-        s.propagator_init()
-        systems.append(s)
-    t_build = time.perf_counter() - t_build
-    # warm-up doubles as the schedule autotuner: every (block size, unroll)
-    # candidate advances the replicas by one sweep; the fastest is kept
-    tuned = fgx.autotune_batch(systems, 1) if args.warmup > 0 else None
-    for _ in range(max(0, args.warmup - 1)):
-        fgx.propagate_batch(systems, 1)
-    torch.cuda.synchronize()
-    dev = []
-    trials = 0
-    acc = 0
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        st = fgx.propagate_batch(systems, 1)
-        dev.append(st["seconds"])
-        trials += st["trials"]
-        acc += st["accepted"]
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    wall = t1 - t0
-    n = args.n
-    value = trials / wall
-    # algorithmic work per trial: 2 (N-1) pair evaluations (old and new, as the
-    # reference counts them) + K k-vector updates
-    from beofaunus_amd.fgx import lib  # noqa: F401
-    nk = 2242 if n == 20000 else None
-    flops_trial = 2 * (n - 1) * FLOPS_PAIR_EWALD_LJ + (nk or 0) * FLOPS_K_PER_TRIAL
-    trials_per_launch = trials / args.steps
-    dev_mean = float(np.mean(dev))
-    achieved = flops_trial * trials_per_launch / dev_mean / 1e12
-    peak = PEAK["fp32" if prec == fgx.MIXED else "fp64"]
-    line = {
-        "metric": "trial-move ΔU evals/s at stated ΔU error vs FP64; p50/p99 sweep latency",
-        "value": value,
-        "unit": "trial dU evals/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * wall / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32 pairs + f64 accumulate" if prec == fgx.MIXED else "f64",
-        "data": "synthetic (seeded jittered-lattice configuration)",
-        "config": {
-            "workload": "Ewald LJ+Coulomb electrolyte: N=%d, L=%.1f A, +-1e 50/50, LJ-LB sigma 4 A "
-                        "eps 0.1 kT uncut, Ewald alpha 0.25/A rc 12 A |n|<=10 spherical, tinfoil, "
-                        "AtomicTranslation dp 1 A, S(k) refresh every 20000 accepts" % (n, L),
-            "replicas": R,
-            "trials_per_step": int(trials_per_launch),
-            "precision": args.precision,
-        },
-        "pairs_per_s": value * 2 * (n - 1),
-        "acceptance": acc / max(1, trials),
-        "sweep_latency_ms": {"p50": 1e3 * float(np.percentile(dev, 50)),
-                             "p99": 1e3 * float(np.percentile(dev, 99))},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
-                     "kernel": "k_sweep_at<%s>" % ("mixed" if prec == fgx.MIXED else "fp64"),
-                     "flops_per_trial": flops_trial},
-        "build_seconds": t_build,
-        "schedule": tuned,
-    }
-    for s in systems:
-        s.close()
-    if not args.no_error_check:
-        try:
-            line["du_error_vs_fp64"] = du_error_vs_fp64(w)
-        except Exception as e:  # reported, never hidden
-            line["du_error_vs_fp64"] = {"error": str(e)}
-    if not args.no_cpu_baseline:
-        try:
-            line["cpu_baseline"] = cpu_baseline(w)
-        except Exception as e:
-            line["cpu_baseline"] = {"error": str(e)}
+    if args.config == "ewald":
+        line = bench_ewald(args, fgx, workloads)
+    else:
+        line = bench_groups(args, fgx, workloads)
     print(json.dumps(line))
 
 

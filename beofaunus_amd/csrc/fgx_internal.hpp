@@ -107,6 +107,10 @@ struct fgx_system {
     size_t device_bytes = 0;
     int launch_threads = 0;                // 0 = default; set by fgx_set_launch / fgx_autotune
     int launch_unroll = 0;
+    // group-pair energy cache of the mixed TranslateRotate sweep (ng x ng)
+    double *d_emat = nullptr;
+    bool emat_on = true;                   // fgx_set_energy_matrix
+    bool emat_valid = false;               // cleared whenever the accepted state or lB changes
 };
 
 namespace fgx {

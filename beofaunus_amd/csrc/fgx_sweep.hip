@@ -65,6 +65,12 @@ struct ChainDev {
     double *du_rec;
     int *acc_rec;
     long cap;
+    // group-pair energy cache (ng x ng, FP64) of the mixed TranslateRotate
+    // sweep, or null
+    double *emat;
+    // fixed-unit pair form of the cached path (cubic box, Coulomb/DH + HS)
+    int fx;
+    float fx_lbs, fx_rs, fx_c1;   // lB / s, sqrt(1 + 1e-6) / s, -kappa s / ln 2
 };
 
 // ------------------------------------------------------------------------
@@ -571,6 +577,285 @@ __device__ __forceinline__ void translate_pt(const Box &b, const TrialTR &T, dou
     boundary_d(b, x, y, z);
 }
 
+// ------------------------------------------------------------------------
+// Mixed-path group energies with the group-pair energy cache
+// (EnergyMatrix, energy.h:225-641: g2All of the trial state is evaluated and
+// kept per partner group; the accepted state's g2g values are read from the
+// matrix instead of being recomputed, and update(true) writes the new row).
+// ------------------------------------------------------------------------
+static constexpr int MAT_MAXG = 256;   // groups per chain for the cached path
+static constexpr int TR_DW = 128;      // queued FP64 pairs per wavefront
+
+// Adds e into slot[h] for each distinct h >= 0 among the lanes of the
+// wavefront, lowest lane's group first (fixed order, so deterministic).
+// Called by all 64 lanes.
+__device__ __forceinline__ void seg_add(double e, int h, double *slot)
+{
+    const int lane = threadIdx.x & 63;
+    uint64_t pending = __ballot(h >= 0);
+    while (pending) {
+        const int lead = __builtin_ctzll(pending);
+        const int hl = __shfl(h, lead, 64);
+        const bool mine = (h == hl);
+        const double v = wave_sum(mine ? e : 0.0);
+        if (lane == 0) slot[hl] += v;
+        pending &= ~__ballot(mine);
+    }
+}
+
+// Staged-site record of the cached path.  Generic form: fixed-point x, y, z,
+// FP32 charge (sp) and FP32 radius (srf).  Fixed-unit form (ChainDev::fx, cubic
+// boxes with Coulomb/Debye-Hueckel and hard spheres): the charge is stored as
+// lB q / s and the radius as r c / s, where s = L / 2^32 is the fixed-point unit
+// and c = sqrt(1 + 1e-6) widens the hard-sphere test, so pair distances stay
+// in fixed-point units and no per-pair rescaling is needed.
+__device__ __forceinline__ void stage_site(const ChainDev &C, int i, int32_t fx, int32_t fy,
+                                           int32_t fz, int4 *sp, float *srf, int k)
+{
+    const float q = __int_as_float(C.pk[i].w);
+    if (C.fx) {
+        sp[k] = make_int4(fx, fy, fz, __float_as_int(q * C.fx_lbs));
+        srf[k] = C.rf[i] * C.fx_rs;
+    } else {
+        sp[k] = make_int4(fx, fy, fz, __float_as_int(q));
+        srf[k] = C.rf[i];
+    }
+}
+
+// FP32 pair energy in fixed-point units (see stage_site); `bad` flags a pair
+// whose value must come from FP64 (strong, non-finite, or within FP32
+// rounding of hard-sphere contact).
+template <int ELEC, int SR>
+__device__ __forceinline__ float pair_fx(const ChainDev &C, float r2, float aq, float bq, float ra,
+                                         float rb, float exact, bool &bad)
+{
+    float u = 0.f;
+    if constexpr (ELEC == ELEC_COULOMB) {
+        u = aq * bq * __builtin_amdgcn_rsqf(r2);
+    } else if constexpr (ELEC == ELEC_DH) {
+        const float rinv = __builtin_amdgcn_rsqf(r2);
+        u = aq * bq * rinv * __builtin_amdgcn_exp2f(C.fx_c1 * (r2 * rinv));
+    }
+    bool b = !(fabsf(u) <= exact);
+    if constexpr (SR == SR_HS) {
+        const float m = ra + rb;
+        b = b || (r2 < m * m);
+    }
+    bad = b;
+    return u;
+}
+
+template <int ELEC, int SR>
+__device__ __forceinline__ double pair_exact_new(const ChainDev &C, const double *__restrict__ tab,
+                                                 int j, int s, const double *sx, const double *sy,
+                                                 const double *sz, const double *sq,
+                                                 const double *sr, const int *st)
+{
+    double r2;
+    if (C.pot.nbkind == NB_VECTOR) {
+        double rx, ry, rz;
+        vdist_d(C.box, sx[s], sy[s], sz[s], C.x[j], C.y[j], C.z[j], rx, ry, rz);
+        r2 = rx * rx + (ry * ry + rz * rz);
+    } else {
+        r2 = sqdist_d(C.box, sx[s], sy[s], sz[s], C.x[j], C.y[j], C.z[j]);
+    }
+    return pair_u64_t<ELEC, SR>(C.pot, tab, r2, sq[s], C.q[j], sr[s], C.r[j], st[s], C.t[j]);
+}
+
+static constexpr int TR_B = 8;   // staged sites per inner-loop batch
+
+// Staged group (see stage_site; FP64 copies sx..sr for the exact pass)
+// against every site outside [front, back].  Energies are summed per partner
+// group into this wavefront's row of wslot (wslot[w * ng + h]); keep[h] == 0
+// drops group h (NonbondedCutg2g).  FP32 pairs are accumulated in FP64, TR_B
+// sites per batch with the site records read from LDS up front; a pair that
+// must be exact (pair_fx) is queued per wavefront and evaluated in FP64
+// afterwards.  Must be called by whole wavefronts; the caller clears wslot
+// before and synchronises after.
+template <int ELEC, int SR, bool FX>
+__device__ __forceinline__ void group_energies_mixed_t(const ChainDev &C, const float *__restrict__ tabf,
+                                       const double *__restrict__ tab, int front, int back, int ns,
+                                       const int4 *sp, const float *srf, const int *st,
+                                       const double *sx, const double *sy, const double *sz,
+                                       const double *sq, const double *sr,
+                                       const unsigned char *keep, double *wslot, int *dl)
+{
+    const int tid = threadIdx.x, nthr = blockDim.x, lane = tid & 63, w = tid >> 6;
+    double *slot = wslot + w * C.ng;
+    int *mydl = dl + w * TR_DW;
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const float fsx = C.box.fscale[0], fsy = C.box.fscale[1], fsz = C.box.fscale[2];
+    const float exact = C.pot.exact_f;
+    const int nfull = ns - ns % TR_B;
+    int nd = 0;   // wavefront-uniform queue length
+    const int iters = (C.n + nthr - 1) / nthr;
+
+    // queue (j, s) for the exact pass, or evaluate it at once when full
+    auto enqueue = [&](bool d, int j, int s, double &e) {
+        const uint64_t m = __ballot(d);
+        if (m) {
+            if (d) {
+                const int k = nd + __popcll(m & below);
+                if (k < TR_DW)
+                    mydl[k] = (j << 9) | s;
+                else
+                    e += pair_exact_new<ELEC, SR>(C, tab, j, s, sx, sy, sz, sq, sr, st);
+            }
+            nd += __popcll(m);
+        }
+    };
+    // one pair in FP32 (either form)
+    auto pair = [&](const int4 a, float ra, int ta, const int4 pj, float bq, float rb, int bt,
+                    bool &bad) -> float {
+        if constexpr (FX) {
+            const float dx = (float)(int32_t)(a.x - pj.x);
+            const float dy = (float)(int32_t)(a.y - pj.y);
+            const float dz = (float)(int32_t)(a.z - pj.z);
+            return pair_fx<ELEC, SR>(C, dx * dx + dy * dy + dz * dz, __int_as_float(a.w), bq, ra,
+                                     rb, exact, bad);
+        } else {
+            const float dx = (float)(int32_t)(a.x - pj.x) * fsx;
+            const float dy = (float)(int32_t)(a.y - pj.y) * fsy;
+            const float dz = (float)(int32_t)(a.z - pj.z) * fsz;
+            const float u = pair_u32_t<ELEC, SR>(C.pot, tabf, dx * dx + dy * dy + dz * dz,
+                                                 __int_as_float(a.w), bq, ra, rb, ta, bt);
+            bad = !(fabsf(u) <= exact);
+            return u;
+        }
+    };
+    constexpr bool NEED_T = (SR == SR_LJ || ELEC == ELEC_EWALD);
+    for (int it = 0; it < iters; ++it) {
+        const int j = it * nthr + tid;
+        int h = -1;
+        if (j < C.n && (j < front || j > back)) {
+            h = C.gid[j];
+            if (keep && !keep[h]) h = -1;
+        }
+        if (__ballot(h >= 0) == 0) continue;
+        const bool act = h >= 0;
+        const int jc = j < C.n ? j : C.n - 1;
+        const int4 pj = C.pk[jc];
+        const float bq = __int_as_float(pj.w);
+        const float br = (SR == SR_HS) ? C.rf[jc] * (FX ? C.fx_rs : 1.f) : 0.f;
+        const int bt = NEED_T ? C.t[jc] : 0;
+        double e = 0.0;
+        for (int s0 = 0; s0 < nfull; s0 += TR_B) {
+            int4 a[TR_B];
+            float ra[TR_B];
+            int ta[TR_B];
+#pragma unroll
+            for (int b = 0; b < TR_B; ++b) {
+                a[b] = sp[s0 + b];
+                ra[b] = (SR == SR_HS) ? srf[s0 + b] : 0.f;
+                ta[b] = NEED_T ? st[s0 + b] : 0;
+            }
+            unsigned dm = 0;
+#pragma unroll
+            for (int b = 0; b < TR_B; ++b) {
+                bool bad;
+                const float u = pair(a[b], ra[b], ta[b], pj, bq, br, bt, bad);
+                // an inactive lane's sum is discarded by seg_add
+                dm |= ((bad && act) ? 1u : 0u) << b;
+                e += (double)(bad ? 0.f : u);
+            }
+            if (__ballot(dm != 0)) {
+                for (int b = 0; b < TR_B; ++b) enqueue((dm >> b) & 1u, j, s0 + b, e);
+            }
+        }
+        for (int s = nfull; s < ns; ++s) {
+            bool bad;
+            const float u = pair(sp[s], (SR == SR_HS) ? srf[s] : 0.f, NEED_T ? st[s] : 0, pj, bq,
+                                 br, bt, bad);
+            e += (double)(bad ? 0.f : u);
+            enqueue(bad && act, j, s, e);
+        }
+        seg_add(e, h, slot);
+    }
+    // exact pass over this wavefront's queue
+    wave_sync_lds();
+    const int nq = min(nd, TR_DW);
+    for (int k0 = 0; k0 < nq; k0 += 64) {
+        const int k = k0 + lane;
+        int h = -1;
+        double e = 0.0;
+        if (k < nq) {
+            const int v = mydl[k];
+            const int j = v >> 9, s = v & 511;
+            e = pair_exact_new<ELEC, SR>(C, tab, j, s, sx, sy, sz, sq, sr, st);
+            h = C.gid[j];
+        }
+        seg_add(e, h, slot);
+    }
+}
+
+template <int ELEC, int SR>
+__device__ __forceinline__ void group_energies_mixed(const ChainDev &C, const float *__restrict__ tabf,
+                                     const double *__restrict__ tab, int front, int back, int ns,
+                                     const int4 *sp, const float *srf, const int *st,
+                                     const double *sx, const double *sy, const double *sz,
+                                     const double *sq, const double *sr, const unsigned char *keep,
+                                     double *wslot, int *dl)
+{
+    constexpr bool FX_OK = (SR == SR_NONE || SR == SR_HS) &&
+                           (ELEC == ELEC_NONE || ELEC == ELEC_COULOMB || ELEC == ELEC_DH);
+    if (FX_OK && C.fx)
+        group_energies_mixed_t<ELEC, SR, FX_OK>(C, tabf, tab, front, back, ns, sp, srf, st, sx, sy,
+                                                sz, sq, sr, keep, wslot, dl);
+    else
+        group_energies_mixed_t<ELEC, SR, false>(C, tabf, tab, front, back, ns, sp, srf, st, sx, sy,
+                                                sz, sq, sr, keep, wslot, dl);
+}
+
+// Fills row g of every chain's cache with the accepted state's g2g energies
+// (grid: groups x chains).  Rows of groups larger than the LDS stage are left
+// untouched: such groups are never moved by the cached sweep.
+template <int ELEC, int SR>
+__global__ __launch_bounds__(AT_THREADS) void k_emat_init(const ChainDev *__restrict__ chains)
+{
+    const ChainDev &C = chains[blockIdx.y];
+    const int g = blockIdx.x, tid = threadIdx.x, nthr = blockDim.x;
+    if (g >= C.ng || !C.emat) return;
+    const GroupDesc gd = C.groups[g];
+    const int ns = gd.back - gd.front + 1;
+    if (ns > MAX_GROUP_SITES) return;
+    __shared__ int4 sp[MAX_GROUP_SITES];
+    __shared__ float srf[MAX_GROUP_SITES];
+    __shared__ int st[MAX_GROUP_SITES];
+    __shared__ double sx[MAX_GROUP_SITES], sy[MAX_GROUP_SITES], sz[MAX_GROUP_SITES];
+    __shared__ double sq[MAX_GROUP_SITES], sr[MAX_GROUP_SITES];
+    __shared__ unsigned char keep[MAT_MAXG];
+    __shared__ double wslot[(AT_THREADS / 64) * MAT_MAXG];
+    __shared__ int dl[(AT_THREADS / 64) * TR_DW];
+    for (int k = tid; k < ns; k += nthr) {
+        const int i = gd.front + k;
+        const int4 p4 = C.pk[i];
+        stage_site(C, i, p4.x, p4.y, p4.z, sp, srf, k);
+        st[k] = C.t[i];
+        sx[k] = C.x[i]; sy[k] = C.y[i]; sz[k] = C.z[i];
+        sq[k] = C.q[i]; sr[k] = C.r[i];
+    }
+    for (int h = tid; h < C.ng; h += nthr) {
+        unsigned char kn = 1;
+        if (C.pot.nbkind == NB_CUTG2G) {
+            const GroupDesc hd = C.groups[h];
+            if (hd.molsize == hd.back - hd.front + 1 && gd.molsize == ns)
+                kn = sqdist_d(C.box, C.cm[6 * g], C.cm[6 * g + 1], C.cm[6 * g + 2], C.cm[6 * h],
+                              C.cm[6 * h + 1], C.cm[6 * h + 2]) > C.pot.g2g_rc2 ? 0 : 1;
+        }
+        keep[h] = kn;
+    }
+    for (int k = tid; k < (nthr >> 6) * C.ng; k += nthr) wslot[k] = 0.0;
+    __syncthreads();
+    group_energies_mixed<ELEC, SR>(C, C.tabf, C.tab, gd.front, gd.back, ns, sp, srf, st, sx, sy, sz,
+                                   sq, sr, keep, wslot, dl);
+    __syncthreads();
+    for (int h = tid; h < C.ng; h += nthr) {
+        double e = 0.0;
+        for (int w = 0; w < (nthr >> 6); ++w) e += wslot[w * C.ng + h];
+        C.emat[(size_t)g * C.ng + h] = (h == g) ? 0.0 : e;
+    }
+}
+
 template <int ELEC, int SR, bool MIX>
 __global__ __launch_bounds__(AT_THREADS) void k_sweep_tr(const ChainDev *__restrict__ chains, int ncalls)
 {
@@ -586,6 +871,13 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_tr(const ChainDev *__restr
     __shared__ int st[MAX_GROUP_SITES];
     __shared__ unsigned char keep_n[MAX_GROUPS_LDS], keep_o[MAX_GROUPS_LDS];
     __shared__ int s_collide;
+    // cached path: packed new sites, per-wavefront group sums, FP64 queue
+    __shared__ int4 snp[MIX ? MAX_GROUP_SITES : 1];
+    __shared__ float srf[MIX ? MAX_GROUP_SITES : 1];
+    __shared__ double wslot[MIX ? (AT_THREADS / 64) * MAT_MAXG : 1];
+    __shared__ int dl[MIX ? (AT_THREADS / 64) * TR_DW : 1];
+    __shared__ double enew[MIX ? MAT_MAXG : 1];
+    const bool cached = MIX && C.emat != nullptr;
 
     for (int k = tid; k < 624; k += nthr) {
         mtg[k] = C.rng[k];
@@ -683,8 +975,13 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_tr(const ChainDev *__restr
                 snf[0][s] = to_fixed(x, C.box.inv[0]);
                 snf[1][s] = to_fixed(y, C.box.inv[1]);
                 snf[2][s] = to_fixed(z, C.box.inv[2]);
+                if (MIX) {
+                    stage_site(C, i, snf[0][s], snf[1][s], snf[2][s], snp, srf, s);
+                }
                 if (collision_d(C.box, x, y, z)) s_collide = 1;
             }
+            if (cached)
+                for (int k = tid; k < (nthr >> 6) * C.ng; k += nthr) wslot[k] = 0.0;
             // ---- NonbondedCutg2g keep flags per other group (energy.h:1343-1358)
             for (int h = tid; h < C.ng; h += nthr) {
                 unsigned char kn = 1, ko = 1;
@@ -704,7 +1001,20 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_tr(const ChainDev *__restr
             }
             __syncthreads();
             double acc = 0.0;
-            for (int j = tid; j < C.n; j += nthr) {
+            if (cached) {
+                // g2All of the trial state per partner group; the accepted
+                // state's values come from the cache
+                group_energies_mixed<ELEC, SR>(C, C.tabf, C.tab, gd.front, gd.back, ns, snp, srf, st,
+                                               sn[0], sn[1], sn[2], sq, sr, keep_n, wslot, dl);
+                __syncthreads();
+                for (int h = tid; h < C.ng; h += nthr) {
+                    double e = 0.0;
+                    for (int w = 0; w < (nthr >> 6); ++w) e += wslot[w * C.ng + h];
+                    enew[h] = e;
+                    if (h != g) acc += e - C.emat[(size_t)g * C.ng + h];
+                }
+            }
+            for (int j = tid; j < (cached ? 0 : C.n); j += nthr) {
                 if (j >= gd.front && j <= gd.back) continue;
                 int h = C.gid[j];
                 bool kn = keep_n[h], ko = keep_o[h];
@@ -801,6 +1111,13 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_tr(const ChainDev *__restr
                 if (tid == 0) {
                     C.cm[6 * g] = tr.cmx; C.cm[6 * g + 1] = tr.cmy; C.cm[6 * g + 2] = tr.cmz;
                     C.cm[6 * g + 3] = tr.cmx; C.cm[6 * g + 4] = tr.cmy; C.cm[6 * g + 5] = tr.cmz;
+                }
+                if (cached) {   // EnergyMatrix::update(true): row and column g
+                    for (int h = tid; h < C.ng; h += nthr) {
+                        if (h == g) continue;
+                        C.emat[(size_t)g * C.ng + h] = enew[h];
+                        C.emat[(size_t)h * C.ng + g] = enew[h];
+                    }
                 }
             } else if (du == __builtin_inf()) {
                 n_inf++;
@@ -913,6 +1230,28 @@ static int prepare_chain(fgx_system *s, ChainDev &C, int **d_gvec, double *du_re
     C.du_rec = du_rec;
     C.acc_rec = acc_rec;
     C.cap = cap;
+    C.emat = nullptr;
+    if (mv.type == MOVE_TR && s->pot.precision == FGX_MIXED && s->emat_on && s->ng <= MAT_MAXG &&
+        s->n < (1 << 22)) {
+        if (!s->d_emat) {
+            FGX_HIP(hipMalloc(&s->d_emat, sizeof(double) * s->ng * s->ng));
+            s->device_bytes += sizeof(double) * s->ng * s->ng;
+            s->emat_valid = false;
+        }
+        C.emat = s->d_emat;
+    }
+    // fixed-unit pair form: one fixed-point unit s = L / 2^32 on all axes
+    C.fx = 0;
+    C.fx_lbs = C.fx_rs = C.fx_c1 = 0.f;
+    if (s->box.len[0] == s->box.len[1] && s->box.len[0] == s->box.len[2] &&
+        (s->pot.sr == SR_NONE || s->pot.sr == SR_HS) &&
+        (s->pot.elec == ELEC_NONE || s->pot.elec == ELEC_COULOMB || s->pot.elec == ELEC_DH)) {
+        const double unit = s->box.len[0] / 4294967296.0;
+        C.fx = 1;
+        C.fx_lbs = (float)(s->pot.lB / unit);
+        C.fx_rs = (float)(std::sqrt(1.000001) / unit);
+        C.fx_c1 = (float)(-s->pot.kappa * unit / std::log(2.0));
+    }
     return FGX_OK;
 }
 
@@ -1004,6 +1343,22 @@ static int launch_at(const Pot &P, hipStream_t st, const ChainDev *d_chains, int
     return FGX_OK;
 }
 
+static int launch_emat_init(const Pot &P, hipStream_t st, const ChainDev *d_chains, int nchains,
+                            int maxng)
+{
+#define FGX_EI(E, S) \
+    if (P.elec == E && P.sr == S) { hipLaunchKernelGGL((k_emat_init<E, S>), dim3(maxng, nchains), dim3(AT_THREADS), 0, st, d_chains); }
+    FGX_EI(ELEC_NONE, SR_HS) else FGX_EI(ELEC_NONE, SR_LJ) else
+    FGX_EI(ELEC_COULOMB, SR_NONE) else FGX_EI(ELEC_COULOMB, SR_HS) else FGX_EI(ELEC_COULOMB, SR_LJ) else
+    FGX_EI(ELEC_DH, SR_NONE) else FGX_EI(ELEC_DH, SR_HS) else FGX_EI(ELEC_DH, SR_LJ) else {
+        set_error("group-pair energy cache: no kernel for this Hamiltonian");
+        return FGX_EUNSUPPORTED;
+    }
+#undef FGX_EI
+    FGX_HIP(hipGetLastError());
+    return FGX_OK;
+}
+
 template <bool MIX>
 static int launch_tr(const Pot &P, hipStream_t st, const ChainDev *d_chains, int nchains, int nthr,
                      int ncalls)
@@ -1064,6 +1419,32 @@ static int run_chains(std::vector<fgx_system *> &sys, int ncalls, double *du_rec
         if (s->has_ewald) dyn = std::max(dyn, at_dyn_lds(s->ew.K));
     }
     ChainDev *d_chains = nullptr;
+    if (!rc) {
+        // group-pair energy caches that no longer match the accepted state
+        std::vector<ChainDev> stale;
+        int maxng = 0;
+        for (int c = 0; c < nc; ++c)
+            if (chains[c].emat && !sys[c]->emat_valid) {
+                stale.push_back(chains[c]);
+                maxng = std::max(maxng, chains[c].ng);
+            }
+        if (!stale.empty()) {
+            ChainDev *d_stale = nullptr;
+            FGX_HIP(hipMalloc(&d_stale, sizeof(ChainDev) * stale.size()));
+            FGX_HIP(hipMemcpyAsync(d_stale, stale.data(), sizeof(ChainDev) * stale.size(),
+                                   hipMemcpyHostToDevice, s0->stream));
+            rc = launch_emat_init(s0->pot, s0->stream, d_stale, (int)stale.size(), maxng);
+            hipError_t err = hipStreamSynchronize(s0->stream);
+            (void)hipFree(d_stale);
+            if (!rc && err != hipSuccess) {
+                set_error(std::string("energy cache kernel failed: ") + hipGetErrorString(err));
+                rc = FGX_EHIP;
+            }
+            if (!rc)
+                for (int c = 0; c < nc; ++c)
+                    if (chains[c].emat) sys[c]->emat_valid = true;
+        }
+    }
     if (!rc) {
         FGX_HIP(hipMalloc(&d_chains, sizeof(ChainDev) * nc));
         FGX_HIP(hipMemcpyAsync(d_chains, chains.data(), sizeof(ChainDev) * nc, hipMemcpyHostToDevice,
@@ -1256,6 +1637,14 @@ extern "C" int fgx_set_launch(fgx_system *s, int threads, int unroll)
               FGX_EARG, "fgx_set_launch: threads in {0,256,512}, unroll in {0,4,8,16}");
     s->launch_threads = threads;
     s->launch_unroll = unroll;
+    return FGX_OK;
+}
+
+extern "C" int fgx_set_energy_matrix(fgx_system *s, int on)
+{
+    FGX_CHECK(s, FGX_EARG, "fgx_set_energy_matrix: null system");
+    s->emat_on = on != 0;
+    s->emat_valid = false;
     return FGX_OK;
 }
 

@@ -807,7 +807,7 @@ extern "C" int fgx_destroy(fgx_system *s)
                     s->d_rf, s->d_groups, s->d_cm, s->d_tab, s->d_tabf, s->d_type_dp, s->d_mask,
                     s->d_partial, s->d_out, s->d_idx, s->d_rng, s->d_stats_i, s->d_stats_d,
                     s->ew.kv, s->ew.aks, s->ew.kn, s->ew.Q, s->ew.Qt, s->ew.eff,
-                    s->ew.cnt_dev};
+                    s->ew.cnt_dev, s->d_emat};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (s->h_out) (void)hipHostFree(s->h_out);
@@ -823,6 +823,7 @@ extern "C" int fgx_set_positions(fgx_system *s, int which, int first, int count,
     FGX_CHECK(s && xyz && which >= 0 && which <= 2 && first >= 0 && count >= 0 &&
                   first + count <= s->n,
               FGX_EARG, "fgx_set_positions: bad arguments");
+    if (which != 1) s->emat_valid = false;
     std::vector<double> t(count);
     for (int w = 0; w < 2; ++w) {
         if (which != 2 && which != w) continue;
@@ -901,6 +902,7 @@ extern "C" int fgx_get_group_cm(fgx_system *s, int g, double cm[3], double cm_tr
 extern "C" int fgx_set_group_cm(fgx_system *s, int g, const double cm[3], const double cm_trial[3])
 {
     FGX_CHECK(s && g >= 0 && g < s->ng, FGX_EARG, "fgx_set_group_cm: bad group");
+    if (cm) s->emat_valid = false;
     for (int k = 0; k < 3; ++k) {
         if (cm) s->cm[3 * g + k] = cm[k];
         if (cm_trial) s->cm_trial[3 * g + k] = cm_trial[k];
@@ -1232,6 +1234,7 @@ extern "C" int fgx_energy_change(fgx_system *s, const fgx_change *c, double *du)
 extern "C" int fgx_commit(fgx_system *s, const fgx_change *c, int accepted, double *drift)
 {
     FGX_CHECK(s, FGX_EARG, "fgx_commit: null system");
+    if (accepted) s->emat_valid = false;
     int nm;
     bool list;
     int rc = stage_change(s, c, &nm, &list);

@@ -119,6 +119,8 @@ extern "C" int fgx_temper_exchange(fgx_temper *t, int *acc, double *du)
             t->cfg[t->slot[r]]->pot.lBf = (float)t->lB[r];
             t->cfg[t->slot[p]]->pot.lB = t->lB[p];
             t->cfg[t->slot[p]]->pot.lBf = (float)t->lB[p];
+            t->cfg[t->slot[r]]->emat_valid = false;   // cached pair energies scale with lB
+            t->cfg[t->slot[p]]->emat_valid = false;
             if (acc) { acc[r] = 1; acc[p] = 1; }
         }
     }

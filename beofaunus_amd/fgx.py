@@ -97,6 +97,7 @@ def lib():
         "fgx_propagate": (I, [P, I, C.POINTER(Stats), _dp, _ip, C.c_long]),
         "fgx_propagate_batch": (I, [pp, I, I, C.POINTER(Stats)]),
         "fgx_set_launch": (I, [P, I, I]),
+        "fgx_set_energy_matrix": (I, [P, I]),
         "fgx_get_launch": (I, [P, _ip, _ip]),
         "fgx_autotune": (I, [P, I, _ip, _ip, _dp]),
         "fgx_autotune_batch": (I, [pp, I, I, _ip, _ip, _dp]),
@@ -271,6 +272,10 @@ class System:
 
     def set_launch(self, threads=0, unroll=0):
         _check(lib().fgx_set_launch(self.h, threads, unroll))
+
+    def set_energy_matrix(self, on):
+        """Group-pair energy cache of the mixed TranslateRotate sweep."""
+        _check(lib().fgx_set_energy_matrix(self.h, 1 if on else 0))
 
     def launch(self):
         t, u = C.c_int(), C.c_int()

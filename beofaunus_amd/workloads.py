@@ -173,18 +173,9 @@ def rigid_molecules(nmol=64, nsite=400, nion=8000, L=250.0, lB=7.1, debye=15.0, 
         radius += list(rr)
         types += [1 if qq < 0 else (2 if qq == 0 else 3) for qq in q]
         groups.append({"front": front, "back": len(xyz) - 1, "molsize": nsite, "molid": 0})
-    # ions: random positions avoiding molecule spheres and each other
-    ions = []
-    while len(ions) < nion:
-        cand = (rng.random((4 * nion, 3)) - 0.5) * L
-        dmin = np.full(len(cand), np.inf)
-        for c in centres:
-            d = cand - c
-            d -= L * np.round(d / L)
-            dmin = np.minimum(dmin, np.linalg.norm(d, axis=1))
-        ions += list(cand[dmin > R + 3.0])
-    ions = np.array(ions[:nion])
-    ions = _separate(ions, L, 4.0, rng)
+    # ions: random sequential addition clear of every site (hard-sphere
+    # contact ri + rj) and of each other
+    ions = _place_clear(rng, nion, L, 2.0, np.array(xyz), np.array(radius))
     front = len(xyz)
     for k in range(nion):
         xyz.append(ions[k])
@@ -208,8 +199,66 @@ def rigid_molecules(nmol=64, nsite=400, nion=8000, L=250.0, lB=7.1, debye=15.0, 
         "move": dict(kind=fgx.MOVE_TR, molid=0, permol=True, dp=dp, dprot=dprot,
                      center_rotation=True),
         "ref_kind": "dhhs",
+        "ref_nonbonded": {"epsr": epsr_for(lB, T), "debyelength": debye},
+        # atom ids follow sorted names: 1 = site -1, 2 = site 0, 3 = site +1,
+        # 4 = cation, 5 = anion; per-particle charge and radius come from the
+        # state file
+        "ref_atoms": {"M1n": {"q": -1.0, "r": 2.0, "mw": 1.0},
+                      "M2z": {"q": 0.0, "r": 2.0, "mw": 1.0},
+                      "M3p": {"q": 1.0, "r": 2.0, "mw": 1.0},
+                      "M4ip": {"q": 1.0, "r": 2.0, "dp": 2.0, "mw": 1.0},
+                      "M5im": {"q": -1.0, "r": 2.0, "dp": 2.0, "mw": 1.0}},
+        # molecule ids follow key order from 0: "amol" = 0, "bions" = 1
+        "ref_molecules": {"amol": {"atoms": " ".join(["M2z"] * nsite), "Ninit": 0},
+                          "bions": {"atoms": "M4ip M5im", "atomic": True, "Ninit": 0}},
+        "ref_moves": lambda per: {"moltransrot": {"center_rotation": True,
+                                                  "amol": {"dp": dp, "dprot": dprot,
+                                                           "permol": bool(per)}}},
         "T": T,
     }
+
+
+def _place_clear(rng, n, L, r_new, placed_xyz, placed_r, max_tries=10_000_000):
+    """n points of radius r_new placed uniformly at random in the periodic
+    cube, each clear of the placed spheres and of the points before it."""
+    from collections import defaultdict
+    rmax = max(float(np.max(placed_r)) if len(placed_r) else 0.0, r_new)
+    cs = r_new + rmax
+    nc = max(1, int(L // cs))
+    cs = L / nc
+    cell = defaultdict(list)
+
+    def key(x):
+        return tuple((((x + 0.5 * L) // cs).astype(int)) % nc)
+
+    for x, r in zip(placed_xyz, placed_r):
+        cell[key(x)].append((x, r))
+    out = []
+    tries = 0
+    while len(out) < n:
+        tries += 1
+        if tries > max_tries:
+            raise RuntimeError("could not place %d points" % n)
+        x = (rng.random(3) - 0.5) * L
+        c = key(x)
+        ok = True
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    for (y, r) in cell.get(((c[0] + dx) % nc, (c[1] + dy) % nc, (c[2] + dz) % nc), ()):
+                        d = y - x
+                        d -= L * np.round(d / L)
+                        if d @ d < (r + r_new) ** 2:
+                            ok = False
+                            break
+                    if not ok:
+                        break
+                if not ok:
+                    break
+        if ok:
+            out.append(x)
+            cell[c].append((x, r_new))
+    return np.array(out)
 
 
 def _separate(p, L, dmin, rng):
@@ -265,16 +314,19 @@ def mt_state_text(seed):
 
 def write_reference_inputs(w, directory, seed=1, peratom=True):
     """Write the reference's JSON config and Space::save state file
-    (space.h:736-764) for an atomic-salt workload, so the reference (or its
-    CPU restatement) can run the identical system."""
+    (space.h:736-764) for a workload, so the reference (or its CPU
+    restatement) can run the identical system.  `peratom` selects a sweep
+    (peratom / permol) or a single trial per move() call."""
     os.makedirs(directory, exist_ok=True)
     L = w["box"]
     cfg = {
         "system": {"temperature": w.get("T", T_DEFAULT), "geometry": {"length": L[0]}},
         "atomlist": w["ref_atoms"],
-        "moleculelist": {"salt": {"atoms": " ".join(w["ref_atoms"].keys()), "atomic": True,
+        "moleculelist": w.get("ref_molecules") or
+                        {"salt": {"atoms": " ".join(w["ref_atoms"].keys()), "atomic": True,
                                   "Ninit": 0}},
-        "moves": {"atomtranslate": {"salt": {"peratom": bool(peratom)}}},
+        "moves": (w["ref_moves"](peratom) if "ref_moves" in w else
+                  {"atomtranslate": {"salt": {"peratom": bool(peratom)}}}),
         "energy": {"nonbonded": w["ref_nonbonded"]},
     }
     cpath = os.path.join(directory, "config.json")

@@ -157,6 +157,11 @@ int fgx_propagate(fgx_system *s, int ncalls, fgx_stats *st, double *du_out, int 
  * picks the defaults.  No counterpart in the reference (a GPU schedule). */
 int fgx_set_launch(fgx_system *s, int threads, int unroll);
 int fgx_get_launch(fgx_system *s, int *threads, int *unroll);
+/* Group-pair energy cache of the mixed TranslateRotate sweep, after
+ * Energy::EnergyMatrix (energy.h:225-641): the accepted state's g2g energies
+ * are kept per group pair (ng <= 256) so only the trial state is evaluated.
+ * On by default; any change of the accepted state from the host rebuilds it. */
+int fgx_set_energy_matrix(fgx_system *s, int on);
 /* Autotuner: times `ncalls` Propagator::move() calls of a probe chain under
  * every schedule and keeps the fastest on the probe (whose state advances). */
 int fgx_autotune(fgx_system *probe, int ncalls, int *threads, int *unroll, double *best_seconds);
