@@ -208,7 +208,9 @@ __device__ __forceinline__ float pair_u32_t(const Pot &P, const float *__restric
         x = x * x * x;
         u2 = tabf[TAB_EPS4 * P.ntypes2 + k] * (x * x - x);
     }
-    return u1 + u2;
+    if constexpr (ELEC == ELEC_NONE) return u2;
+    else if constexpr (SR == SR_NONE) return u1;
+    else return u1 + u2;
 }
 
 // fixed-point coordinates: x -> round(x / L * 2^32) as a wrapping int32, so

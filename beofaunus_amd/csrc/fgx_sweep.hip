@@ -68,23 +68,26 @@ struct ChainDev {
     // group-pair energy cache (ng x ng, FP64) of the mixed TranslateRotate
     // sweep, or null
     double *emat;
-    // fixed-unit pair form of the cached path (cubic box, Coulomb/DH + HS)
+    // fixed-unit pair form (cubic box; Coulomb/DH + HS, or Ewald + LJ): pair
+    // distances stay in units of s = L / 2^32
     int fx;
     float fx_lbs, fx_rs, fx_c1;   // lB / s, sqrt(1 + 1e-6) / s, -kappa s / ln 2
+    float fx_rc2, fx_s2scale;     // Ewald rc^2 / s^2; 1 / s^2 for LJ sigma^2
+    // LJ parameters when every type pair present shares them (fixed units)
+    int lj_uniform, lj_check;
+    float lj_s2f, lj_eps4f;
 };
 
 // ------------------------------------------------------------------------
 // pair-loop helpers
 // ------------------------------------------------------------------------
-// u(new, j) - u(old, j) for one partner j, FP64
+// u(new, j) - u(old, j) for one partner with given FP64 data
 template <int ELEC, int SR>
-__device__ __forceinline__ double du_pair64(const ChainDev &C, const double *__restrict__ tab,
-                                            double nx, double ny, double nz, double ox, double oy,
-                                            double oz, double aq, double ar, int at, int j)
+__device__ __forceinline__ double du_pair64_v(const ChainDev &C, const double *__restrict__ tab,
+                                              double nx, double ny, double nz, double ox, double oy,
+                                              double oz, double aq, double ar, int at, double bx,
+                                              double by, double bz, double bq, double br, int bt)
 {
-    double bx = C.x[j], by = C.y[j], bz = C.z[j];
-    double bq = C.q[j], br = C.r[j];
-    int bt = C.t[j];
     double r2n, r2o;
     if (ELEC == ELEC_EWALD || C.pot.nbkind == NB_VECTOR) {
         double rx, ry, rz;
@@ -100,79 +103,233 @@ __device__ __forceinline__ double du_pair64(const ChainDev &C, const double *__r
            pair_u64_t<ELEC, SR>(C.pot, tab, r2o, aq, bq, ar, br, at, bt);
 }
 
-static constexpr int DCAP = 2048;   // deferred-pair list capacity (LDS)
+// u(new, j) - u(old, j) for one partner j, FP64
+template <int ELEC, int SR>
+__device__ __forceinline__ double du_pair64(const ChainDev &C, const double *__restrict__ tab,
+                                            double nx, double ny, double nz, double ox, double oy,
+                                            double oz, double aq, double ar, int at, int j)
+{
+    return du_pair64_v<ELEC, SR>(C, tab, nx, ny, nz, ox, oy, oz, aq, ar, at, C.x[j], C.y[j], C.z[j],
+                                 C.q[j], C.r[j], C.t[j]);
+}
+
+// FP32 pair energy in fixed-point units: distances in units of s = L / 2^32,
+// charge as lB q / s, radius as r sqrt(1 + 1e-6) / s; `bad` flags a pair
+// whose value must come from FP64 (strong, non-finite, or within FP32
+// rounding of hard-sphere contact).
+template <int ELEC, int SR>
+__device__ __forceinline__ float pair_fx(const ChainDev &C, float r2, float aq, float bq, float ra,
+                                         float rb, float exact, bool &bad)
+{
+    float u = 0.f;
+    if constexpr (ELEC == ELEC_COULOMB) {
+        u = aq * bq * __builtin_amdgcn_rsqf(r2);
+    } else if constexpr (ELEC == ELEC_DH) {
+        const float rinv = __builtin_amdgcn_rsqf(r2);
+        u = aq * bq * rinv * __builtin_amdgcn_exp2f(C.fx_c1 * (r2 * rinv));
+    }
+    bool b = !(fabsf(u) <= exact);
+    if constexpr (SR == SR_HS) {
+        const float m = ra + rb;
+        b = b || (r2 < m * m);
+    }
+    bad = b;
+    return u;
+}
+
+static constexpr int DCAP = 2048;   // deferred-pair queue capacity (LDS), all wavefronts
 static constexpr int LDS_TYPES2 = 256;  // type tables staged in LDS up to 16 types
 
 // Mixed-path pair loop of one AtomicTranslation trial.  Partners are read as
-// packed 16-byte records (fixed-point x, y, z and FP32 charge), four per
-// thread in flight.  The bulk of the pairs are evaluated in FP32 and summed in
-// FP64.  Partners inside the Ewald real-space cutoff, and pairs whose FP32
-// energy is strong (|u| > Pot::exact_f) or a hard-sphere contact, are appended
-// to an LDS list and evaluated afterwards in FP64 by all threads, so the
-// erfc/exp work is compacted instead of idling divergent lanes.
-template <int ELEC, int SR, int U>
+// packed 16-byte records (fixed-point x, y, z and FP32 charge), U per thread
+// per batch, and evaluated in FP32 for the old and the new position with the
+// difference summed in FP64.  Partners inside the Ewald real-space cutoff, and
+// pairs whose FP32 energy is strong (|u| > Pot::exact_f), non-finite or a
+// hard-sphere contact, are flagged in a per-batch bit mask and appended to
+// this wavefront's queue (dq, `qcap` entries; the wavefront's count in *nq),
+// to be evaluated in FP64 afterwards, so the erfc/exp work is compacted
+// instead of idling divergent lanes.  Queue order is fixed by lane order, so
+// results are deterministic.  Called by whole wavefronts.
+// Loads through a global-address-space pointer: the ChainDev arrays are
+// device memory, and saying so lets the compiler use global (not flat)
+// loads, which do not hold back LDS traffic.
+template <typename T>
+__device__ __forceinline__ T gload(const T *p, int i)
+{
+    return ((const __attribute__((address_space(1))) T *)p)[i];
+}
+__device__ __forceinline__ int4 gload(const int4 *p, int i)
+{
+    typedef int v4i __attribute__((ext_vector_type(4)));
+    const v4i v = ((const __attribute__((address_space(1))) v4i *)p)[i];
+    return make_int4(v.x, v.y, v.z, v.w);
+}
+
+template <int U, bool HS, bool NEED_T>
+struct PartnerBatch {
+    int4 P[U];
+    float R[U];
+    int T[U];
+    __device__ __forceinline__ void load(const ChainDev &C, int base, int nthr, int tid)
+    {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = base + u * nthr + tid;
+            const int jc = j < C.n ? j : C.n - 1;
+            P[u] = gload(C.pk, jc);
+            R[u] = HS ? gload(C.rf, jc) : 0.f;
+            T[u] = NEED_T ? gload(C.t, jc) : 0;
+        }
+    }
+};
+
+// true where the mixed AtomicTranslation loop has a fixed-unit form
+template <int ELEC, int SR>
+__host__ __device__ constexpr bool at_fx_ok()
+{
+    return (ELEC == ELEC_EWALD && SR == SR_LJ) ||
+           ((ELEC == ELEC_NONE || ELEC == ELEC_COULOMB || ELEC == ELEC_DH) &&
+            (SR == SR_NONE || SR == SR_HS));
+}
+
+template <int ELEC, int SR, int U, bool FX>
 __device__ __forceinline__ double pair_loop_mixed(const ChainDev &C, const float *__restrict__ tabf,
                                                   const double *__restrict__ tab, int i, double tnx,
                                                   double tny, double tnz, double tox, double toy,
-                                                  double toz, int *dlist, int *dcount)
+                                                  double toz, int *dq, int qcap, int &nq)
 {
-    const int tid = threadIdx.x, nthr = blockDim.x;
-    const int4 me = C.pk[i];
+    const int tid = threadIdx.x, nthr = blockDim.x, lane = tid & 63;
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const int4 me = gload(C.pk, i);
     const float aq = __int_as_float(me.w);
-    const float ar = (SR == SR_HS) ? C.rf[i] : 0.f;
-    const int at = C.t[i];
+    const float ar = (SR == SR_HS) ? gload(C.rf, i) : 0.f;
+    const int at = gload(C.t, i);
     const int32_t nx = to_fixed(tnx, C.box.inv[0]), ny = to_fixed(tny, C.box.inv[1]),
                   nz = to_fixed(tnz, C.box.inv[2]);
     const float sx = C.box.fscale[0], sy = C.box.fscale[1], sz = C.box.fscale[2];
     const float exact = C.pot.exact_f;
+    constexpr bool NEED_T = (SR == SR_LJ || ELEC == ELEC_EWALD);
+    // fixed-unit form: charge lB q / s, radius r c / s (see stage_site)
+    const float aqx = FX ? aq * C.fx_lbs : aq;
+    const float arx = FX ? ar * C.fx_rs : ar;
+    int cnt = 0;   // wavefront-uniform
     double acc = 0.0;
-    for (int j0 = tid; j0 < C.n; j0 += U * nthr) {
-        int4 P[U];
-        float R[U];
-        int TT[U];
+    // two batches alternate: one is evaluated while the other's loads are in
+    // flight (the loop body is unrolled twice so neither is copied)
+    PartnerBatch<U, SR == SR_HS, NEED_T> A, B;
+    // FAST: one shared LJ parameter pair and no strong-pair test beyond rc
+    // (decided once per launch, so the unrolled body has no branches)
+    auto eval = [&](auto fast, const PartnerBatch<U, SR == SR_HS, NEED_T> &X, int base) {
+        constexpr bool FAST = decltype(fast)::value;
+        unsigned dm = 0;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            int j = j0 + u * nthr;
-            bool ok = j < C.n;
-            P[u] = ok ? C.pk[j] : make_int4(0, 0, 0, 0);
-            R[u] = (SR == SR_HS && ok) ? C.rf[j] : 0.f;
-            TT[u] = ((SR == SR_LJ || ELEC == ELEC_EWALD) && ok) ? C.t[j] : 0;
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int j = j0 + u * nthr;
-            if (j >= C.n || j == i) continue;
-            float dx = (float)(int32_t)(nx - P[u].x) * sx, dy = (float)(int32_t)(ny - P[u].y) * sy,
-                  dz = (float)(int32_t)(nz - P[u].z) * sz;
-            const float r2n = dx * dx + dy * dy + dz * dz;
-            dx = (float)(int32_t)(me.x - P[u].x) * sx; dy = (float)(int32_t)(me.y - P[u].y) * sy;
-            dz = (float)(int32_t)(me.z - P[u].z) * sz;
-            const float r2o = dx * dx + dy * dy + dz * dz;
-            const float bq = __int_as_float(P[u].w);
-            bool defer = false;
-            float un = 0.f, uo = 0.f;
-            if constexpr (ELEC == ELEC_EWALD) {
-                defer = (r2n <= C.pot.ew_rc2f) || (r2o <= C.pot.ew_rc2f);
-                if (!defer) {   // beyond rc the real-space term is zero
-                    un = pair_u32_t<ELEC_NONE, SR>(C.pot, tabf, r2n, aq, bq, ar, R[u], at, TT[u]);
-                    uo = pair_u32_t<ELEC_NONE, SR>(C.pot, tabf, r2o, aq, bq, ar, R[u], at, TT[u]);
+            const int j = base + u * nthr + tid;
+            const bool ok = j < C.n && j != i;
+            const int4 P = X.P[u];
+            if constexpr (FX) {
+                float dx = (float)(int32_t)(nx - P.x), dy = (float)(int32_t)(ny - P.y),
+                      dz = (float)(int32_t)(nz - P.z);
+                const float r2n = dx * dx + dy * dy + dz * dz;
+                dx = (float)(int32_t)(me.x - P.x);
+                dy = (float)(int32_t)(me.y - P.y);
+                dz = (float)(int32_t)(me.z - P.z);
+                const float r2o = dx * dx + dy * dy + dz * dz;
+                float d;
+                bool bad;
+                if constexpr (ELEC == ELEC_EWALD) {
+                    // LJ only beyond rc (real-space Ewald is zero there);
+                    // partners inside rc go to the FP64 pass
+                    float s2 = C.lj_s2f, e4 = C.lj_eps4f;
+                    if constexpr (!FAST) {
+                        const int k = at * C.pot.ntypes + X.T[u];
+                        s2 = tabf[TAB_S2 * C.pot.ntypes2 + k];
+                        e4 = tabf[TAB_EPS4 * C.pot.ntypes2 + k];
+                    }
+                    float xn = s2 * __builtin_amdgcn_rcpf(r2n);
+                    float xo = s2 * __builtin_amdgcn_rcpf(r2o);
+                    xn = xn * xn * xn;
+                    xo = xo * xo * xo;
+                    const float vn = xn * xn - xn, vo = xo * xo - xo;
+                    bad = fminf(r2n, r2o) <= C.fx_rc2;
+                    if constexpr (!FAST)
+                        bad = bad || !(fabsf(e4 * vn) <= exact) || !(fabsf(e4 * vo) <= exact);
+                    d = e4 * (vn - vo);
+                } else {
+                    bool bn, bo;
+                    const float rb = X.R[u] * C.fx_rs;
+                    const float un = pair_fx<ELEC, SR>(C, r2n, aqx, __int_as_float(P.w), arx, rb,
+                                                       exact, bn);
+                    const float uo = pair_fx<ELEC, SR>(C, r2o, aqx, __int_as_float(P.w), arx, rb,
+                                                       exact, bo);
+                    bad = bn || bo;
+                    d = un - uo;
                 }
-            } else {
-                un = pair_u32_t<ELEC, SR>(C.pot, tabf, r2n, aq, bq, ar, R[u], at, TT[u]);
-                uo = pair_u32_t<ELEC, SR>(C.pot, tabf, r2o, aq, bq, ar, R[u], at, TT[u]);
+                const bool take = ok && !bad;
+                dm |= ((ok && bad) ? 1u : 0u) << u;
+                // select on the FP32 bits, then widen: one select, not two
+                acc += (double)__int_as_float(__float_as_int(d) & (take ? -1 : 0));
+                continue;
             }
-            defer = defer || !(fabsf(un) <= exact) || !(fabsf(uo) <= exact);
-            if (defer) {
-                int slot = atomicAdd(dcount, 1);
-                if (slot < DCAP)
-                    dlist[slot] = j;
-                else
-                    acc += du_pair64<ELEC, SR>(C, tab, tnx, tny, tnz, tox, toy, toz, C.q[i], C.r[i], at, j);
+            float dx = (float)(int32_t)(nx - P.x) * sx, dy = (float)(int32_t)(ny - P.y) * sy,
+                  dz = (float)(int32_t)(nz - P.z) * sz;
+            const float r2n = dx * dx + dy * dy + dz * dz;
+            dx = (float)(int32_t)(me.x - P.x) * sx;
+            dy = (float)(int32_t)(me.y - P.y) * sy;
+            dz = (float)(int32_t)(me.z - P.z) * sz;
+            const float r2o = dx * dx + dy * dy + dz * dz;
+            const float bq = __int_as_float(P.w);
+            float un, uo;
+            bool bad;
+            if constexpr (ELEC == ELEC_EWALD) {
+                // beyond rc the real-space term is zero; inside, FP64 decides
+                un = pair_u32_t<ELEC_NONE, SR>(C.pot, tabf, r2n, aq, bq, ar, X.R[u], at, X.T[u]);
+                uo = pair_u32_t<ELEC_NONE, SR>(C.pot, tabf, r2o, aq, bq, ar, X.R[u], at, X.T[u]);
+                bad = (r2n <= C.pot.ew_rc2f) || (r2o <= C.pot.ew_rc2f);
             } else {
-                acc += (double)(un - uo);
+                un = pair_u32_t<ELEC, SR>(C.pot, tabf, r2n, aq, bq, ar, X.R[u], at, X.T[u]);
+                uo = pair_u32_t<ELEC, SR>(C.pot, tabf, r2o, aq, bq, ar, X.R[u], at, X.T[u]);
+                bad = false;
+            }
+            bad = bad || !(fabsf(un) <= exact) || !(fabsf(uo) <= exact);
+            dm |= ((ok && bad) ? 1u : 0u) << u;
+            acc += (double)((ok && !bad) ? un - uo : 0.f);
+        }
+        if (__ballot(dm != 0)) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const bool d = (dm >> u) & 1u;
+                const uint64_t m = __ballot(d);
+                if (m) {
+                    if (d) {
+                        const int j = base + u * nthr + tid;
+                        const int k = cnt + __popcll(m & below);
+                        if (k < qcap)
+                            dq[k] = j;
+                        else
+                            acc += du_pair64<ELEC, SR>(C, tab, tnx, tny, tnz, tox, toy, toz,
+                                                       C.q[i], C.r[i], at, j);
+                    }
+                    cnt += __popcll(m);
+                }
             }
         }
-    }
+    };
+    const int step = U * nthr;
+    auto run = [&](auto fast) {
+        A.load(C, 0, nthr, tid);
+        for (int base = 0; base < C.n; base += 2 * step) {
+            B.load(C, base + step, nthr, tid);        // clamped: always in bounds
+            eval(fast, A, base);
+            A.load(C, base + 2 * step, nthr, tid);
+            if (base + step < C.n) eval(fast, B, base + step);
+        }
+    };
+    if (FX && ELEC == ELEC_EWALD && C.lj_uniform && !C.lj_check)
+        run(std::true_type{});
+    else
+        run(std::false_type{});
+    nq = min(cnt, qcap);
     return acc;
 }
 
@@ -232,10 +389,11 @@ __device__ __forceinline__ void kunpack(int v, int &nx, int &ny, int &nz)
     nz = ((v >> 16) & 0xff) - 128;
 }
 
-// bytes of dynamic LDS a chain needs: S(k) (double2), A_k (double), packed n (int)
-__host__ __device__ inline size_t at_dyn_lds(int K) { return (size_t)K * (16 + 8 + 4); }
+// bytes of dynamic LDS a chain needs: S(k) and the trial's dS(k) (double2),
+// A_k (double), packed n (int)
+__host__ __device__ inline size_t at_dyn_lds(int K) { return (size_t)K * (16 + 16 + 8 + 4); }
 
-template <int ELEC, int SR, bool MIX, int U>
+template <int ELEC, int SR, bool MIX, int U, bool FX>
 __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restrict__ chains,
                                                          int ncalls)
 {
@@ -249,15 +407,15 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
     __shared__ double2 etab[2][3 * MAXK_AXIS];   // old / new per-axis phase tables
     __shared__ double2 rtab[EW ? REFRESH_CHUNK * 3 * REFRESH_KAX : 1];
     __shared__ double rw[EW ? REFRESH_CHUNK : 1];
-    __shared__ int dlist[MIX ? DCAP : 1];
-    __shared__ int dcount;
+    __shared__ int dlist[MIX ? DCAP : 1];   // per-wavefront FP64 queues
     // per-type pair tables in LDS (the partner loop indexes them per lane)
     __shared__ double ltab[TAB_COUNT * LDS_TYPES2];
     __shared__ float ltabf[MIX ? TAB_COUNT * LDS_TYPES2 : 1];
     // Ewald state for the whole launch, LDS-resident: S(k) FP64, A_k, n-triples
     extern __shared__ double2 dyn[];
     double2 *S = dyn;
-    double *Ak = (double *)(S + (EW ? C.K : 0));
+    double2 *dS = S + (EW ? C.K : 0);
+    double *Ak = (double *)(dS + (EW ? C.K : 0));
     int *kid = (int *)(Ak + (EW ? C.K : 0));
 
     for (int k = tid; k < 624; k += nthr) {
@@ -270,7 +428,8 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
         for (int k = tid; k < TAB_COUNT * C.pot.ntypes2; k += nthr) {
             int t = k / C.pot.ntypes2, e = k % C.pot.ntypes2;
             ltab[t * C.pot.ntypes2 + e] = C.tab[k];
-            if (MIX) ltabf[t * C.pot.ntypes2 + e] = C.tabf[k];
+            // the fixed-unit loop takes sigma^2 in units of s^2
+            if (MIX) ltabf[t * C.pot.ntypes2 + e] = C.tabf[k] * ((FX && t == TAB_S2) ? C.fx_s2scale : 1.f);
         }
     }
     const double *tab = lds_tab ? ltab : C.tab;
@@ -335,7 +494,6 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
                     tr.ox = ox; tr.oy = oy; tr.oz = oz;
                     tr.nx = nx; tr.ny = ny; tr.nz = nz;
                     tr.collide = collision_d(C.box, nx, ny, nz) ? 1 : 0;
-                    dcount = 0;
                 }
             }
             __syncthreads();
@@ -355,6 +513,11 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
                 }
             }
             // ---- pair part: sum_{j != i} u(i', j) - u(i, j)
+            int nq = 0;            // this wavefront's FP64 queue (mixed path)
+            int *dq = dlist;
+            int pj = -1;           // this lane's first queued partner, data preloaded
+            double pbx = 0, pby = 0, pbz = 0, pbq = 0, pbr = 0;
+            int pbt = 0;
             if constexpr (!MIX) {
                 const double aq = C.q[i], ar = C.r[i];
                 const int at = C.t[i];
@@ -365,15 +528,25 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
                     acc += du_pair64<ELEC, SR>(C, tab, nx, ny, nz, ox, oy, oz, aq, ar, at, j);
                 }
             } else {
-                acc = pair_loop_mixed<ELEC, SR, U>(C, tabf, tab, i, tr.nx, tr.ny, tr.nz, tr.ox, tr.oy,
-                                                   tr.oz, dlist, &dcount);
-                __syncthreads();   // deferred list complete
-                const int nd = min(dcount, DCAP);
-                const double aq64 = C.q[i], ar64 = C.r[i];
-                const int at = C.t[i];
-                for (int k = tid; k < nd; k += nthr)
-                    acc += du_pair64<ELEC, SR>(C, tab, tr.nx, tr.ny, tr.nz, tr.ox, tr.oy, tr.oz, aq64,
-                                               ar64, at, dlist[k]);
+                const int qcap = DCAP / (nthr >> 6);
+                dq = dlist + (tid >> 6) * qcap;
+                // (the fixed-unit form is only launched with LDS tables)
+                if (FX || lds_tab)
+                    acc = pair_loop_mixed<ELEC, SR, U, FX>(C, ltabf, ltab, i, tr.nx, tr.ny, tr.nz,
+                                                           tr.ox, tr.oy, tr.oz, dq, qcap, nq);
+                else
+                    acc = pair_loop_mixed<ELEC, SR, U, false>(C, C.tabf, C.tab, i, tr.nx, tr.ny,
+                                                              tr.nz, tr.ox, tr.oy, tr.oz, dq, qcap,
+                                                              nq);
+                wave_sync_lds();
+                // loads for the FP64 pass are issued before the k-space work
+                // so their latency overlaps it
+                const int lane = tid & 63;
+                if (lane < nq) {
+                    pj = dq[lane];
+                    pbx = C.x[pj]; pby = C.y[pj]; pbz = C.z[pj];
+                    pbq = C.q[pj]; pbr = C.r[pj]; pbt = C.t[pj];
+                }
             }
             // ---- reciprocal part: A_k (|S+dS|^2 - |S|^2) per k, FP64
             const double wq = EW ? C.q[i] * C.eff[C.t[i]] : 0.0;
@@ -388,9 +561,21 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
                     ew_phase(etab[0], kcc1, nx, ny, nz, orr, oi);
                     double dr = wq * (er - orr), di = wq * (ei - oi);
                     double2 sk = S[k];
+                    dS[k] = make_double2(dr, di);
                     de += Ak[k] * (2.0 * (sk.x * dr + sk.y * di) + (dr * dr + di * di));
                 }
                 acc += C.recip_pref * de;
+            }
+            // ---- FP64 pass of the mixed path over this wavefront's queue
+            if constexpr (MIX) {
+                const double aq64 = C.q[i], ar64 = C.r[i];
+                const int at = C.t[i];
+                if (pj >= 0)
+                    acc += du_pair64_v<ELEC, SR>(C, tab, tr.nx, tr.ny, tr.nz, tr.ox, tr.oy, tr.oz,
+                                                 aq64, ar64, at, pbx, pby, pbz, pbq, pbr, pbt);
+                for (int k = 64 + (tid & 63); k < nq; k += 64)
+                    acc += du_pair64<ELEC, SR>(C, tab, tr.nx, tr.ny, tr.nz, tr.ox, tr.oy, tr.oz,
+                                               aq64, ar64, at, dq[k]);
             }
             double du = block_sum(acc, red);
             if (EW && C.surf_pref != 0.0) {
@@ -425,13 +610,8 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
                     Mx += wq * (tr.nx - tr.ox); My += wq * (tr.ny - tr.oy); Mz += wq * (tr.nz - tr.oz);
                     // S(k) += dS(k): same k ownership as above, so no race
                     for (int k = tid; k < C.K; k += nthr) {
-                        int nx, ny, nz;
-                        kunpack(kid[k], nx, ny, nz);
-                        double er, ei, orr, oi;
-                        ew_phase(etab[1], kcc1, nx, ny, nz, er, ei);
-                        ew_phase(etab[0], kcc1, nx, ny, nz, orr, oi);
-                        double2 sk = S[k];
-                        S[k] = make_double2(sk.x + wq * (er - orr), sk.y + wq * (ei - oi));
+                        const double2 sk = S[k], d = dS[k];
+                        S[k] = make_double2(sk.x + d.x, sk.y + d.y);
                     }
                     // NonbondedEwald::update (ewald.h:626-643): full recompute of
                     // S(k) every update_frequency accepted moves; the returned
@@ -620,29 +800,6 @@ __device__ __forceinline__ void stage_site(const ChainDev &C, int i, int32_t fx,
         sp[k] = make_int4(fx, fy, fz, __float_as_int(q));
         srf[k] = C.rf[i];
     }
-}
-
-// FP32 pair energy in fixed-point units (see stage_site); `bad` flags a pair
-// whose value must come from FP64 (strong, non-finite, or within FP32
-// rounding of hard-sphere contact).
-template <int ELEC, int SR>
-__device__ __forceinline__ float pair_fx(const ChainDev &C, float r2, float aq, float bq, float ra,
-                                         float rb, float exact, bool &bad)
-{
-    float u = 0.f;
-    if constexpr (ELEC == ELEC_COULOMB) {
-        u = aq * bq * __builtin_amdgcn_rsqf(r2);
-    } else if constexpr (ELEC == ELEC_DH) {
-        const float rinv = __builtin_amdgcn_rsqf(r2);
-        u = aq * bq * rinv * __builtin_amdgcn_exp2f(C.fx_c1 * (r2 * rinv));
-    }
-    bool b = !(fabsf(u) <= exact);
-    if constexpr (SR == SR_HS) {
-        const float m = ra + rb;
-        b = b || (r2 < m * m);
-    }
-    bad = b;
-    return u;
 }
 
 template <int ELEC, int SR>
@@ -1242,15 +1399,48 @@ static int prepare_chain(fgx_system *s, ChainDev &C, int **d_gvec, double *du_re
     }
     // fixed-unit pair form: one fixed-point unit s = L / 2^32 on all axes
     C.fx = 0;
-    C.fx_lbs = C.fx_rs = C.fx_c1 = 0.f;
-    if (s->box.len[0] == s->box.len[1] && s->box.len[0] == s->box.len[2] &&
-        (s->pot.sr == SR_NONE || s->pot.sr == SR_HS) &&
-        (s->pot.elec == ELEC_NONE || s->pot.elec == ELEC_COULOMB || s->pot.elec == ELEC_DH)) {
+    C.fx_lbs = C.fx_rs = C.fx_c1 = C.fx_rc2 = C.fx_s2scale = 0.f;
+    C.lj_uniform = 0;
+    C.lj_check = 1;
+    C.lj_s2f = C.lj_eps4f = 0.f;
+    const bool cubic = s->box.len[0] == s->box.len[1] && s->box.len[0] == s->box.len[2];
+    const bool dh_like = (s->pot.sr == SR_NONE || s->pot.sr == SR_HS) &&
+                         (s->pot.elec == ELEC_NONE || s->pot.elec == ELEC_COULOMB ||
+                          s->pot.elec == ELEC_DH);
+    const bool ew_lj = s->pot.elec == ELEC_EWALD && s->pot.sr == SR_LJ &&
+                       s->pot.ntypes2 <= LDS_TYPES2;
+    if (cubic && (dh_like || ew_lj)) {
         const double unit = s->box.len[0] / 4294967296.0;
         C.fx = 1;
         C.fx_lbs = (float)(s->pot.lB / unit);
         C.fx_rs = (float)(std::sqrt(1.000001) / unit);
         C.fx_c1 = (float)(-s->pot.kappa * unit / std::log(2.0));
+        if (ew_lj) {
+            C.fx_rc2 = (float)(s->pot.ew_rc2 / (unit * unit));
+            C.fx_s2scale = (float)(1.0 / (unit * unit));
+            // LJ parameters over the type pairs present; and whether any LJ
+            // pair beyond rc can be strong (|x^6 - x^3| <= y + y^2 with
+            // y = (sigma^2 / rc^2)^3), which would need the |u| test
+            const int nt = s->pot.ntypes, nt2 = s->pot.ntypes2;
+            std::vector<char> present(nt, 0);
+            for (int t : s->h_type) present[t] = 1;
+            bool uni = true, first = true;
+            double s2u = 0, e4u = 0, bound = 0;
+            for (int a = 0; a < nt; ++a)
+                for (int b = 0; b < nt; ++b) {
+                    if (!present[a] || !present[b]) continue;
+                    const double s2 = s->tab[TAB_S2 * nt2 + a * nt + b];
+                    const double e4 = s->tab[TAB_EPS4 * nt2 + a * nt + b];
+                    if (first) { s2u = s2; e4u = e4; first = false; }
+                    else if (s2 != s2u || e4 != e4u) uni = false;
+                    const double y = std::pow(s2 / s->pot.ew_rc2, 3.0);
+                    bound = std::max(bound, std::fabs(e4) * (y + y * y));
+                }
+            C.lj_uniform = uni ? 1 : 0;
+            C.lj_s2f = (float)(s2u / (unit * unit));
+            C.lj_eps4f = (float)e4u;
+            C.lj_check = bound < 0.5 * s->pot.exact_f ? 0 : 1;
+        }
     }
     return FGX_OK;
 }
@@ -1303,34 +1493,46 @@ static int init_mass_centres(fgx_system *s)
     return FGX_OK;
 }
 
-template <int ELEC, int SR, bool MIX, int U>
-static void launch_at_u(hipStream_t st, const ChainDev *d_chains, int nchains, int nthr,
+template <int ELEC, int SR, bool MIX, int U, bool FX>
+static void launch_at_f(hipStream_t st, const ChainDev *d_chains, int nchains, int nthr,
                         size_t dyn, int ncalls)
 {
     if (dyn > 0)
-        (void)hipFuncSetAttribute((const void *)k_sweep_at<ELEC, SR, MIX, U>,
+        (void)hipFuncSetAttribute((const void *)k_sweep_at<ELEC, SR, MIX, U, FX>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
-    hipLaunchKernelGGL((k_sweep_at<ELEC, SR, MIX, U>), dim3(nchains), dim3(nthr), dyn, st,
+    hipLaunchKernelGGL((k_sweep_at<ELEC, SR, MIX, U, FX>), dim3(nchains), dim3(nthr), dyn, st,
                        d_chains, ncalls);
+}
+
+template <int ELEC, int SR, bool MIX, int U>
+static void launch_at_u(hipStream_t st, const ChainDev *d_chains, int nchains, int nthr,
+                        size_t dyn, int ncalls, bool fx)
+{
+    if constexpr (MIX && at_fx_ok<ELEC, SR>()) {
+        if (fx) {
+            launch_at_f<ELEC, SR, MIX, U, true>(st, d_chains, nchains, nthr, dyn, ncalls);
+            return;
+        }
+    }
+    launch_at_f<ELEC, SR, MIX, U, false>(st, d_chains, nchains, nthr, dyn, ncalls);
 }
 
 // partner-loop unroll depth: the autotuned knob of the mixed path (the FP64
 // path keeps a plain strided loop)
 template <int ELEC, int SR, bool MIX>
 static void launch_at_k(hipStream_t st, const ChainDev *d_chains, int nchains, int nthr,
-                        size_t dyn, int ncalls, int unroll)
+                        size_t dyn, int ncalls, int unroll, bool fx)
 {
-    if (!MIX || unroll <= 4) launch_at_u<ELEC, SR, MIX, 4>(st, d_chains, nchains, nthr, dyn, ncalls);
-    else if (unroll <= 8) launch_at_u<ELEC, SR, MIX, 8>(st, d_chains, nchains, nthr, dyn, ncalls);
-    else launch_at_u<ELEC, SR, MIX, 16>(st, d_chains, nchains, nthr, dyn, ncalls);
+    if (!MIX || unroll <= 4) launch_at_u<ELEC, SR, MIX, 4>(st, d_chains, nchains, nthr, dyn, ncalls, fx);
+    else launch_at_u<ELEC, SR, MIX, 8>(st, d_chains, nchains, nthr, dyn, ncalls, fx);
 }
 
 template <bool MIX>
 static int launch_at(const Pot &P, hipStream_t st, const ChainDev *d_chains, int nchains, int nthr,
-                     size_t dyn, int ncalls, int unroll)
+                     size_t dyn, int ncalls, int unroll, bool fx)
 {
 #define FGX_AT(E, S) \
-    if (P.elec == E && P.sr == S) { launch_at_k<E, S, MIX>(st, d_chains, nchains, nthr, dyn, ncalls, unroll); }
+    if (P.elec == E && P.sr == S) { launch_at_k<E, S, MIX>(st, d_chains, nchains, nthr, dyn, ncalls, unroll, fx); }
     FGX_AT(ELEC_NONE, SR_HS) else FGX_AT(ELEC_NONE, SR_LJ) else
     FGX_AT(ELEC_COULOMB, SR_NONE) else FGX_AT(ELEC_COULOMB, SR_HS) else FGX_AT(ELEC_COULOMB, SR_LJ) else
     FGX_AT(ELEC_DH, SR_NONE) else FGX_AT(ELEC_DH, SR_HS) else FGX_AT(ELEC_DH, SR_LJ) else
@@ -1386,6 +1588,7 @@ static int run_chains(std::vector<fgx_system *> &sys, int ncalls, double *du_rec
     int rc = FGX_OK;
     int nthr = 0;
     size_t dyn = 0;
+    bool fx = true;   // fixed-unit form only when every chain has it
     const int unroll = s0->launch_unroll > 0 ? s0->launch_unroll : 4;
     for (int c = 0; c < nc && !rc; ++c) {
         fgx_system *s = sys[c];
@@ -1417,6 +1620,7 @@ static int run_chains(std::vector<fgx_system *> &sys, int ncalls, double *du_rec
         FGX_HIP(hipStreamSynchronize(s->stream));
         nthr = std::max(nthr, s->launch_threads > 0 ? s->launch_threads : chain_block(s));
         if (s->has_ewald) dyn = std::max(dyn, at_dyn_lds(s->ew.K));
+        if (!rc && !chains[c].fx) fx = false;
     }
     ChainDev *d_chains = nullptr;
     if (!rc) {
@@ -1456,8 +1660,8 @@ static int run_chains(std::vector<fgx_system *> &sys, int ncalls, double *du_rec
         FGX_HIP(hipEventRecord(e0, s0->stream));
         if (s0->moves[0].type == MOVE_AT) {
             rc = s0->pot.precision == FGX_MIXED
-                     ? launch_at<true>(s0->pot, s0->stream, d_chains, nc, nthr, dyn, ncalls, unroll)
-                     : launch_at<false>(s0->pot, s0->stream, d_chains, nc, nthr, dyn, ncalls, unroll);
+                     ? launch_at<true>(s0->pot, s0->stream, d_chains, nc, nthr, dyn, ncalls, unroll, fx)
+                     : launch_at<false>(s0->pot, s0->stream, d_chains, nc, nthr, dyn, ncalls, unroll, fx);
         } else {
             rc = s0->pot.precision == FGX_MIXED
                      ? launch_tr<true>(s0->pot, s0->stream, d_chains, nc, nthr, ncalls)
@@ -1633,8 +1837,8 @@ extern "C" int fgx_propagate_batch(fgx_system **systems, int nsys, int ncalls, f
 extern "C" int fgx_set_launch(fgx_system *s, int threads, int unroll)
 {
     FGX_CHECK(s && (threads == 0 || threads == 256 || threads == 512) &&
-                  (unroll == 0 || unroll == 4 || unroll == 8 || unroll == 16),
-              FGX_EARG, "fgx_set_launch: threads in {0,256,512}, unroll in {0,4,8,16}");
+                  (unroll == 0 || unroll == 4 || unroll == 8),
+              FGX_EARG, "fgx_set_launch: threads in {0,256,512}, unroll in {0,4,8}");
     s->launch_threads = threads;
     s->launch_unroll = unroll;
     return FGX_OK;
@@ -1664,7 +1868,7 @@ extern "C" int fgx_autotune_batch(fgx_system **systems, int nsys, int ncalls, in
 {
     FGX_CHECK(systems && nsys > 0 && ncalls > 0, FGX_EARG, "fgx_autotune_batch: bad arguments");
     const int cand_t[2] = {256, 512};
-    const int cand_u[3] = {4, 8, 16};
+    const int cand_u[2] = {4, 8};
     double best = 1e300;
     int bt = 0, bu = 0;
     for (int t : cand_t)

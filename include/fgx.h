@@ -153,7 +153,7 @@ int fgx_propagate(fgx_system *s, int ncalls, fgx_stats *st, double *du_out, int 
                   long cap);
 
 /* Launch schedule of the device sweeps: workgroup size (256 or 512 threads
- * per chain) and partner-loop unroll depth (4, 8, 16) of the mixed path; 0
+ * per chain) and partner-loop batch depth (4 or 8) of the mixed path; 0
  * picks the defaults.  No counterpart in the reference (a GPU schedule). */
 int fgx_set_launch(fgx_system *s, int threads, int unroll);
 int fgx_get_launch(fgx_system *s, int *threads, int *unroll);

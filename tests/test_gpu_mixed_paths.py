@@ -1,0 +1,80 @@
+"""Mixed-precision AtomicTranslation sweep: every pair-loop form against the
+FP64 device path on the same RNG stream.
+
+The mixed sweep picks a fixed-unit pair loop for cubic boxes (Debye-Hueckel/
+Coulomb + hard spheres; Ewald + Lennard-Jones with one shared or a per-type
+LJ table) and a generic loop otherwise.  The FP64 device path is pinned to
+the reference by tests/test_gpu_parity.py; here each mixed form must give
+|dU - dU_FP64| <= 1e-5 kT for every trial while the two chains agree, and
+agree on which trials are infinite (hard-sphere overlaps).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL_DU = 1e-5   # kT
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from beofaunus_amd import fgx
+    return fgx
+
+
+def _cases():
+    from beofaunus_amd import workloads as W
+    out = {}
+    out["dh_cubic"] = W.salt_dh(n=300, L=40.0, seed=3)
+    w = W.salt_dh(n=300, L=38.0, seed=4)
+    w["box"] = [40.0, 44.0, 38.0]              # positions lie inside the cube of 38 A
+    out["dh_noncubic"] = w
+    out["ewald_cubic_uniform_lj"] = W.ewald_electrolyte(n=2000, L=55.7, seed=5)
+    w = W.ewald_electrolyte(n=2000, L=55.7, seed=6)
+    w["ham"] = dict(w["ham"], type_sigma=[0.0, 4.0, 3.2], type_eps=[0.0, 0.1, 0.15])
+    out["ewald_cubic_lj_table"] = w
+    w = W.ewald_electrolyte(n=2000, L=53.0, seed=7)
+    w["box"] = [53.0, 56.0, 55.0]
+    out["ewald_noncubic"] = w
+    return out
+
+
+CASE_NAMES = ["dh_cubic", "dh_noncubic", "ewald_cubic_uniform_lj", "ewald_cubic_lj_table",
+              "ewald_noncubic"]
+
+
+@pytest.mark.parametrize("name", CASE_NAMES)
+def test_mixed_forms_match_fp64(gpu, name):
+    from beofaunus_amd import workloads
+    w = _cases()[name]
+    say("[mixed-form] %s n=%d box=%s" % (name, len(w["charge"]), w["box"]))
+    rec = {}
+    for prec in (gpu.FP64, gpu.MIXED):
+        s = workloads.make_system(w, precision=prec)
+        s.rng_seed(29)
+        s.propagator_init()
+        st, du, acc = s.propagate(1, record=len(w["charge"]))
+        rec[prec] = (du, acc)
+        s.close()
+    d64, a64 = rec[gpu.FP64]
+    dmx, amx = rec[gpu.MIXED]
+    n = min(len(d64), len(dmx))
+    mism = np.nonzero(a64[:n] != amx[:n])[0]
+    upto = int(mism[0]) if len(mism) else n
+    f64, fmx = np.isfinite(d64[:upto]), np.isfinite(dmx[:upto])
+    assert np.array_equal(f64, fmx)
+    err = np.abs(dmx[:upto][f64] - d64[:upto][f64])
+    say("  trials %d, compared %d, max |err| %.3e kT, first accept mismatch %s"
+        % (n, upto, err.max(), mism[0] if len(mism) else None))
+    assert upto >= 200
+    assert err.max() <= TOL_DU
