@@ -50,7 +50,17 @@ struct EwaldDev {
     int update_frequency, cnt_accepted;
     double surf, surf_trial, reci, reci_trial, update_drift;
     int *cnt_dev = nullptr;   // device copy of cnt_accepted for the sweeps
+    // Chebyshev series of erfcx(x) = exp(x^2) erfc(x) on [0, cheb_xmax] for
+    // the real-space pairs of the mixed path's FP64 pass (plain Ewald only)
+    double *cheb = nullptr;
+    int ncheb = 0;
+    double cheb_xmax = 0;
 };
+
+static constexpr int EW_CHEB_MAX = 48;
+// records past the end of the partner arrays (packed records, types, FP32
+// radii), zero-filled, so batched partner loads need no clamping
+static constexpr int PARTNER_PAD = 2 * 8 * 512;
 
 // device pointers of one particle vector (FP64 SoA)
 struct Vec64 {
@@ -107,6 +117,17 @@ struct fgx_system {
     size_t device_bytes = 0;
     int launch_threads = 0;                // 0 = default; set by fgx_set_launch / fgx_autotune
     int launch_unroll = 0;
+    // The device copies of the RNG states and the Ewald accept counter are the
+    // current ones after a device sweep (state_on_dev); host-side users pull
+    // them first (pull_device_state)
+    bool state_on_dev = false;
+    int *d_gvec = nullptr;                 // move's group list (device sweeps)
+    int gvec_n = 0;
+    // per-batch launch buffers owned by the batch's first system
+    void *d_batch = nullptr;               // ChainDev array
+    int batch_cap = 0;
+    long *d_bstat_i = nullptr;             // 8 per chain
+    double *d_bstat_d = nullptr;
     // group-pair energy cache of the mixed TranslateRotate sweep (ng x ng)
     double *d_emat = nullptr;
     bool emat_on = true;                   // fgx_set_energy_matrix
@@ -119,4 +140,5 @@ int ensure_partials(fgx_system *s, int nblocks);
 int ewald_full_recompute(fgx_system *s, int which, double *Qdst);
 int ewald_reciprocal(fgx_system *s, const double *Q, double V, double *out_host);
 void mt_seed_host(uint32_t *state625, uint32_t seed);
+int pull_device_state(fgx_system *s);
 }  // namespace fgx

@@ -76,6 +76,10 @@ struct ChainDev {
     // LJ parameters when every type pair present shares them (fixed units)
     int lj_uniform, lj_check;
     float lj_s2f, lj_eps4f;
+    // erfcx Chebyshev series for the mixed path's FP64 pass (plain Ewald)
+    const double *ew_cheb;
+    int ew_ncheb;
+    double ew_cheb_t;             // 2 / xmax
 };
 
 // ------------------------------------------------------------------------
@@ -137,6 +141,55 @@ __device__ __forceinline__ float pair_fx(const ChainDev &C, float r2, float aq, 
     return u;
 }
 
+// erfc(x) for x in [0, xmax] from the erfcx series (Clenshaw)
+__device__ __forceinline__ double erfc_series(const double *c, int n, double tscale, double x)
+{
+    const double t = fma(x, tscale, -1.0);
+    const double t2 = t + t;
+    double b1 = 0.0, b2 = 0.0;
+    for (int k = n - 1; k >= 1; --k) {
+        const double b0 = fma(t2, b1, c[k] - b2);
+        b2 = b1;
+        b1 = b0;
+    }
+    return exp(-x * x) * fma(t, b1, c[0] - b2);
+}
+
+// u(new, j) - u(old, j) in FP64 for the mixed path's exact pass.  As
+// du_pair64, except that a plain real-space Ewald term (kappa -> 0, where the
+// screened form of ewald.h:109 reduces to erfc(alpha r) / r) takes erfc from
+// the staged series instead of the library call.
+template <int ELEC, int SR>
+__device__ __forceinline__ double du_pair64_mixed(const ChainDev &C, const double *__restrict__ tab,
+                                                  const double *cheb, double nx, double ny,
+                                                  double nz, double ox, double oy, double oz,
+                                                  double aq, double ar, int at, int j)
+{
+    if constexpr (ELEC == ELEC_EWALD) {
+        if (C.ew_ncheb > 0) {
+            const double bx = C.x[j], by = C.y[j], bz = C.z[j], bq = C.q[j], br = C.r[j];
+            const int bt = C.t[j];
+            double rx, ry, rz;
+            vdist_d(C.box, nx, ny, nz, bx, by, bz, rx, ry, rz);
+            const double r2n = rx * rx + (ry * ry + rz * rz);
+            vdist_d(C.box, ox, oy, oz, bx, by, bz, rx, ry, rz);
+            const double r2o = rx * rx + (ry * ry + rz * rz);
+            const int k = at * C.pot.ntypes + bt;
+            const double qq = aq * bq * tab[TAB_QEQE * C.pot.ntypes2 + k] * C.pot.lB;
+            auto u = [&](double r2) {
+                double e = 0.0;
+                if (r2 <= C.pot.ew_rc2) {
+                    const double r = sqrt(r2);
+                    e = qq * erfc_series(cheb, C.ew_ncheb, C.ew_cheb_t, C.pot.ew_alpha * r) / r;
+                }
+                return e + pair_u64_t<ELEC_NONE, SR>(C.pot, tab, r2, aq, bq, ar, br, at, bt);
+            };
+            return u(r2n) - u(r2o);
+        }
+    }
+    return du_pair64<ELEC, SR>(C, tab, nx, ny, nz, ox, oy, oz, aq, ar, at, j);
+}
+
 static constexpr int DCAP = 2048;   // deferred-pair queue capacity (LDS), all wavefronts
 static constexpr int LDS_TYPES2 = 256;  // type tables staged in LDS up to 16 types
 
@@ -170,15 +223,16 @@ struct PartnerBatch {
     int4 P[U];
     float R[U];
     int T[U];
+    // rows base + u nthr; loads past n read the zero padding (at most two
+    // batches of 8 x 512 records beyond the end)
     __device__ __forceinline__ void load(const ChainDev &C, int base, int nthr, int tid)
     {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int j = base + u * nthr + tid;
-            const int jc = j < C.n ? j : C.n - 1;
-            P[u] = gload(C.pk, jc);
-            R[u] = HS ? gload(C.rf, jc) : 0.f;
-            T[u] = NEED_T ? gload(C.t, jc) : 0;
+            const int row = base + u * nthr;   // wavefront-uniform
+            P[u] = gload(C.pk + row, tid);
+            R[u] = HS ? gload(C.rf + row, tid) : 0.f;
+            T[u] = NEED_T ? gload(C.t + row, tid) : 0;
         }
     }
 };
@@ -194,22 +248,24 @@ __host__ __device__ constexpr bool at_fx_ok()
 
 template <int ELEC, int SR, int U, bool FX>
 __device__ __forceinline__ double pair_loop_mixed(const ChainDev &C, const float *__restrict__ tabf,
-                                                  const double *__restrict__ tab, int i, double tnx,
-                                                  double tny, double tnz, double tox, double toy,
-                                                  double toz, int *dq, int qcap, int &nq)
+                                                  const double *__restrict__ tab, int i, int4 me,
+                                                  float ar, int at, double aq64, double ar64,
+                                                  double tnx, double tny, double tnz, double tox,
+                                                  double toy, double toz, int *dq, int qcap,
+                                                  int &nq)
 {
     const int tid = threadIdx.x, nthr = blockDim.x, lane = tid & 63;
     const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    const int4 me = gload(C.pk, i);
     const float aq = __int_as_float(me.w);
-    const float ar = (SR == SR_HS) ? gload(C.rf, i) : 0.f;
-    const int at = gload(C.t, i);
     const int32_t nx = to_fixed(tnx, C.box.inv[0]), ny = to_fixed(tny, C.box.inv[1]),
                   nz = to_fixed(tnz, C.box.inv[2]);
     const float sx = C.box.fscale[0], sy = C.box.fscale[1], sz = C.box.fscale[2];
     const float exact = C.pot.exact_f;
     constexpr bool NEED_T = (SR == SR_LJ || ELEC == ELEC_EWALD);
     // fixed-unit form: charge lB q / s, radius r c / s (see stage_site)
+    // Ewald partners inside rc (with a margin above FP32 rounding) go to the
+    // FP64 pass, which applies the exact cut
+    const float rc2m = C.pot.ew_rc2f * 1.00001f;
     const float aqx = FX ? aq * C.fx_lbs : aq;
     const float arx = FX ? ar * C.fx_rs : ar;
     int cnt = 0;   // wavefront-uniform
@@ -285,7 +341,7 @@ __device__ __forceinline__ double pair_loop_mixed(const ChainDev &C, const float
                 // beyond rc the real-space term is zero; inside, FP64 decides
                 un = pair_u32_t<ELEC_NONE, SR>(C.pot, tabf, r2n, aq, bq, ar, X.R[u], at, X.T[u]);
                 uo = pair_u32_t<ELEC_NONE, SR>(C.pot, tabf, r2o, aq, bq, ar, X.R[u], at, X.T[u]);
-                bad = (r2n <= C.pot.ew_rc2f) || (r2o <= C.pot.ew_rc2f);
+                bad = (r2n <= rc2m) || (r2o <= rc2m);
             } else {
                 un = pair_u32_t<ELEC, SR>(C.pot, tabf, r2n, aq, bq, ar, X.R[u], at, X.T[u]);
                 uo = pair_u32_t<ELEC, SR>(C.pot, tabf, r2o, aq, bq, ar, X.R[u], at, X.T[u]);
@@ -308,7 +364,7 @@ __device__ __forceinline__ double pair_loop_mixed(const ChainDev &C, const float
                             dq[k] = j;
                         else
                             acc += du_pair64<ELEC, SR>(C, tab, tnx, tny, tnz, tox, toy, toz,
-                                                       C.q[i], C.r[i], at, j);
+                                                       aq64, ar64, at, j);
                     }
                     cnt += __popcll(m);
                 }
@@ -366,13 +422,25 @@ __device__ __forceinline__ void ew_phase(const double2 *tab, int kcc1, int nx, i
 // ------------------------------------------------------------------------
 // AtomicTranslation sweep
 // ------------------------------------------------------------------------
+// One AtomicTranslation trial: its random numbers, the moved particle's data
+// and the proposed position.  `pf` marks a record whose draws and particle
+// data were fetched ahead, during the previous trial of the same call.
 struct TrialAT {
     int i;
     int go;
     int acc;
     int collide;
+    int pf;
+    int ti;                       // type of i
+    int4 pk;                      // packed record of i (old fixed-point position)
+    float rf;                     // FP32 radius of i
+    double q64, r64, wq;          // FP64 charge, radius, Ewald weight q eff
+    double t3[3], u;              // displacement canonicals, Metropolis uniform
     double ox, oy, oz, nx, ny, nz;
 };
+
+static constexpr int AT_LDS_GROUPS = 256;   // group table staged in LDS up to this size
+static constexpr int AT_LDS_TYPES = 64;
 
 static constexpr int AT_THREADS = 512;
 static constexpr int REFRESH_CHUNK = 16;   // particles per LDS table chunk
@@ -412,6 +480,15 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
     __shared__ double ltab[TAB_COUNT * LDS_TYPES2];
     __shared__ float ltabf[MIX ? TAB_COUNT * LDS_TYPES2 : 1];
     // Ewald state for the whole launch, LDS-resident: S(k) FP64, A_k, n-triples
+    // move's group list, group table and per-type dp for the trial draw
+    __shared__ int lgvec[AT_LDS_GROUPS];
+    __shared__ GroupDesc lgroups[AT_LDS_GROUPS];
+    __shared__ double ltdp[AT_LDS_TYPES], leff[AT_LDS_TYPES];
+    __shared__ int nqs[AT_THREADS / 64];          // FP64 queue length per wavefront
+    // chain scalars kept once in LDS (written by thread 0 only): surface
+    // dipole M, utot, dusum
+    __shared__ double sst[5];
+    __shared__ double lcheb[EW && MIX ? EW_CHEB_MAX : 1];
     extern __shared__ double2 dyn[];
     double2 *S = dyn;
     double2 *dS = S + (EW ? C.K : 0);
@@ -423,7 +500,48 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
         mti[k] = C.rng[625 + k];
     }
     WaveMT G{mtg, (int)C.rng[624]}, I{mti, (int)C.rng[625 + 624]};
+    const bool lds_grp = C.ngvec <= AT_LDS_GROUPS && C.ng <= AT_LDS_GROUPS &&
+                         C.pot.ntypes <= AT_LDS_TYPES;
+    if (lds_grp) {
+        for (int k = tid; k < C.ngvec; k += nthr) lgvec[k] = C.gvec[k];
+        for (int k = tid; k < C.ng; k += nthr) lgroups[k] = C.groups[k];
+        for (int k = tid; k < C.pot.ntypes; k += nthr) {
+            ltdp[k] = C.type_dp[k];
+            leff[k] = EW ? C.eff[k] : 0.0;
+        }
+    }
+    if constexpr (EW && MIX)
+        for (int k = tid; k < C.ew_ncheb; k += nthr) lcheb[k] = C.ew_cheb[k];
+    if (tid == 0) {
+        T[0].pf = T[1].pf = 0;
+        T[0].acc = T[1].acc = 0;
+    }
     const bool lds_tab = C.pot.ntypes2 <= LDS_TYPES2;
+    // one trial's draws from the global stream, in the reference's order:
+    // group, particle (move.h:825, group.h:132), three displacement
+    // canonicals (move.h:843-845), Metropolis uniform (move.h:570).  Called
+    // by all 64 lanes of wave 0.
+    auto at_draw = [&](TrialAT &t) {
+        const int gk = mt_range(G, 0, C.ngvec - 1);
+        const int g = lds_grp ? lgvec[gk] : C.gvec[gk];
+        const GroupDesc gd = lds_grp ? lgroups[g] : C.groups[g];
+        t.i = gd.front + mt_range(G, 0, gd.back - gd.front);
+        t.t3[0] = mt_canonical(G);
+        t.t3[1] = mt_canonical(G);
+        t.t3[2] = mt_canonical(G);
+        t.u = mt_canonical(G);
+    };
+    // the moved particle's data, loads issued together
+    auto at_fetch = [&](TrialAT &t) {
+        const int i = t.i;
+        t.ti = gload(C.t, i);
+        t.ox = gload(C.x, i); t.oy = gload(C.y, i); t.oz = gload(C.z, i);
+        t.pk = gload(C.pk, i);
+        t.rf = gload(C.rf, i);
+        t.q64 = gload(C.q, i);
+        t.r64 = gload(C.r, i);
+        t.wq = EW ? t.q64 * (lds_grp ? leff[t.ti] : C.eff[t.ti]) : 0.0;
+    };
     if (lds_tab) {
         for (int k = tid; k < TAB_COUNT * C.pot.ntypes2; k += nthr) {
             int t = k / C.pot.ntypes2, e = k % C.pot.ntypes2;
@@ -443,19 +561,21 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
         }
     }
     // surface dipole M = sum q eff r (getSurfaceEnergy, ewald.h:278-292)
-    double Mx = 0, My = 0, Mz = 0;
     int ew_cnt = EW ? *C.ew_cnt : 0;
     long n_trials = 0, n_acc = 0, n_inf = 0, n_ref = 0;
-    double utot = 0.0, dusum = 0.0;
+    if (tid == 0)
+        for (int q = 0; q < 5; ++q) sst[q] = 0.0;
     long rec = 0;
     if (EW && C.surf_pref != 0.0) {
+        double mx = 0, my = 0, mz = 0;
         for (int j = tid; j < C.n; j += nthr) {
             double w = C.q[j] * C.eff[C.t[j]];
-            Mx += w * C.x[j]; My += w * C.y[j]; Mz += w * C.z[j];
+            mx += w * C.x[j]; my += w * C.y[j]; mz += w * C.z[j];
         }
-        Mx = block_sum(Mx, red);
-        My = block_sum(My, red);
-        Mz = block_sum(Mz, red);
+        mx = block_sum(mx, red);
+        my = block_sum(my, red);
+        mz = block_sum(mz, red);
+        if (tid == 0) { sst[0] = mx; sst[1] = my; sst[2] = mz; }
     }
     __syncthreads();
 
@@ -472,32 +592,52 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
         if (!T[par].go) { par ^= 1; continue; }
         for (int rep = 0; rep < C.repeat; ++rep) {
             TrialAT &tr = T[par];
+            TrialAT &tn = T[par ^ 1];   // previous trial's record; next trial's prefetch
             // ---- AtomicTranslation::_trialMove (move.h:818-864), wave 0
             if (wave0) {
-                int g = C.gvec[mt_range(G, 0, C.ngvec - 1)];
-                GroupDesc gd = C.groups[g];
-                int i = gd.front + mt_range(G, 0, gd.back - gd.front);
-                double dp = C.type_dp[C.t[i]];
-                double ox = C.x[i], oy = C.y[i], oz = C.z[i];
+                TrialAT c;
+                const bool pf = tr.pf;
+                if (pf) {
+                    c = tr;
+                } else {
+                    at_draw(c);
+                    at_fetch(c);
+                }
+                // a prefetched position is stale if the previous trial moved i
+                double ox = c.ox, oy = c.oy, oz = c.oz;
+                int4 pk = c.pk;
+                if (pf && tn.acc && tn.i == c.i) {
+                    ox = tn.nx; oy = tn.ny; oz = tn.nz;
+                    pk = make_int4(to_fixed(ox, C.box.inv[0]), to_fixed(oy, C.box.inv[1]),
+                                   to_fixed(oz, C.box.inv[2]), pk.w);
+                }
+                const double dp = lds_grp ? ltdp[c.ti] : C.type_dp[c.ti];
                 double nx, ny, nz;
                 {
 #pragma clang fp contract(off)
                     double tx = C.mv.dir[0] * dp, ty = C.mv.dir[1] * dp, tz = C.mv.dir[2] * dp;
-                    tx *= mt_canonical(G) - 0.5;
-                    ty *= mt_canonical(G) - 0.5;
-                    tz *= mt_canonical(G) - 0.5;
+                    tx *= c.t3[0] - 0.5;
+                    ty *= c.t3[1] - 0.5;
+                    tz *= c.t3[2] - 0.5;
                     nx = ox + tx; ny = oy + ty; nz = oz + tz;
                     boundary_d(C.box, nx, ny, nz);
                 }
                 if (tid == 0) {
-                    tr.i = i;
+                    if (!pf) {
+                        tr.i = c.i; tr.ti = c.ti; tr.rf = c.rf;
+                        tr.q64 = c.q64; tr.r64 = c.r64; tr.wq = c.wq; tr.u = c.u;
+                    }
+                    tr.pf = 0;
+                    tr.pk = pk;
                     tr.ox = ox; tr.oy = oy; tr.oz = oz;
                     tr.nx = nx; tr.ny = ny; tr.nz = nz;
                     tr.collide = collision_d(C.box, nx, ny, nz) ? 1 : 0;
+                    tn.pf = 0;
                 }
             }
             __syncthreads();
             const int i = tr.i;
+            const bool ahead = rep + 1 < C.repeat;   // next trial in this call
             double acc = 0.0;
             // ---- Ewald phase tables exp(i 2 pi n x / L) for old and new position
             if constexpr (EW) {
@@ -513,14 +653,11 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
                 }
             }
             // ---- pair part: sum_{j != i} u(i', j) - u(i, j)
-            int nq = 0;            // this wavefront's FP64 queue (mixed path)
-            int *dq = dlist;
-            int pj = -1;           // this lane's first queued partner, data preloaded
-            double pbx = 0, pby = 0, pbz = 0, pbq = 0, pbr = 0;
-            int pbt = 0;
+            const int nw = nthr >> 6, wv = tid >> 6;
+            const int qcap = DCAP / nw;
             if constexpr (!MIX) {
-                const double aq = C.q[i], ar = C.r[i];
-                const int at = C.t[i];
+                const double aq = tr.q64, ar = tr.r64;
+                const int at = tr.ti;
                 const double nx = tr.nx, ny = tr.ny, nz = tr.nz, ox = tr.ox, oy = tr.oy,
                              oz = tr.oz;
                 for (int j = tid; j < C.n; j += nthr) {
@@ -528,32 +665,78 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
                     acc += du_pair64<ELEC, SR>(C, tab, nx, ny, nz, ox, oy, oz, aq, ar, at, j);
                 }
             } else {
-                const int qcap = DCAP / (nthr >> 6);
-                dq = dlist + (tid >> 6) * qcap;
+                int nq = 0;
+                int *dq = dlist + wv * qcap;
                 // (the fixed-unit form is only launched with LDS tables)
                 if (FX || lds_tab)
-                    acc = pair_loop_mixed<ELEC, SR, U, FX>(C, ltabf, ltab, i, tr.nx, tr.ny, tr.nz,
+                    acc = pair_loop_mixed<ELEC, SR, U, FX>(C, ltabf, ltab, i, tr.pk, tr.rf, tr.ti,
+                                                           tr.q64, tr.r64, tr.nx, tr.ny, tr.nz,
                                                            tr.ox, tr.oy, tr.oz, dq, qcap, nq);
                 else
-                    acc = pair_loop_mixed<ELEC, SR, U, false>(C, C.tabf, C.tab, i, tr.nx, tr.ny,
+                    acc = pair_loop_mixed<ELEC, SR, U, false>(C, C.tabf, C.tab, i, tr.pk, tr.rf,
+                                                              tr.ti, tr.q64, tr.r64, tr.nx, tr.ny,
                                                               tr.nz, tr.ox, tr.oy, tr.oz, dq, qcap,
                                                               nq);
-                wave_sync_lds();
-                // loads for the FP64 pass are issued before the k-space work
-                // so their latency overlaps it
-                const int lane = tid & 63;
-                if (lane < nq) {
-                    pj = dq[lane];
-                    pbx = C.x[pj]; pby = C.y[pj]; pbz = C.z[pj];
-                    pbq = C.q[pj]; pbr = C.r[pj]; pbt = C.t[pj];
+                if ((tid & 63) == 0) nqs[wv] = nq;
+            }
+            if (EW || MIX) __syncthreads();   // etab and FP64 queues complete
+            // ---- draw-ahead: the next trial's random numbers and particle
+            // data (wave 0, while the other wavefronts start the reciprocal
+            // sum and the FP64 pass; the reference's draw order is unchanged)
+            if (wave0 && ahead) {
+                TrialAT nxt;
+                at_draw(nxt);
+                at_fetch(nxt);
+                if (tid == 0) {
+                    tn.pf = 1;
+                    tn.i = nxt.i; tn.ti = nxt.ti; tn.pk = nxt.pk; tn.rf = nxt.rf;
+                    tn.q64 = nxt.q64; tn.r64 = nxt.r64; tn.wq = nxt.wq;
+                    tn.t3[0] = nxt.t3[0]; tn.t3[1] = nxt.t3[1]; tn.t3[2] = nxt.t3[2];
+                    tn.u = nxt.u;
+                    tn.ox = nxt.ox; tn.oy = nxt.oy; tn.oz = nxt.oz;
                 }
             }
+            // FP64 work of the mixed path goes to the last F wavefronts, the
+            // reciprocal-space sum to the others, so the two run side by side
+            int Qn = 0, F = 0;
+            if constexpr (MIX) {
+                for (int w = 0; w < nw; ++w) Qn += nqs[w];
+                F = min((Qn + 63) >> 6, EW ? nw / 2 : nw);
+            }
+            const int nk = 64 * (nw - F);    // threads on the reciprocal sum
             // ---- reciprocal part: A_k (|S+dS|^2 - |S|^2) per k, FP64
-            const double wq = EW ? C.q[i] * C.eff[C.t[i]] : 0.0;
+            const double wq = tr.wq;
             if constexpr (EW) {
-                __syncthreads();   // etab complete
                 double de = 0.0;
-                for (int k = tid; k < C.K; k += nthr) {
+                // four k per pass, their LDS reads issued before any arithmetic
+                constexpr int KB = 4;
+                int k0 = tid;
+                for (; tid < nk && k0 + (KB - 1) * nk < C.K; k0 += KB * nk) {
+                    int kv[KB];
+                    double2 sk[KB];
+                    double ak[KB];
+#pragma unroll
+                    for (int b = 0; b < KB; ++b) {
+                        kv[b] = kid[k0 + b * nk];
+                        sk[b] = S[k0 + b * nk];
+                        ak[b] = Ak[k0 + b * nk];
+                    }
+                    double2 d[KB];
+#pragma unroll
+                    for (int b = 0; b < KB; ++b) {
+                        int nx, ny, nz;
+                        kunpack(kv[b], nx, ny, nz);
+                        double er, ei, orr, oi;
+                        ew_phase(etab[1], kcc1, nx, ny, nz, er, ei);
+                        ew_phase(etab[0], kcc1, nx, ny, nz, orr, oi);
+                        const double dr = wq * (er - orr), di = wq * (ei - oi);
+                        d[b] = make_double2(dr, di);
+                        de += ak[b] * (2.0 * (sk[b].x * dr + sk[b].y * di) + (dr * dr + di * di));
+                    }
+#pragma unroll
+                    for (int b = 0; b < KB; ++b) dS[k0 + b * nk] = d[b];
+                }
+                for (int k = k0; tid < nk && k < C.K; k += nk) {
                     int nx, ny, nz;
                     kunpack(kid[k], nx, ny, nz);
                     double er, ei, orr, oi;
@@ -566,31 +749,29 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
                 }
                 acc += C.recip_pref * de;
             }
-            // ---- FP64 pass of the mixed path over this wavefront's queue
+            // ---- FP64 pass of the mixed path over all queued partners
             if constexpr (MIX) {
-                const double aq64 = C.q[i], ar64 = C.r[i];
-                const int at = C.t[i];
-                if (pj >= 0)
-                    acc += du_pair64_v<ELEC, SR>(C, tab, tr.nx, tr.ny, tr.nz, tr.ox, tr.oy, tr.oz,
-                                                 aq64, ar64, at, pbx, pby, pbz, pbq, pbr, pbt);
-                for (int k = 64 + (tid & 63); k < nq; k += 64)
-                    acc += du_pair64<ELEC, SR>(C, tab, tr.nx, tr.ny, tr.nz, tr.ox, tr.oy, tr.oz,
-                                               aq64, ar64, at, dq[k]);
+                for (int e = tid - nk; e >= 0 && e < Qn; e += 64 * F) {
+                    int w = 0, k = e;
+                    while (k >= nqs[w]) { k -= nqs[w]; ++w; }
+                    const int j = dlist[w * qcap + k];
+                    acc += du_pair64_mixed<ELEC, SR>(C, tab, lcheb, tr.nx, tr.ny, tr.nz, tr.ox,
+                                                     tr.oy, tr.oz, tr.q64, tr.r64, tr.ti, j);
+                }
             }
             double du = block_sum(acc, red);
             if (EW && C.surf_pref != 0.0) {
+                const double Mx = sst[0], My = sst[1], Mz = sst[2];
                 double dMx = wq * (tr.nx - tr.ox), dMy = wq * (tr.ny - tr.oy), dMz = wq * (tr.nz - tr.oz);
                 double Nx = Mx + dMx, Ny = My + dMy, Nz = Mz + dMz;
                 du += C.surf_pref * ((Nx * Nx + (Ny * Ny + Nz * Nz)) - (Mx * Mx + (My * My + Mz * Mz)));
             }
             if (tr.collide) du = __builtin_inf();   // energy.h:3597-3607
-            // ---- Movebase::metropolis (move.h:567-573), wave 0
-            if (wave0) {
-                bool accept = !(mt_canonical(G) > exp(-du));
-                if (tid == 0) tr.acc = accept ? 1 : 0;
-            }
-            __syncthreads();
-            const bool accept = tr.acc;
+            // ---- Movebase::metropolis (move.h:567-573): the uniform was drawn
+            // with the trial (it is the next number of the same stream); every
+            // thread takes the same decision from the same du
+            const bool accept = !(tr.u > exp(-du));
+            if (tid == 0) tr.acc = accept ? 1 : 0;
             n_trials++;
             if (tid == 0 && C.du_rec && rec < C.cap) {
                 C.du_rec[rec] = du;
@@ -599,16 +780,19 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
             rec++;
             if (accept) {
                 n_acc++;
-                dusum += du;
-                utot += du;
+                if (tid == 0) { sst[4] += du; sst[3] += du; }
                 if (tid == 0) {   // AtomicTranslation::_acceptMove (move.h:866-877)
                     C.x[i] = tr.nx; C.y[i] = tr.ny; C.z[i] = tr.nz;
                     C.pk_w[i] = make_int4(to_fixed(tr.nx, C.box.inv[0]), to_fixed(tr.ny, C.box.inv[1]),
-                                          to_fixed(tr.nz, C.box.inv[2]), C.pk[i].w);
+                                          to_fixed(tr.nz, C.box.inv[2]), tr.pk.w);
                 }
                 if constexpr (EW) {
-                    Mx += wq * (tr.nx - tr.ox); My += wq * (tr.ny - tr.oy); Mz += wq * (tr.nz - tr.oz);
-                    // S(k) += dS(k): same k ownership as above, so no race
+                    if (tid == 0) {
+                        sst[0] += wq * (tr.nx - tr.ox);
+                        sst[1] += wq * (tr.ny - tr.oy);
+                        sst[2] += wq * (tr.nz - tr.oz);
+                    }
+                    // S(k) += dS(k) (dS complete: block_sum synchronised)
                     for (int k = tid; k < C.K; k += nthr) {
                         const double2 sk = S[k], d = dS[k];
                         S[k] = make_double2(sk.x + d.x, sk.y + d.y);
@@ -674,11 +858,12 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
                                 double w2 = C.q[j] * C.eff[C.t[j]];
                                 mx += w2 * C.x[j]; my += w2 * C.y[j]; mz += w2 * C.z[j];
                             }
-                            Mx = block_sum(mx, red);
-                            My = block_sum(my, red);
-                            Mz = block_sum(mz, red);
+                            mx = block_sum(mx, red);
+                            my = block_sum(my, red);
+                            mz = block_sum(mz, red);
+                            if (tid == 0) { sst[0] = mx; sst[1] = my; sst[2] = mz; }
                         }
-                        utot += C.recip_pref * (ea - eb);
+                        if (tid == 0) sst[3] += C.recip_pref * (ea - eb);
                         ew_cnt = 0;
                         n_ref++;
                     }
@@ -698,12 +883,12 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
     if (tid == 0) {
         C.rng[624] = (uint32_t)G.p;
         C.rng[625 + 624] = (uint32_t)I.p;
-        C.stats_i[0] += n_trials;
-        C.stats_i[1] += n_acc;
-        C.stats_i[2] += n_inf;
-        C.stats_i[3] += n_ref;
-        C.stats_d[0] += utot;
-        C.stats_d[1] += dusum;
+        C.stats_i[0] = n_trials;
+        C.stats_i[1] = n_acc;
+        C.stats_i[2] = n_inf;
+        C.stats_i[3] = n_ref;
+        C.stats_d[0] = sst[3];
+        C.stats_d[1] = sst[4];
         if (EW) *C.ew_cnt = ew_cnt;
     }
     if constexpr (EW) {
@@ -1291,11 +1476,12 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_tr(const ChainDev *__restr
     if (tid == 0) {
         C.rng[624] = (uint32_t)G.p;
         C.rng[625 + 624] = (uint32_t)I.p;
-        C.stats_i[0] += n_trials;
-        C.stats_i[1] += n_acc;
-        C.stats_i[2] += n_inf;
-        C.stats_d[0] += utot;
-        C.stats_d[1] += dusum;
+        C.stats_i[0] = n_trials;
+        C.stats_i[1] = n_acc;
+        C.stats_i[2] = n_inf;
+        C.stats_i[3] = 0;
+        C.stats_d[0] = utot;
+        C.stats_d[1] = dusum;
     }
 }
 
@@ -1314,7 +1500,7 @@ struct Prepared {
     int *d_gvec = nullptr;
 };
 
-static int prepare_chain(fgx_system *s, ChainDev &C, int **d_gvec, double *du_rec, int *acc_rec,
+static int prepare_chain(fgx_system *s, ChainDev &C, hipStream_t st, double *du_rec, int *acc_rec,
                          long cap)
 {
     FGX_CHECK(s->moves.size() == 1, FGX_EUNSUPPORTED,
@@ -1342,9 +1528,14 @@ static int prepare_chain(fgx_system *s, ChainDev &C, int **d_gvec, double *du_re
             FGX_CHECK(s->groups[g].back - s->groups[g].front + 1 <= MAX_GROUP_SITES,
                       FGX_EUNSUPPORTED, "device TranslateRotate: group larger than LDS stage");
     }
-    FGX_HIP(hipMalloc(d_gvec, sizeof(int) * gvec.size()));
-    FGX_HIP(hipMemcpyAsync(*d_gvec, gvec.data(), sizeof(int) * gvec.size(), hipMemcpyHostToDevice,
-                           s->stream));
+    if (!s->d_gvec || s->gvec_n != (int)gvec.size()) {
+        if (s->d_gvec) (void)hipFree(s->d_gvec);
+        s->d_gvec = nullptr;
+        FGX_HIP(hipMalloc(&s->d_gvec, sizeof(int) * gvec.size()));
+        FGX_HIP(hipMemcpyAsync(s->d_gvec, gvec.data(), sizeof(int) * gvec.size(),
+                               hipMemcpyHostToDevice, st));
+        s->gvec_n = (int)gvec.size();
+    }
     C.box = s->box;
     C.pot = s->pot;
     C.n = s->n;
@@ -1361,7 +1552,7 @@ static int prepare_chain(fgx_system *s, ChainDev &C, int **d_gvec, double *du_re
     C.mv = mv;
     C.nmoves = (int)s->moves.size();
     C.repeat = repeat;
-    C.gvec = *d_gvec;
+    C.gvec = s->d_gvec;
     C.ngvec = (int)gvec.size();
     C.rng = s->d_rng;
     if (s->has_ewald) {
@@ -1378,7 +1569,13 @@ static int prepare_chain(fgx_system *s, ChainDev &C, int **d_gvec, double *du_re
                           ? s->ew.const_inf * (2 * pi / ((2 * s->ew.eps_surf + 1) * s->ew.V)) * s->pot.lB
                           : 0.0;
         C.update_frequency = s->ew.update_frequency;
+        C.ew_cheb = s->ew.cheb;
+        C.ew_ncheb = s->ew.ncheb;
+        C.ew_cheb_t = s->ew.ncheb ? 2.0 / s->ew.cheb_xmax : 0.0;
     } else {
+        C.ew_cheb = nullptr;
+        C.ew_ncheb = 0;
+        C.ew_cheb_t = 0.0;
         C.K = 0;
         C.kcc = 0;
     }
@@ -1416,7 +1613,7 @@ static int prepare_chain(fgx_system *s, ChainDev &C, int **d_gvec, double *du_re
         C.fx_rs = (float)(std::sqrt(1.000001) / unit);
         C.fx_c1 = (float)(-s->pot.kappa * unit / std::log(2.0));
         if (ew_lj) {
-            C.fx_rc2 = (float)(s->pot.ew_rc2 / (unit * unit));
+            C.fx_rc2 = (float)(s->pot.ew_rc2 / (unit * unit) * 1.00001);   // margin: see pair loop
             C.fx_s2scale = (float)(1.0 / (unit * unit));
             // LJ parameters over the type pairs present; and whether any LJ
             // pair beyond rc can be strong (|x^6 - x^3| <= y + y^2 with
@@ -1578,18 +1775,51 @@ static int launch_tr(const Pot &P, hipStream_t st, const ChainDev *d_chains, int
     return FGX_OK;
 }
 
+// the device RNG states and Ewald accept counter become the host's again
+int pull_device_state(fgx_system *s)
+{
+    if (!s->state_on_dev) return FGX_OK;
+    FGX_HIP(hipMemcpyAsync(s->rng_global, s->d_rng, sizeof(uint32_t) * 625, hipMemcpyDeviceToHost,
+                           s->stream));
+    FGX_HIP(hipMemcpyAsync(s->rng_internal, s->d_rng + 625, sizeof(uint32_t) * 625,
+                           hipMemcpyDeviceToHost, s->stream));
+    if (s->has_ewald && s->ew.cnt_dev)
+        FGX_HIP(hipMemcpyAsync(&s->ew.cnt_accepted, s->ew.cnt_dev, sizeof(int),
+                               hipMemcpyDeviceToHost, s->stream));
+    FGX_HIP(hipStreamSynchronize(s->stream));
+    s->state_on_dev = false;
+    return FGX_OK;
+}
+
+// One launch advancing every chain by `ncalls` Propagator::move() calls.  All
+// transfers go on the first system's stream with a single synchronisation at
+// the end; RNG states and counters stay on the device between launches.
+// Per-chain statistics land in `stats` (8 longs + 8 doubles per chain).
 static int run_chains(std::vector<fgx_system *> &sys, int ncalls, double *du_rec, int *acc_rec,
-                      long cap, double *seconds)
+                      long cap, double *seconds, std::vector<long> *stats_i,
+                      std::vector<double> *stats_d)
 {
     fgx_system *s0 = sys[0];
     const int nc = (int)sys.size();
+    hipStream_t st = s0->stream;
     std::vector<ChainDev> chains(nc);
-    std::vector<int *> gvecs(nc, nullptr);
     int rc = FGX_OK;
     int nthr = 0;
     size_t dyn = 0;
     bool fx = true;   // fixed-unit form only when every chain has it
     const int unroll = s0->launch_unroll > 0 ? s0->launch_unroll : 4;
+    // batch buffers, kept by the first system
+    if (s0->batch_cap < nc) {
+        if (s0->d_batch) (void)hipFree(s0->d_batch);
+        if (s0->d_bstat_i) (void)hipFree(s0->d_bstat_i);
+        if (s0->d_bstat_d) (void)hipFree(s0->d_bstat_d);
+        s0->d_batch = nullptr; s0->d_bstat_i = nullptr; s0->d_bstat_d = nullptr;
+        s0->batch_cap = 0;
+        FGX_HIP(hipMalloc(&s0->d_batch, sizeof(ChainDev) * nc));
+        FGX_HIP(hipMalloc(&s0->d_bstat_i, sizeof(long) * 8 * nc));
+        FGX_HIP(hipMalloc(&s0->d_bstat_d, sizeof(double) * 8 * nc));
+        s0->batch_cap = nc;
+    }
     for (int c = 0; c < nc && !rc; ++c) {
         fgx_system *s = sys[c];
         FGX_CHECK(s->moves.size() == s0->moves.size() && !s->moves.empty() &&
@@ -1604,25 +1834,30 @@ static int run_chains(std::vector<fgx_system *> &sys, int ncalls, double *du_rec
         s->propagator_started = true;
         if (s->move_cnt.empty()) s->move_cnt.assign(s->moves.size(), 0);
         if (s->move_cnt[0] == 0 && (rc = init_mass_centres(s))) break;
-        FGX_HIP(hipMemcpyAsync(s->d_rng, s->rng_global, sizeof(uint32_t) * 625, hipMemcpyHostToDevice, s->stream));
-        FGX_HIP(hipMemcpyAsync(s->d_rng + 625, s->rng_internal, sizeof(uint32_t) * 625, hipMemcpyHostToDevice, s->stream));
-        FGX_HIP(hipMemsetAsync(s->d_stats_i, 0, sizeof(long) * 16, s->stream));
-        FGX_HIP(hipMemsetAsync(s->d_stats_d, 0, sizeof(double) * 16, s->stream));
-        if (s->has_ewald) {
-            if (!s->ew.cnt_dev) {
-                FGX_HIP(hipMalloc(&s->ew.cnt_dev, sizeof(int)));
-            }
-            FGX_HIP(hipMemcpyAsync(s->ew.cnt_dev, &s->ew.cnt_accepted, sizeof(int), hipMemcpyHostToDevice, s->stream));
+        if (s->has_ewald && !s->ew.cnt_dev) FGX_HIP(hipMalloc(&s->ew.cnt_dev, sizeof(int)));
+        // work queued on the system's own stream precedes the sweep
+        if (s != s0) FGX_HIP(hipStreamSynchronize(s->stream));
+        if (!s->state_on_dev) {
+            FGX_HIP(hipMemcpyAsync(s->d_rng, s->rng_global, sizeof(uint32_t) * 625,
+                                   hipMemcpyHostToDevice, st));
+            FGX_HIP(hipMemcpyAsync(s->d_rng + 625, s->rng_internal, sizeof(uint32_t) * 625,
+                                   hipMemcpyHostToDevice, st));
+            if (s->has_ewald)
+                FGX_HIP(hipMemcpyAsync(s->ew.cnt_dev, &s->ew.cnt_accepted, sizeof(int),
+                                       hipMemcpyHostToDevice, st));
+            FGX_HIP(hipStreamSynchronize(st));   // host buffers are read at once
+            s->state_on_dev = true;
         }
-        rc = prepare_chain(s, chains[c], &gvecs[c], c == 0 ? du_rec : nullptr,
+        rc = prepare_chain(s, chains[c], st, c == 0 ? du_rec : nullptr,
                            c == 0 ? acc_rec : nullptr, c == 0 ? cap : 0);
-        if (!rc && s->has_ewald) chains[c].ew_cnt = s->ew.cnt_dev;
-        FGX_HIP(hipStreamSynchronize(s->stream));
+        if (rc) break;
+        if (s->has_ewald) chains[c].ew_cnt = s->ew.cnt_dev;
+        chains[c].stats_i = s0->d_bstat_i + 8 * c;
+        chains[c].stats_d = s0->d_bstat_d + 8 * c;
         nthr = std::max(nthr, s->launch_threads > 0 ? s->launch_threads : chain_block(s));
         if (s->has_ewald) dyn = std::max(dyn, at_dyn_lds(s->ew.K));
-        if (!rc && !chains[c].fx) fx = false;
+        if (!chains[c].fx) fx = false;
     }
-    ChainDev *d_chains = nullptr;
     if (!rc) {
         // group-pair energy caches that no longer match the accepted state
         std::vector<ChainDev> stale;
@@ -1636,9 +1871,9 @@ static int run_chains(std::vector<fgx_system *> &sys, int ncalls, double *du_rec
             ChainDev *d_stale = nullptr;
             FGX_HIP(hipMalloc(&d_stale, sizeof(ChainDev) * stale.size()));
             FGX_HIP(hipMemcpyAsync(d_stale, stale.data(), sizeof(ChainDev) * stale.size(),
-                                   hipMemcpyHostToDevice, s0->stream));
-            rc = launch_emat_init(s0->pot, s0->stream, d_stale, (int)stale.size(), maxng);
-            hipError_t err = hipStreamSynchronize(s0->stream);
+                                   hipMemcpyHostToDevice, st));
+            rc = launch_emat_init(s0->pot, st, d_stale, (int)stale.size(), maxng);
+            hipError_t err = hipStreamSynchronize(st);
             (void)hipFree(d_stale);
             if (!rc && err != hipSuccess) {
                 set_error(std::string("energy cache kernel failed: ") + hipGetErrorString(err));
@@ -1650,60 +1885,55 @@ static int run_chains(std::vector<fgx_system *> &sys, int ncalls, double *du_rec
         }
     }
     if (!rc) {
-        FGX_HIP(hipMalloc(&d_chains, sizeof(ChainDev) * nc));
+        ChainDev *d_chains = (ChainDev *)s0->d_batch;
         FGX_HIP(hipMemcpyAsync(d_chains, chains.data(), sizeof(ChainDev) * nc, hipMemcpyHostToDevice,
-                               s0->stream));
-        FGX_HIP(hipStreamSynchronize(s0->stream));
+                               st));
         hipEvent_t e0, e1;
         FGX_HIP(hipEventCreate(&e0));
         FGX_HIP(hipEventCreate(&e1));
-        FGX_HIP(hipEventRecord(e0, s0->stream));
+        FGX_HIP(hipEventRecord(e0, st));
         if (s0->moves[0].type == MOVE_AT) {
             rc = s0->pot.precision == FGX_MIXED
-                     ? launch_at<true>(s0->pot, s0->stream, d_chains, nc, nthr, dyn, ncalls, unroll, fx)
-                     : launch_at<false>(s0->pot, s0->stream, d_chains, nc, nthr, dyn, ncalls, unroll, fx);
+                     ? launch_at<true>(s0->pot, st, d_chains, nc, nthr, dyn, ncalls, unroll, fx)
+                     : launch_at<false>(s0->pot, st, d_chains, nc, nthr, dyn, ncalls, unroll, fx);
         } else {
             rc = s0->pot.precision == FGX_MIXED
-                     ? launch_tr<true>(s0->pot, s0->stream, d_chains, nc, nthr, ncalls)
-                     : launch_tr<false>(s0->pot, s0->stream, d_chains, nc, nthr, ncalls);
+                     ? launch_tr<true>(s0->pot, st, d_chains, nc, nthr, ncalls)
+                     : launch_tr<false>(s0->pot, st, d_chains, nc, nthr, ncalls);
         }
-        FGX_HIP(hipEventRecord(e1, s0->stream));
-        FGX_HIP(hipEventSynchronize(e1));
-        float ms = 0;
-        FGX_HIP(hipEventElapsedTime(&ms, e0, e1));
-        if (seconds) *seconds = ms * 1e-3;
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        hipError_t err = hipStreamSynchronize(s0->stream);
+        FGX_HIP(hipEventRecord(e1, st));
+        // trial vector, trial S(k) and mass centres mirror the accepted state
+        // (undo/accept leave p == trial between moves)
+        for (int c = 0; c < nc && !rc; ++c) {
+            fgx_system *s = sys[c];
+            if (s->has_ewald)
+                FGX_HIP(hipMemcpyAsync(s->ew.Qt, s->ew.Q, sizeof(double) * 2 * s->ew.K,
+                                       hipMemcpyDeviceToDevice, st));
+            for (int k = 0; k < 3; ++k) {
+                double *src = k == 0 ? s->v[0].x : k == 1 ? s->v[0].y : s->v[0].z;
+                double *dst = k == 0 ? s->v[1].x : k == 1 ? s->v[1].y : s->v[1].z;
+                FGX_HIP(hipMemcpyAsync(dst, src, sizeof(double) * s->n, hipMemcpyDeviceToDevice, st));
+            }
+        }
+        if (stats_i) {
+            stats_i->assign(8 * (size_t)nc, 0);
+            stats_d->assign(8 * (size_t)nc, 0.0);
+            FGX_HIP(hipMemcpyAsync(stats_i->data(), s0->d_bstat_i, sizeof(long) * 8 * nc,
+                                   hipMemcpyDeviceToHost, st));
+            FGX_HIP(hipMemcpyAsync(stats_d->data(), s0->d_bstat_d, sizeof(double) * 8 * nc,
+                                   hipMemcpyDeviceToHost, st));
+        }
+        hipError_t err = hipStreamSynchronize(st);
         if (err != hipSuccess && !rc) {
             set_error(std::string("sweep kernel failed: ") + hipGetErrorString(err));
             rc = FGX_EHIP;
         }
+        float ms = 0;
+        if (!rc) FGX_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (seconds) *seconds = ms * 1e-3;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
     }
-    // pull back RNG state and bookkeeping
-    for (int c = 0; c < nc && !rc; ++c) {
-        fgx_system *s = sys[c];
-        FGX_HIP(hipMemcpyAsync(s->rng_global, s->d_rng, sizeof(uint32_t) * 625, hipMemcpyDeviceToHost, s->stream));
-        FGX_HIP(hipMemcpyAsync(s->rng_internal, s->d_rng + 625, sizeof(uint32_t) * 625, hipMemcpyDeviceToHost, s->stream));
-        if (s->has_ewald) {
-            FGX_HIP(hipMemcpyAsync(&s->ew.cnt_accepted, s->ew.cnt_dev, sizeof(int), hipMemcpyDeviceToHost, s->stream));
-        }
-        FGX_HIP(hipStreamSynchronize(s->stream));
-        if (s->has_ewald) {
-            // trial S(k) mirrors the current one between trials (undo/accept)
-            FGX_HIP(hipMemcpyAsync(s->ew.Qt, s->ew.Q, sizeof(double) * 2 * s->ew.K, hipMemcpyDeviceToDevice, s->stream));
-        }
-        // trial vector and mass centres mirror the accepted state
-        for (int k = 0; k < 3; ++k) {
-            double *src = k == 0 ? s->v[0].x : k == 1 ? s->v[0].y : s->v[0].z;
-            double *dst = k == 0 ? s->v[1].x : k == 1 ? s->v[1].y : s->v[1].z;
-            FGX_HIP(hipMemcpyAsync(dst, src, sizeof(double) * s->n, hipMemcpyDeviceToDevice, s->stream));
-        }
-        FGX_HIP(hipStreamSynchronize(s->stream));
-    }
-    for (auto *p : gvecs)
-        if (p) (void)hipFree(p);
-    if (d_chains) (void)hipFree(d_chains);
     return rc;
 }
 
@@ -1738,6 +1968,7 @@ extern "C" int fgx_add_move(fgx_system *s, int type, int molid, double prob, int
 extern "C" int fgx_rng_seed(fgx_system *s, uint32_t seed)
 {
     FGX_CHECK(s, FGX_EARG, "fgx_rng_seed: null system");
+    if (int rc = pull_device_state(s)) return rc;
     mt_seed_host(s->rng_global, seed);
     return FGX_OK;
 }
@@ -1745,6 +1976,7 @@ extern "C" int fgx_rng_seed(fgx_system *s, uint32_t seed)
 extern "C" int fgx_rng_set_state(fgx_system *s, const uint32_t words[624], int pos)
 {
     FGX_CHECK(s && words && pos >= 0 && pos <= 624, FGX_EARG, "fgx_rng_set_state: bad arguments");
+    if (int rc = pull_device_state(s)) return rc;
     std::memcpy(s->rng_global, words, sizeof(uint32_t) * 624);
     s->rng_global[624] = (uint32_t)pos;
     return FGX_OK;
@@ -1753,6 +1985,7 @@ extern "C" int fgx_rng_set_state(fgx_system *s, const uint32_t words[624], int p
 extern "C" int fgx_rng_get_state(fgx_system *s, uint32_t words[624], int *pos)
 {
     FGX_CHECK(s && words && pos, FGX_EARG, "fgx_rng_get_state: bad arguments");
+    if (int rc = pull_device_state(s)) return rc;
     std::memcpy(words, s->rng_global, sizeof(uint32_t) * 624);
     *pos = (int)s->rng_global[624];
     return FGX_OK;
@@ -1761,19 +1994,17 @@ extern "C" int fgx_rng_get_state(fgx_system *s, uint32_t words[624], int *pos)
 extern "C" int fgx_propagator_init(fgx_system *s)
 {
     FGX_CHECK(s, FGX_EARG, "fgx_propagator_init: null system");
+    if (int rc = pull_device_state(s)) return rc;
     std::memcpy(s->rng_internal, s->rng_global, sizeof(s->rng_global));   // move.h:7781
     s->propagator_started = false;
     s->dusum = 0;
     return FGX_OK;
 }
 
-static int collect_stats(fgx_system *s, fgx_stats *st, double seconds)
+// folds one chain's launch statistics into the system and the caller's sum
+static void collect_stats(fgx_system *s, const long *si, const double *sd, fgx_stats *st,
+                          double seconds)
 {
-    long si[16];
-    double sd[16];
-    FGX_HIP(hipMemcpyAsync(si, s->d_stats_i, sizeof(si), hipMemcpyDeviceToHost, s->stream));
-    FGX_HIP(hipMemcpyAsync(sd, s->d_stats_d, sizeof(sd), hipMemcpyDeviceToHost, s->stream));
-    FGX_HIP(hipStreamSynchronize(s->stream));
     if (si[0] > 0) s->move_cnt[0] += si[0];
     s->dusum += sd[1];
     if (st) {
@@ -1785,7 +2016,6 @@ static int collect_stats(fgx_system *s, fgx_stats *st, double seconds)
         st->dusum += sd[1];
         st->seconds = seconds;
     }
-    return FGX_OK;
 }
 
 extern "C" int fgx_propagate(fgx_system *s, int ncalls, fgx_stats *st, double *du_out,
@@ -1803,11 +2033,12 @@ extern "C" int fgx_propagate(fgx_system *s, int ncalls, fgx_stats *st, double *d
     }
     std::vector<fgx_system *> sys{s};
     double secs = 0;
-    int rc = run_chains(sys, ncalls, d_du, d_acc, cap, &secs);
-    if (!rc) rc = collect_stats(s, st, secs);
+    std::vector<long> si;
+    std::vector<double> sd;
+    int rc = run_chains(sys, ncalls, d_du, d_acc, cap, &secs, &si, &sd);
+    if (!rc) collect_stats(s, si.data(), sd.data(), st, secs);
     if (!rc && cap > 0) {
-        long ntr = st ? st->trials : cap;
-        long ncopy = std::min(cap, ntr);
+        const long ncopy = std::min(cap, si[0]);
         if (ncopy > 0) {
             FGX_HIP(hipMemcpy(du_out, d_du, sizeof(double) * ncopy, hipMemcpyDeviceToHost));
             FGX_HIP(hipMemcpy(acc_out, d_acc, sizeof(int) * ncopy, hipMemcpyDeviceToHost));
@@ -1823,10 +2054,16 @@ extern "C" int fgx_propagate_batch(fgx_system **systems, int nsys, int ncalls, f
     FGX_CHECK(systems && nsys > 0 && ncalls >= 0, FGX_EARG, "fgx_propagate_batch: bad arguments");
     std::vector<fgx_system *> sys(systems, systems + nsys);
     for (auto *s : sys) FGX_CHECK(s, FGX_EARG, "fgx_propagate_batch: null system");
+    for (int a = 0; a < nsys; ++a)
+        for (int b = a + 1; b < nsys; ++b)
+            FGX_CHECK(sys[a] != sys[b], FGX_EARG, "fgx_propagate_batch: a system appears twice");
     if (st) std::memset(st, 0, sizeof(*st));
     double secs = 0;
-    int rc = run_chains(sys, ncalls, nullptr, nullptr, 0, &secs);
-    for (int c = 0; c < nsys && !rc; ++c) rc = collect_stats(sys[c], st, secs);
+    std::vector<long> si;
+    std::vector<double> sd;
+    int rc = run_chains(sys, ncalls, nullptr, nullptr, 0, &secs, &si, &sd);
+    for (int c = 0; c < nsys && !rc; ++c)
+        collect_stats(sys[c], &si[8 * (size_t)c], &sd[8 * (size_t)c], st, secs);
     return rc;
 }
 

@@ -612,6 +612,42 @@ static int ewald_set_space(fgx_system *s)
     return FGX_OK;
 }
 
+// Chebyshev coefficients of erfcx(x) = exp(x^2) erfc(x) on [0, xmax], xmax =
+// alpha rc (the largest real-space argument), from long-double samples at the
+// Chebyshev nodes; the series is cut where the remaining coefficients are
+// below 1e-17 (erfcx is entire, so they fall off faster than geometrically).
+// Used by the mixed path's FP64 pass: erfc(x) = exp(-x^2) * series(x).
+static int build_erfc_series(fgx_system *s)
+{
+    const int M = 96;
+    const long double xmax = (long double)s->pot.ew_alpha * std::sqrt((long double)s->pot.ew_rc2) *
+                             (1.0L + 1e-9L);
+    const long double pi = 3.14159265358979323846264338327950288L;
+    std::vector<long double> f(M), c(M);
+    for (int k = 0; k < M; ++k) {
+        const long double t = std::cos(pi * (k + 0.5L) / M);
+        const long double x = 0.5L * (t + 1.0L) * xmax;
+        f[k] = std::exp(x * x) * std::erfc(x);
+    }
+    for (int j = 0; j < M; ++j) {
+        long double a = 0;
+        for (int k = 0; k < M; ++k) a += f[k] * std::cos(pi * j * (k + 0.5L) / M);
+        c[j] = 2.0L * a / M;
+    }
+    c[0] *= 0.5L;
+    int n = 0;
+    for (int j = M - 1; j >= 0; --j)
+        if (std::fabs(c[j]) > 1e-17L) { n = j + 1; break; }
+    if (n > EW_CHEB_MAX || n == 0) return FGX_OK;   // series not used: generic erfc stays
+    std::vector<double> cd(n);
+    for (int j = 0; j < n; ++j) cd[j] = (double)c[j];
+    int rc;
+    if ((rc = dalloc(s, &s->ew.cheb, n)) || (rc = upload(s, s->ew.cheb, cd.data(), n))) return rc;
+    s->ew.ncheb = n;
+    s->ew.cheb_xmax = (double)xmax;
+    return FGX_OK;
+}
+
 extern "C" int fgx_create(const fgx_hamiltonian *h, const double box[3], int n, const double *xyz,
                           const double *charge, const double *radius, const double *mw,
                           const int *type, int ng, const int *front, const int *back,
@@ -735,9 +771,9 @@ extern "C" int fgx_create(const fgx_hamiltonian *h, const double box[3], int n, 
             return fail(rc);
     }
     if ((rc = dalloc(s, &s->d_q, n)) || (rc = dalloc(s, &s->d_r, n)) ||
-        (rc = dalloc(s, &s->d_mw, n)) || (rc = dalloc(s, &s->d_type, n)) ||
-        (rc = dalloc(s, &s->d_gid, n)) || (rc = dalloc(s, &s->d_pk, n)) ||
-        (rc = dalloc(s, &s->d_qf, n)) || (rc = dalloc(s, &s->d_rf, n)) ||
+        (rc = dalloc(s, &s->d_mw, n)) || (rc = dalloc(s, &s->d_type, n + PARTNER_PAD)) ||
+        (rc = dalloc(s, &s->d_gid, n)) || (rc = dalloc(s, &s->d_pk, n + PARTNER_PAD)) ||
+        (rc = dalloc(s, &s->d_qf, n)) || (rc = dalloc(s, &s->d_rf, n + PARTNER_PAD)) ||
         (rc = dalloc(s, &s->d_groups, ng)) || (rc = dalloc(s, &s->d_cm, 6 * (size_t)ng)) ||
         (rc = dalloc(s, &s->d_tab, s->tab.size())) || (rc = dalloc(s, &s->d_tabf, s->tab.size())) ||
         (rc = dalloc(s, &s->d_type_dp, T)) || (rc = dalloc(s, &s->d_mask, 2 * (size_t)ng + 2)) ||
@@ -750,6 +786,12 @@ extern "C" int fgx_create(const fgx_hamiltonian *h, const double box[3], int n, 
         return fail(FGX_EHIP);
     }
     if ((rc = ensure_partials(s, 1024))) return fail(rc);
+    if (hipMemsetAsync(s->d_type + n, 0, sizeof(int) * PARTNER_PAD, s->stream) != hipSuccess ||
+        hipMemsetAsync(s->d_pk + n, 0, sizeof(int4) * PARTNER_PAD, s->stream) != hipSuccess ||
+        hipMemsetAsync(s->d_rf + n, 0, sizeof(float) * PARTNER_PAD, s->stream) != hipSuccess) {
+        set_error("fgx_create: hipMemsetAsync failed");
+        return fail(FGX_EHIP);
+    }
     std::vector<double> tmp(n);
     for (int k = 0; k < 3; ++k) {
         for (int i = 0; i < n; ++i) tmp[i] = xyz[3 * i + k];
@@ -787,6 +829,7 @@ extern "C" int fgx_create(const fgx_hamiltonian *h, const double box[3], int n, 
     if (s->has_ewald) {
         if ((rc = build_ewald(s, h))) return fail(rc);
         if ((rc = ewald_set_space(s))) return fail(rc);
+        if (P.ew_plain && (rc = build_erfc_series(s))) return fail(rc);
     }
     mt_seed_host(s->rng_global, 5489u);
     mt_seed_host(s->rng_internal, 5489u);
@@ -807,7 +850,8 @@ extern "C" int fgx_destroy(fgx_system *s)
                     s->d_rf, s->d_groups, s->d_cm, s->d_tab, s->d_tabf, s->d_type_dp, s->d_mask,
                     s->d_partial, s->d_out, s->d_idx, s->d_rng, s->d_stats_i, s->d_stats_d,
                     s->ew.kv, s->ew.aks, s->ew.kn, s->ew.Q, s->ew.Qt, s->ew.eff,
-                    s->ew.cnt_dev, s->d_emat};
+                    s->ew.cnt_dev, s->d_emat, s->ew.cheb, s->d_gvec, s->d_batch,
+                    s->d_bstat_i, s->d_bstat_d};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (s->h_out) (void)hipHostFree(s->h_out);
@@ -1235,6 +1279,8 @@ extern "C" int fgx_commit(fgx_system *s, const fgx_change *c, int accepted, doub
 {
     FGX_CHECK(s, FGX_EARG, "fgx_commit: null system");
     if (accepted) s->emat_valid = false;
+    if (int prc = pull_device_state(s)) return prc;
+    s->state_on_dev = false;
     int nm;
     bool list;
     int rc = stage_change(s, c, &nm, &list);
