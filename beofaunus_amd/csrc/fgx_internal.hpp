@@ -121,6 +121,8 @@ struct fgx_system {
     // current ones after a device sweep (state_on_dev); host-side users pull
     // them first (pull_device_state)
     bool state_on_dev = false;
+    double chain_seconds = 0;              // this chain's wall time in the last device launch
+    int wall_khz = 0;
     int *d_gvec = nullptr;                 // move's group list (device sweeps)
     int gvec_n = 0;
     // per-batch launch buffers owned by the batch's first system

@@ -563,6 +563,7 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
     // surface dipole M = sum q eff r (getSurfaceEnergy, ewald.h:278-292)
     int ew_cnt = EW ? *C.ew_cnt : 0;
     long n_trials = 0, n_acc = 0, n_inf = 0, n_ref = 0;
+    const long long t_start = wall_clock64();   // chain's own wall time (stats slot 4)
     if (tid == 0)
         for (int q = 0; q < 5; ++q) sst[q] = 0.0;
     long rec = 0;
@@ -887,6 +888,7 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
         C.stats_i[1] = n_acc;
         C.stats_i[2] = n_inf;
         C.stats_i[3] = n_ref;
+        C.stats_i[4] = wall_clock64() - t_start;
         C.stats_d[0] = sst[3];
         C.stats_d[1] = sst[4];
         if (EW) *C.ew_cnt = ew_cnt;
@@ -1227,6 +1229,7 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_tr(const ChainDev *__restr
     }
     WaveMT G{mtg, (int)C.rng[624]}, I{mti, (int)C.rng[625 + 624]};
     long n_trials = 0, n_acc = 0, n_inf = 0;
+    const long long t_start = wall_clock64();
     double utot = 0.0, dusum = 0.0;
     long rec = 0;
     __syncthreads();
@@ -1480,6 +1483,7 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_tr(const ChainDev *__restr
         C.stats_i[1] = n_acc;
         C.stats_i[2] = n_inf;
         C.stats_i[3] = 0;
+        C.stats_i[4] = wall_clock64() - t_start;
         C.stats_d[0] = utot;
         C.stats_d[1] = dusum;
     }
@@ -2007,6 +2011,13 @@ static void collect_stats(fgx_system *s, const long *si, const double *sd, fgx_s
 {
     if (si[0] > 0) s->move_cnt[0] += si[0];
     s->dusum += sd[1];
+    if (s->wall_khz <= 0) {
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, s->device) != hipSuccess)
+            khz = 0;
+        s->wall_khz = khz;
+    }
+    s->chain_seconds = s->wall_khz > 0 ? (double)si[4] / (1e3 * s->wall_khz) : 0.0;
     if (st) {
         st->trials += si[0];
         st->accepted += si[1];
@@ -2047,6 +2058,13 @@ extern "C" int fgx_propagate(fgx_system *s, int ncalls, fgx_stats *st, double *d
     if (d_du) (void)hipFree(d_du);
     if (d_acc) (void)hipFree(d_acc);
     return rc;
+}
+
+extern "C" int fgx_chain_seconds(const fgx_system *s, double *seconds)
+{
+    FGX_CHECK(s && seconds, FGX_EARG, "fgx_chain_seconds: bad arguments");
+    *seconds = s->chain_seconds;
+    return FGX_OK;
 }
 
 extern "C" int fgx_propagate_batch(fgx_system **systems, int nsys, int ncalls, fgx_stats *st)

@@ -98,6 +98,7 @@ def lib():
         "fgx_propagate_batch": (I, [pp, I, I, C.POINTER(Stats)]),
         "fgx_set_launch": (I, [P, I, I]),
         "fgx_set_energy_matrix": (I, [P, I]),
+        "fgx_chain_seconds": (I, [P, C.POINTER(C.c_double)]),
         "fgx_get_launch": (I, [P, _ip, _ip]),
         "fgx_autotune": (I, [P, I, _ip, _ip, _dp]),
         "fgx_autotune_batch": (I, [pp, I, I, _ip, _ip, _dp]),
@@ -272,6 +273,12 @@ class System:
 
     def set_launch(self, threads=0, unroll=0):
         _check(lib().fgx_set_launch(self.h, threads, unroll))
+
+    def chain_seconds(self):
+        """Wall time this chain spent in the last device launch."""
+        v = C.c_double()
+        _check(lib().fgx_chain_seconds(self.h, C.byref(v)))
+        return v.value
 
     def set_energy_matrix(self, on):
         """Group-pair energy cache of the mixed TranslateRotate sweep."""

@@ -151,6 +151,32 @@ def ewald_electrolyte(n=20000, L=120.0, lB=7.1, sigma=4.0, eps_kt=0.1, rc=12.0, 
     }
 
 
+def pt_ladders(nladder=6, rungs=8, nmin=1000, nmax=16000, tmin=280.0, tmax=420.0,
+               lB_ref=7.1, T_ref=T_DEFAULT, rho=500.0 / 60.0 ** 3, seed=4,
+               precision=fgx.MIXED):
+    """Batched parallel tempering: `nladder` ladders of `rungs` replicas.
+    Each ladder has one N (log-uniform in [nmin, nmax], seeded) at the salt
+    density of the 500-ion / 60 A system, and geometric temperatures
+    T_r = tmin (tmax/tmin)^(r/(rungs-1)).  The solvent permittivity is fixed,
+    so lB_r = lB_ref T_ref / T_r (physconst.h:91-94); Debye-Hueckel 10 A, hard
+    spheres 2 A, AtomicTranslation dp 2 A.  Exchanges stay inside a ladder
+    (the reference swaps configurations of equal N, move.h:6683-6687)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for l in range(nladder):
+        n = int(round(math.exp(rng.uniform(math.log(nmin), math.log(nmax)))))
+        n += n % 2
+        L = (n / rho) ** (1.0 / 3.0)
+        reps = []
+        for r in range(rungs):
+            T = tmin * (tmax / tmin) ** (r / (rungs - 1))
+            w = salt_dh(n=n, L=L, lB=lB_ref * T_ref / T, debye=10.0, radius=2.0, dp=2.0,
+                        seed=seed * 1000 + 10 * l + r, T=T, precision=precision)
+            reps.append(w)
+        out.append({"n": n, "L": L, "T": [w["T"] for w in reps], "replicas": reps})
+    return out
+
+
 def rigid_molecules(nmol=64, nsite=400, nion=8000, L=250.0, lB=7.1, debye=15.0, R=20.0,
                     dp=5.0, dprot=0.5, seed=3, T=T_DEFAULT, precision=fgx.MIXED):
     """Rigid molecules (site charges from {-1,0,+1} with p = 0.2/0.6/0.2,

@@ -175,6 +175,10 @@ int fgx_autotune_batch(fgx_system **systems, int nsys, int ncalls, int *threads,
  * launch (one workgroup per chain).  Systems keep their own state.
  * ------------------------------------------------------------------------ */
 int fgx_propagate_batch(fgx_system **systems, int nsys, int ncalls, fgx_stats *st);
+/* wall time this chain spent inside the last launch that advanced it
+ * (device clock; chains of a batch finish at different times).  No
+ * counterpart in the reference (per-chain sweep latency). */
+int fgx_chain_seconds(const fgx_system *s, double *seconds);
 
 /* ------------------------------------------------------------------------
  * ParallelTempering (move.h:6520-6772) for R replicas on one device.  The

@@ -54,3 +54,36 @@ def test_temper_matches_oracle():
     t.close()
     for s in gpu:
         s.close()
+
+
+def test_batched_ladders_drive():
+    """Several equal-N ladders advanced by one batched launch, then exchanges
+    inside each ladder: every chain reports its own device time, the trial
+    count is the sum of the sweeps, and each ladder's slots stay a
+    permutation of its replicas."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from beofaunus_amd import fgx, workloads
+    print("[temper] 2 ladders x 4 replicas, batched", flush=True)
+    ladders = workloads.pt_ladders(nladder=2, rungs=4, nmin=200, nmax=400, seed=9)
+    systems, tempers = [], []
+    for l, lad in enumerate(ladders):
+        reps = [workloads.make_system(w) for w in lad["replicas"]]
+        for r, s in enumerate(reps):
+            s.rng_seed(31 + 10 * l + r)
+            s.propagator_init()
+        systems += reps
+        tempers.append(fgx.Temper(reps, ctl_seed=3 + l, glob_seed=5 + l))
+    for step in range(3):
+        st = fgx.propagate_batch(systems, 2)
+        assert st["trials"] == 2 * sum(l["n"] * len(l["replicas"]) for l in ladders)
+        assert all(s.chain_seconds() > 0 for s in systems)
+        for t in tempers:
+            acc, du = t.exchange()
+            assert np.all(np.isfinite(du))
+    for t, lad in zip(tempers, ladders):
+        assert sorted(t.slots().tolist()) == list(range(len(lad["replicas"])))
+        t.close()
+    for s in systems:
+        s.close()
