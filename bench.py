@@ -358,14 +358,88 @@ def bench_temper(args, fgx, workloads):
     return line
 
 
+def bench_serve(args, fgx, workloads):
+    """Served job stream: jobs of N log-uniform 500-50000 (half Ewald
+    electrolytes, half Debye-Hueckel salts) arrive as a Poisson process and
+    are served by fgx_server under an HBM budget.  Reports job latency
+    (arrival to completion), per-job sweep latency and aggregate throughput."""
+    import torch
+    jobs = workloads.serve_jobs(njobs=args.jobs, seed=args.seed)
+    rng = np.random.default_rng(args.seed + 1)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, size=len(jobs)))
+    budget = int(args.hbm_mb * 1024 * 1024)
+    sv = fgx.Server(budget, tick_seconds=args.tick_ms * 1e-3)
+    ids, arr_of = [], {}
+    res = {}
+    submitted = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    while len(res) < len(jobs):
+        now = time.perf_counter() - t0
+        while submitted < len(jobs) and arrivals[submitted] <= now:
+            jid = sv.submit(jobs[submitted]["w"], seed=1000 + submitted, sweeps=args.job_sweeps)
+            arr_of[jid] = (submitted, arrivals[submitted])
+            ids.append(jid)
+            submitted += 1
+        inf = sv.info()
+        if inf["active"] or inf["queued"]:
+            sv.step()
+        elif submitted < len(jobs):
+            time.sleep(max(0.0, min(0.01, arrivals[submitted] - now)))
+        for jid in sv.poll():
+            r = sv.result(jid)
+            r["done_at"] = time.perf_counter() - t0
+            res[jid] = r
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    peak = sv.info()["peak"]
+    sv.close()
+    lat = [r["done_at"] - arr_of[j][1] for j, r in res.items()]
+    sweep = [r["mean_sweep_seconds"] for r in res.values()]
+    trials = sum(r["trials"] for r in res.values())
+    pairs = sum(r["trials"] * 2 * (jobs[arr_of[j][0]]["n"] - 1) for j, r in res.items())
+    bad = [j for j, r in res.items() if r["status"] != 0]
+    line = base_line(args, trials / wall, wall, "f32 pairs + f64 accumulate",
+                     {"workload": "served job stream: %d jobs, Poisson arrivals %.1f jobs/s, N "
+                                  "log-uniform 500-50000 at 0.01 A^-3, half Ewald LJ+Coulomb "
+                                  "electrolyte (alpha 0.25, rc 12, |n|<=10), half Debye-Hueckel "
+                                  "10 A + hard spheres, AtomicTranslation, %d sweeps per job, "
+                                  "HBM budget %d MB"
+                                  % (len(jobs), args.rate, args.job_sweeps, args.hbm_mb),
+                      "jobs": len(jobs), "sweeps_per_job": args.job_sweeps,
+                      "precision": "mixed"})
+    line["ms_per_step"] = None
+    line["steps"] = None
+    line.update({
+        "pairs_per_s": pairs / wall,
+        "job_latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                          "max": float(np.max(lat))},
+        "sweep_latency_ms": {"p50": 1e3 * float(np.percentile(sweep, 50)),
+                             "p99": 1e3 * float(np.percentile(sweep, 99))},
+        "queue_wait_s": {"p50": float(np.percentile([r["queue_seconds"] for r in res.values()], 50)),
+                         "p99": float(np.percentile([r["queue_seconds"] for r in res.values()], 99))},
+        "peak_resident_mb": peak / 2 ** 20,
+        "failed_jobs": len(bad),
+        "wall_s": wall,
+    })
+    return line
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", choices=["ewald", "groups", "temper"], default="ewald",
+    ap.add_argument("--config", choices=["ewald", "groups", "temper", "serve"], default="ewald",
                     help="ewald: N=20000 Ewald electrolyte (headline); groups: 64 x 400-site "
-                         "rigid molecules + 8000 ions; temper: 48-replica parallel tempering")
+                         "rigid molecules + 8000 ions; temper: 48-replica parallel tempering; "
+                         "serve: 256-job stream through the job server")
+    ap.add_argument("--jobs", type=int, default=256)
+    ap.add_argument("--rate", type=float, default=50.0, help="job arrivals per second")
+    ap.add_argument("--job-sweeps", type=int, default=4)
+    ap.add_argument("--hbm-mb", type=float, default=256.0)
+    ap.add_argument("--tick-ms", type=float, default=50.0)
+    ap.add_argument("--seed", type=int, default=5)
     ap.add_argument("--ladders", type=int, default=6)
     ap.add_argument("--rungs", type=int, default=8)
     ap.add_argument("--exchange-every", type=int, default=10, help="sweeps between exchanges")
@@ -390,8 +464,10 @@ def main():
         line = bench_ewald(args, fgx, workloads)
     elif args.config == "groups":
         line = bench_groups(args, fgx, workloads)
-    else:
+    elif args.config == "temper":
         line = bench_temper(args, fgx, workloads)
+    else:
+        line = bench_serve(args, fgx, workloads)
     print(json.dumps(line))
 
 

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "libfgx.so")
-SOURCES = ["fgx_system.hip", "fgx_sweep.hip", "fgx_temper.hip"]
+SOURCES = ["fgx_system.hip", "fgx_sweep.hip", "fgx_temper.hip", "fgx_server.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

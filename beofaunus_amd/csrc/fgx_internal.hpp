@@ -143,4 +143,15 @@ int ewald_full_recompute(fgx_system *s, int which, double *Qdst);
 int ewald_reciprocal(fgx_system *s, const double *Q, double V, double *out_host);
 void mt_seed_host(uint32_t *state625, uint32_t seed);
 int pull_device_state(fgx_system *s);
+// asynchronous batched sweeps (fgx_sweep.hip), used by the job server:
+// chain c advances by ncalls[c] Propagator::move() calls
+int batch_launch(std::vector<fgx_system *> &sys, const std::vector<int> &ncalls, void **handle);
+int batch_finish(void *handle, double *seconds, std::vector<long> *stats_i,
+                 std::vector<double> *stats_d);
+}  // namespace fgx
+// folds one chain's launch statistics (8 longs, 8 doubles) into the system
+// and into *st (may be null)
+void collect_stats(fgx_system *s, const long *si, const double *sd, fgx_stats *st,
+                   double seconds);
+namespace fgx {
 }  // namespace fgx

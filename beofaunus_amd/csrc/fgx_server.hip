@@ -1,0 +1,373 @@
+// fgx_server.hip -- serving independent simulation jobs on one device.
+//
+// A job is one Faunus simulation (a Space, its Hamiltonian, one registered
+// move, a seed) and a number of Propagator::move() calls (sweeps).  Submitted
+// jobs wait in a FIFO queue on the host; fgx_server_step admits queued jobs
+// while their estimated device footprint fits the HBM budget (creating their
+// device state), advances every resident job by one tick of batched sweeps,
+// and retires finished jobs, releasing their memory so that later arrivals
+// are admitted.  Jobs of one kernel kind (move, precision, Hamiltonian form)
+// share one launch; launches of different kinds run on different streams at
+// the same time.  Each job advances by as many sweeps per tick as fit the
+// tick time at its own measured sweep rate, so small jobs are not held to
+// the pace of large ones (continuous batching).  A job's result does not
+// depend on the other jobs it shares launches with: chains are independent
+// workgroups and every reduction has a fixed order.
+//
+// The reference runs one simulation per process (src/examples/bulk.cpp) and
+// has no counterpart to this component.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "fgx_internal.hpp"
+
+using namespace fgx;
+
+namespace {
+
+double now_s()
+{
+    using clk = std::chrono::steady_clock;
+    static const clk::time_point t0 = clk::now();
+    return std::chrono::duration<double>(clk::now() - t0).count();
+}
+
+struct JobSpec {
+    fgx_hamiltonian h{};
+    std::vector<double> tsig, teps, tq, tr, tdp;
+    double box[3]{};
+    int n = 0, ng = 0;
+    std::vector<double> xyz, q, r, mw;
+    std::vector<int> type, front, back, molsize, molid;
+    int move_type = 0, move_molid = 0, peratom = 0, permol = 0, center_rotation = 0;
+    double prob = 1, dp = 0, dprot = 0;
+    double dir[3] = {1, 1, 1}, dir2[3] = {1, 1, 1};
+    uint32_t seed = 0;
+    int sweeps = 0;
+    int keep_positions = 0;
+};
+
+struct Job {
+    int id = 0;
+    JobSpec spec;
+    fgx_system *sys = nullptr;
+    size_t est_bytes = 0, bytes = 0;
+    int done = 0;                 // sweeps completed
+    int state = 0;                // 0 queued, 1 resident, 2 finished, 3 failed
+    double t_submit = 0, t_admit = 0, t_done = 0;
+    double sweep_sum = 0, sweep_max = 0, sweep_est = 0;
+    int sweep_samples = 0;
+    fgx_stats st{};
+    std::vector<double> final_xyz;
+    int error = FGX_OK;
+};
+
+// device bytes fgx_create and a sweep allocate for a job, from its sizes
+// (the padding, FP64/FP32 copies and Ewald tables of fgx_system)
+size_t estimate_bytes(const JobSpec &j)
+{
+    const size_t n = (size_t)j.n;
+    size_t b = n * (6 * 8 + 3 * 8 + 2 * 4 + 16 + 2 * 4);
+    b += (size_t)PARTNER_PAD * (16 + 4 + 4);
+    b += (size_t)j.ng * (16 + 48 + 4) + 4 * (size_t)j.h.ntypes * j.h.ntypes * 12;
+    b += 2 * 625 * 4 + 64 * 1024;   // RNG, reduction scratch and small buffers
+    if (j.h.elec == FGX_ELEC_EWALD) {
+        const double kc = j.h.ewald_kc;
+        const size_t K = (size_t)(2.5 * kc * kc * kc + 64);
+        b += K * (3 * 8 + 8 + 3 * 4 + 2 * 16) + (size_t)j.h.ntypes * 8;
+    }
+    if (j.move_type == FGX_MOVE_TR && j.h.precision == FGX_MIXED && j.ng <= 256)
+        b += (size_t)j.ng * j.ng * 8;
+    return b;
+}
+
+// kernel kind: chains of one kind share a launch
+long kind_of(const JobSpec &j)
+{
+    return ((((long)j.move_type * 4 + j.h.precision) * 8 + j.h.elec) * 8 + j.h.sr) * 4 + j.h.nbkind;
+}
+
+}  // namespace
+
+struct fgx_server {
+    size_t budget = 0, resident = 0, peak_resident = 0;
+    int max_active = 0;
+    double tick = 0.05;
+    int next_id = 1;
+    std::deque<int> queue;
+    std::map<int, std::unique_ptr<Job>> jobs;
+    std::vector<int> finished;   // not yet polled
+    long ticks = 0;
+};
+
+static int admit(fgx_server *sv, Job &J)
+{
+    const JobSpec &j = J.spec;
+    fgx_system *s = nullptr;
+    int rc = fgx_create(&j.h, j.box, j.n, j.xyz.data(), j.q.data(), j.r.data(), j.mw.data(),
+                        j.type.data(), j.ng, j.front.data(), j.back.data(), j.molsize.data(),
+                        j.molid.data(), &s);
+    if (!rc)
+        rc = fgx_add_move(s, j.move_type, j.move_molid, j.prob, j.peratom, j.permol, j.dir, j.dir2,
+                          j.dp, j.dprot, j.center_rotation);
+    if (!rc) rc = fgx_rng_seed(s, j.seed);
+    if (!rc) rc = fgx_propagator_init(s);
+    // one fixed schedule for every job, so a result does not depend on what
+    // else is resident
+    if (!rc) rc = fgx_set_launch(s, 512, 4);
+    if (rc) {
+        if (s) fgx_destroy(s);
+        return rc;
+    }
+    J.sys = s;
+    J.bytes = fgx_device_bytes(s);
+    J.state = 1;
+    J.t_admit = now_s();
+    sv->resident += J.bytes;
+    sv->peak_resident = std::max(sv->peak_resident, sv->resident);
+    // first sweep-time guess: one CU at 1e9 pair evaluations per second
+    J.sweep_est = 2.0 * (double)j.n * (double)j.n / 1e9;
+    return FGX_OK;
+}
+
+static void retire(fgx_server *sv, Job &J, int error)
+{
+    if (J.sys) {
+        if (!error && J.spec.keep_positions) {
+            J.final_xyz.resize(3 * (size_t)J.spec.n);
+            if (fgx_get_positions(J.sys, 0, J.final_xyz.data())) J.final_xyz.clear();
+        }
+        sv->resident -= std::min(sv->resident, J.bytes);
+        fgx_destroy(J.sys);
+        J.sys = nullptr;
+    }
+    J.state = error ? 3 : 2;
+    J.error = error;
+    J.t_done = now_s();
+    J.spec.xyz.clear();
+    J.spec.xyz.shrink_to_fit();
+    sv->finished.push_back(J.id);
+}
+
+extern "C" int fgx_server_create(size_t hbm_budget, int max_active, double tick_seconds,
+                                 fgx_server **out)
+{
+    FGX_CHECK(out && hbm_budget > 0 && max_active > 0 && tick_seconds > 0, FGX_EARG,
+              "fgx_server_create: bad arguments");
+    fgx_server *sv = new fgx_server;
+    sv->budget = hbm_budget;
+    sv->max_active = max_active;
+    sv->tick = tick_seconds;
+    *out = sv;
+    return FGX_OK;
+}
+
+extern "C" int fgx_server_submit(fgx_server *sv, const fgx_job *job, int *id)
+{
+    FGX_CHECK(sv && job && job->h && job->n > 0 && job->ng > 0 && job->xyz && job->charge &&
+                  job->radius && job->mw && job->type && job->front && job->back &&
+                  job->molsize && job->molid && job->sweeps >= 0,
+              FGX_EARG, "fgx_server_submit: bad job");
+    auto J = std::make_unique<Job>();
+    JobSpec &j = J->spec;
+    j.h = *job->h;
+    const int T = j.h.ntypes;
+    auto copyt = [&](const double *src, std::vector<double> &dst, const double **field) {
+        if (src) dst.assign(src, src + T);
+        *field = src ? dst.data() : nullptr;
+    };
+    copyt(job->h->type_sigma, j.tsig, &j.h.type_sigma);
+    copyt(job->h->type_eps, j.teps, &j.h.type_eps);
+    copyt(job->h->type_charge, j.tq, &j.h.type_charge);
+    copyt(job->h->type_radius, j.tr, &j.h.type_radius);
+    copyt(job->h->type_dp, j.tdp, &j.h.type_dp);
+    std::memcpy(j.box, job->box, sizeof(j.box));
+    j.n = job->n;
+    j.ng = job->ng;
+    j.xyz.assign(job->xyz, job->xyz + 3 * (size_t)j.n);
+    j.q.assign(job->charge, job->charge + j.n);
+    j.r.assign(job->radius, job->radius + j.n);
+    j.mw.assign(job->mw, job->mw + j.n);
+    j.type.assign(job->type, job->type + j.n);
+    j.front.assign(job->front, job->front + j.ng);
+    j.back.assign(job->back, job->back + j.ng);
+    j.molsize.assign(job->molsize, job->molsize + j.ng);
+    j.molid.assign(job->molid, job->molid + j.ng);
+    j.move_type = job->move_type;
+    j.move_molid = job->move_molid;
+    j.prob = job->prob;
+    j.peratom = job->peratom;
+    j.permol = job->permol;
+    j.dp = job->dp;
+    j.dprot = job->dprot;
+    j.center_rotation = job->center_rotation;
+    for (int k = 0; k < 3; ++k) {
+        if (job->dir) j.dir[k] = job->dir[k];
+        if (job->dir2) j.dir2[k] = job->dir2[k];
+    }
+    j.seed = job->seed;
+    j.sweeps = job->sweeps;
+    j.keep_positions = job->keep_positions;
+    J->est_bytes = estimate_bytes(j);
+    FGX_CHECK(J->est_bytes <= sv->budget, FGX_ENOMEM,
+              "fgx_server_submit: job larger than the HBM budget");
+    J->id = sv->next_id++;
+    J->t_submit = now_s();
+    if (id) *id = J->id;
+    sv->queue.push_back(J->id);
+    sv->jobs[J->id] = std::move(J);
+    return FGX_OK;
+}
+
+extern "C" int fgx_server_step(fgx_server *sv, int *n_admitted, int *n_finished)
+{
+    FGX_CHECK(sv, FGX_EARG, "fgx_server_step: null server");
+    int adm = 0, fin = 0;
+    // ---- admission: FIFO while the footprint fits the budget
+    int active = 0;
+    for (auto &kv : sv->jobs) active += kv.second->state == 1;
+    while (!sv->queue.empty() && active < sv->max_active) {
+        Job &J = *sv->jobs[sv->queue.front()];
+        if (sv->resident + J.est_bytes > sv->budget) break;
+        sv->queue.pop_front();
+        const int rc = admit(sv, J);
+        if (rc) {
+            retire(sv, J, rc);
+            ++fin;
+            continue;
+        }
+        ++adm;
+        ++active;
+    }
+    // ---- one tick: a batched launch per kernel kind, all in flight together
+    std::map<long, std::vector<Job *>> groups;
+    for (auto &kv : sv->jobs) {
+        Job &J = *kv.second;
+        if (J.state != 1) continue;
+        if (J.done >= J.spec.sweeps) {   // zero-sweep jobs
+            retire(sv, J, FGX_OK);
+            ++fin;
+            continue;
+        }
+        groups[kind_of(J.spec)].push_back(&J);
+    }
+    struct InFlight {
+        std::vector<Job *> jobs;
+        std::vector<int> ncalls;
+        void *h = nullptr;
+        int rc = FGX_OK;
+    };
+    std::vector<InFlight> fl;
+    for (auto &g : groups) {
+        InFlight f;
+        f.jobs = g.second;
+        std::vector<fgx_system *> sys;
+        for (Job *J : f.jobs) {
+            const int left = J->spec.sweeps - J->done;
+            int k = (int)std::floor(sv->tick / std::max(J->sweep_est, 1e-7));
+            k = std::max(1, std::min(left, k));
+            f.ncalls.push_back(k);
+            sys.push_back(J->sys);
+        }
+        f.rc = batch_launch(sys, f.ncalls, &f.h);
+        fl.push_back(std::move(f));
+    }
+    int rc_all = FGX_OK;
+    for (auto &f : fl) {
+        double secs = 0;
+        std::vector<long> si;
+        std::vector<double> sd;
+        int rc = batch_finish(f.h, &secs, f.rc ? nullptr : &si, &sd);
+        if (f.rc) rc = f.rc;
+        for (size_t c = 0; c < f.jobs.size(); ++c) {
+            Job &J = *f.jobs[c];
+            if (rc) {
+                retire(sv, J, rc);
+                ++fin;
+                continue;
+            }
+            collect_stats(J.sys, &si[8 * c], &sd[8 * c], &J.st, secs);
+            const double per = J.sys->chain_seconds / f.ncalls[c];
+            J.sweep_sum += J.sys->chain_seconds;
+            J.sweep_samples += f.ncalls[c];
+            J.sweep_max = std::max(J.sweep_max, per);
+            J.sweep_est = per;
+            J.done += f.ncalls[c];
+            if (J.done >= J.spec.sweeps) {
+                retire(sv, J, FGX_OK);
+                ++fin;
+            }
+        }
+        if (rc && !rc_all) rc_all = rc;
+    }
+    ++sv->ticks;
+    if (n_admitted) *n_admitted = adm;
+    if (n_finished) *n_finished = fin;
+    return rc_all;
+}
+
+extern "C" int fgx_server_poll(fgx_server *sv, int *ids, int cap, int *count)
+{
+    FGX_CHECK(sv && count && (ids || cap == 0), FGX_EARG, "fgx_server_poll: bad arguments");
+    const int n = std::min(cap, (int)sv->finished.size());
+    for (int k = 0; k < n; ++k) ids[k] = sv->finished[k];
+    sv->finished.erase(sv->finished.begin(), sv->finished.begin() + n);
+    *count = n;
+    return FGX_OK;
+}
+
+extern "C" int fgx_server_result(fgx_server *sv, int id, fgx_job_result *r, double *xyz)
+{
+    FGX_CHECK(sv && r, FGX_EARG, "fgx_server_result: bad arguments");
+    auto it = sv->jobs.find(id);
+    FGX_CHECK(it != sv->jobs.end(), FGX_EARG, "fgx_server_result: unknown job");
+    const Job &J = *it->second;
+    FGX_CHECK(J.state >= 2, FGX_ESTATE, "fgx_server_result: job not finished");
+    std::memset(r, 0, sizeof(*r));
+    r->status = J.error;
+    r->trials = J.st.trials;
+    r->accepted = J.st.accepted;
+    r->utot = J.st.utot;
+    r->sweeps = J.done;
+    r->queue_seconds = J.t_admit > 0 ? J.t_admit - J.t_submit : J.t_done - J.t_submit;
+    r->latency_seconds = J.t_done - J.t_submit;
+    r->mean_sweep_seconds = J.sweep_samples ? J.sweep_sum / J.sweep_samples : 0.0;
+    r->max_sweep_seconds = J.sweep_max;
+    r->device_bytes = J.bytes;
+    if (xyz) {
+        FGX_CHECK(!J.final_xyz.empty(), FGX_ESTATE,
+                  "fgx_server_result: positions were not kept for this job");
+        std::memcpy(xyz, J.final_xyz.data(), sizeof(double) * J.final_xyz.size());
+    }
+    return FGX_OK;
+}
+
+extern "C" int fgx_server_info(const fgx_server *sv, size_t *resident, size_t *peak, int *active,
+                               int *queued)
+{
+    FGX_CHECK(sv, FGX_EARG, "fgx_server_info: null server");
+    int a = 0;
+    for (auto &kv : sv->jobs) a += kv.second->state == 1;
+    if (resident) *resident = sv->resident;
+    if (peak) *peak = sv->peak_resident;
+    if (active) *active = a;
+    if (queued) *queued = (int)sv->queue.size();
+    return FGX_OK;
+}
+
+extern "C" int fgx_server_destroy(fgx_server *sv)
+{
+    if (!sv) return FGX_OK;
+    for (auto &kv : sv->jobs)
+        if (kv.second->sys) fgx_destroy(kv.second->sys);
+    delete sv;
+    return FGX_OK;
+}

@@ -65,6 +65,7 @@ struct ChainDev {
     double *du_rec;
     int *acc_rec;
     long cap;
+    int ncalls;                        // Propagator::move() calls for this chain
     // group-pair energy cache (ng x ng, FP64) of the mixed TranslateRotate
     // sweep, or null
     double *emat;
@@ -581,7 +582,7 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
     __syncthreads();
 
     int par = 0;
-    for (int call = 0; call < ncalls; ++call) {
+    for (int call = 0; call < C.ncalls; ++call) {
         // ---- Propagator::move (move.h:7887) + Movebase::move prologue
         if (wave0) {
             (void)mt_range(I, 0, C.nmoves - 1);       // pick the move
@@ -1234,7 +1235,7 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_tr(const ChainDev *__restr
     long rec = 0;
     __syncthreads();
     int par = 0;
-    for (int call = 0; call < ncalls; ++call) {
+    for (int call = 0; call < C.ncalls; ++call) {
         if (wave0) {
             (void)mt_range(I, 0, C.nmoves - 1);
             (void)mt_range(I, 0, 0);
@@ -1795,17 +1796,27 @@ int pull_device_state(fgx_system *s)
     return FGX_OK;
 }
 
-// One launch advancing every chain by `ncalls` Propagator::move() calls.  All
-// transfers go on the first system's stream with a single synchronisation at
-// the end; RNG states and counters stay on the device between launches.
-// Per-chain statistics land in `stats` (8 longs + 8 doubles per chain).
-static int run_chains(std::vector<fgx_system *> &sys, int ncalls, double *du_rec, int *acc_rec,
-                      long cap, double *seconds, std::vector<long> *stats_i,
-                      std::vector<double> *stats_d)
+// One batched launch in flight (launch_chains .. finish_chains).
+struct Launch {
+    std::vector<fgx_system *> sys;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = FGX_OK;
+};
+
+// Enqueues one launch advancing chain c by ncalls[c] Propagator::move()
+// calls.  All transfers go on the first system's stream; RNG states and
+// counters stay on the device between launches.  Nothing waits for the
+// sweep itself, so launches of different Hamiltonians on different first
+// systems overlap on the device.
+static int launch_chains(std::vector<fgx_system *> &sys, const std::vector<int> &ncalls,
+                         double *du_rec, int *acc_rec, long cap, Launch &L)
 {
     fgx_system *s0 = sys[0];
     const int nc = (int)sys.size();
     hipStream_t st = s0->stream;
+    L.sys = sys;
+    L.st = st;
     std::vector<ChainDev> chains(nc);
     int rc = FGX_OK;
     int nthr = 0;
@@ -1856,6 +1867,7 @@ static int run_chains(std::vector<fgx_system *> &sys, int ncalls, double *du_rec
                            c == 0 ? acc_rec : nullptr, c == 0 ? cap : 0);
         if (rc) break;
         if (s->has_ewald) chains[c].ew_cnt = s->ew.cnt_dev;
+        chains[c].ncalls = ncalls[c];
         chains[c].stats_i = s0->d_bstat_i + 8 * c;
         chains[c].stats_d = s0->d_bstat_d + 8 * c;
         nthr = std::max(nthr, s->launch_threads > 0 ? s->launch_threads : chain_block(s));
@@ -1892,20 +1904,20 @@ static int run_chains(std::vector<fgx_system *> &sys, int ncalls, double *du_rec
         ChainDev *d_chains = (ChainDev *)s0->d_batch;
         FGX_HIP(hipMemcpyAsync(d_chains, chains.data(), sizeof(ChainDev) * nc, hipMemcpyHostToDevice,
                                st));
-        hipEvent_t e0, e1;
-        FGX_HIP(hipEventCreate(&e0));
-        FGX_HIP(hipEventCreate(&e1));
-        FGX_HIP(hipEventRecord(e0, st));
+        FGX_HIP(hipEventCreate(&L.e0));
+        FGX_HIP(hipEventCreate(&L.e1));
+        FGX_HIP(hipEventRecord(L.e0, st));
+        const int ncall0 = ncalls.empty() ? 0 : ncalls[0];
         if (s0->moves[0].type == MOVE_AT) {
             rc = s0->pot.precision == FGX_MIXED
-                     ? launch_at<true>(s0->pot, st, d_chains, nc, nthr, dyn, ncalls, unroll, fx)
-                     : launch_at<false>(s0->pot, st, d_chains, nc, nthr, dyn, ncalls, unroll, fx);
+                     ? launch_at<true>(s0->pot, st, d_chains, nc, nthr, dyn, ncall0, unroll, fx)
+                     : launch_at<false>(s0->pot, st, d_chains, nc, nthr, dyn, ncall0, unroll, fx);
         } else {
             rc = s0->pot.precision == FGX_MIXED
-                     ? launch_tr<true>(s0->pot, st, d_chains, nc, nthr, ncalls)
-                     : launch_tr<false>(s0->pot, st, d_chains, nc, nthr, ncalls);
+                     ? launch_tr<true>(s0->pot, st, d_chains, nc, nthr, ncall0)
+                     : launch_tr<false>(s0->pot, st, d_chains, nc, nthr, ncall0);
         }
-        FGX_HIP(hipEventRecord(e1, st));
+        FGX_HIP(hipEventRecord(L.e1, st));
         // trial vector, trial S(k) and mass centres mirror the accepted state
         // (undo/accept leave p == trial between moves)
         for (int c = 0; c < nc && !rc; ++c) {
@@ -1919,26 +1931,68 @@ static int run_chains(std::vector<fgx_system *> &sys, int ncalls, double *du_rec
                 FGX_HIP(hipMemcpyAsync(dst, src, sizeof(double) * s->n, hipMemcpyDeviceToDevice, st));
             }
         }
-        if (stats_i) {
-            stats_i->assign(8 * (size_t)nc, 0);
-            stats_d->assign(8 * (size_t)nc, 0.0);
-            FGX_HIP(hipMemcpyAsync(stats_i->data(), s0->d_bstat_i, sizeof(long) * 8 * nc,
-                                   hipMemcpyDeviceToHost, st));
-            FGX_HIP(hipMemcpyAsync(stats_d->data(), s0->d_bstat_d, sizeof(double) * 8 * nc,
-                                   hipMemcpyDeviceToHost, st));
-        }
-        hipError_t err = hipStreamSynchronize(st);
-        if (err != hipSuccess && !rc) {
-            set_error(std::string("sweep kernel failed: ") + hipGetErrorString(err));
-            rc = FGX_EHIP;
-        }
-        float ms = 0;
-        if (!rc) FGX_HIP(hipEventElapsedTime(&ms, e0, e1));
-        if (seconds) *seconds = ms * 1e-3;
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
+    }
+    L.rc = rc;
+    return rc;
+}
+
+// Waits for a launch; its device time, and per-chain statistics (8 longs +
+// 8 doubles per chain) when asked for.
+static int finish_chains(Launch &L, double *seconds, std::vector<long> *stats_i,
+                         std::vector<double> *stats_d)
+{
+    int rc = L.rc;
+    if (L.sys.empty()) return rc;
+    fgx_system *s0 = L.sys[0];
+    const int nc = (int)L.sys.size();
+    hipError_t err = hipStreamSynchronize(L.st);
+    if (err != hipSuccess && !rc) {
+        set_error(std::string("sweep kernel failed: ") + hipGetErrorString(err));
+        rc = FGX_EHIP;
+    }
+    float ms = 0;
+    if (!rc && L.e0 && L.e1) FGX_HIP(hipEventElapsedTime(&ms, L.e0, L.e1));
+    if (seconds) *seconds = ms * 1e-3;
+    if (L.e0) (void)hipEventDestroy(L.e0);
+    if (L.e1) (void)hipEventDestroy(L.e1);
+    L.e0 = L.e1 = nullptr;
+    if (!rc && stats_i) {
+        stats_i->assign(8 * (size_t)nc, 0);
+        stats_d->assign(8 * (size_t)nc, 0.0);
+        FGX_HIP(hipMemcpy(stats_i->data(), s0->d_bstat_i, sizeof(long) * 8 * nc,
+                          hipMemcpyDeviceToHost));
+        FGX_HIP(hipMemcpy(stats_d->data(), s0->d_bstat_d, sizeof(double) * 8 * nc,
+                          hipMemcpyDeviceToHost));
     }
     return rc;
+}
+
+// asynchronous batch launches for the job server (fgx_server.hip)
+int batch_launch(std::vector<fgx_system *> &sys, const std::vector<int> &ncalls, void **handle)
+{
+    Launch *L = new Launch;
+    *handle = L;
+    return launch_chains(sys, ncalls, nullptr, nullptr, 0, *L);
+}
+
+int batch_finish(void *handle, double *seconds, std::vector<long> *stats_i,
+                 std::vector<double> *stats_d)
+{
+    Launch *L = (Launch *)handle;
+    int rc = finish_chains(*L, seconds, stats_i, stats_d);
+    delete L;
+    return rc;
+}
+
+static int run_chains(std::vector<fgx_system *> &sys, int ncalls, double *du_rec, int *acc_rec,
+                      long cap, double *seconds, std::vector<long> *stats_i,
+                      std::vector<double> *stats_d)
+{
+    Launch L;
+    std::vector<int> nv(sys.size(), ncalls);
+    int rc = launch_chains(sys, nv, du_rec, acc_rec, cap, L);
+    int rc2 = finish_chains(L, seconds, rc ? nullptr : stats_i, stats_d);
+    return rc ? rc : rc2;
 }
 
 }  // namespace fgx
@@ -2006,7 +2060,7 @@ extern "C" int fgx_propagator_init(fgx_system *s)
 }
 
 // folds one chain's launch statistics into the system and the caller's sum
-static void collect_stats(fgx_system *s, const long *si, const double *sd, fgx_stats *st,
+void collect_stats(fgx_system *s, const long *si, const double *sd, fgx_stats *st,
                           double seconds)
 {
     if (si[0] > 0) s->move_cnt[0] += si[0];

@@ -50,6 +50,31 @@ class Stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Job(C.Structure):
+    _fields_ = [
+        ("h", C.POINTER(Hamiltonian)), ("box", C.c_double * 3), ("n", C.c_int),
+        ("xyz", _dp), ("charge", _dp), ("radius", _dp), ("mw", _dp), ("type", _ip),
+        ("ng", C.c_int), ("front", _ip), ("back", _ip), ("molsize", _ip), ("molid", _ip),
+        ("move_type", C.c_int), ("move_molid", C.c_int), ("peratom", C.c_int),
+        ("permol", C.c_int), ("center_rotation", C.c_int),
+        ("prob", C.c_double), ("dp", C.c_double), ("dprot", C.c_double),
+        ("dir", _dp), ("dir2", _dp), ("seed", C.c_uint32), ("sweeps", C.c_int),
+        ("keep_positions", C.c_int),
+    ]
+
+
+class JobResult(C.Structure):
+    _fields_ = [
+        ("status", C.c_int), ("sweeps", C.c_int), ("trials", C.c_long), ("accepted", C.c_long),
+        ("utot", C.c_double), ("queue_seconds", C.c_double), ("latency_seconds", C.c_double),
+        ("mean_sweep_seconds", C.c_double), ("max_sweep_seconds", C.c_double),
+        ("device_bytes", C.c_size_t),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class Change(C.Structure):
     _fields_ = [("group", C.c_int), ("count", C.c_int), ("index", _ip)]
 
@@ -99,6 +124,13 @@ def lib():
         "fgx_set_launch": (I, [P, I, I]),
         "fgx_set_energy_matrix": (I, [P, I]),
         "fgx_chain_seconds": (I, [P, C.POINTER(C.c_double)]),
+        "fgx_server_create": (I, [C.c_size_t, I, C.c_double, pp]),
+        "fgx_server_submit": (I, [P, C.POINTER(Job), _ip]),
+        "fgx_server_step": (I, [P, _ip, _ip]),
+        "fgx_server_poll": (I, [P, _ip, I, _ip]),
+        "fgx_server_result": (I, [P, I, C.POINTER(JobResult), _dp]),
+        "fgx_server_info": (I, [P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), _ip, _ip]),
+        "fgx_server_destroy": (I, [P]),
         "fgx_get_launch": (I, [P, _ip, _ip]),
         "fgx_autotune": (I, [P, I, _ip, _ip, _dp]),
         "fgx_autotune_batch": (I, [pp, I, I, _ip, _ip, _dp]),
@@ -137,6 +169,31 @@ def device_info():
     return {"arch": name.value.decode(), "cus": cu.value, "hbm_bytes": mem.value}
 
 
+def make_hamiltonian(arr, ntypes, *, elec, sr, nbkind=NB_PLAIN, precision=FP64, lB=7.0,
+                     kappa=0.0, g2g_cut=0.0, type_sigma=None, type_eps=None, type_charge=None,
+                     type_radius=None, type_dp=None, ewald=None):
+    """fgx_hamiltonian from keyword parameters; `arr(values, n)` must keep the
+    per-type arrays alive for as long as the structure is used."""
+    h = Hamiltonian()
+    h.elec, h.sr, h.nbkind, h.precision = elec, sr, nbkind, precision
+    h.lB, h.kappa, h.g2g_cut = lB, kappa, g2g_cut
+    h.ntypes = ntypes
+    h.type_sigma = _d(arr(type_sigma, ntypes))
+    h.type_eps = _d(arr(type_eps, ntypes))
+    h.type_charge = _d(arr(type_charge, ntypes))
+    h.type_radius = _d(arr(type_radius, ntypes))
+    h.type_dp = _d(arr(type_dp, ntypes))
+    if ewald:
+        h.ewald_alpha = ewald["alpha"]
+        h.ewald_rc = ewald["cutoff"]
+        h.ewald_kc = ewald["cutoffK"]
+        h.ewald_debyelength = ewald["debyelength"]
+        h.ewald_eps_surf = ewald["eps_surf"]
+        h.ewald_spherical_sum = int(ewald.get("spherical_sum", True))
+        h.ewald_update_frequency = int(ewald.get("update_frequency", -1))
+    return h
+
+
 class System:
     """One Space<Cuboid,PointParticle> + non-bonded Hamiltonian on the GPU."""
 
@@ -155,23 +212,10 @@ class System:
             self._keep.append(a)
             return a
 
-        h = Hamiltonian()
-        h.elec, h.sr, h.nbkind, h.precision = elec, sr, nbkind, precision
-        h.lB, h.kappa, h.g2g_cut = lB, kappa, g2g_cut
-        h.ntypes = ntypes
-        h.type_sigma = _d(arr(type_sigma, ntypes))
-        h.type_eps = _d(arr(type_eps, ntypes))
-        h.type_charge = _d(arr(type_charge, ntypes))
-        h.type_radius = _d(arr(type_radius, ntypes))
-        h.type_dp = _d(arr(type_dp, ntypes))
-        if ewald:
-            h.ewald_alpha = ewald["alpha"]
-            h.ewald_rc = ewald["cutoff"]
-            h.ewald_kc = ewald["cutoffK"]
-            h.ewald_debyelength = ewald["debyelength"]
-            h.ewald_eps_surf = ewald["eps_surf"]
-            h.ewald_spherical_sum = int(ewald.get("spherical_sum", True))
-            h.ewald_update_frequency = int(ewald.get("update_frequency", -1))
+        h = make_hamiltonian(arr, ntypes, elec=elec, sr=sr, nbkind=nbkind, precision=precision,
+                             lB=lB, kappa=kappa, g2g_cut=g2g_cut, type_sigma=type_sigma,
+                             type_eps=type_eps, type_charge=type_charge,
+                             type_radius=type_radius, type_dp=type_dp, ewald=ewald)
         self.ham = h
         self.n = n
         self.groups = [dict(g) for g in groups]
@@ -358,6 +402,102 @@ class Temper:
     def close(self):
         if getattr(self, "h", None):
             lib().fgx_temper_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Job server (fgx_server_*): independent simulations admitted by HBM
+    footprint and advanced together in batched launches."""
+
+    def __init__(self, hbm_budget, max_active=4096, tick_seconds=0.05):
+        h = C.c_void_p()
+        _check(lib().fgx_server_create(int(hbm_budget), int(max_active), float(tick_seconds),
+                                       C.byref(h)))
+        self.h = h
+        self.n = {}
+
+    def submit(self, w, seed, sweeps, precision=None, keep_positions=False):
+        """Queue a workload dict (beofaunus_amd.workloads) for `sweeps`
+        Propagator::move() calls; returns the job id."""
+        keep = []
+
+        def arr(v, n, dtype=np.float64):
+            a = np.ascontiguousarray(np.asarray(v if v is not None else np.zeros(n), dtype=dtype))
+            keep.append(a)
+            return a
+        ham = dict(w["ham"])
+        if precision is not None:
+            ham["precision"] = precision
+        ntypes = len(ham["type_sigma"])
+        h = make_hamiltonian(arr, ntypes, **ham)
+        n = len(w["charge"])
+        mv = dict(w["move"])
+        j = Job()
+        j.h = C.pointer(h)
+        j.box = (C.c_double * 3)(*w["box"])
+        j.n = n
+        j.xyz = _d(arr(np.asarray(w["xyz"]).reshape(-1), 3 * n))
+        j.charge = _d(arr(w["charge"], n))
+        j.radius = _d(arr(w["radius"], n))
+        j.mw = _d(arr(w["mw"], n))
+        j.type = _i(arr(w["types"], n, np.int32))
+        g = w["groups"]
+        j.ng = len(g)
+        j.front = _i(arr([x["front"] for x in g], len(g), np.int32))
+        j.back = _i(arr([x["back"] for x in g], len(g), np.int32))
+        j.molsize = _i(arr([x["molsize"] for x in g], len(g), np.int32))
+        j.molid = _i(arr([x["molid"] for x in g], len(g), np.int32))
+        j.move_type = mv["kind"]
+        j.move_molid = mv["molid"]
+        j.peratom = int(mv.get("peratom", False))
+        j.permol = int(mv.get("permol", False))
+        j.center_rotation = int(mv.get("center_rotation", True))
+        j.prob = mv.get("prob", 1.0)
+        j.dp = mv.get("dp", 0.0)
+        j.dprot = mv.get("dprot", 0.0)
+        j.seed = seed
+        j.sweeps = sweeps
+        j.keep_positions = int(keep_positions)
+        jid = C.c_int()
+        _check(lib().fgx_server_submit(self.h, C.byref(j), C.byref(jid)))
+        self.n[jid.value] = n
+        return jid.value
+
+    def step(self):
+        a, f = C.c_int(), C.c_int()
+        _check(lib().fgx_server_step(self.h, C.byref(a), C.byref(f)))
+        return a.value, f.value
+
+    def poll(self, cap=4096):
+        ids = np.zeros(cap, dtype=np.int32)
+        cnt = C.c_int()
+        _check(lib().fgx_server_poll(self.h, _i(ids), cap, C.byref(cnt)))
+        return ids[:cnt.value].tolist()
+
+    def result(self, jid, positions=False):
+        r = JobResult()
+        xyz = np.zeros(3 * self.n[jid]) if positions else None
+        _check(lib().fgx_server_result(self.h, jid, C.byref(r), _d(xyz) if positions else None))
+        out = r.as_dict()
+        if positions:
+            out["xyz"] = xyz.reshape(-1, 3)
+        return out
+
+    def info(self):
+        res, peak = C.c_size_t(), C.c_size_t()
+        act, q = C.c_int(), C.c_int()
+        _check(lib().fgx_server_info(self.h, C.byref(res), C.byref(peak), C.byref(act), C.byref(q)))
+        return {"resident": res.value, "peak": peak.value, "active": act.value, "queued": q.value}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fgx_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

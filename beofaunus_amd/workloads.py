@@ -177,6 +177,26 @@ def pt_ladders(nladder=6, rungs=8, nmin=1000, nmax=16000, tmin=280.0, tmax=420.0
     return out
 
 
+def serve_jobs(njobs=256, nmin=500, nmax=50000, rho=0.01, seed=5, precision=fgx.MIXED):
+    """Served job stream: N log-uniform in [nmin, nmax] at number density rho;
+    even-numbered draws are Ewald LJ+Coulomb electrolytes (the N=20000
+    electrolyte's parameters), the others Debye-Hueckel + hard-sphere salt
+    (the 500-ion salt's parameters).  Starting configurations are jittered
+    lattices (hard-sphere free), seeded per job."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(njobs):
+        n = int(round(math.exp(rng.uniform(math.log(nmin), math.log(nmax)))))
+        n += n % 2
+        L = (n / rho) ** (1.0 / 3.0)
+        if k % 2 == 0:
+            w = ewald_electrolyte(n=n, L=L, seed=seed * 10007 + k, precision=precision)
+        else:
+            w = salt_dh(n=n, L=L, seed=seed * 10007 + k, lattice=True, precision=precision)
+        out.append({"n": n, "L": L, "kind": "ewald" if k % 2 == 0 else "dh", "w": w})
+    return out
+
+
 def rigid_molecules(nmol=64, nsite=400, nion=8000, L=250.0, lB=7.1, debye=15.0, R=20.0,
                     dp=5.0, dprot=0.5, seed=3, T=T_DEFAULT, precision=fgx.MIXED):
     """Rigid molecules (site charges from {-1,0,+1} with p = 0.2/0.6/0.2,

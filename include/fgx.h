@@ -181,6 +181,51 @@ int fgx_propagate_batch(fgx_system **systems, int nsys, int ncalls, fgx_stats *s
 int fgx_chain_seconds(const fgx_system *s, double *seconds);
 
 /* ------------------------------------------------------------------------
+ * Job server: independent simulations of different size and Hamiltonian
+ * served on one device.  Jobs are admitted in arrival order while their
+ * device footprint fits the HBM budget, advance together in batched
+ * launches (one per kernel kind, concurrent streams), and release their
+ * memory when done.  No counterpart in the reference (one simulation per
+ * process there).
+ * ------------------------------------------------------------------------ */
+typedef struct fgx_server fgx_server;
+typedef struct {
+    const fgx_hamiltonian *h;              /* copied at submission */
+    double box[3];
+    int n;
+    const double *xyz, *charge, *radius, *mw;
+    const int *type;
+    int ng;
+    const int *front, *back, *molsize, *molid;
+    /* the job's one move, as fgx_add_move */
+    int move_type, move_molid, peratom, permol, center_rotation;
+    double prob, dp, dprot;
+    const double *dir, *dir2;              /* NULL: (1,1,1) */
+    uint32_t seed;                         /* global slump seed (fgx_rng_seed) */
+    int sweeps;                            /* Propagator::move() calls to run */
+    int keep_positions;                    /* keep final positions for fgx_server_result */
+} fgx_job;
+typedef struct {
+    int status;                            /* FGX_OK or the error that ended the job */
+    int sweeps;
+    long trials, accepted;
+    double utot;                           /* sum of accepted dU + Ewald drift terms */
+    double queue_seconds;                  /* submission to admission */
+    double latency_seconds;                /* submission to completion */
+    double mean_sweep_seconds, max_sweep_seconds;   /* device time per sweep */
+    size_t device_bytes;
+} fgx_job_result;
+int fgx_server_create(size_t hbm_budget, int max_active, double tick_seconds, fgx_server **out);
+int fgx_server_submit(fgx_server *sv, const fgx_job *job, int *id);
+/* admit what fits, advance every resident job by one tick, retire finished */
+int fgx_server_step(fgx_server *sv, int *n_admitted, int *n_finished);
+int fgx_server_poll(fgx_server *sv, int *ids, int cap, int *count);
+int fgx_server_result(fgx_server *sv, int id, fgx_job_result *r, double *xyz);
+int fgx_server_info(const fgx_server *sv, size_t *resident, size_t *peak, int *active,
+                    int *queued);
+int fgx_server_destroy(fgx_server *sv);
+
+/* ------------------------------------------------------------------------
  * ParallelTempering (move.h:6520-6772) for R replicas on one device.  The
  * replicas differ in Bjerrum length (temperature enters through lB,
  * physconst.h:91-94).  An exchange draws the partner direction from a
