@@ -124,6 +124,7 @@ struct fgx_system {
     double chain_seconds = 0;              // this chain's wall time in the last device launch
     int wall_khz = 0;
     int *d_gvec = nullptr;                 // move's group list (device sweeps)
+    int *d_resume = nullptr;               // unfinished Propagator::move() call (4 ints)
     int gvec_n = 0;
     // per-batch launch buffers owned by the batch's first system
     void *d_batch = nullptr;               // ChainDev array
@@ -144,8 +145,11 @@ int ewald_reciprocal(fgx_system *s, const double *Q, double V, double *out_host)
 void mt_seed_host(uint32_t *state625, uint32_t seed);
 int pull_device_state(fgx_system *s);
 // asynchronous batched sweeps (fgx_sweep.hip), used by the job server:
-// chain c advances by ncalls[c] Propagator::move() calls
-int batch_launch(std::vector<fgx_system *> &sys, const std::vector<int> &ncalls, void **handle);
+// chain c runs up to ncalls[c] Propagator::move() calls and at most
+// max_trials[c] trials (> 0); a call cut short resumes in the next launch.
+// Statistics slot 5 holds the calls completed.
+int batch_launch(std::vector<fgx_system *> &sys, const std::vector<int> &ncalls,
+                 const std::vector<long> &max_trials, void **handle);
 int batch_finish(void *handle, double *seconds, std::vector<long> *stats_i,
                  std::vector<double> *stats_d);
 }  // namespace fgx

@@ -8,9 +8,11 @@
 // and retires finished jobs, releasing their memory so that later arrivals
 // are admitted.  Jobs of one kernel kind (move, precision, Hamiltonian form)
 // share one launch; launches of different kinds run on different streams at
-// the same time.  Each job advances by as many sweeps per tick as fit the
-// tick time at its own measured sweep rate, so small jobs are not held to
-// the pace of large ones (continuous batching).  A job's result does not
+// the same time.  Each job gets a trial budget per tick from its own
+// measured trial rate, so every launch lasts about one tick whatever the
+// job sizes: a large job's sweep spans several ticks (the kernel resumes an
+// unfinished Propagator::move() call), a small job runs several sweeps per
+// tick, and arrivals join at the next tick (continuous batching).  A job's result does not
 // depend on the other jobs it shares launches with: chains are independent
 // workgroups and every reduction has a fixed order.
 //
@@ -63,8 +65,9 @@ struct Job {
     int done = 0;                 // sweeps completed
     int state = 0;                // 0 queued, 1 resident, 2 finished, 3 failed
     double t_submit = 0, t_admit = 0, t_done = 0;
-    double sweep_sum = 0, sweep_max = 0, sweep_est = 0;
-    int sweep_samples = 0;
+    double chain_sum = 0;          // device seconds over all launches
+    double trial_max = 0;          // slowest launch, seconds per trial
+    double trial_est = 0;          // current estimate, seconds per trial
     fgx_stats st{};
     std::vector<double> final_xyz;
     int error = FGX_OK;
@@ -133,8 +136,8 @@ static int admit(fgx_server *sv, Job &J)
     J.t_admit = now_s();
     sv->resident += J.bytes;
     sv->peak_resident = std::max(sv->peak_resident, sv->resident);
-    // first sweep-time guess: one CU at 1e9 pair evaluations per second
-    J.sweep_est = 2.0 * (double)j.n * (double)j.n / 1e9;
+    // first trial-time guess: one CU at 1e9 pair evaluations per second
+    J.trial_est = 2.0 * (double)j.n / 1e9 + 2e-6;
     return FGX_OK;
 }
 
@@ -262,6 +265,7 @@ extern "C" int fgx_server_step(fgx_server *sv, int *n_admitted, int *n_finished)
     struct InFlight {
         std::vector<Job *> jobs;
         std::vector<int> ncalls;
+        std::vector<long> budget;
         void *h = nullptr;
         int rc = FGX_OK;
     };
@@ -271,13 +275,12 @@ extern "C" int fgx_server_step(fgx_server *sv, int *n_admitted, int *n_finished)
         f.jobs = g.second;
         std::vector<fgx_system *> sys;
         for (Job *J : f.jobs) {
-            const int left = J->spec.sweeps - J->done;
-            int k = (int)std::floor(sv->tick / std::max(J->sweep_est, 1e-7));
-            k = std::max(1, std::min(left, k));
-            f.ncalls.push_back(k);
+            f.ncalls.push_back(J->spec.sweeps - J->done);
+            const double b = sv->tick / std::max(J->trial_est, 1e-8);
+            f.budget.push_back(std::max(64L, (long)std::min(b, 1e12)));
             sys.push_back(J->sys);
         }
-        f.rc = batch_launch(sys, f.ncalls, &f.h);
+        f.rc = batch_launch(sys, f.ncalls, f.budget, &f.h);
         fl.push_back(std::move(f));
     }
     int rc_all = FGX_OK;
@@ -295,12 +298,13 @@ extern "C" int fgx_server_step(fgx_server *sv, int *n_admitted, int *n_finished)
                 continue;
             }
             collect_stats(J.sys, &si[8 * c], &sd[8 * c], &J.st, secs);
-            const double per = J.sys->chain_seconds / f.ncalls[c];
-            J.sweep_sum += J.sys->chain_seconds;
-            J.sweep_samples += f.ncalls[c];
-            J.sweep_max = std::max(J.sweep_max, per);
-            J.sweep_est = per;
-            J.done += f.ncalls[c];
+            const long tr = si[8 * c];
+            J.chain_sum += J.sys->chain_seconds;
+            if (tr > 0) {
+                J.trial_est = J.sys->chain_seconds / tr;
+                J.trial_max = std::max(J.trial_max, J.trial_est);
+            }
+            J.done += (int)si[8 * c + 5];
             if (J.done >= J.spec.sweeps) {
                 retire(sv, J, FGX_OK);
                 ++fin;
@@ -339,8 +343,10 @@ extern "C" int fgx_server_result(fgx_server *sv, int id, fgx_job_result *r, doub
     r->sweeps = J.done;
     r->queue_seconds = J.t_admit > 0 ? J.t_admit - J.t_submit : J.t_done - J.t_submit;
     r->latency_seconds = J.t_done - J.t_submit;
-    r->mean_sweep_seconds = J.sweep_samples ? J.sweep_sum / J.sweep_samples : 0.0;
-    r->max_sweep_seconds = J.sweep_max;
+    // per sweep: device time over completed sweeps; the slowest launch's
+    // trial rate times the trials of one sweep
+    r->mean_sweep_seconds = J.done ? J.chain_sum / J.done : 0.0;
+    r->max_sweep_seconds = J.done ? J.trial_max * ((double)J.st.trials / J.done) : 0.0;
     r->device_bytes = J.bytes;
     if (xyz) {
         FGX_CHECK(!J.final_xyz.empty(), FGX_ESTATE,

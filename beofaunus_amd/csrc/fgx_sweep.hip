@@ -66,6 +66,8 @@ struct ChainDev {
     int *acc_rec;
     long cap;
     int ncalls;                        // Propagator::move() calls for this chain
+    long max_trials;                   // trial budget of this launch (<= 0: none)
+    int *resume;                       // unfinished call: [in progress, rep, go]
     // group-pair energy cache (ng x ng, FP64) of the mixed TranslateRotate
     // sweep, or null
     double *emat;
@@ -582,17 +584,35 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
     __syncthreads();
 
     int par = 0;
-    for (int call = 0; call < C.ncalls; ++call) {
-        // ---- Propagator::move (move.h:7887) + Movebase::move prologue
-        if (wave0) {
+    // a launch may end inside a Propagator::move() call when its trial
+    // budget runs out; the next launch resumes that call where it stopped
+    const long budget = C.max_trials > 0 ? C.max_trials : LONG_MAX;
+    long used = 0;
+    int calls_done = 0;
+    bool stop = false;
+    int resume_rep = (C.resume && C.resume[0]) ? C.resume[1] : -1;
+    for (int call = 0; call < C.ncalls && !stop; ++call) {
+        int rep0 = 0;
+        if (resume_rep >= 0) {
+            if (tid == 0) T[par].go = 1;
+            rep0 = resume_rep;
+            resume_rep = -1;
+        } else if (wave0) {
+            // ---- Propagator::move (move.h:7887) + Movebase::move prologue
             (void)mt_range(I, 0, C.nmoves - 1);       // pick the move
             (void)mt_range(I, 0, 0);                  // randomMolId: mollist element
             bool run = mt_canonical(I) < C.mv.prob;   // Movebase::run (move.h:575-581)
             if (tid == 0) T[par].go = run ? 1 : 0;
         }
         __syncthreads();
-        if (!T[par].go) { par ^= 1; continue; }
-        for (int rep = 0; rep < C.repeat; ++rep) {
+        if (!T[par].go) { par ^= 1; ++calls_done; continue; }
+        for (int rep = rep0; rep < C.repeat; ++rep) {
+            if (used == budget) {
+                stop = true;
+                if (tid == 0 && C.resume) { C.resume[0] = 1; C.resume[1] = rep; }
+                break;
+            }
+            ++used;
             TrialAT &tr = T[par];
             TrialAT &tn = T[par ^ 1];   // previous trial's record; next trial's prefetch
             // ---- AtomicTranslation::_trialMove (move.h:818-864), wave 0
@@ -639,7 +659,8 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
             }
             __syncthreads();
             const int i = tr.i;
-            const bool ahead = rep + 1 < C.repeat;   // next trial in this call
+            // next trial in this call and in this launch
+            const bool ahead = rep + 1 < C.repeat && used < budget;
             double acc = 0.0;
             // ---- Ewald phase tables exp(i 2 pi n x / L) for old and new position
             if constexpr (EW) {
@@ -875,7 +896,9 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
             }
             par ^= 1;
         }
+        if (!stop) ++calls_done;
     }
+    if (!stop && tid == 0 && C.resume) C.resume[0] = 0;
     // ---- write back chain state
     __syncthreads();
     for (int k = tid; k < 624; k += nthr) {
@@ -890,6 +913,7 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
         C.stats_i[2] = n_inf;
         C.stats_i[3] = n_ref;
         C.stats_i[4] = wall_clock64() - t_start;
+        C.stats_i[5] = calls_done;
         C.stats_d[0] = sst[3];
         C.stats_d[1] = sst[4];
         if (EW) *C.ew_cnt = ew_cnt;
@@ -1235,16 +1259,32 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_tr(const ChainDev *__restr
     long rec = 0;
     __syncthreads();
     int par = 0;
-    for (int call = 0; call < C.ncalls; ++call) {
-        if (wave0) {
+    const long budget = C.max_trials > 0 ? C.max_trials : LONG_MAX;
+    long used = 0;
+    int calls_done = 0;
+    bool stop = false;
+    int resume_rep = (C.resume && C.resume[0]) ? C.resume[1] : -1;
+    for (int call = 0; call < C.ncalls && !stop; ++call) {
+        int rep0 = 0;
+        if (resume_rep >= 0) {
+            if (tid == 0) T[par].go = 1;
+            rep0 = resume_rep;
+            resume_rep = -1;
+        } else if (wave0) {
             (void)mt_range(I, 0, C.nmoves - 1);
             (void)mt_range(I, 0, 0);
             bool run = mt_canonical(I) < C.mv.prob;
             if (tid == 0) T[par].go = run ? 1 : 0;
         }
         __syncthreads();
-        if (!T[par].go) { par ^= 1; continue; }
-        for (int rep = 0; rep < C.repeat; ++rep) {
+        if (!T[par].go) { par ^= 1; ++calls_done; continue; }
+        for (int rep = rep0; rep < C.repeat; ++rep) {
+            if (used == budget) {
+                stop = true;
+                if (tid == 0 && C.resume) { C.resume[0] = 1; C.resume[1] = rep; }
+                break;
+            }
+            ++used;
             TrialTR &tr = T[par];
             // ---- TranslateRotate::_trialMove (move.h:1620-1686), wave 0
             if (wave0) {
@@ -1471,7 +1511,9 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_tr(const ChainDev *__restr
             __syncthreads();   // committed sites / cm visible to the next trial
             par ^= 1;
         }
+        if (!stop) ++calls_done;
     }
+    if (!stop && tid == 0 && C.resume) C.resume[0] = 0;
     __syncthreads();
     for (int k = tid; k < 624; k += nthr) {
         C.rng[k] = mtg[k];
@@ -1485,6 +1527,7 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_tr(const ChainDev *__restr
         C.stats_i[2] = n_inf;
         C.stats_i[3] = 0;
         C.stats_i[4] = wall_clock64() - t_start;
+        C.stats_i[5] = calls_done;
         C.stats_d[0] = utot;
         C.stats_d[1] = dusum;
     }
@@ -1810,7 +1853,8 @@ struct Launch {
 // sweep itself, so launches of different Hamiltonians on different first
 // systems overlap on the device.
 static int launch_chains(std::vector<fgx_system *> &sys, const std::vector<int> &ncalls,
-                         double *du_rec, int *acc_rec, long cap, Launch &L)
+                         const std::vector<long> *max_trials, double *du_rec, int *acc_rec,
+                         long cap, Launch &L)
 {
     fgx_system *s0 = sys[0];
     const int nc = (int)sys.size();
@@ -1868,6 +1912,8 @@ static int launch_chains(std::vector<fgx_system *> &sys, const std::vector<int> 
         if (rc) break;
         if (s->has_ewald) chains[c].ew_cnt = s->ew.cnt_dev;
         chains[c].ncalls = ncalls[c];
+        chains[c].max_trials = max_trials ? (*max_trials)[c] : 0;
+        chains[c].resume = s->d_resume;
         chains[c].stats_i = s0->d_bstat_i + 8 * c;
         chains[c].stats_d = s0->d_bstat_d + 8 * c;
         nthr = std::max(nthr, s->launch_threads > 0 ? s->launch_threads : chain_block(s));
@@ -1968,11 +2014,12 @@ static int finish_chains(Launch &L, double *seconds, std::vector<long> *stats_i,
 }
 
 // asynchronous batch launches for the job server (fgx_server.hip)
-int batch_launch(std::vector<fgx_system *> &sys, const std::vector<int> &ncalls, void **handle)
+int batch_launch(std::vector<fgx_system *> &sys, const std::vector<int> &ncalls,
+                 const std::vector<long> &max_trials, void **handle)
 {
     Launch *L = new Launch;
     *handle = L;
-    return launch_chains(sys, ncalls, nullptr, nullptr, 0, *L);
+    return launch_chains(sys, ncalls, &max_trials, nullptr, nullptr, 0, *L);
 }
 
 int batch_finish(void *handle, double *seconds, std::vector<long> *stats_i,
@@ -1990,7 +2037,7 @@ static int run_chains(std::vector<fgx_system *> &sys, int ncalls, double *du_rec
 {
     Launch L;
     std::vector<int> nv(sys.size(), ncalls);
-    int rc = launch_chains(sys, nv, du_rec, acc_rec, cap, L);
+    int rc = launch_chains(sys, nv, nullptr, du_rec, acc_rec, cap, L);
     int rc2 = finish_chains(L, seconds, rc ? nullptr : stats_i, stats_d);
     return rc ? rc : rc2;
 }

@@ -778,7 +778,8 @@ extern "C" int fgx_create(const fgx_hamiltonian *h, const double box[3], int n, 
         (rc = dalloc(s, &s->d_tab, s->tab.size())) || (rc = dalloc(s, &s->d_tabf, s->tab.size())) ||
         (rc = dalloc(s, &s->d_type_dp, T)) || (rc = dalloc(s, &s->d_mask, 2 * (size_t)ng + 2)) ||
         (rc = dalloc(s, &s->d_out, 16)) || (rc = dalloc(s, &s->d_rng, 2 * 625)) ||
-        (rc = dalloc(s, &s->d_stats_i, 16)) || (rc = dalloc(s, &s->d_stats_d, 16)))
+        (rc = dalloc(s, &s->d_stats_i, 16)) || (rc = dalloc(s, &s->d_stats_d, 16)) ||
+        (rc = dalloc(s, &s->d_resume, 4)))
         return fail(rc);
     if (hipHostMalloc((void **)&s->h_out, sizeof(double) * 16, hipHostMallocDefault) !=
         hipSuccess) {
@@ -786,7 +787,8 @@ extern "C" int fgx_create(const fgx_hamiltonian *h, const double box[3], int n, 
         return fail(FGX_EHIP);
     }
     if ((rc = ensure_partials(s, 1024))) return fail(rc);
-    if (hipMemsetAsync(s->d_type + n, 0, sizeof(int) * PARTNER_PAD, s->stream) != hipSuccess ||
+    if (hipMemsetAsync(s->d_resume, 0, sizeof(int) * 4, s->stream) != hipSuccess ||
+        hipMemsetAsync(s->d_type + n, 0, sizeof(int) * PARTNER_PAD, s->stream) != hipSuccess ||
         hipMemsetAsync(s->d_pk + n, 0, sizeof(int4) * PARTNER_PAD, s->stream) != hipSuccess ||
         hipMemsetAsync(s->d_rf + n, 0, sizeof(float) * PARTNER_PAD, s->stream) != hipSuccess) {
         set_error("fgx_create: hipMemsetAsync failed");
@@ -851,7 +853,7 @@ extern "C" int fgx_destroy(fgx_system *s)
                     s->d_partial, s->d_out, s->d_idx, s->d_rng, s->d_stats_i, s->d_stats_d,
                     s->ew.kv, s->ew.aks, s->ew.kn, s->ew.Q, s->ew.Qt, s->ew.eff,
                     s->ew.cnt_dev, s->d_emat, s->ew.cheb, s->d_gvec, s->d_batch,
-                    s->d_bstat_i, s->d_bstat_d};
+                    s->d_bstat_i, s->d_bstat_d, s->d_resume};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (s->h_out) (void)hipHostFree(s->h_out);
