@@ -119,15 +119,16 @@ def du_error_vs_fp64(w, seed=11, ntrials=4000, ncalls=1):
 METRIC = "trial-move ΔU evals/s at stated ΔU error vs FP64; p50/p99 sweep latency"
 
 
-def timed_batch(fgx, systems, steps):
-    """`steps` launches of one Propagator::move() per chain; wall clock around
-    all of them (synchronised), device time of each from HIP events."""
+def timed_batch(fgx, systems, steps, ncalls=1):
+    """`steps` launches of `ncalls` Propagator::move() calls per chain; wall
+    clock around all of them (synchronised), device time of each from HIP
+    events."""
     import torch
     torch.cuda.synchronize()
     dev, trials, acc = [], 0, 0
     t0 = time.perf_counter()
     for _ in range(steps):
-        st = fgx.propagate_batch(systems, 1)
+        st = fgx.propagate_batch(systems, ncalls)
         dev.append(st["seconds"])
         trials += st["trials"]
         acc += st["accepted"]
@@ -425,15 +426,94 @@ def bench_serve(args, fgx, workloads):
     return line
 
 
+def bench_salt(args, fgx, workloads):
+    """The 500-ion salt solution (Debye-Hueckel 10 A, hard spheres 2 A, lB 7.1
+    A, L = 60 A, AtomicTranslation dp 2 A): batched FP64 chains (the
+    reference's own precision for this system) and mixed chains; the
+    mixed path's acceptance-mismatch rate against FP64 on identical RNG
+    streams, and <U> and g(r) of the two ensembles compared within
+    statistical error."""
+    from beofaunus_amd import analysis
+    import torch
+    R = args.replicas or 256
+    sweeps = args.sweeps_per_step
+    res = {}
+    for key, prec in (("fp64", fgx.FP64), ("mixed", fgx.MIXED)):
+        systems = []
+        for r in range(R):
+            w = workloads.salt_dh(seed=1 + r % 16, precision=prec)
+            s = workloads.make_system(w)
+            s.rng_seed(1000 + r)
+            s.propagator_init()
+            systems.append(s)
+        tuned = fgx.autotune_batch(systems, 1) if args.warmup > 0 else None
+        res[key] = timed_batch(fgx, systems, args.steps, sweeps)
+        res[key]["schedule"] = tuned
+        for s in systems:
+            s.close()
+    f, m = res["fp64"], res["mixed"]
+    line = base_line(args, f["value"], f["wall"], "f64",
+                     {"workload": "salt solution: 250 + 250 ions, L=60 A, hard spheres 2 A, "
+                                  "Debye-Hueckel 10 A, lB 7.1 A, AtomicTranslation dp 2 A "
+                                  "(%d sweeps per step)" % sweeps,
+                      "replicas": R, "trials_per_step": int(f["trials"] / args.steps),
+                      "precision": "fp64"})
+    line.update({
+        "pairs_per_s": f["value"] * 2 * 499,
+        "acceptance": f["accepted"] / max(1, f["trials"]),
+        "sweep_latency_ms": {k: v / sweeps for k, v in f["sweep_latency_ms"].items()},
+        "mixed": {"value": m["value"], "pairs_per_s": m["value"] * 2 * 499,
+                  "sweep_latency_ms": {k: v / sweeps for k, v in m["sweep_latency_ms"].items()}},
+        "schedule": {"fp64": f["schedule"], "mixed": m["schedule"]},
+    })
+    if not args.no_error_check:
+        w = workloads.salt_dh(seed=3)
+        errs, mism, compared = [], 0, 0
+        for c in range(8):
+            e = du_error_vs_fp64(w, seed=50 + c, ntrials=20000, ncalls=40)
+            compared += e["compared_trials"]
+            mism += e["first_accept_mismatch"] is not None
+            if e["max_abs_kT"] is not None:
+                errs.append(e["max_abs_kT"])
+        line["du_error_vs_fp64"] = {"max_abs_kT": max(errs), "compared_trials": compared,
+                                    "chains_diverged": mism,
+                                    "accept_mismatch_rate": mism / max(1, compared)}
+        out = {}
+        for prec in (fgx.FP64, fgx.MIXED):
+            systems = []
+            for c in range(64):
+                w = workloads.salt_dh(seed=300 + c, precision=prec)
+                s = workloads.make_system(w)
+                s.rng_seed(5000 + c)
+                s.propagator_init()
+                systems.append(s)
+            out[prec] = analysis.sample_chains(systems, 100, 40, 5, [(1, 2), (1, 1)], 0.1)
+            for s in systems:
+                s.close()
+        (ua, ca), (ub, cb) = out[fgx.FP64], out[fgx.MIXED]
+        line["ensemble_mixed_vs_fp64"] = analysis.compare(ua, ub, ca, cb, 0.1, 60.0 ** 3)
+        line["ensemble_mixed_vs_fp64"]["sample"] = ("64 chains per precision, 100 sweeps "
+                                                    "equilibration, 40 samples 5 sweeps apart")
+    if not args.no_cpu_baseline:
+        try:
+            line["cpu_baseline"] = cpu_baseline(workloads.salt_dh(seed=1))
+        except Exception as e:
+            line["cpu_baseline"] = {"error": str(e)}
+    torch.cuda.synchronize()
+    return line
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", choices=["ewald", "groups", "temper", "serve"], default="ewald",
-                    help="ewald: N=20000 Ewald electrolyte (headline); groups: 64 x 400-site "
-                         "rigid molecules + 8000 ions; temper: 48-replica parallel tempering; "
-                         "serve: 256-job stream through the job server")
+    ap.add_argument("--config", choices=["ewald", "salt", "groups", "temper", "serve"],
+                    default="ewald",
+                    help="ewald: N=20000 Ewald electrolyte (headline); salt: 500-ion DH salt; "
+                         "groups: 64 x 400-site rigid molecules + 8000 ions; temper: 48-replica "
+                         "parallel tempering; serve: 256-job stream through the job server")
+    ap.add_argument("--sweeps-per-step", type=int, default=20, help="salt: sweeps per launch")
     ap.add_argument("--jobs", type=int, default=256)
     ap.add_argument("--rate", type=float, default=50.0, help="job arrivals per second")
     ap.add_argument("--job-sweeps", type=int, default=4)
@@ -462,6 +542,8 @@ def main():
     from beofaunus_amd import fgx, workloads
     if args.config == "ewald":
         line = bench_ewald(args, fgx, workloads)
+    elif args.config == "salt":
+        line = bench_salt(args, fgx, workloads)
     elif args.config == "groups":
         line = bench_groups(args, fgx, workloads)
     elif args.config == "temper":

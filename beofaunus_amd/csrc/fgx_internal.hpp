@@ -59,8 +59,10 @@ struct EwaldDev {
 
 static constexpr int EW_CHEB_MAX = 48;
 // records past the end of the partner arrays (packed records, types, FP32
-// radii), zero-filled, so batched partner loads need no clamping
-static constexpr int PARTNER_PAD = 2 * 8 * 512;
+// radii), zero-filled, so batched partner loads need no clamping: the
+// double-buffered loop reads up to three batches (3 x unroll x block) beyond
+// the last partner
+static constexpr int PARTNER_PAD = 3 * 8 * 512;
 
 // device pointers of one particle vector (FP64 SoA)
 struct Vec64 {

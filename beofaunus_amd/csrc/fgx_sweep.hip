@@ -226,8 +226,9 @@ struct PartnerBatch {
     int4 P[U];
     float R[U];
     int T[U];
-    // rows base + u nthr; loads past n read the zero padding (at most two
-    // batches of 8 x 512 records beyond the end)
+    // rows base + u nthr; loads past n read the zero padding (the loop reads
+    // at most three batches beyond the end: PARTNER_PAD >= 3 U nthr, checked
+    // at launch)
     __device__ __forceinline__ void load(const ChainDev &C, int base, int nthr, int tid)
     {
 #pragma unroll
@@ -1917,6 +1918,9 @@ static int launch_chains(std::vector<fgx_system *> &sys, const std::vector<int> 
         chains[c].stats_i = s0->d_bstat_i + 8 * c;
         chains[c].stats_d = s0->d_bstat_d + 8 * c;
         nthr = std::max(nthr, s->launch_threads > 0 ? s->launch_threads : chain_block(s));
+        // the partner loop's prefetch must stay inside the zero padding
+        FGX_CHECK(3 * 8 * nthr <= PARTNER_PAD, FGX_EUNSUPPORTED,
+                  "device sweep: block size exceeds the partner-array padding");
         if (s->has_ewald) dyn = std::max(dyn, at_dyn_lds(s->ew.K));
         if (!chains[c].fx) fx = false;
     }

@@ -960,6 +960,87 @@ extern "C" int fgx_set_group_cm(fgx_system *s, int g, const double cm[3], const 
 
 static int check_which(fgx_system *s, int which) { return s && (which == 0 || which == 1); }
 
+// ------------------------------------------------------------- analysis
+
+static constexpr int RDF_TILE = 256;
+static constexpr int RDF_MAXBINS = 8192;
+
+// Analysis::AtomRDF::update (analysis.h:2119-2134): every pair i < j whose
+// types are {ta, tb} adds one count at bin int(r / dr + 0.5) (Table2D::round,
+// auxiliary.h:831), r the minimum-image distance (Cuboid::dist = sqrt of
+// sqdist).  Block (bx, by) takes partners j of tile bx against rows i of
+// tile by; counts go to an LDS histogram first, then to `hist`.
+__global__ __launch_bounds__(RDF_TILE) void k_rdf(Box b, PV v, int n, int ta, int tb, double dr,
+                                                  int nbins, unsigned long long *hist)
+{
+    __shared__ unsigned int lh[RDF_MAXBINS];
+    __shared__ double sx[RDF_TILE], sy[RDF_TILE], sz[RDF_TILE];
+    __shared__ int st[RDF_TILE];
+    for (int k = threadIdx.x; k < nbins; k += blockDim.x) lh[k] = 0;
+    const int i0 = blockIdx.y * RDF_TILE;
+    const int ni = min(RDF_TILE, n - i0);
+    if ((int)threadIdx.x < ni) {
+        const int i = i0 + threadIdx.x;
+        sx[threadIdx.x] = v.x[i]; sy[threadIdx.x] = v.y[i]; sz[threadIdx.x] = v.z[i];
+        st[threadIdx.x] = v.t[i];
+    }
+    __syncthreads();
+    const int j = blockIdx.x * RDF_TILE + threadIdx.x;
+    if (j < n && blockIdx.x >= blockIdx.y) {
+        const double bx = v.x[j], by = v.y[j], bz = v.z[j];
+        const int bt = v.t[j];
+        for (int k = 0; k < ni; ++k) {
+            const int i = i0 + k;
+            if (i >= j) break;
+            const int at = st[k];
+            if (!((at == ta && bt == tb) || (at == tb && bt == ta))) continue;
+            const double r = sqrt(sqdist_d(b, sx[k], sy[k], sz[k], bx, by, bz));
+            const int bin = (int)(r / dr + 0.5);
+            if (bin < nbins) atomicAdd(&lh[bin], 1u);
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < nbins; k += blockDim.x)
+        if (lh[k]) atomicAdd(&hist[k], (unsigned long long)lh[k]);
+}
+
+extern "C" int fgx_rdf(fgx_system *s, int which, int type_a, int type_b, double dr, int nbins,
+                       unsigned long long *counts)
+{
+    FGX_CHECK(check_which(s, which) && counts && dr > 0 && nbins > 0 && nbins <= RDF_MAXBINS,
+              FGX_EARG, "fgx_rdf: bad arguments");
+    // every minimum-image distance must land in a bin
+    const double rmax = 0.5 * std::sqrt(s->box.len[0] * s->box.len[0] +
+                                        s->box.len[1] * s->box.len[1] +
+                                        s->box.len[2] * s->box.len[2]);
+    FGX_CHECK((int)(rmax / dr + 0.5) < nbins, FGX_EARG,
+              "fgx_rdf: nbins too small for the box diagonal");
+    unsigned long long *d = nullptr;
+    FGX_HIP(hipMalloc(&d, sizeof(unsigned long long) * nbins));
+    int rc = FGX_OK;
+    if (hipMemsetAsync(d, 0, sizeof(unsigned long long) * nbins, s->stream) != hipSuccess) {
+        set_error("fgx_rdf: hipMemsetAsync failed");
+        rc = FGX_EHIP;
+    }
+    const int nt = (s->n + RDF_TILE - 1) / RDF_TILE;
+    if (!rc && s->n > 1) {
+        hipLaunchKernelGGL(k_rdf, dim3(nt, nt), dim3(RDF_TILE), 0, s->stream, s->box, pv(s, which),
+                           s->n, type_a, type_b, dr, nbins, d);
+        if (hipGetLastError() != hipSuccess) {
+            set_error("fgx_rdf: launch failed");
+            rc = FGX_EHIP;
+        }
+    }
+    if (!rc && (hipMemcpyAsync(counts, d, sizeof(unsigned long long) * nbins, hipMemcpyDeviceToHost,
+                               s->stream) != hipSuccess ||
+                hipStreamSynchronize(s->stream) != hipSuccess)) {
+        set_error("fgx_rdf: device error");
+        rc = FGX_EHIP;
+    }
+    (void)hipFree(d);
+    return rc;
+}
+
 extern "C" int fgx_i2i(fgx_system *s, int which, int i, int j, double *u)
 {
     FGX_CHECK(check_which(s, which) && u && i >= 0 && j >= 0 && i < s->n && j < s->n, FGX_EARG,

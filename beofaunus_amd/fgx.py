@@ -124,6 +124,7 @@ def lib():
         "fgx_set_launch": (I, [P, I, I]),
         "fgx_set_energy_matrix": (I, [P, I]),
         "fgx_chain_seconds": (I, [P, C.POINTER(C.c_double)]),
+        "fgx_rdf": (I, [P, I, I, I, C.c_double, I, C.POINTER(C.c_ulonglong)]),
         "fgx_server_create": (I, [C.c_size_t, I, C.c_double, pp]),
         "fgx_server_submit": (I, [P, C.POINTER(Job), _ip]),
         "fgx_server_step": (I, [P, _ip, _ip]),
@@ -218,6 +219,7 @@ class System:
                              type_radius=type_radius, type_dp=type_dp, ewald=ewald)
         self.ham = h
         self.n = n
+        self.box_len = [float(v) for v in box]
         self.groups = [dict(g) for g in groups]
         self.ng = len(groups)
         b = arr(box, 3)
@@ -317,6 +319,15 @@ class System:
 
     def set_launch(self, threads=0, unroll=0):
         _check(lib().fgx_set_launch(self.h, threads, unroll))
+
+    def rdf_counts(self, type_a, type_b, dr, which=0):
+        """Pair counts per bin int(r/dr + 0.5) (Analysis::AtomRDF sampling)."""
+        L = np.asarray(self.box_len, dtype=np.float64)
+        nb = int(0.5 * np.sqrt((L * L).sum()) / dr + 0.5) + 2
+        out = np.zeros(nb, dtype=np.uint64)
+        _check(lib().fgx_rdf(self.h, which, type_a, type_b, dr, nb,
+                             out.ctypes.data_as(C.POINTER(C.c_ulonglong))))
+        return out
 
     def chain_seconds(self):
         """Wall time this chain spent in the last device launch."""

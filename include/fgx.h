@@ -111,6 +111,13 @@ int fgx_external(fgx_system *s, int which, double *u);                 /* ewald.
 /* Energy::systemEnergy (energy.h:3486-3491 -> 169-180) */
 int fgx_system_energy(fgx_system *s, int which, double *u);
 
+/* Analysis::AtomRDF sampling (analysis.h:2117-2148): pair counts of types
+ * {type_a, type_b} over i < j at bin int(r/dr + 0.5), r the minimum-image
+ * distance; nbins must cover half the box diagonal.  Normalise as
+ * PairFunctionBase::normalize (src/analysis.cpp:190-206). */
+int fgx_rdf(fgx_system *s, int which, int type_a, int type_b, double dr, int nbins,
+            unsigned long long *counts);
+
 /* A Space::Change (space.h:287-318) restricted to one moved group:
  * `count` moved indices of group g, or count = 0 meaning the whole group
  * (TranslateRotate).  Trial coordinates must already be uploaded with

@@ -78,3 +78,31 @@ def test_mixed_forms_match_fp64(gpu, name):
         % (n, upto, err.max(), mism[0] if len(mism) else None))
     assert upto >= 200
     assert err.max() <= TOL_DU
+
+
+@pytest.mark.parametrize("threads,unroll", [(256, 4), (256, 8), (512, 4), (512, 8)])
+def test_every_schedule_small_system(gpu, threads, unroll):
+    """Every autotuner schedule on a system much smaller than one partner
+    batch (the loop's prefetch reads into the partner arrays' zero padding):
+    same accept/reject sequence as the FP64 path, dU within 1e-5 kT."""
+    from beofaunus_amd import workloads
+    w = workloads.salt_dh(n=300, L=50.0, seed=8)
+    say("[schedule] salt n=300, %d threads, unroll %d" % (threads, unroll))
+    rec = {}
+    for prec in (gpu.FP64, gpu.MIXED):
+        s = workloads.make_system(w, precision=prec)
+        s.set_launch(threads, unroll)
+        s.rng_seed(77)
+        s.propagator_init()
+        st, du, acc = s.propagate(4, record=1200)
+        rec[prec] = (du, acc)
+        s.close()
+    d64, a64 = rec[gpu.FP64]
+    dmx, amx = rec[gpu.MIXED]
+    assert len(d64) == len(dmx) == 1200
+    mism = np.nonzero(a64 != amx)[0]
+    upto = int(mism[0]) if len(mism) else 1200
+    fin = np.isfinite(d64[:upto])
+    assert np.array_equal(fin, np.isfinite(dmx[:upto]))
+    assert upto >= 200
+    assert np.abs(dmx[:upto][fin] - d64[:upto][fin]).max() <= TOL_DU
