@@ -36,6 +36,17 @@ HBM_PEAK_GBS = 8000.0
 FLOPS_PAIR_EWALD_LJ = 31                       # EwaldReal(kappa->0) + LJ-LB per pair
 FLOPS_K_PER_TRIAL = 32                         # exp(i k.r) update per k-vector
 FLOPS_PAIR_DHHS = 27                           # Debye-Hueckel + hard sphere per pair
+# Algorithmic bytes per trial: one partner record per partner (20 B FP32 SoA,
+# 40 B FP64), the moved particle twice, and per k-vector A_k + S(k) read
+# (24 B) plus S(k) read+write on an accepted move (32 B).
+BYTES_PARTNER = {"mixed": 20, "fp64": 40}
+BYTES_K_READ, BYTES_K_ACCEPT = 24, 32
+# HBM traffic per launch of the default Ewald step (256 replicas x N=20,000,
+# mixed, 512 threads, batch depth 4), from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE of bench.py --steps 1, scaled by the counters' reading for a
+# 1 GiB copy (FETCH x2.0, WRITE x1.0):
+# 1,024,463,194 KiB x 2 x 1024 + 432,350 KiB x 1024 bytes.
+MEASURED_TRAFFIC = {("ewald", 20000, 256, "mixed"): 1024463194 * 2 * 1024 + 432350 * 1024}
 
 
 def cpu_baseline(w, seconds_target=12.0):
@@ -116,6 +127,44 @@ def du_error_vs_fp64(w, seed=11, ntrials=4000, ncalls=1):
     return out
 
 
+def du_error_batched(w, chains=64, ncalls=2, cap=256, seed=11):
+    """Mixed-path Delta-U error against the FP64 device path for many chains
+    at once (one batched launch per precision, identical RNG streams per
+    chain): every chain compared trial by trial up to its first accept/reject
+    mismatch."""
+    from beofaunus_amd import fgx, workloads
+    rec = {}
+    for prec in (fgx.FP64, fgx.MIXED):
+        systems = []
+        for c in range(chains):
+            sy = workloads.make_system(w, precision=prec)
+            sy.rng_seed(seed + c)
+            sy.propagator_init()
+            systems.append(sy)
+        rec[prec] = fgx.propagate_batch_record(systems, ncalls, cap)[1:]
+        for sy in systems:
+            sy.close()
+    errs, compared, finite, diverged, overlap_mismatch = [], 0, 0, 0, 0
+    for c in range(chains):
+        d64, a64 = rec[fgx.FP64][0][c], rec[fgx.FP64][1][c]
+        dmx, amx = rec[fgx.MIXED][0][c], rec[fgx.MIXED][1][c]
+        n = min(len(d64), len(dmx))
+        mism = np.nonzero(a64[:n] != amx[:n])[0]
+        upto = int(mism[0]) if len(mism) else n
+        diverged += len(mism) > 0
+        fin = np.isfinite(d64[:upto]) & np.isfinite(dmx[:upto])
+        overlap_mismatch += int((np.isfinite(d64[:upto]) != np.isfinite(dmx[:upto])).sum())
+        compared += upto
+        finite += int(fin.sum())
+        errs.extend(np.abs(dmx[:upto][fin] - d64[:upto][fin]).tolist())
+    errs = np.array(errs)
+    return {"chains": chains, "compared_trials": compared, "finite_trials": finite,
+            "max_abs_kT": float(errs.max()) if len(errs) else None,
+            "p99_abs_kT": float(np.percentile(errs, 99)) if len(errs) else None,
+            "chains_diverged": diverged, "overlap_mismatch": overlap_mismatch,
+            "accept_mismatch_rate": diverged / max(1, compared)}
+
+
 METRIC = "trial-move ΔU evals/s at stated ΔU error vs FP64; p50/p99 sweep latency"
 
 
@@ -186,8 +235,14 @@ def bench_ewald(args, fgx, workloads):
     nk = 2242 if n == 20000 else None
     flops_trial = 2 * (n - 1) * FLOPS_PAIR_EWALD_LJ + (nk or 0) * FLOPS_K_PER_TRIAL
     trials_per_launch = t["trials"] / args.steps
-    achieved = flops_trial * trials_per_launch / float(np.mean(t["dev"])) / 1e12
+    launch_s = float(np.mean(t["dev"]))
+    tflops = flops_trial * trials_per_launch / launch_s / 1e12
     peak = PEAK["fp32" if prec == fgx.MIXED else "fp64"]
+    acc_rate = t["accepted"] / max(1, t["trials"])
+    k = nk or 0
+    bytes_trial = (BYTES_PARTNER[args.precision] * (n + 1) + BYTES_K_READ * k
+                   + acc_rate * BYTES_K_ACCEPT * k)
+    gbs = bytes_trial * trials_per_launch / launch_s / 1e9
     line = base_line(args, t["value"], t["wall"],
                      "f32 pairs + f64 accumulate" if prec == fgx.MIXED else "f64",
                      {"workload": "Ewald LJ+Coulomb electrolyte: N=%d, L=%.1f A, +-1e 50/50, LJ-LB "
@@ -200,10 +255,16 @@ def bench_ewald(args, fgx, workloads):
         "pairs_per_s": t["value"] * 2 * (n - 1),
         "acceptance": t["accepted"] / max(1, t["trials"]),
         "sweep_latency_ms": t["sweep_latency_ms"],
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
+        # one-vs-all streaming of the partner records: HBM roofline on the
+        # algorithmic bytes; the flop rate against the vector peak beside it
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs / HBM_PEAK_GBS,
+                     "traffic": MEASURED_TRAFFIC.get(("ewald", n, R, args.precision)),
                      "kernel": "k_sweep_at<%s>" % ("mixed" if prec == fgx.MIXED else "fp64"),
-                     "flops_per_trial": flops_trial},
+                     "bytes_per_trial": bytes_trial, "trials_per_launch": int(trials_per_launch),
+                     "launch_ms": 1e3 * launch_s,
+                     "valu": {"achieved": tflops, "peak": peak, "unit": "TFLOP/s",
+                              "frac": tflops / peak, "flops_per_trial": flops_trial}},
         "build_seconds": t_build,
         "schedule": tuned,
     })
@@ -252,6 +313,10 @@ def bench_groups(args, fgx, workloads):
     trials_per_launch = m["trials"] / args.steps
     pairs_exec = nsite * (n - nsite)              # cached path: trial state only
     achieved = pairs_exec * FLOPS_PAIR_DHHS * trials_per_launch / float(np.mean(m["dev"])) / 1e12
+    # each partner record is read once per trial and reused against all
+    # staged sites, so the kernel sits far right of the HBM ridge
+    bytes_trial = BYTES_PARTNER["mixed"] * (n - nsite) + 2 * BYTES_PARTNER["mixed"] * nsite
+    gbs = bytes_trial * trials_per_launch / float(np.mean(m["dev"])) / 1e9
     line = base_line(args, m["value"], m["wall"], "f32 pairs + f64 accumulate",
                      {"workload": "rigid g2g: 64 molecules x 400 sites (q -1/0/+1 p 0.2/0.6/0.2, "
                                   "radii U[1.5,3] A, 20 A spheres) + 8000 ions, L=250 A, "
@@ -269,14 +334,16 @@ def bench_groups(args, fgx, workloads):
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK["fp32"],
                      "unit": "TFLOP/s", "frac": achieved / PEAK["fp32"], "traffic": None,
                      "kernel": "k_sweep_tr<DH,HS,mixed> (cached)",
-                     "flops_per_trial": pairs_exec * FLOPS_PAIR_DHHS},
+                     "flops_per_trial": pairs_exec * FLOPS_PAIR_DHHS,
+                     "hbm": {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": gbs / HBM_PEAK_GBS, "bytes_per_trial": bytes_trial}},
     })
     if "mixed_uncached" in res:
         u = res["mixed_uncached"]
         line["mixed_uncached"] = {"value": u["value"], "sweep_latency_ms": u["sweep_latency_ms"]}
     if not args.no_error_check:
         try:
-            line["du_error_vs_fp64"] = du_error_vs_fp64(w, ntrials=512, ncalls=8)
+            line["du_error_vs_fp64"] = du_error_batched(w, chains=64, ncalls=2, cap=128)
         except Exception as e:
             line["du_error_vs_fp64"] = {"error": str(e)}
     if not args.no_cpu_baseline:

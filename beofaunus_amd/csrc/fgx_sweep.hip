@@ -1908,8 +1908,9 @@ static int launch_chains(std::vector<fgx_system *> &sys, const std::vector<int> 
             FGX_HIP(hipStreamSynchronize(st));   // host buffers are read at once
             s->state_on_dev = true;
         }
-        rc = prepare_chain(s, chains[c], st, c == 0 ? du_rec : nullptr,
-                           c == 0 ? acc_rec : nullptr, c == 0 ? cap : 0);
+        // chain c records into entries [c cap, (c + 1) cap) of the buffers
+        rc = prepare_chain(s, chains[c], st, du_rec ? du_rec + (size_t)c * cap : nullptr,
+                           acc_rec ? acc_rec + (size_t)c * cap : nullptr, du_rec ? cap : 0);
         if (rc) break;
         if (s->has_ewald) chains[c].ew_cnt = s->ew.cnt_dev;
         chains[c].ncalls = ncalls[c];
@@ -2187,6 +2188,47 @@ extern "C" int fgx_propagate_batch(fgx_system **systems, int nsys, int ncalls, f
     int rc = run_chains(sys, ncalls, nullptr, nullptr, 0, &secs, &si, &sd);
     for (int c = 0; c < nsys && !rc; ++c)
         collect_stats(sys[c], &si[8 * (size_t)c], &sd[8 * (size_t)c], st, secs);
+    return rc;
+}
+
+extern "C" int fgx_propagate_batch_record(fgx_system **systems, int nsys, int ncalls,
+                                          fgx_stats *st, double *du_out, int *acc_out, long cap,
+                                          long *counts)
+{
+    FGX_CHECK(systems && nsys > 0 && ncalls >= 0 && du_out && acc_out && cap > 0 && counts,
+              FGX_EARG, "fgx_propagate_batch_record: bad arguments");
+    std::vector<fgx_system *> sys(systems, systems + nsys);
+    for (auto *s : sys) FGX_CHECK(s, FGX_EARG, "fgx_propagate_batch_record: null system");
+    for (int a = 0; a < nsys; ++a)
+        for (int b = a + 1; b < nsys; ++b)
+            FGX_CHECK(sys[a] != sys[b], FGX_EARG,
+                      "fgx_propagate_batch_record: a system appears twice");
+    if (st) std::memset(st, 0, sizeof(*st));
+    const size_t tot = (size_t)nsys * (size_t)cap;
+    double *d_du = nullptr;
+    int *d_acc = nullptr;
+    FGX_HIP(hipMalloc(&d_du, sizeof(double) * tot));
+    if (hipMalloc(&d_acc, sizeof(int) * tot) != hipSuccess) {
+        (void)hipFree(d_du);
+        set_error("fgx_propagate_batch_record: out of device memory");
+        return FGX_EHIP;
+    }
+    double secs = 0;
+    std::vector<long> si;
+    std::vector<double> sd;
+    int rc = run_chains(sys, ncalls, d_du, d_acc, cap, &secs, &si, &sd);
+    for (int c = 0; c < nsys && !rc; ++c)
+        collect_stats(sys[c], &si[8 * (size_t)c], &sd[8 * (size_t)c], st, secs);
+    if (!rc) {
+        for (int c = 0; c < nsys; ++c) counts[c] = std::min(cap, si[8 * (size_t)c]);
+        if (hipMemcpy(du_out, d_du, sizeof(double) * tot, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(acc_out, d_acc, sizeof(int) * tot, hipMemcpyDeviceToHost) != hipSuccess) {
+            set_error("fgx_propagate_batch_record: copy of the records failed");
+            rc = FGX_EHIP;
+        }
+    }
+    (void)hipFree(d_du);
+    (void)hipFree(d_acc);
     return rc;
 }
 

@@ -121,6 +121,8 @@ def lib():
         "fgx_propagator_init": (I, [P]),
         "fgx_propagate": (I, [P, I, C.POINTER(Stats), _dp, _ip, C.c_long]),
         "fgx_propagate_batch": (I, [pp, I, I, C.POINTER(Stats)]),
+        "fgx_propagate_batch_record": (I, [pp, I, I, C.POINTER(Stats), _dp, _ip, C.c_long,
+                                           C.POINTER(C.c_long)]),
         "fgx_set_launch": (I, [P, I, I]),
         "fgx_set_energy_matrix": (I, [P, I]),
         "fgx_chain_seconds": (I, [P, C.POINTER(C.c_double)]),
@@ -386,6 +388,22 @@ def propagate_batch(systems, ncalls):
     st = Stats()
     _check(lib().fgx_propagate_batch(arr, len(systems), ncalls, C.byref(st)))
     return st.as_dict()
+
+
+def propagate_batch_record(systems, ncalls, cap):
+    """propagate_batch with every chain's per-trial record: returns the
+    statistics and lists of (dU, accept) arrays, one per chain."""
+    arr = (C.c_void_p * len(systems))(*[s.h for s in systems])
+    st = Stats()
+    R = len(systems)
+    du = np.zeros(R * cap)
+    acc = np.zeros(R * cap, dtype=np.int32)
+    cnt = (C.c_long * R)()
+    _check(lib().fgx_propagate_batch_record(arr, R, ncalls, C.byref(st), _d(du), _i(acc), cap,
+                                            cnt))
+    dus = [du[c * cap:c * cap + cnt[c]].copy() for c in range(R)]
+    accs = [acc[c * cap:c * cap + cnt[c]].copy() for c in range(R)]
+    return st.as_dict(), dus, accs
 
 
 class Temper:

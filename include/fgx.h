@@ -182,6 +182,11 @@ int fgx_autotune_batch(fgx_system **systems, int nsys, int ncalls, int *threads,
  * launch (one workgroup per chain).  Systems keep their own state.
  * ------------------------------------------------------------------------ */
 int fgx_propagate_batch(fgx_system **systems, int nsys, int ncalls, fgx_stats *st);
+/* the same with fgx_propagate's per-trial record for every chain: chain c
+ * writes its first min(cap, trials) dU values and accept flags to
+ * du_out[c cap ...], acc_out[c cap ...] and that count to counts[c]. */
+int fgx_propagate_batch_record(fgx_system **systems, int nsys, int ncalls, fgx_stats *st,
+                               double *du_out, int *acc_out, long cap, long *counts);
 /* wall time this chain spent inside the last launch that advanced it
  * (device clock; chains of a batch finish at different times).  No
  * counterpart in the reference (per-chain sweep latency). */
