@@ -58,7 +58,9 @@ def cpu_baseline(w, seconds_target=12.0):
     what = "single-group TranslateRotate" if w["move"]["kind"] == 2 else "single-particle"
     with tempfile.TemporaryDirectory() as td:
         cfg, state = workloads.write_reference_inputs(w, td, seed=7, peratom=False)
-        if os.path.exists(harness):
+        # the reference's NonbondedEwald cannot take a cut LJ: the C
+        # restatement stands in for it there
+        if os.path.exists(harness) and w["ref_kind"] != "ewaldljcut":
             # calibrate on a short run, then a sample sized for ~seconds_target
             probe_n = 200 if what == "single-particle" else 2
             probe = json.loads(subprocess.run([harness, "mctime", w["ref_kind"], cfg, state,
@@ -80,13 +82,13 @@ def cpu_baseline(w, seconds_target=12.0):
     # C restatement fallback
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_py import OracleSystem, lib as olib
-    kind = {"ewaldlj": "ewaldlj", "dhhs": "dhhs"}[w["ref_kind"]]
+    kind = w["ref_kind"]
     h = w["ham"]
     o = OracleSystem(kind, w["box"], w["xyz"], w["charge"], w["radius"], w["mw"], w["types"],
                      w["groups"], h["lB"], kappa=h.get("kappa", 0.0),
                      type_sigma=h["type_sigma"], type_eps=h["type_eps"],
                      type_charge=h["type_charge"], type_radius=h["type_radius"],
-                     type_dp=h["type_dp"], ewald=h.get("ewald"))
+                     type_dp=h["type_dp"], ewald=h.get("ewald"), lj_cut=h.get("lj_cut", 0.0))
     olib().or_add_move(o.h, 1, 0, 1.0, 0, 0, None, None, 0.0, 0.0, 1)
     o.seed(7)
     olib().or_propagator_init(o.h)
@@ -207,12 +209,33 @@ def base_line(args, value, wall, dtype, config):
     }
 
 
-def bench_ewald(args, fgx, workloads):
-    """Ewald LJ+Coulomb electrolyte, AtomicTranslation sweeps, R replicas."""
+def cells_vs_screen(w, seed=13, ntrials=4000):
+    """The cell-list sweep against the all-pairs screen of the same mixed
+    sweep (same Hamiltonian): accept sequences and the largest dU difference."""
+    from beofaunus_amd import fgx, workloads
+    rec = {}
+    for mode in (0, 1):
+        s = workloads.make_system(w, precision=fgx.MIXED)
+        s.set_cell_list(mode)
+        s.rng_seed(seed)
+        s.propagator_init()
+        st, du, acc = s.propagate(1, record=ntrials)
+        rec[mode] = (du[:ntrials], acc[:ntrials], s.cell_list())
+        s.close()
+    (d0, a0, _), (d1, a1, cl) = rec[0], rec[1]
+    return {"trials": int(len(d1)), "accept_sequences_equal": bool(np.array_equal(a0, a1)),
+            "max_abs_diff_kT": float(np.max(np.abs(d1 - d0))), "cells_per_axis": cl[0],
+            "cell_capacity": cl[1]}
+
+
+def bench_ewald(args, fgx, workloads, cut=False):
+    """Ewald LJ+Coulomb electrolyte, AtomicTranslation sweeps, R replicas.
+    cut=True: LJ as CutShift<LennardJonesLB> at rc (every pair term vanishes
+    beyond rc) and the cell-list sweep."""
     prec = fgx.MIXED if args.precision == "mixed" else fgx.FP64
     n = args.n or 20000
     L = 120.0 * (n / 20000.0) ** (1.0 / 3.0)
-    w = workloads.ewald_electrolyte(n=n, L=L, precision=prec)
+    w = workloads.ewald_electrolyte(n=n, L=L, precision=prec, lj_cut=cut)
     R = args.replicas or 256
     t_build = time.perf_counter()
     systems = []
@@ -228,38 +251,43 @@ def bench_ewald(args, fgx, workloads):
     for _ in range(max(0, args.warmup - 1)):
         fgx.propagate_batch(systems, 1)
     t = timed_batch(fgx, systems, args.steps)
+    cells = systems[0].cell_list()
     for s in systems:
         s.close()
     # algorithmic work per trial: 2 (N-1) pair evaluations (old and new, as the
-    # reference counts them) + K k-vector updates
+    # reference counts them) + K k-vector updates; with a cell list, the
+    # candidates of the 27 cells around the moved particle
     nk = 2242 if n == 20000 else None
-    flops_trial = 2 * (n - 1) * FLOPS_PAIR_EWALD_LJ + (nk or 0) * FLOPS_K_PER_TRIAL
+    partners = n - 1 if not cells[0] else 27.0 * n / cells[0] ** 3
+    flops_trial = 2 * partners * FLOPS_PAIR_EWALD_LJ + (nk or 0) * FLOPS_K_PER_TRIAL
     trials_per_launch = t["trials"] / args.steps
     launch_s = float(np.mean(t["dev"]))
     tflops = flops_trial * trials_per_launch / launch_s / 1e12
     peak = PEAK["fp32" if prec == fgx.MIXED else "fp64"]
     acc_rate = t["accepted"] / max(1, t["trials"])
     k = nk or 0
-    bytes_trial = (BYTES_PARTNER[args.precision] * (n + 1) + BYTES_K_READ * k
+    bytes_trial = (BYTES_PARTNER[args.precision] * (partners + 2) + BYTES_K_READ * k
                    + acc_rate * BYTES_K_ACCEPT * k)
     gbs = bytes_trial * trials_per_launch / launch_s / 1e9
     line = base_line(args, t["value"], t["wall"],
                      "f32 pairs + f64 accumulate" if prec == fgx.MIXED else "f64",
                      {"workload": "Ewald LJ+Coulomb electrolyte: N=%d, L=%.1f A, +-1e 50/50, LJ-LB "
-                                  "sigma 4 A eps 0.1 kT uncut, Ewald alpha 0.25/A rc 12 A |n|<=10 "
+                                  "sigma 4 A eps 0.1 kT %s, Ewald alpha 0.25/A rc 12 A |n|<=10 "
                                   "spherical, tinfoil, AtomicTranslation dp 1 A, S(k) refresh every "
-                                  "20000 accepts" % (n, L),
+                                  "20000 accepts" % (n, L, "cut and shifted at 12 A (cell list "
+                                                     "%d^3)" % cells[0] if cut else "uncut"),
                       "replicas": R, "trials_per_step": int(trials_per_launch),
                       "precision": args.precision})
     line.update({
-        "pairs_per_s": t["value"] * 2 * (n - 1),
+        "pairs_per_s": t["value"] * 2 * partners,
         "acceptance": t["accepted"] / max(1, t["trials"]),
         "sweep_latency_ms": t["sweep_latency_ms"],
         # one-vs-all streaming of the partner records: HBM roofline on the
         # algorithmic bytes; the flop rate against the vector peak beside it
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gbs / HBM_PEAK_GBS,
-                     "traffic": MEASURED_TRAFFIC.get(("ewald", n, R, args.precision)),
+                     "traffic": MEASURED_TRAFFIC.get(("ewald-cells" if cut else "ewald", n, R,
+                                                      args.precision)),
                      "kernel": "k_sweep_at<%s>" % ("mixed" if prec == fgx.MIXED else "fp64"),
                      "bytes_per_trial": bytes_trial, "trials_per_launch": int(trials_per_launch),
                      "launch_ms": 1e3 * launch_s,
@@ -273,6 +301,11 @@ def bench_ewald(args, fgx, workloads):
             line["du_error_vs_fp64"] = du_error_vs_fp64(w)
         except Exception as e:  # reported, never hidden
             line["du_error_vs_fp64"] = {"error": str(e)}
+        if cut and prec == fgx.MIXED:
+            try:
+                line["cells_vs_all_pairs"] = cells_vs_screen(w)
+            except Exception as e:
+                line["cells_vs_all_pairs"] = {"error": str(e)}
     if not args.no_cpu_baseline:
         try:
             line["cpu_baseline"] = cpu_baseline(w)
@@ -575,9 +608,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", choices=["ewald", "salt", "groups", "temper", "serve"],
+    ap.add_argument("--config", choices=["ewald", "cells", "salt", "groups", "temper", "serve"],
                     default="ewald",
-                    help="ewald: N=20000 Ewald electrolyte (headline); salt: 500-ion DH salt; "
+                    help="ewald: N=20000 Ewald electrolyte (headline); cells: the same with LJ "
+                         "cut and shifted at rc and a cell list; salt: 500-ion DH salt; "
                          "groups: 64 x 400-site rigid molecules + 8000 ions; temper: 48-replica "
                          "parallel tempering; serve: 256-job stream through the job server")
     ap.add_argument("--sweeps-per-step", type=int, default=20, help="salt: sweeps per launch")
@@ -609,6 +643,8 @@ def main():
     from beofaunus_amd import fgx, workloads
     if args.config == "ewald":
         line = bench_ewald(args, fgx, workloads)
+    elif args.config == "cells":
+        line = bench_ewald(args, fgx, workloads, cut=True)
     elif args.config == "salt":
         line = bench_salt(args, fgx, workloads)
     elif args.config == "groups":

@@ -16,7 +16,8 @@ SOURCES = ["fgx_system.hip", "fgx_sweep.hip", "fgx_temper.hip", "fgx_server.hip"
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
-    # packed FP32 instructions stay off (see DESIGN: numerics)
+    # packed FP32 instructions stay off: some of their forms have returned
+    # wrong results on this hardware without any error
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result", "-Wno-unused-variable",
 ]

@@ -63,6 +63,38 @@ __device__ __forceinline__ void vdist_d(const Box &b, double ax, double ay, doub
 // sigma^2 / 4 eps, Ewald effective-charge products) are read through `tab`.
 // -------------------------------------------------------------------------
 
+// LennardJonesMixed<LorentzBerthelot> (potentials.h:2035-2040), or
+// CutShift<LennardJonesLB> (potentials.h:1805-1811) when Pot::lj_rc2 > 0:
+// zero beyond the cutoff, minus the energy at the cutoff inside it.
+__device__ __forceinline__ double lj64(const Pot &P, const double *__restrict__ tab, double r2,
+                                       int k)
+{
+    const double s2 = tab[TAB_S2 * P.ntypes2 + k], e4 = tab[TAB_EPS4 * P.ntypes2 + k];
+    double x = s2 / r2;
+    x = x * x * x;
+    double u = e4 * (x * x - x);
+    if (P.lj_rc2 > 0.0) {
+        double xc = s2 / P.lj_rc2;
+        xc = xc * xc * xc;
+        u = (r2 > P.lj_rc2) ? 0.0 : u - e4 * (xc * xc - xc);
+    }
+    return u;
+}
+
+__device__ __forceinline__ float lj32(const Pot &P, const float *__restrict__ tabf, float r2, int k)
+{
+    const float s2 = tabf[TAB_S2 * P.ntypes2 + k], e4 = tabf[TAB_EPS4 * P.ntypes2 + k];
+    float x = s2 * __builtin_amdgcn_rcpf(r2);
+    x = x * x * x;
+    float u = e4 * (x * x - x);
+    if (P.lj_rc2f > 0.f) {
+        float xc = s2 * __builtin_amdgcn_rcpf(P.lj_rc2f);
+        xc = xc * xc * xc;
+        u = (r2 > P.lj_rc2f) ? 0.f : u - e4 * (xc * xc - xc);
+    }
+    return u;
+}
+
 // FP64 pair energy from squared distance (and the vector for Ewald, which
 // the reference evaluates with vdist).  Same formulas as the reference's
 // CombinedPairPotential (potentials.h:3768-3771): elec + short-range.
@@ -90,10 +122,7 @@ __device__ __forceinline__ double pair_u64(const Pot &P, const double *__restric
         double m = ra + rb;
         u2 = (r2 < m * m) ? __builtin_inf() : 0.0;
     } else if (P.sr == SR_LJ) {
-        int k = ta * P.ntypes + tb;
-        double x = tab[TAB_S2 * P.ntypes2 + k] / r2;
-        x = x * x * x;
-        u2 = tab[TAB_EPS4 * P.ntypes2 + k] * (x * x - x);
+        u2 = lj64(P, tab, r2, ta * P.ntypes + tb);
     }
     return u1 + u2;
 }
@@ -128,10 +157,7 @@ __device__ __forceinline__ float pair_u32(const Pot &P, const float *__restrict_
         float m = ra + rb;
         u2 = (r2 < m * m) ? __builtin_inff() : 0.f;
     } else if (P.sr == SR_LJ) {
-        int k = ta * P.ntypes + tb;
-        float x = tabf[TAB_S2 * P.ntypes2 + k] * __frcp_rn(r2);
-        x = x * x * x;
-        u2 = tabf[TAB_EPS4 * P.ntypes2 + k] * (x * x - x);
+        u2 = lj32(P, tabf, r2, ta * P.ntypes + tb);
     }
     return u1 + u2;
 }
@@ -164,10 +190,7 @@ __device__ __forceinline__ double pair_u64_t(const Pot &P, const double *__restr
         double m = ra + rb;
         u2 = (r2 < m * m) ? __builtin_inf() : 0.0;
     } else if constexpr (SR == SR_LJ) {
-        int k = ta * P.ntypes + tb;
-        double x = tab[TAB_S2 * P.ntypes2 + k] / r2;
-        x = x * x * x;
-        u2 = tab[TAB_EPS4 * P.ntypes2 + k] * (x * x - x);
+        u2 = lj64(P, tab, r2, ta * P.ntypes + tb);
     }
     return u1 + u2;
 }
@@ -203,10 +226,7 @@ __device__ __forceinline__ float pair_u32_t(const Pot &P, const float *__restric
         float m = ra + rb;
         u2 = (r2 < m * m * 1.000001f) ? __builtin_inff() : 0.f;
     } else if constexpr (SR == SR_LJ) {
-        int k = ta * P.ntypes + tb;
-        float x = tabf[TAB_S2 * P.ntypes2 + k] * __builtin_amdgcn_rcpf(r2);
-        x = x * x * x;
-        u2 = tabf[TAB_EPS4 * P.ntypes2 + k] * (x * x - x);
+        u2 = lj32(P, tabf, r2, ta * P.ntypes + tb);
     }
     if constexpr (ELEC == ELEC_NONE) return u2;
     else if constexpr (SR == SR_NONE) return u1;

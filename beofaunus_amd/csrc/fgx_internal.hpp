@@ -137,6 +137,18 @@ struct fgx_system {
     double *d_emat = nullptr;
     bool emat_on = true;                   // fgx_set_energy_matrix
     bool emat_valid = false;               // cleared whenever the accepted state or lB changes
+    // cell list of the mixed AtomicTranslation sweep when every pair term is
+    // cut at the Ewald rc (fgx_set_cell_list): mode 0 off, 1 automatic
+    // capacity, >= 2 that many records per cell
+    int cell_mode = 1;
+    int cell_nc = 0, cell_cap = 0;         // cells per axis, records per cell (allocated)
+    int cell_cap_min = 0;                  // automatic capacity after an overflow
+    int *d_cell_cnt = nullptr;             // nc^3 counts + overflow word
+    double4 *d_cell_rec = nullptr;         // nc^3 * cap records (FP64 x, y, z, charge)
+    int *d_cell_idx = nullptr;             // nc^3 * cap particle indices
+    int *d_cell_slot = nullptr;            // n: cell * cap + slot
+    size_t cell_bytes = 0;
+    bool cell_valid = false;               // cleared whenever the accepted state changes
 };
 
 namespace fgx {

@@ -83,6 +83,20 @@ struct ChainDev {
     const double *ew_cheb;
     int ew_ncheb;
     double ew_cheb_t;             // 2 / xmax
+    double lj_rc2inv;             // 1 / CutShift cutoff^2 (0: uncut LJ)
+    // every pair term vanishes beyond the Ewald cutoff (CutShift LJ at rc):
+    // the mixed partner loop only screens for partners inside rc
+    int lj_screen;
+    // cell list of the mixed AtomicTranslation sweep (cubic box, lj_screen):
+    // nc^3 cells of side >= rc + the largest displacement per axis, each with
+    // `cell_cap` records {fixed x, y, z, particle}; counts live in LDS during
+    // a launch.  cell_nc == 0: no cell list.
+    int cell_nc, cell_cap;
+    int cell_type;                // partner types must be read (type tables not uniform)
+    int *cell_cnt;
+    double4 *cell_rec;            // FP64 {x, y, z, charge} per slot
+    int *cell_idx;                // particle per slot
+    int *cell_slot;               // per particle: cell * cap + slot
 };
 
 // ------------------------------------------------------------------------
@@ -158,6 +172,39 @@ __device__ __forceinline__ double erfc_series(const double *c, int n, double tsc
     return exp(-x * x) * fma(t, b1, c[0] - b2);
 }
 
+// Pair energy of the mixed path's FP64 evaluations for plain Ewald
+// (kappa -> 0): real-space erfc(alpha r) / r from the staged series inside
+// rc, plus the short-range term (LJ, or CutShift LJ zero beyond its cutoff
+// and shifted by its value there).  One division per pair: 1/r and the LJ
+// ratio come from 1/r^2.
+template <int SR>
+__device__ __forceinline__ double u_ewald64(const ChainDev &C, const double *__restrict__ tab,
+                                            const double *cheb, double r2, double qq, double aq,
+                                            double bq, double ar, double br, int at, int bt)
+{
+    const double rinv2 = 1.0 / r2;
+    double e = 0.0;
+    if (r2 <= C.pot.ew_rc2) {
+        const double r = sqrt(r2);
+        e = qq * erfc_series(cheb, C.ew_ncheb, C.ew_cheb_t, C.pot.ew_alpha * r) * (r * rinv2);
+    }
+    if constexpr (SR == SR_LJ) {
+        const int k = at * C.pot.ntypes + bt;
+        const double s2 = tab[TAB_S2 * C.pot.ntypes2 + k], e4 = tab[TAB_EPS4 * C.pot.ntypes2 + k];
+        double x = s2 * rinv2;
+        x = x * x * x;
+        double lj = e4 * (x * x - x);
+        if (C.pot.lj_rc2 > 0.0) {
+            double xc = s2 * C.lj_rc2inv;
+            xc = xc * xc * xc;
+            lj = (r2 > C.pot.lj_rc2) ? 0.0 : lj - e4 * (xc * xc - xc);
+        }
+        return e + lj;
+    } else {
+        return e + pair_u64_t<ELEC_NONE, SR>(C.pot, tab, r2, aq, bq, ar, br, at, bt);
+    }
+}
+
 // u(new, j) - u(old, j) in FP64 for the mixed path's exact pass.  As
 // du_pair64, except that a plain real-space Ewald term (kappa -> 0, where the
 // screened form of ewald.h:109 reduces to erfc(alpha r) / r) takes erfc from
@@ -179,15 +226,8 @@ __device__ __forceinline__ double du_pair64_mixed(const ChainDev &C, const doubl
             const double r2o = rx * rx + (ry * ry + rz * rz);
             const int k = at * C.pot.ntypes + bt;
             const double qq = aq * bq * tab[TAB_QEQE * C.pot.ntypes2 + k] * C.pot.lB;
-            auto u = [&](double r2) {
-                double e = 0.0;
-                if (r2 <= C.pot.ew_rc2) {
-                    const double r = sqrt(r2);
-                    e = qq * erfc_series(cheb, C.ew_ncheb, C.ew_cheb_t, C.pot.ew_alpha * r) / r;
-                }
-                return e + pair_u64_t<ELEC_NONE, SR>(C.pot, tab, r2, aq, bq, ar, br, at, bt);
-            };
-            return u(r2n) - u(r2o);
+            return u_ewald64<SR>(C, tab, cheb, r2n, qq, aq, bq, ar, br, at, bt) -
+                   u_ewald64<SR>(C, tab, cheb, r2o, qq, aq, bq, ar, br, at, bt);
         }
     }
     return du_pair64<ELEC, SR>(C, tab, nx, ny, nz, ox, oy, oz, aq, ar, at, j);
@@ -213,6 +253,12 @@ template <typename T>
 __device__ __forceinline__ T gload(const T *p, int i)
 {
     return ((const __attribute__((address_space(1))) T *)p)[i];
+}
+__device__ __forceinline__ double4 gload(const double4 *p, int i)
+{
+    typedef double v4d __attribute__((ext_vector_type(4)));
+    const v4d v = ((const __attribute__((address_space(1))) v4d *)p)[i];
+    return make_double4(v.x, v.y, v.z, v.w);
 }
 __device__ __forceinline__ int4 gload(const int4 *p, int i)
 {
@@ -277,10 +323,13 @@ __device__ __forceinline__ double pair_loop_mixed(const ChainDev &C, const float
     // two batches alternate: one is evaluated while the other's loads are in
     // flight (the loop body is unrolled twice so neither is copied)
     PartnerBatch<U, SR == SR_HS, NEED_T> A, B;
-    // FAST: one shared LJ parameter pair and no strong-pair test beyond rc
-    // (decided once per launch, so the unrolled body has no branches)
-    auto eval = [&](auto fast, const PartnerBatch<U, SR == SR_HS, NEED_T> &X, int base) {
-        constexpr bool FAST = decltype(fast)::value;
+    // mode 1 (FAST): one shared LJ parameter pair and no strong-pair test
+    // beyond rc; mode 2 (SCREEN): nothing beyond rc (CutShift LJ at rc), so
+    // the loop only finds the partners inside rc.  Decided once per launch,
+    // so the unrolled body has no branches.
+    auto eval = [&](auto mode, const PartnerBatch<U, SR == SR_HS, NEED_T> &X, int base) {
+        constexpr bool FAST = decltype(mode)::value >= 1;
+        constexpr bool SCREEN = decltype(mode)::value == 2;
         unsigned dm = 0;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -297,7 +346,10 @@ __device__ __forceinline__ double pair_loop_mixed(const ChainDev &C, const float
                 const float r2o = dx * dx + dy * dy + dz * dz;
                 float d;
                 bool bad;
-                if constexpr (ELEC == ELEC_EWALD) {
+                if constexpr (ELEC == ELEC_EWALD && SCREEN) {
+                    bad = fminf(r2n, r2o) <= C.fx_rc2;
+                    d = 0.f;
+                } else if constexpr (ELEC == ELEC_EWALD) {
                     // LJ only beyond rc (real-space Ewald is zero there);
                     // partners inside rc go to the FP64 pass
                     float s2 = C.lj_s2f, e4 = C.lj_eps4f;
@@ -376,21 +428,186 @@ __device__ __forceinline__ double pair_loop_mixed(const ChainDev &C, const float
         }
     };
     const int step = U * nthr;
-    auto run = [&](auto fast) {
+    auto run = [&](auto mode) {
         A.load(C, 0, nthr, tid);
         for (int base = 0; base < C.n; base += 2 * step) {
             B.load(C, base + step, nthr, tid);        // clamped: always in bounds
-            eval(fast, A, base);
+            eval(mode, A, base);
             A.load(C, base + 2 * step, nthr, tid);
-            if (base + step < C.n) eval(fast, B, base + step);
+            if (base + step < C.n) eval(mode, B, base + step);
         }
     };
-    if (FX && ELEC == ELEC_EWALD && C.lj_uniform && !C.lj_check)
-        run(std::true_type{});
+    if (FX && ELEC == ELEC_EWALD && C.lj_screen)
+        run(std::integral_constant<int, 2>{});
+    else if (FX && ELEC == ELEC_EWALD && C.lj_uniform && !C.lj_check)
+        run(std::integral_constant<int, 1>{});
     else
-        run(std::false_type{});
+        run(std::integral_constant<int, 0>{});
     nq = min(cnt, qcap);
     return acc;
+}
+
+// ------------------------------------------------------------------------
+// Cell list of the mixed AtomicTranslation sweep.  Cells split each axis of
+// the fixed-point coordinate range into nc equal parts: floor((x/L + 1/2) nc).
+// Two coordinates whose minimum-image distance is at most one cell side lie
+// in the same or in adjacent cells (periodically), so with a side of at least
+// rc plus the largest displacement of one trial per axis, the 27 cells around
+// the moved particle's old cell hold every partner inside rc of its old and
+// of its new position.
+// ------------------------------------------------------------------------
+// neighbour cells per half-wavefront: cell-list sweeps run 512 threads, whose
+// waves 1..7 hold 14 half-wavefronts for the 27 cells
+static constexpr int CELL_NB = 2;
+
+__device__ __forceinline__ int cell_axis(int32_t f, int nc)
+{
+    const uint32_t u = (uint32_t)f ^ 0x80000000u;
+    return (int)(((uint64_t)u * (uint64_t)(uint32_t)nc) >> 32);
+}
+
+__device__ __forceinline__ int cell_of(int32_t x, int32_t y, int32_t z, int nc)
+{
+    return cell_axis(x, nc) + nc * (cell_axis(y, nc) + nc * cell_axis(z, nc));
+}
+
+// u(new, j) - u(old, j) in FP64 from a cell record (FP64 position and
+// charge of j), as du_pair64_mixed computes it from the particle arrays;
+// zero when j lies beyond rc of both positions (every pair term is cut there)
+template <int ELEC, int SR>
+__device__ __forceinline__ double du_pair64_rec(const ChainDev &C, const double *__restrict__ tab,
+                                                const double *cheb, double nx, double ny,
+                                                double nz, double ox, double oy, double oz,
+                                                double aq, double ar, int at, double4 p, int j)
+{
+    double rx, ry, rz;
+    vdist_d(C.box, nx, ny, nz, p.x, p.y, p.z, rx, ry, rz);
+    const double r2n = rx * rx + (ry * ry + rz * rz);
+    vdist_d(C.box, ox, oy, oz, p.x, p.y, p.z, rx, ry, rz);
+    const double r2o = rx * rx + (ry * ry + rz * rz);
+    if (r2n > C.pot.ew_rc2 && r2o > C.pot.ew_rc2) return 0.0;
+    const int bt = C.cell_type ? gload(C.t, j) : at;
+    const double bq = p.w;
+    if (ELEC == ELEC_EWALD && C.ew_ncheb > 0) {
+        const int k = at * C.pot.ntypes + bt;
+        const double qq = aq * bq * tab[TAB_QEQE * C.pot.ntypes2 + k] * C.pot.lB;
+        return u_ewald64<SR>(C, tab, cheb, r2n, qq, aq, bq, ar, 0.0, at, bt) -
+               u_ewald64<SR>(C, tab, cheb, r2o, qq, aq, bq, ar, 0.0, at, bt);
+    }
+    return pair_u64_t<ELEC, SR>(C.pot, tab, r2n, aq, bq, ar, 0.0, at, bt) -
+           pair_u64_t<ELEC, SR>(C.pot, tab, r2o, aq, bq, ar, 0.0, at, bt);
+}
+
+// Sum over partners inside rc of u(new, j) - u(old, j), FP64 (every pair
+// term vanishes beyond rc).  Wavefront 0 is drawing the next trial, so the
+// half-wavefronts of the other waves take the 27 neighbour cells, cell
+// h, h + H, ... for half-wave h of H, one slot per lane; the first 32 slots
+// of all its cells are loaded before any arithmetic.  Fixed order: the sum
+// is deterministic.  Called by all threads of the block.
+template <int ELEC, int SR>
+__device__ __forceinline__ double cell_loop(const ChainDev &C, const double *__restrict__ tab,
+                                            const double *cheb, const int *lcnt, int i, int4 me,
+                                            double aq64, double ar64, int at, double tnx,
+                                            double tny, double tnz, double tox, double toy,
+                                            double toz)
+{
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    if (tid < 64) return 0.0;
+    const int hw = (tid >> 5) - 2, H = (nthr >> 5) - 2, l = tid & 31;
+    const int nc = C.cell_nc, cap = C.cell_cap;
+    const int cx = cell_axis(me.x, nc), cy = cell_axis(me.y, nc), cz = cell_axis(me.z, nc);
+    int cb[CELL_NB], nb[CELL_NB], jb[CELL_NB];
+    double4 pb[CELL_NB];
+#pragma unroll
+    for (int b = 0; b < CELL_NB; ++b) {
+        const int k = hw + b * H;
+        cb[b] = 0;
+        nb[b] = 0;
+        if (k < 27) {
+            int ax = cx + k % 3 - 1, ay = cy + (k / 3) % 3 - 1, az = cz + k / 9 - 1;
+            ax = ax < 0 ? nc - 1 : (ax >= nc ? 0 : ax);
+            ay = ay < 0 ? nc - 1 : (ay >= nc ? 0 : ay);
+            az = az < 0 ? nc - 1 : (az >= nc ? 0 : az);
+            cb[b] = (ax + nc * (ay + nc * az)) * cap;
+            nb[b] = lcnt[ax + nc * (ay + nc * az)];
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < CELL_NB; ++b) {
+        jb[b] = -1;
+        if (l < nb[b]) {
+            pb[b] = gload(C.cell_rec, cb[b] + l);
+            jb[b] = gload(C.cell_idx, cb[b] + l);
+        }
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int b = 0; b < CELL_NB; ++b)
+        if (jb[b] >= 0 && jb[b] != i)
+            acc += du_pair64_rec<ELEC, SR>(C, tab, cheb, tnx, tny, tnz, tox, toy, toz, aq64, ar64,
+                                           at, pb[b], jb[b]);
+    // cells holding more than 32 records
+#pragma unroll
+    for (int b = 0; b < CELL_NB; ++b)
+        for (int q = l + 32; q < nb[b]; q += 32) {
+            const int j = gload(C.cell_idx, cb[b] + q);
+            if (j != i)
+                acc += du_pair64_rec<ELEC, SR>(C, tab, cheb, tnx, tny, tnz, tox, toy, toz, aq64,
+                                               ar64, at, gload(C.cell_rec, cb[b] + q), j);
+        }
+    return acc;
+}
+
+// Builds a chain's cell list from its accepted vector (one block per chain;
+// cells from the fixed-point coordinates, records in FP64).  Slots follow particle order within each cell, so the list (and
+// the order of every later sum over it) is deterministic.  A cell that would
+// exceed `cell_cap` sets the overflow word cell_cnt[nc^3].
+__global__ __launch_bounds__(256) void k_cell_build(const ChainDev *__restrict__ chains)
+{
+    const ChainDev &C = chains[blockIdx.x];
+    if (C.cell_nc <= 0) return;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int nc = C.cell_nc, ncell = nc * nc * nc, cap = C.cell_cap;
+    extern __shared__ int bcnt[];
+    for (int c = tid; c < ncell; c += blockDim.x) bcnt[c] = 0;
+    __syncthreads();
+    if (tid < 64) {
+        int over = 0;
+        for (int j0 = 0; j0 < C.n; j0 += 64) {
+            const int j = j0 + lane;
+            int4 p = make_int4(0, 0, 0, 0);
+            int c = -1;
+            if (j < C.n) {
+                p = C.pk[j];
+                c = cell_of(p.x, p.y, p.z, nc);
+            }
+            // rank among lower lanes of this chunk in the same cell
+            int below = 0, above = 0;
+            for (int l = 0; l < 64; ++l) {
+                const int o = __shfl(c, l, 64);
+                below += (l < lane && o == c) ? 1 : 0;
+                above += (l > lane && o == c) ? 1 : 0;
+            }
+            int slot = 0;
+            if (c >= 0) slot = bcnt[c] + below;
+            wave_sync_lds();
+            if (c >= 0) {
+                if (slot < cap) {
+                    C.cell_rec[c * cap + slot] = make_double4(C.x[j], C.y[j], C.z[j], C.q[j]);
+                    C.cell_idx[c * cap + slot] = j;
+                    C.cell_slot[j] = c * cap + slot;
+                } else {
+                    over = 1;
+                }
+                if (above == 0) bcnt[c] = slot + 1;   // last of its cell in this chunk
+            }
+            wave_sync_lds();
+        }
+        over = __ballot(over) != 0ull;
+        if (lane == 0) C.cell_cnt[ncell] = over;
+    }
+    __syncthreads();
+    for (int c = tid; c < ncell; c += blockDim.x) C.cell_cnt[c] = min(bcnt[c], cap);
 }
 
 // ------------------------------------------------------------------------
@@ -436,6 +653,7 @@ struct TrialAT {
     int collide;
     int pf;
     int ti;                       // type of i
+    int slot;                     // cell-list record of i (cell * cap + slot)
     int4 pk;                      // packed record of i (old fixed-point position)
     float rf;                     // FP32 radius of i
     double q64, r64, wq;          // FP64 charge, radius, Ewald weight q eff
@@ -465,7 +683,7 @@ __device__ __forceinline__ void kunpack(int v, int &nx, int &ny, int &nz)
 // A_k (double), packed n (int)
 __host__ __device__ inline size_t at_dyn_lds(int K) { return (size_t)K * (16 + 16 + 8 + 4); }
 
-template <int ELEC, int SR, bool MIX, int U, bool FX>
+template <int ELEC, int SR, bool MIX, int U, bool FX, bool CELL = false>
 __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restrict__ chains,
                                                          int ncalls)
 {
@@ -498,6 +716,9 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
     double2 *dS = S + (EW ? C.K : 0);
     double *Ak = (double *)(dS + (EW ? C.K : 0));
     int *kid = (int *)(Ak + (EW ? C.K : 0));
+    // cell counts (cell-list sweeps): LDS-resident for the launch
+    int *lcnt = kid + (EW ? C.K : 0);
+    __shared__ int lcell_ok;
 
     for (int k = tid; k < 624; k += nthr) {
         mtg[k] = C.rng[k];
@@ -519,6 +740,11 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
     if (tid == 0) {
         T[0].pf = T[1].pf = 0;
         T[0].acc = T[1].acc = 0;
+    }
+    if constexpr (CELL) {
+        const int ncell = C.cell_nc * C.cell_nc * C.cell_nc;
+        for (int c = tid; c < ncell; c += nthr) lcnt[c] = C.cell_cnt[c];
+        if (tid == 0) lcell_ok = (C.cell_nc > 0 && C.cell_cnt[ncell] == 0) ? 1 : 0;
     }
     const bool lds_tab = C.pot.ntypes2 <= LDS_TYPES2;
     // one trial's draws from the global stream, in the reference's order:
@@ -545,6 +771,7 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
         t.q64 = gload(C.q, i);
         t.r64 = gload(C.r, i);
         t.wq = EW ? t.q64 * (lds_grp ? leff[t.ti] : C.eff[t.ti]) : 0.0;
+        t.slot = (CELL && C.cell_nc > 0) ? gload(C.cell_slot, i) : 0;
     };
     if (lds_tab) {
         for (int k = tid; k < TAB_COUNT * C.pot.ntypes2; k += nthr) {
@@ -647,7 +874,7 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
                 }
                 if (tid == 0) {
                     if (!pf) {
-                        tr.i = c.i; tr.ti = c.ti; tr.rf = c.rf;
+                        tr.i = c.i; tr.ti = c.ti; tr.rf = c.rf; tr.slot = c.slot;
                         tr.q64 = c.q64; tr.r64 = c.r64; tr.wq = c.wq; tr.u = c.u;
                     }
                     tr.pf = 0;
@@ -676,6 +903,26 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
                     etab[which][axis * kcc1 + nn] = make_double2(cs, sn);
                 }
             }
+            // ---- draw-ahead: the next trial's random numbers and particle
+            // data (wave 0; the reference's draw order is unchanged).  With a
+            // cell list it overlaps the other waves' cell scan, otherwise the
+            // reciprocal sum and the FP64 pass after the pair loop.
+            auto draw_ahead = [&]() {
+                TrialAT nxt;
+                at_draw(nxt);
+                at_fetch(nxt);
+                if (tid == 0) {
+                    tn.pf = 1;
+                    tn.i = nxt.i; tn.ti = nxt.ti; tn.pk = nxt.pk; tn.rf = nxt.rf;
+                    tn.slot = nxt.slot;
+                    tn.q64 = nxt.q64; tn.r64 = nxt.r64; tn.wq = nxt.wq;
+                    tn.t3[0] = nxt.t3[0]; tn.t3[1] = nxt.t3[1]; tn.t3[2] = nxt.t3[2];
+                    tn.u = nxt.u;
+                    tn.ox = nxt.ox; tn.oy = nxt.oy; tn.oz = nxt.oz;
+                }
+            };
+            const bool use_cells = CELL && lcell_ok;
+            if (use_cells && wave0 && ahead) draw_ahead();
             // ---- pair part: sum_{j != i} u(i', j) - u(i, j)
             const int nw = nthr >> 6, wv = tid >> 6;
             const int qcap = DCAP / nw;
@@ -692,7 +939,10 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
                 int nq = 0;
                 int *dq = dlist + wv * qcap;
                 // (the fixed-unit form is only launched with LDS tables)
-                if (FX || lds_tab)
+                if (use_cells)
+                    acc = cell_loop<ELEC, SR>(C, ltab, lcheb, lcnt, i, tr.pk, tr.q64, tr.r64, tr.ti,
+                                              tr.nx, tr.ny, tr.nz, tr.ox, tr.oy, tr.oz);
+                else if (FX || lds_tab)
                     acc = pair_loop_mixed<ELEC, SR, U, FX>(C, ltabf, ltab, i, tr.pk, tr.rf, tr.ti,
                                                            tr.q64, tr.r64, tr.nx, tr.ny, tr.nz,
                                                            tr.ox, tr.oy, tr.oz, dq, qcap, nq);
@@ -704,22 +954,7 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
                 if ((tid & 63) == 0) nqs[wv] = nq;
             }
             if (EW || MIX) __syncthreads();   // etab and FP64 queues complete
-            // ---- draw-ahead: the next trial's random numbers and particle
-            // data (wave 0, while the other wavefronts start the reciprocal
-            // sum and the FP64 pass; the reference's draw order is unchanged)
-            if (wave0 && ahead) {
-                TrialAT nxt;
-                at_draw(nxt);
-                at_fetch(nxt);
-                if (tid == 0) {
-                    tn.pf = 1;
-                    tn.i = nxt.i; tn.ti = nxt.ti; tn.pk = nxt.pk; tn.rf = nxt.rf;
-                    tn.q64 = nxt.q64; tn.r64 = nxt.r64; tn.wq = nxt.wq;
-                    tn.t3[0] = nxt.t3[0]; tn.t3[1] = nxt.t3[1]; tn.t3[2] = nxt.t3[2];
-                    tn.u = nxt.u;
-                    tn.ox = nxt.ox; tn.oy = nxt.oy; tn.oz = nxt.oz;
-                }
-            }
+            if (!use_cells && wave0 && ahead) draw_ahead();
             // FP64 work of the mixed path goes to the last F wavefronts, the
             // reciprocal-space sum to the others, so the two run side by side
             int Qn = 0, F = 0;
@@ -807,8 +1042,42 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
                 if (tid == 0) { sst[4] += du; sst[3] += du; }
                 if (tid == 0) {   // AtomicTranslation::_acceptMove (move.h:866-877)
                     C.x[i] = tr.nx; C.y[i] = tr.ny; C.z[i] = tr.nz;
-                    C.pk_w[i] = make_int4(to_fixed(tr.nx, C.box.inv[0]), to_fixed(tr.ny, C.box.inv[1]),
-                                          to_fixed(tr.nz, C.box.inv[2]), tr.pk.w);
+                    const int4 np = make_int4(to_fixed(tr.nx, C.box.inv[0]),
+                                              to_fixed(tr.ny, C.box.inv[1]),
+                                              to_fixed(tr.nz, C.box.inv[2]), tr.pk.w);
+                    C.pk_w[i] = np;
+                    if (CELL && lcell_ok) {
+                        // the moved record changes in place, or moves to its
+                        // new cell (the old cell's last record fills the hole)
+                        const int nc = C.cell_nc, cap = C.cell_cap;
+                        const int co = cell_of(tr.pk.x, tr.pk.y, tr.pk.z, nc);
+                        const int cn = cell_of(np.x, np.y, np.z, nc);
+                        // (slots are fetched with the trial; a prefetched
+                        // next trial's slot is corrected here when it moves)
+                        const int so = tr.slot;
+                        const double4 rec = make_double4(tr.nx, tr.ny, tr.nz, tr.q64);
+                        if (co == cn) {
+                            C.cell_rec[so] = rec;
+                        } else if (lcnt[cn] >= cap) {
+                            // full cell: the chain continues on the all-pairs screen
+                            lcell_ok = 0;
+                        } else {
+                            const int last = co * cap + (--lcnt[co]);
+                            if (so != last) {
+                                const double4 m = C.cell_rec[last];
+                                const int mj = C.cell_idx[last];
+                                C.cell_rec[so] = m;
+                                C.cell_idx[so] = mj;
+                                C.cell_slot[mj] = so;
+                                if (tn.pf && tn.i == mj) tn.slot = so;
+                            }
+                            const int sn = cn * cap + (lcnt[cn]++);
+                            C.cell_rec[sn] = rec;
+                            C.cell_idx[sn] = i;
+                            C.cell_slot[i] = sn;
+                            if (tn.pf && tn.i == i) tn.slot = sn;
+                        }
+                    }
                 }
                 if constexpr (EW) {
                     if (tid == 0) {
@@ -915,12 +1184,20 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
         C.stats_i[3] = n_ref;
         C.stats_i[4] = wall_clock64() - t_start;
         C.stats_i[5] = calls_done;
+        C.stats_i[6] = (CELL && C.cell_nc > 0 && !lcell_ok) ? 1 : 0;
         C.stats_d[0] = sst[3];
         C.stats_d[1] = sst[4];
         if (EW) *C.ew_cnt = ew_cnt;
     }
     if constexpr (EW) {
         for (int k = tid; k < C.K; k += nthr) { C.Q[2 * k] = S[k].x; C.Q[2 * k + 1] = S[k].y; }
+    }
+    if constexpr (CELL) {
+        if (C.cell_nc > 0) {
+            const int ncell = C.cell_nc * C.cell_nc * C.cell_nc;
+            for (int c = tid; c < ncell; c += nthr) C.cell_cnt[c] = lcnt[c];
+            if (tid == 0) C.cell_cnt[ncell] = lcell_ok ? 0 : 1;
+        }
     }
 }
 
@@ -1549,6 +1826,82 @@ struct Prepared {
     int *d_gvec = nullptr;
 };
 
+// LDS bytes of the Ewald state of a mixed AtomicTranslation chain with a
+// cell list of nc^3 cells
+static size_t at_dyn_lds_cells(int K, int nc) { return at_dyn_lds(K) + 4 * (size_t)nc * nc * nc; }
+static constexpr size_t AT_DYN_LDS_MAX = 110 * 1024;
+
+// Cell list geometry and storage of a chain (cubic box, every pair term cut
+// at rc).  Cells of side >= rc + the largest single-trial displacement per
+// axis, at least 3 per axis; otherwise the chain keeps the all-pairs screen.
+static int setup_cells(fgx_system *s, ChainDev &C, const MoveDesc &mv)
+{
+    double dpmax = 0;
+    std::vector<char> present(s->pot.ntypes, 0);
+    for (int t : s->h_type) present[t] = 1;
+    for (int t = 0; t < s->pot.ntypes; ++t)
+        if (present[t]) dpmax = std::max(dpmax, std::fabs(s->type_dp[t]));
+    const double dmax = std::max({std::fabs(mv.dir[0]), std::fabs(mv.dir[1]), std::fabs(mv.dir[2])});
+    const double reach = (std::sqrt(s->pot.ew_rc2) + 0.5 * dpmax * dmax) * (1.0 + 1e-6);
+    const int nc = (int)std::floor(s->box.len[0] / reach);
+    const int K = s->has_ewald ? s->ew.K : 0;
+    if (nc < 3 || at_dyn_lds_cells(K, nc) > AT_DYN_LDS_MAX) return FGX_OK;
+    const int ncell = nc * nc * nc;
+    int cap = s->cell_mode >= 2 ? s->cell_mode : 0;
+    if (!cap) {
+        // twice the mean occupancy plus a margin, grown on overflow
+        const double mean = (double)s->n / ncell;
+        cap = std::max(32, ((int)std::ceil(2.0 * mean + 32.0) + 31) / 32 * 32);
+        cap = std::max(cap, s->cell_cap_min);
+    }
+    if (s->cell_nc != nc || s->cell_cap != cap || !s->d_cell_cnt) {
+        if (s->d_cell_cnt) (void)hipFree(s->d_cell_cnt);
+        if (s->d_cell_rec) (void)hipFree(s->d_cell_rec);
+        if (s->d_cell_idx) (void)hipFree(s->d_cell_idx);
+        if (s->d_cell_slot) (void)hipFree(s->d_cell_slot);
+        s->d_cell_cnt = nullptr; s->d_cell_rec = nullptr; s->d_cell_idx = nullptr;
+        s->d_cell_slot = nullptr;
+        s->device_bytes -= s->cell_bytes;
+        s->cell_bytes = 0;
+        s->cell_nc = s->cell_cap = 0;
+        FGX_HIP(hipMalloc(&s->d_cell_cnt, sizeof(int) * (ncell + 1)));
+        FGX_HIP(hipMalloc(&s->d_cell_rec, sizeof(double4) * (size_t)ncell * cap));
+        FGX_HIP(hipMalloc(&s->d_cell_idx, sizeof(int) * (size_t)ncell * cap));
+        FGX_HIP(hipMalloc(&s->d_cell_slot, sizeof(int) * (size_t)s->n));
+        s->cell_bytes = sizeof(int) * (ncell + 1) + (sizeof(double4) + sizeof(int)) * (size_t)ncell * cap +
+                        sizeof(int) * (size_t)s->n;
+        s->device_bytes += s->cell_bytes;
+        s->cell_nc = nc;
+        s->cell_cap = cap;
+        s->cell_valid = false;
+    }
+    C.cell_nc = nc;
+    C.cell_cap = cap;
+    C.cell_cnt = s->d_cell_cnt;
+    C.cell_rec = (double4 *)s->d_cell_rec;
+    C.cell_idx = s->d_cell_idx;
+    C.cell_slot = s->d_cell_slot;
+    // partner types are needed unless every present type pair shares the
+    // LJ and Ewald effective-charge tables
+    {
+        const int nt = s->pot.ntypes, nt2 = s->pot.ntypes2;
+        bool uni = true, first = true;
+        double ref[TAB_COUNT] = {0};
+        for (int a = 0; a < nt && uni; ++a)
+            for (int b2 = 0; b2 < nt && uni; ++b2) {
+                if (!present[a] || !present[b2]) continue;
+                for (int t = 0; t < TAB_COUNT; ++t) {
+                    const double v = s->tab[t * nt2 + a * nt + b2];
+                    if (first) ref[t] = v;
+                    else if (v != ref[t]) uni = false;
+                }
+                first = false;
+            }
+        C.cell_type = uni ? 0 : 1;
+    }
+    return FGX_OK;
+}
+
 static int prepare_chain(fgx_system *s, ChainDev &C, hipStream_t st, double *du_rec, int *acc_rec,
                          long cap)
 {
@@ -1621,7 +1974,9 @@ static int prepare_chain(fgx_system *s, ChainDev &C, hipStream_t st, double *du_
         C.ew_cheb = s->ew.cheb;
         C.ew_ncheb = s->ew.ncheb;
         C.ew_cheb_t = s->ew.ncheb ? 2.0 / s->ew.cheb_xmax : 0.0;
+        C.lj_rc2inv = s->pot.lj_rc2 > 0.0 ? 1.0 / s->pot.lj_rc2 : 0.0;
     } else {
+        C.lj_rc2inv = 0.0;
         C.ew_cheb = nullptr;
         C.ew_ncheb = 0;
         C.ew_cheb_t = 0.0;
@@ -1653,8 +2008,18 @@ static int prepare_chain(fgx_system *s, ChainDev &C, hipStream_t st, double *du_
     const bool dh_like = (s->pot.sr == SR_NONE || s->pot.sr == SR_HS) &&
                          (s->pot.elec == ELEC_NONE || s->pot.elec == ELEC_COULOMB ||
                           s->pot.elec == ELEC_DH);
+    // CutShift LJ: the fixed-unit Ewald form takes it only at the Ewald rc,
+    // where every pair term vanishes beyond rc (the loop then only screens)
+    const bool lj_at_rc = s->pot.lj_rc2 > 0.0 && s->pot.lj_rc2 == s->pot.ew_rc2;
     const bool ew_lj = s->pot.elec == ELEC_EWALD && s->pot.sr == SR_LJ &&
-                       s->pot.ntypes2 <= LDS_TYPES2;
+                       s->pot.ntypes2 <= LDS_TYPES2 && (s->pot.lj_rc2 == 0.0 || lj_at_rc);
+    C.lj_screen = 0;
+    C.cell_nc = C.cell_cap = 0;
+    C.cell_cnt = nullptr;
+    C.cell_rec = nullptr;
+    C.cell_idx = nullptr;
+    C.cell_slot = nullptr;
+    C.cell_type = 1;
     if (cubic && (dh_like || ew_lj)) {
         const double unit = s->box.len[0] / 4294967296.0;
         C.fx = 1;
@@ -1686,7 +2051,12 @@ static int prepare_chain(fgx_system *s, ChainDev &C, hipStream_t st, double *du_
             C.lj_s2f = (float)(s2u / (unit * unit));
             C.lj_eps4f = (float)e4u;
             C.lj_check = bound < 0.5 * s->pot.exact_f ? 0 : 1;
+            C.lj_screen = lj_at_rc ? 1 : 0;
         }
+    }
+    if (C.lj_screen && mv.type == MOVE_AT && s->pot.precision == FGX_MIXED && s->cell_mode != 0) {
+        int rc = setup_cells(s, C, mv);
+        if (rc) return rc;
     }
     return FGX_OK;
 }
@@ -1739,22 +2109,29 @@ static int init_mass_centres(fgx_system *s)
     return FGX_OK;
 }
 
-template <int ELEC, int SR, bool MIX, int U, bool FX>
+template <int ELEC, int SR, bool MIX, int U, bool FX, bool CELL = false>
 static void launch_at_f(hipStream_t st, const ChainDev *d_chains, int nchains, int nthr,
                         size_t dyn, int ncalls)
 {
     if (dyn > 0)
-        (void)hipFuncSetAttribute((const void *)k_sweep_at<ELEC, SR, MIX, U, FX>,
+        (void)hipFuncSetAttribute((const void *)k_sweep_at<ELEC, SR, MIX, U, FX, CELL>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
-    hipLaunchKernelGGL((k_sweep_at<ELEC, SR, MIX, U, FX>), dim3(nchains), dim3(nthr), dyn, st,
-                       d_chains, ncalls);
+    hipLaunchKernelGGL((k_sweep_at<ELEC, SR, MIX, U, FX, CELL>), dim3(nchains), dim3(nthr), dyn,
+                       st, d_chains, ncalls);
 }
 
 template <int ELEC, int SR, bool MIX, int U>
 static void launch_at_u(hipStream_t st, const ChainDev *d_chains, int nchains, int nthr,
-                        size_t dyn, int ncalls, bool fx)
+                        size_t dyn, int ncalls, bool fx, bool cell)
 {
     if constexpr (MIX && at_fx_ok<ELEC, SR>()) {
+        if constexpr (ELEC == ELEC_EWALD && SR == SR_LJ && U == 4) {
+            // chains with a cell list (others in the batch keep the screen)
+            if (fx && cell) {
+                launch_at_f<ELEC, SR, MIX, U, true, true>(st, d_chains, nchains, nthr, dyn, ncalls);
+                return;
+            }
+        }
         if (fx) {
             launch_at_f<ELEC, SR, MIX, U, true>(st, d_chains, nchains, nthr, dyn, ncalls);
             return;
@@ -1764,21 +2141,24 @@ static void launch_at_u(hipStream_t st, const ChainDev *d_chains, int nchains, i
 }
 
 // partner-loop unroll depth: the autotuned knob of the mixed path (the FP64
-// path keeps a plain strided loop)
+// path keeps a plain strided loop; cell-list launches use depth 4 for the
+// all-pairs screen they fall back to)
 template <int ELEC, int SR, bool MIX>
 static void launch_at_k(hipStream_t st, const ChainDev *d_chains, int nchains, int nthr,
-                        size_t dyn, int ncalls, int unroll, bool fx)
+                        size_t dyn, int ncalls, int unroll, bool fx, bool cell)
 {
-    if (!MIX || unroll <= 4) launch_at_u<ELEC, SR, MIX, 4>(st, d_chains, nchains, nthr, dyn, ncalls, fx);
-    else launch_at_u<ELEC, SR, MIX, 8>(st, d_chains, nchains, nthr, dyn, ncalls, fx);
+    if (!MIX || unroll <= 4 || cell)
+        launch_at_u<ELEC, SR, MIX, 4>(st, d_chains, nchains, nthr, dyn, ncalls, fx, cell);
+    else
+        launch_at_u<ELEC, SR, MIX, 8>(st, d_chains, nchains, nthr, dyn, ncalls, fx, cell);
 }
 
 template <bool MIX>
 static int launch_at(const Pot &P, hipStream_t st, const ChainDev *d_chains, int nchains, int nthr,
-                     size_t dyn, int ncalls, int unroll, bool fx)
+                     size_t dyn, int ncalls, int unroll, bool fx, bool cell)
 {
 #define FGX_AT(E, S) \
-    if (P.elec == E && P.sr == S) { launch_at_k<E, S, MIX>(st, d_chains, nchains, nthr, dyn, ncalls, unroll, fx); }
+    if (P.elec == E && P.sr == S) { launch_at_k<E, S, MIX>(st, d_chains, nchains, nthr, dyn, ncalls, unroll, fx, cell); }
     FGX_AT(ELEC_NONE, SR_HS) else FGX_AT(ELEC_NONE, SR_LJ) else
     FGX_AT(ELEC_COULOMB, SR_NONE) else FGX_AT(ELEC_COULOMB, SR_HS) else FGX_AT(ELEC_COULOMB, SR_LJ) else
     FGX_AT(ELEC_DH, SR_NONE) else FGX_AT(ELEC_DH, SR_HS) else FGX_AT(ELEC_DH, SR_LJ) else
@@ -1867,6 +2247,7 @@ static int launch_chains(std::vector<fgx_system *> &sys, const std::vector<int> 
     int nthr = 0;
     size_t dyn = 0;
     bool fx = true;   // fixed-unit form only when every chain has it
+    bool cell = false;   // some chain sweeps with a cell list
     const int unroll = s0->launch_unroll > 0 ? s0->launch_unroll : 4;
     // batch buffers, kept by the first system
     if (s0->batch_cap < nc) {
@@ -1922,8 +2303,41 @@ static int launch_chains(std::vector<fgx_system *> &sys, const std::vector<int> 
         // the partner loop's prefetch must stay inside the zero padding
         FGX_CHECK(3 * 8 * nthr <= PARTNER_PAD, FGX_EUNSUPPORTED,
                   "device sweep: block size exceeds the partner-array padding");
-        if (s->has_ewald) dyn = std::max(dyn, at_dyn_lds(s->ew.K));
+        if (s->has_ewald)
+            dyn = std::max(dyn, chains[c].cell_nc > 0 ? at_dyn_lds_cells(s->ew.K, chains[c].cell_nc)
+                                                      : at_dyn_lds(s->ew.K));
         if (!chains[c].fx) fx = false;
+        if (chains[c].cell_nc > 0) cell = true;
+    }
+    // the cell scan spreads 27 cells over waves 1..7 of a 512-thread block
+    if (cell) nthr = AT_THREADS;
+    if (!rc) {
+        // cell lists that no longer match the accepted state
+        std::vector<ChainDev> stale;
+        int maxcell = 0;
+        for (int c = 0; c < nc; ++c)
+            if (chains[c].cell_nc > 0 && !sys[c]->cell_valid) {
+                stale.push_back(chains[c]);
+                maxcell = std::max(maxcell, chains[c].cell_nc * chains[c].cell_nc * chains[c].cell_nc);
+            }
+        if (!stale.empty()) {
+            ChainDev *d_stale = nullptr;
+            FGX_HIP(hipMalloc(&d_stale, sizeof(ChainDev) * stale.size()));
+            FGX_HIP(hipMemcpyAsync(d_stale, stale.data(), sizeof(ChainDev) * stale.size(),
+                                   hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_cell_build, dim3((unsigned)stale.size()), dim3(256),
+                               sizeof(int) * (size_t)maxcell, st, d_stale);
+            hipError_t err = hipGetLastError();
+            if (err == hipSuccess) err = hipStreamSynchronize(st);
+            (void)hipFree(d_stale);
+            if (err != hipSuccess) {
+                set_error(std::string("cell list kernel failed: ") + hipGetErrorString(err));
+                rc = FGX_EHIP;
+            }
+            if (!rc)
+                for (int c = 0; c < nc; ++c)
+                    if (chains[c].cell_nc > 0) sys[c]->cell_valid = true;
+        }
     }
     if (!rc) {
         // group-pair energy caches that no longer match the accepted state
@@ -1961,8 +2375,9 @@ static int launch_chains(std::vector<fgx_system *> &sys, const std::vector<int> 
         const int ncall0 = ncalls.empty() ? 0 : ncalls[0];
         if (s0->moves[0].type == MOVE_AT) {
             rc = s0->pot.precision == FGX_MIXED
-                     ? launch_at<true>(s0->pot, st, d_chains, nc, nthr, dyn, ncall0, unroll, fx)
-                     : launch_at<false>(s0->pot, st, d_chains, nc, nthr, dyn, ncall0, unroll, fx);
+                     ? launch_at<true>(s0->pot, st, d_chains, nc, nthr, dyn, ncall0, unroll, fx, cell)
+                     : launch_at<false>(s0->pot, st, d_chains, nc, nthr, dyn, ncall0, unroll, fx,
+                                        false);
         } else {
             rc = s0->pot.precision == FGX_MIXED
                      ? launch_tr<true>(s0->pot, st, d_chains, nc, nthr, ncall0)
@@ -2124,6 +2539,13 @@ void collect_stats(fgx_system *s, const long *si, const double *sd, fgx_stats *s
         s->wall_khz = khz;
     }
     s->chain_seconds = s->wall_khz > 0 ? (double)si[4] / (1e3 * s->wall_khz) : 0.0;
+    if (s->cell_nc > 0 && si[6]) {
+        // a cell filled up: the chain finished the launch on the all-pairs
+        // screen; the next launch rebuilds the list, with twice the capacity
+        // unless a capacity was set
+        s->cell_valid = false;
+        if (s->cell_mode == 1) s->cell_cap_min = 2 * s->cell_cap;
+    }
     if (st) {
         st->trials += si[0];
         st->accepted += si[1];
@@ -2251,6 +2673,24 @@ extern "C" int fgx_set_energy_matrix(fgx_system *s, int on)
     FGX_CHECK(s, FGX_EARG, "fgx_set_energy_matrix: null system");
     s->emat_on = on != 0;
     s->emat_valid = false;
+    return FGX_OK;
+}
+
+extern "C" int fgx_set_cell_list(fgx_system *s, int mode)
+{
+    FGX_CHECK(s && mode >= 0 && (mode <= 1 || (mode >= 2 && mode <= 4096)), FGX_EARG,
+              "fgx_set_cell_list: mode is 0 (off), 1 (automatic) or a capacity 2..4096");
+    s->cell_mode = mode;
+    s->cell_cap_min = 0;
+    s->cell_valid = false;
+    return FGX_OK;
+}
+
+extern "C" int fgx_get_cell_list(const fgx_system *s, int *cells_per_axis, int *capacity)
+{
+    FGX_CHECK(s && cells_per_axis && capacity, FGX_EARG, "fgx_get_cell_list: bad arguments");
+    *cells_per_axis = s->cell_mode ? s->cell_nc : 0;
+    *capacity = s->cell_mode ? s->cell_cap : 0;
     return FGX_OK;
 }
 

@@ -698,6 +698,11 @@ extern "C" int fgx_create(const fgx_hamiltonian *h, const double box[3], int n, 
     P.g2g_rc2 = h->g2g_cut > 0 ? h->g2g_cut * h->g2g_cut : HUGE_VAL;
     P.lBf = (float)h->lB; P.kappaf = (float)h->kappa;
     P.exact_f = 0.5f;
+    // CutShift: rc2 = pow(cutoff, 2) (potentials.h:1786)
+    FGX_CHECK(h->lj_cut <= 0 || h->sr == SR_LJ, FGX_EARG,
+              "fgx_create: lj_cut needs the Lennard-Jones short-range term");
+    P.lj_rc2 = h->lj_cut > 0 ? std::pow(h->lj_cut, 2) : 0.0;
+    P.lj_rc2f = (float)P.lj_rc2;
     // tables: LennardJonesMixed<LorentzBerthelot>::init (potentials.h:2010-2019)
     const int T = h->ntypes;
     s->tab.assign((size_t)TAB_COUNT * T * T, 0.0);
@@ -853,7 +858,8 @@ extern "C" int fgx_destroy(fgx_system *s)
                     s->d_partial, s->d_out, s->d_idx, s->d_rng, s->d_stats_i, s->d_stats_d,
                     s->ew.kv, s->ew.aks, s->ew.kn, s->ew.Q, s->ew.Qt, s->ew.eff,
                     s->ew.cnt_dev, s->d_emat, s->ew.cheb, s->d_gvec, s->d_batch,
-                    s->d_bstat_i, s->d_bstat_d, s->d_resume};
+                    s->d_bstat_i, s->d_bstat_d, s->d_resume, s->d_cell_cnt, s->d_cell_rec,
+                    s->d_cell_idx, s->d_cell_slot};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (s->h_out) (void)hipHostFree(s->h_out);
@@ -869,7 +875,7 @@ extern "C" int fgx_set_positions(fgx_system *s, int which, int first, int count,
     FGX_CHECK(s && xyz && which >= 0 && which <= 2 && first >= 0 && count >= 0 &&
                   first + count <= s->n,
               FGX_EARG, "fgx_set_positions: bad arguments");
-    if (which != 1) s->emat_valid = false;
+    if (which != 1) s->emat_valid = s->cell_valid = false;
     std::vector<double> t(count);
     for (int w = 0; w < 2; ++w) {
         if (which != 2 && which != w) continue;
@@ -1361,7 +1367,7 @@ extern "C" int fgx_energy_change(fgx_system *s, const fgx_change *c, double *du)
 extern "C" int fgx_commit(fgx_system *s, const fgx_change *c, int accepted, double *drift)
 {
     FGX_CHECK(s, FGX_EARG, "fgx_commit: null system");
-    if (accepted) s->emat_valid = false;
+    if (accepted) s->emat_valid = s->cell_valid = false;
     if (int prc = pull_device_state(s)) return prc;
     s->state_on_dev = false;
     int nm;

@@ -29,6 +29,9 @@ struct Pot {
     float lBf, kappaf, ew_alphaf, ew_kappaf, ew_rc2f, ew_sf;
     int ew_plain;  // kappa*rc below 1e-12: screened real space reduces to erfc(ar)/r
     float exact_f; // mixed path: pairs with |u| above this (kT) are redone in FP64
+    // CutShift<LennardJonesLB> cutoff^2 (0: uncut LJ)
+    double lj_rc2;
+    float lj_rc2f;
 };
 
 // One registered move (Movebase::MolListData, move.h:305-330, plus the

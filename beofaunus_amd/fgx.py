@@ -36,6 +36,7 @@ class Hamiltonian(C.Structure):
         ("ewald_alpha", C.c_double), ("ewald_rc", C.c_double), ("ewald_kc", C.c_double),
         ("ewald_debyelength", C.c_double), ("ewald_eps_surf", C.c_double),
         ("ewald_spherical_sum", C.c_int), ("ewald_update_frequency", C.c_int),
+        ("lj_cut", C.c_double),
     ]
 
 
@@ -125,6 +126,8 @@ def lib():
                                            C.POINTER(C.c_long)]),
         "fgx_set_launch": (I, [P, I, I]),
         "fgx_set_energy_matrix": (I, [P, I]),
+        "fgx_set_cell_list": (I, [P, I]),
+        "fgx_get_cell_list": (I, [P, _ip, _ip]),
         "fgx_chain_seconds": (I, [P, C.POINTER(C.c_double)]),
         "fgx_rdf": (I, [P, I, I, I, C.c_double, I, C.POINTER(C.c_ulonglong)]),
         "fgx_server_create": (I, [C.c_size_t, I, C.c_double, pp]),
@@ -174,12 +177,14 @@ def device_info():
 
 def make_hamiltonian(arr, ntypes, *, elec, sr, nbkind=NB_PLAIN, precision=FP64, lB=7.0,
                      kappa=0.0, g2g_cut=0.0, type_sigma=None, type_eps=None, type_charge=None,
-                     type_radius=None, type_dp=None, ewald=None):
+                     type_radius=None, type_dp=None, ewald=None, lj_cut=0.0):
     """fgx_hamiltonian from keyword parameters; `arr(values, n)` must keep the
-    per-type arrays alive for as long as the structure is used."""
+    per-type arrays alive for as long as the structure is used.  `lj_cut` > 0
+    makes the LJ term CutShift<LennardJonesLB> at that cutoff."""
     h = Hamiltonian()
     h.elec, h.sr, h.nbkind, h.precision = elec, sr, nbkind, precision
     h.lB, h.kappa, h.g2g_cut = lB, kappa, g2g_cut
+    h.lj_cut = lj_cut or 0.0
     h.ntypes = ntypes
     h.type_sigma = _d(arr(type_sigma, ntypes))
     h.type_eps = _d(arr(type_eps, ntypes))
@@ -203,7 +208,7 @@ class System:
     def __init__(self, box, xyz, charge, radius, mw, types, groups, *, elec, sr,
                  nbkind=NB_PLAIN, precision=FP64, lB=7.0, kappa=0.0, g2g_cut=0.0,
                  type_sigma=None, type_eps=None, type_charge=None, type_radius=None,
-                 type_dp=None, ewald=None):
+                 type_dp=None, ewald=None, lj_cut=0.0):
         L = lib()
         n = len(charge)
         ntypes = len(type_sigma) if type_sigma is not None else int(np.max(types)) + 1
@@ -218,7 +223,8 @@ class System:
         h = make_hamiltonian(arr, ntypes, elec=elec, sr=sr, nbkind=nbkind, precision=precision,
                              lB=lB, kappa=kappa, g2g_cut=g2g_cut, type_sigma=type_sigma,
                              type_eps=type_eps, type_charge=type_charge,
-                             type_radius=type_radius, type_dp=type_dp, ewald=ewald)
+                             type_radius=type_radius, type_dp=type_dp, ewald=ewald,
+                             lj_cut=lj_cut)
         self.ham = h
         self.n = n
         self.box_len = [float(v) for v in box]
@@ -340,6 +346,18 @@ class System:
     def set_energy_matrix(self, on):
         """Group-pair energy cache of the mixed TranslateRotate sweep."""
         _check(lib().fgx_set_energy_matrix(self.h, 1 if on else 0))
+
+    def set_cell_list(self, mode):
+        """Cell list of the mixed AtomicTranslation sweep: 0 off, 1 automatic,
+        n >= 2 records per cell."""
+        _check(lib().fgx_set_cell_list(self.h, int(mode)))
+
+    def cell_list(self):
+        """(cells per axis, records per cell) of the last sweep; (0, 0) when
+        the sweep used no cell list."""
+        nc, cap = C.c_int(), C.c_int()
+        _check(lib().fgx_get_cell_list(self.h, C.byref(nc), C.byref(cap)))
+        return nc.value, cap.value
 
     def launch(self):
         t, u = C.c_int(), C.c_int()

@@ -120,10 +120,13 @@ def salt_dh(n=500, L=60.0, lB=7.1, debye=10.0, radius=2.0, dp=2.0, seed=1, T=T_D
 
 def ewald_electrolyte(n=20000, L=120.0, lB=7.1, sigma=4.0, eps_kt=0.1, rc=12.0, alpha=0.25,
                       kc=10.0, dp=1.0, update_frequency=20000, seed=2, T=T_DEFAULT,
-                      precision=fgx.MIXED):
+                      precision=fgx.MIXED, lj_cut=False):
     """Ewald LJ + Coulomb electrolyte (NonbondedEwald<LennardJonesLB>): +-1
     charges 50/50, Lorentz-Berthelot LJ, screened Ewald with kappa -> 0, tinfoil
-    boundary, spherical k-space cutoff."""
+    boundary, spherical k-space cutoff.  lj_cut=True makes the LJ term
+    CutShift<LennardJonesLB> at rc, so every pair term vanishes beyond rc
+    (the cell-list form; the reference's NonbondedEwald cannot take a cut LJ,
+    so this Hamiltonian has no reference run)."""
     rng = np.random.default_rng(seed)
     xyz = jittered_lattice(rng, n, L, 0.3)
     types = rng.permutation(np.array([1] * (n // 2) + [2] * (n - n // 2), dtype=np.int32))
@@ -141,9 +144,9 @@ def ewald_electrolyte(n=20000, L=120.0, lB=7.1, sigma=4.0, eps_kt=0.1, rc=12.0, 
         "ham": dict(elec=fgx.ELEC_EWALD, sr=fgx.SR_LJ, nbkind=fgx.NB_VECTOR, precision=precision,
                     lB=lB, type_sigma=[0.0, sigma, sigma], type_eps=[0.0, eps, eps],
                     type_charge=[0.0, 1.0, -1.0], type_radius=[0.0, r, r],
-                    type_dp=[0.0, dp, dp], ewald=ewald),
+                    type_dp=[0.0, dp, dp], ewald=ewald, lj_cut=rc if lj_cut else 0.0),
         "move": dict(kind=fgx.MOVE_AT, molid=0, peratom=True),
-        "ref_kind": "ewaldlj",
+        "ref_kind": "ewaldljcut" if lj_cut else "ewaldlj",
         "ref_nonbonded": {"epsr": epsr_for(lB, T), "ewald": ewald},
         "ref_atoms": {"T1p": {"q": 1.0, "sigma": sigma, "eps": eps_json, "dp": dp, "mw": 1.0},
                       "T2m": {"q": -1.0, "sigma": sigma, "eps": eps_json, "dp": dp, "mw": 1.0}},

@@ -51,13 +51,18 @@ struct GPUParams {
     double lB = 7.0, kappa = 0.0, g2g_cut = 0.0;
     double alpha = 0, rc = 0, kc = 0, debyelength = 0, eps_surf = 0;
     int spherical_sum = 1, update_frequency = -1;
+    double lj_cut = 0;   /* CutShift<LennardJonesLB> cutoff; 0 = uncut */
 
     /* energy.nonbonded section as read by Potential::Coulomb / DebyeHuckel
      * (src/potentials.cpp:357-363, 770-782), NonbondedCutg2g
      * (energy.h:1363-1369) and NonbondedEwald (ewald.h:507-527) */
-    static GPUParams fromJson(Tmjson &j, int elec, int sr, int nbkind, int precision = FGX_FP64)
+    static GPUParams fromJson(Tmjson &j, int elec, int sr, int nbkind, int precision = FGX_FP64,
+                              bool lj_cutshift = false)
     {
         GPUParams p;
+        /* Potential::CutShift<LennardJonesLB>: energy.nonbonded.cutoff
+         * (potentials.h:1785-1786) */
+        if (lj_cutshift) p.lj_cut = j["energy"]["nonbonded"].at("cutoff");
         p.elec = elec; p.sr = sr; p.nbkind = nbkind; p.precision = precision;
         auto nb = j["energy"]["nonbonded"];
         p.lB = pc::lB(nb.value("epsr", 80.0));
@@ -116,6 +121,7 @@ fgx_system *createSystem(Tspace &spc, const GPUParams &prm)
     h.ewald_alpha = prm.alpha; h.ewald_rc = prm.rc; h.ewald_kc = prm.kc;
     h.ewald_debyelength = prm.debyelength; h.ewald_eps_surf = prm.eps_surf;
     h.ewald_spherical_sum = prm.spherical_sum; h.ewald_update_frequency = prm.update_frequency;
+    h.lj_cut = prm.lj_cut;
     double box[3] = {spc.geo.len.x(), spc.geo.len.y(), spc.geo.len.z()};
     fgx_system *s = nullptr;
     check(fgx_create(&h, box, n, xyz.data(), q.data(), r.data(), mw.data(), t.data(),

@@ -68,6 +68,12 @@ typedef struct {
     /* NonbondedEwald (ewald.h:507-527) */
     double ewald_alpha, ewald_rc, ewald_kc, ewald_debyelength, ewald_eps_surf;
     int ewald_spherical_sum, ewald_update_frequency;
+    /* LJ as Potential::CutShift<LennardJonesLB> (potentials.h:1780-1812):
+     * zero beyond lj_cut (A), shifted by its value at lj_cut inside; <= 0
+     * keeps the uncut LennardJonesLB.  With FGX_ELEC_EWALD and lj_cut equal
+     * to ewald_rc every pair term vanishes beyond rc, and the mixed
+     * AtomicTranslation sweep uses a cell list (fgx_set_cell_list). */
+    double lj_cut;
 } fgx_hamiltonian;
 
 typedef struct {
@@ -169,6 +175,17 @@ int fgx_get_launch(fgx_system *s, int *threads, int *unroll);
  * are kept per group pair (ng <= 256) so only the trial state is evaluated.
  * On by default; any change of the accepted state from the host rebuilds it. */
 int fgx_set_energy_matrix(fgx_system *s, int on);
+/* Cell list of the mixed AtomicTranslation sweep, used when every pair term
+ * is cut at the Ewald rc (lj_cut == ewald_rc) in a cubic box with at least 3
+ * cells of side rc + the largest displacement per axis: each trial visits the
+ * 27 cells around the moved particle instead of all N partners.  The same
+ * Hamiltonian as the all-pairs sweep; the reference's NonbondedEwald has no
+ * cell list.  mode 0 = off, 1 = automatic capacity (default), 2..4096 =
+ * records per cell.  A chain whose cell fills up finishes its launch on the
+ * all-pairs screen and rebuilds the list (automatic mode: twice as large). */
+int fgx_set_cell_list(fgx_system *s, int mode);
+/* cells per axis and records per cell of the last sweep (0, 0: no list) */
+int fgx_get_cell_list(const fgx_system *s, int *cells_per_axis, int *capacity);
 /* Autotuner: times `ncalls` Propagator::move() calls of a probe chain under
  * every schedule and keeps the fastest on the probe (whose state advances). */
 int fgx_autotune(fgx_system *probe, int ncalls, int *threads, int *unroll, double *best_seconds);

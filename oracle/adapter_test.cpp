@@ -96,7 +96,7 @@ static std::unique_ptr<Move::Movebase<Tspace>> make_move(Tmjson &in, Tenergy &po
 
 template <class Eref>
 static int run(const std::string &kind, Tmjson &in, const char *state, int ncalls, int elec, int sr,
-               int nbkind)
+               int nbkind, bool cutshift = false)
 {
     bool tr = in.at("moves").count("moltransrot") > 0;
     std::printf("{\n");
@@ -122,7 +122,7 @@ static int run(const std::string &kind, Tmjson &in, const char *state, int ncall
     {
         Tspace spc(in);
         spc.load(state, Tspace::RESIZE);
-        auto prm = AMD::GPUParams::fromJson(in, elec, sr, nbkind);
+        auto prm = AMD::GPUParams::fromJson(in, elec, sr, nbkind, FGX_FP64, cutshift);
         Energy::GPUNonbonded<Tspace> pot(prm);
         Rec rec;
         std::unique_ptr<Move::Movebase<Tspace>> mv =
@@ -141,7 +141,7 @@ static int run(const std::string &kind, Tmjson &in, const char *state, int ncall
     {
         Tspace spc(in);
         spc.load(state, Tspace::RESIZE);
-        auto prm = AMD::GPUParams::fromJson(in, elec, sr, nbkind);
+        auto prm = AMD::GPUParams::fromJson(in, elec, sr, nbkind, FGX_FP64, cutshift);
         Energy::GPUNonbonded<Tspace> pot(prm);
         Move::GPUPropagator<Tspace> prop(in, pot, spc);
         prop.move(ncalls);
@@ -164,11 +164,13 @@ int main(int argc, char **argv)
     typedef Energy::Nonbonded<Tspace, CombinedPairPotential<Coulomb, HardSphere>> E_coulhs;
     typedef Energy::Nonbonded<Tspace, LennardJonesLB> E_lj;
     typedef Energy::NonbondedEwald<Tspace, LennardJonesLB> E_ewaldlj;
+    typedef Energy::Nonbonded<Tspace, CutShift<LennardJonesLB>> E_ljcut;
     if (kind == "dhhs") return run<E_dhhs>(kind, in, argv[3], ncalls, FGX_ELEC_DH, FGX_SR_HS, FGX_NB_PLAIN);
     if (kind == "dhhs_cut") return run<E_dhhs_cut>(kind, in, argv[3], ncalls, FGX_ELEC_DH, FGX_SR_HS, FGX_NB_CUTG2G);
     if (kind == "coulhs") return run<E_coulhs>(kind, in, argv[3], ncalls, FGX_ELEC_COULOMB, FGX_SR_HS, FGX_NB_PLAIN);
     if (kind == "lj") return run<E_lj>(kind, in, argv[3], ncalls, FGX_ELEC_NONE, FGX_SR_LJ, FGX_NB_PLAIN);
     if (kind == "ewaldlj") return run<E_ewaldlj>(kind, in, argv[3], ncalls, FGX_ELEC_EWALD, FGX_SR_LJ, FGX_NB_VECTOR);
+    if (kind == "ljcut") return run<E_ljcut>(kind, in, argv[3], ncalls, FGX_ELEC_NONE, FGX_SR_LJ, FGX_NB_PLAIN, true);
     std::fprintf(stderr, "unknown kind\n");
     return 2;
 }

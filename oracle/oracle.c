@@ -240,6 +240,7 @@ struct or_sys {
     double lB, kappa, g2g_rc2;
     int ntypes;
     double *s2, *eps4;   /* LennardJonesMixed tables (potentials.h:2010-2023) */
+    double lj_rc2;       /* CutShift cutoff^2 (potentials.h:1785-1786); 0 = uncut */
     double *tdp;
     or_ewald *ew;
     /* RNG: global slump and the Movebase-internal _slump (move.h:356-360) */
@@ -302,9 +303,23 @@ static inline double pair_u(const or_sys *s, const or_particle *a, const or_part
         break;
     }
     case OR_SR_LJ: {      /* LennardJonesMixed, potentials.h:2035-2040 */
-        double x = s->s2[a->id * s->ntypes + b->id] / r2;
+        const int k = a->id * s->ntypes + b->id;
+        if (s->lj_rc2 > 0) {
+            /* CutShift<LennardJonesLB> (potentials.h:1805-1811): zero beyond
+             * the cutoff, shifted by the energy at the cutoff inside it */
+            if (r2 > s->lj_rc2) {
+                u2 = 0;
+            } else {
+                double x = s->s2[k] / r2, xc = s->s2[k] / s->lj_rc2;
+                x = x * x * x;
+                xc = xc * xc * xc;
+                u2 = s->eps4[k] * (x * x - x) - s->eps4[k] * (xc * xc - xc);
+            }
+            break;
+        }
+        double x = s->s2[k] / r2;
         x = x * x * x;
-        u2 = s->eps4[a->id * s->ntypes + b->id] * (x * x - x);
+        u2 = s->eps4[k] * (x * x - x);
         break;
     }
     default: break;
@@ -858,6 +873,7 @@ or_sys *or_create(const or_params *prm, const double box[3], int n,
         q->charge = charge[i]; q->radius = radius[i]; q->mw = mw[i]; q->id = id[i];
         s->trial[i] = *q;
     }
+    s->lj_rc2 = prm->lj_cut > 0 ? pow(prm->lj_cut, 2) : 0.0;   /* potentials.h:1786 */
     s->ng = ng;
     s->g = calloc((size_t)(ng > 0 ? ng : 1), sizeof(or_group));
     for (int k = 0; k < ng; ++k) {

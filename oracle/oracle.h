@@ -65,6 +65,8 @@ typedef struct {
     /* Ewald */
     double alpha, rc, kc, debyelength, eps_surf;
     int spherical_sum, update_frequency;
+    /* Potential::CutShift<LennardJonesLB> cutoff (A); <= 0 = uncut LJ */
+    double lj_cut;
 } or_params;
 
 typedef struct or_sys or_sys;

@@ -12,7 +12,7 @@
 //   ref_harness energy <kind> <config.json> <statefile>
 //   ref_harness mc <kind> <config.json> <statefile> <ncalls>
 //   ref_harness mctime <kind> <config.json> <statefile> <ncalls>   (CPU baseline)
-// kind: dhhs | coulhs | lj | dh | ewaldlj | dhhs_cut | dhhs_vec
+// kind: dhhs | coulhs | lj | dh | ewaldlj | dhhs_cut | dhhs_vec | ljcut
 // Output: one JSON object on stdout, doubles printed with %.17g.
 
 #include <faunus/faunus.h>
@@ -294,6 +294,9 @@ int main(int argc, char **argv)
     typedef Energy::Nonbonded<Tspace, DebyeHuckel> E_dh;
     typedef Energy::Nonbonded<Tspace, LennardJonesLB> E_lj;
     typedef Energy::NonbondedEwald<Tspace, LennardJonesLB> E_ewaldlj;
+    // cut and shifted LJ (energy.nonbonded.cutoff); NonbondedEwald cannot
+    // take it (potentials.h:3770 hides the vector-distance overload)
+    typedef Energy::Nonbonded<Tspace, CutShift<LennardJonesLB>> E_ljcut;
     if (kind == "dhhs") { DISPATCH(E_dhhs); }
     if (kind == "dhhs_vec") { DISPATCH(E_dhhs_vec); }
     if (kind == "dhhs_cut") { DISPATCH(E_dhhs_cut); }
@@ -301,6 +304,7 @@ int main(int argc, char **argv)
     if (kind == "dh") { DISPATCH(E_dh); }
     if (kind == "lj") { DISPATCH(E_lj); }
     if (kind == "ewaldlj") { DISPATCH(E_ewaldlj); }
+    if (kind == "ljcut") { DISPATCH(E_ljcut); }
     std::fprintf(stderr, "unknown kind %s\n", kind.c_str());
     return 2;
 }

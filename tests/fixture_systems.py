@@ -1,6 +1,6 @@
 """Build HIP-engine systems from the golden fixtures (test helper)."""
 from beofaunus_amd import fgx
-from oracle_py import KINDS
+from oracle_py import KINDS, lj_cut_of
 
 ELEC = {"none": fgx.ELEC_NONE, "coulomb": fgx.ELEC_COULOMB, "dh": fgx.ELEC_DH,
         "ewald": fgx.ELEC_EWALD}
@@ -23,7 +23,7 @@ def fgx_system_from_fixture(fx, precision=fgx.FP64, moves=True, ewald_override=N
                    g2g_cut=nb.get("cutoff_g2g", 0.0),
                    type_sigma=e["type_sigma"], type_eps=e["type_eps"],
                    type_charge=e["type_charge"], type_radius=e["type_radius"],
-                   type_dp=e["type_dp"], ewald=ew)
+                   type_dp=e["type_dp"], ewald=ew, lj_cut=lj_cut_of(fx["kind"], nb))
     if moves:
         for mv in fx["moves"]:
             kind = fgx.MOVE_AT if mv["type"] == "AT" else fgx.MOVE_TR

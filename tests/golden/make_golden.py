@@ -152,7 +152,7 @@ def salt(name, kind, n, L, seed, ncalls, elec_cfg, dp=2.0, r=2.0):
     return build_fixture(name, kind, cfg, [L, L, L], parts, groups, seed, ncalls, moves)
 
 
-def lj_mix(name, n, L, seed, ncalls):
+def lj_mix(name, n, L, seed, ncalls, cutoff=None):
     rng = np.random.default_rng(seed)
     half = n // 2
     # sigma 3 and 4.5 A, eps 0.15 / 0.25 kJ/mol (Lorentz-Berthelot mixing)
@@ -165,12 +165,13 @@ def lj_mix(name, n, L, seed, ncalls):
                      "T2b": {"eps": 0.25, "sigma": 4.5, "dp": 1.0, "mw": 1.0}},
         "moleculelist": {"fluid": {"atoms": "T1a T2b", "atomic": True, "Ninit": 0}},
         "moves": {"atomtranslate": {"fluid": {"peratom": True}}},
-        "energy": {"nonbonded": {}},
+        "energy": {"nonbonded": {} if cutoff is None else {"cutoff": cutoff}},
     }
     parts = {"xyz": xyz, "charge": np.zeros(n), "radius": radii, "mw": np.ones(n), "id": ids}
     groups = [{"front": 0, "back": n - 1, "molid": 0, "molsize": 1}]
     moves = [{"type": "AT", "molid": 0, "prob": 1.0, "peratom": 1, "permol": 0}]
-    return build_fixture(name, "lj", cfg, [L, L, L], parts, groups, seed, ncalls, moves)
+    kind = "lj" if cutoff is None else "ljcut"
+    return build_fixture(name, kind, cfg, [L, L, L], parts, groups, seed, ncalls, moves)
 
 
 def ewald(name, n, L, seed, ncalls, eps_surf, update_frequency, alpha=0.35, rc=7.0, kc=4.0):
@@ -282,6 +283,7 @@ def main():
     salt("salt_coulhs", "coulhs", 60, 30.0, 13, 10, {"epsr": lB_epsr(7.1)})
     salt("salt_dh", "dh", 40, 20.0, 14, 8, dh)
     lj_mix("lj_mix", 50, 25.0, 15, 10)
+    lj_mix("lj_cutshift", 80, 26.0, 21, 10, cutoff=8.0)
     ewald("ewald_tinfoil", 40, 20.0, 16, 6, eps_surf=0.5, update_frequency=7)
     ewald("ewald_surface", 30, 18.0, 17, 4, eps_surf=1.0, update_frequency=-1, kc=3.0)
     molecules("mol_dhhs", "dhhs", 4, 12, 16, 60.0, 18, 12)

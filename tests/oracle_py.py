@@ -27,7 +27,15 @@ KINDS = {
     "dh": ("dh", "none", "plain"),
     "lj": ("none", "lj", "plain"),
     "ewaldlj": ("ewald", "lj", "vector"),
+    # LJ as Potential::CutShift<LennardJonesLB> at energy.nonbonded.cutoff
+    "ljcut": ("none", "lj", "plain"),
+    "ewaldljcut": ("ewald", "lj", "vector"),
 }
+
+
+def lj_cut_of(kind, nonbonded):
+    """CutShift cutoff of a fixture kind (0 = uncut LJ)."""
+    return float(nonbonded["cutoff"]) if kind.endswith("ljcut") else 0.0
 
 
 class OrParams(C.Structure):
@@ -41,6 +49,7 @@ class OrParams(C.Structure):
         ("alpha", C.c_double), ("rc", C.c_double), ("kc", C.c_double),
         ("debyelength", C.c_double), ("eps_surf", C.c_double),
         ("spherical_sum", C.c_int), ("update_frequency", C.c_int),
+        ("lj_cut", C.c_double),
     ]
 
 
@@ -126,7 +135,7 @@ class OracleSystem:
 
     def __init__(self, kind, box, xyz, charge, radius, mw, ids, groups, lB, kappa=0.0,
                  type_sigma=None, type_eps=None, type_charge=None, type_radius=None,
-                 type_dp=None, g2g_cut=0.0, ewald=None):
+                 type_dp=None, g2g_cut=0.0, ewald=None, lj_cut=0.0):
         L = lib()
         elec, sr, nb = KINDS[kind]
         self._keep = []
@@ -140,6 +149,7 @@ class OracleSystem:
         prm = OrParams()
         prm.elec, prm.sr, prm.nbkind = ELEC[elec], SR[sr], NB[nb]
         prm.lB, prm.kappa, prm.g2g_cut = lB, kappa, g2g_cut
+        prm.lj_cut = lj_cut
         prm.ntypes = T
         prm.sigma = dptr(arr(type_sigma))
         prm.eps = dptr(arr(type_eps))
@@ -181,7 +191,7 @@ class OracleSystem:
                    type_sigma=e["type_sigma"], type_eps=e["type_eps"],
                    type_charge=e["type_charge"], type_radius=e["type_radius"],
                    type_dp=e["type_dp"], g2g_cut=nb.get("cutoff_g2g", 0.0),
-                   ewald=nb.get("ewald"))
+                   ewald=nb.get("ewald"), lj_cut=lj_cut_of(fx["kind"], nb))
         sys_.add_moves(fx)
         return sys_
 

@@ -39,8 +39,9 @@ def write_inputs(fx, d):
     return cpath, spath
 
 
-@pytest.mark.parametrize("name", ["salt_dhhs", "salt_coulhs", "lj_mix", "ewald_tinfoil",
-                                  "ewald_surface", "mol_dhhs", "mol_dhhs_offcenter", "mol_dhhs_cut"])
+@pytest.mark.parametrize("name", ["salt_dhhs", "salt_coulhs", "lj_mix", "lj_cutshift",
+                                  "ewald_tinfoil", "ewald_surface", "mol_dhhs",
+                                  "mol_dhhs_offcenter", "mol_dhhs_cut"])
 def test_reference_moves_on_gpu_energy(name):
     import torch
     if not torch.cuda.is_available():

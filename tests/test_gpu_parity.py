@@ -107,8 +107,8 @@ def test_device_propagator_matches_reference(gpu, name):
     s.close()
 
 
-@pytest.mark.parametrize("name", ["salt_dhhs", "salt_coulhs", "lj_mix", "ewald_tinfoil",
-                                  "ewald_surface", "mol_dhhs", "mol_dhhs_cut"])
+@pytest.mark.parametrize("name", ["salt_dhhs", "salt_coulhs", "lj_mix", "lj_cutshift",
+                                  "ewald_tinfoil", "ewald_surface", "mol_dhhs", "mol_dhhs_cut"])
 def test_per_call_energy_change(gpu, name):
     """Host-driven drop-in path: the oracle proposes each trial; the GPU
     computes Energy::energyChange for it and commits the oracle's decision."""
@@ -191,8 +191,8 @@ def _oracle_update_change(o, mv):
                                   ia.ctypes.data_as(P))
 
 
-@pytest.mark.parametrize("name", ["salt_dhhs", "salt_coulhs", "lj_mix", "ewald_tinfoil",
-                                  "mol_dhhs", "mol_dhhs_cut"])
+@pytest.mark.parametrize("name", ["salt_dhhs", "salt_coulhs", "lj_mix", "lj_cutshift",
+                                  "ewald_tinfoil", "mol_dhhs", "mol_dhhs_cut"])
 def test_mixed_precision_error(gpu, name):
     """Mixed path (FP32 pairs on fixed-point coordinates, FP64 accumulation,
     FP64 S(k) and Metropolis): |dU - dU_FP64| <= 1e-5 kT while the two
