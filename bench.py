@@ -311,6 +311,21 @@ def bench_ewald(args, fgx, workloads, cut=False):
             line["cpu_baseline"] = cpu_baseline(w)
         except Exception as e:
             line["cpu_baseline"] = {"error": str(e)}
+    if not cut and args.config == "ewald" and prec == fgx.MIXED:
+        # the same electrolyte with every pair term cut at rc (CutShift LJ),
+        # swept with a cell list: a different Hamiltonian from the
+        # reference's uncut NonbondedEwald<LennardJonesLB>, reported beside it
+        sub = argparse.Namespace(**vars(args))
+        sub.no_cpu_baseline = True
+        c = bench_ewald(sub, fgx, workloads, cut=True)
+        line["cut_lj_cell_list"] = {k: c[k] for k in ("value", "ms_per_step", "sweep_latency_ms",
+                                                        "acceptance", "pairs_per_s")
+                                    if k in c}
+        line["cut_lj_cell_list"]["workload"] = c["config"]["workload"]
+        for k in ("du_error_vs_fp64", "cells_vs_all_pairs"):
+            if k in c:
+                line["cut_lj_cell_list"][k] = c[k]
+        line["cut_lj_cell_list"]["launch_ms"] = c["roofline"]["launch_ms"]
     return line
 
 

@@ -95,7 +95,8 @@ size_t estimate_bytes(const JobSpec &j)
 // kernel kind: chains of one kind share a launch
 long kind_of(const JobSpec &j)
 {
-    return ((((long)j.move_type * 4 + j.h.precision) * 8 + j.h.elec) * 8 + j.h.sr) * 4 + j.h.nbkind;
+    return (((((long)j.move_type * 4 + j.h.precision) * 8 + j.h.elec) * 8 + j.h.sr) * 4 +
+            j.h.nbkind) * 2 + (j.h.lj_cut > 0 ? 1 : 0);
 }
 
 }  // namespace

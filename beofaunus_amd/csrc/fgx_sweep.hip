@@ -158,50 +158,70 @@ __device__ __forceinline__ float pair_fx(const ChainDev &C, float r2, float aq, 
     return u;
 }
 
-// erfc(x) for x in [0, xmax] from the erfcx series (Clenshaw)
-__device__ __forceinline__ double erfc_series(const double *c, int n, double tscale, double x)
+// erfc(x) for x in [0, xmax] = exp(-x^2) erfcx(x), erfcx from its Chebyshev
+// series (coefficients c[0..n), argument t = 2x/xmax - 1, Clenshaw), at two
+// arguments in one pass: the two recurrences share each coefficient load.
+__device__ __forceinline__ void erfc_series2(const double *c, int n, double tscale, double xa,
+                                             double xb, double &fa, double &fb)
 {
-    const double t = fma(x, tscale, -1.0);
-    const double t2 = t + t;
-    double b1 = 0.0, b2 = 0.0;
+    const double ta = fma(xa, tscale, -1.0), tb = fma(xb, tscale, -1.0);
+    const double ta2 = ta + ta, tb2 = tb + tb;
+    double a1 = 0.0, a2 = 0.0, b1 = 0.0, b2 = 0.0;
+#pragma unroll 4
     for (int k = n - 1; k >= 1; --k) {
-        const double b0 = fma(t2, b1, c[k] - b2);
+        const double ck = c[k];
+        const double a0 = fma(ta2, a1, ck - a2);
+        a2 = a1;
+        a1 = a0;
+        const double b0 = fma(tb2, b1, ck - b2);
         b2 = b1;
         b1 = b0;
     }
-    return exp(-x * x) * fma(t, b1, c[0] - b2);
+    fa = exp(-xa * xa) * fma(ta, a1, c[0] - a2);
+    fb = exp(-xb * xb) * fma(tb, b1, c[0] - b2);
 }
 
-// Pair energy of the mixed path's FP64 evaluations for plain Ewald
+// u(r2n) - u(r2o) for the mixed path's FP64 evaluations with plain Ewald
 // (kappa -> 0): real-space erfc(alpha r) / r from the staged series inside
-// rc, plus the short-range term (LJ, or CutShift LJ zero beyond its cutoff
-// and shifted by its value there).  One division per pair: 1/r and the LJ
-// ratio come from 1/r^2.
+// rc (both positions in one pass), plus the short-range term (LJ, or
+// CutShift LJ zero beyond its cutoff and shifted by its value there).  One
+// division per position: 1/r and the LJ ratio come from 1/r^2.
 template <int SR>
-__device__ __forceinline__ double u_ewald64(const ChainDev &C, const double *__restrict__ tab,
-                                            const double *cheb, double r2, double qq, double aq,
-                                            double bq, double ar, double br, int at, int bt)
+__device__ __forceinline__ double du_ewald64(const ChainDev &C, const double *__restrict__ tab,
+                                             const double *cheb, double r2n, double r2o,
+                                             double qq, double aq, double bq, double ar,
+                                             double br, int at, int bt)
 {
-    const double rinv2 = 1.0 / r2;
-    double e = 0.0;
-    if (r2 <= C.pot.ew_rc2) {
-        const double r = sqrt(r2);
-        e = qq * erfc_series(cheb, C.ew_ncheb, C.ew_cheb_t, C.pot.ew_alpha * r) * (r * rinv2);
+    const double rc2 = C.pot.ew_rc2;
+    const double rin2 = 1.0 / r2n, rio2 = 1.0 / r2o;
+    double en = 0.0, eo = 0.0;
+    const bool inn = r2n <= rc2, ino = r2o <= rc2;
+    if (inn || ino) {
+        const double rn = sqrt(r2n), ro = sqrt(r2o);
+        double fn, fo;
+        erfc_series2(cheb, C.ew_ncheb, C.ew_cheb_t, inn ? C.pot.ew_alpha * rn : 0.0,
+                     ino ? C.pot.ew_alpha * ro : 0.0, fn, fo);
+        if (inn) en = qq * fn * (rn * rin2);
+        if (ino) eo = qq * fo * (ro * rio2);
     }
     if constexpr (SR == SR_LJ) {
         const int k = at * C.pot.ntypes + bt;
         const double s2 = tab[TAB_S2 * C.pot.ntypes2 + k], e4 = tab[TAB_EPS4 * C.pot.ntypes2 + k];
-        double x = s2 * rinv2;
-        x = x * x * x;
-        double lj = e4 * (x * x - x);
+        double xn = s2 * rin2, xo = s2 * rio2;
+        xn = xn * xn * xn;
+        xo = xo * xo * xo;
+        double ljn = e4 * (xn * xn - xn), ljo = e4 * (xo * xo - xo);
         if (C.pot.lj_rc2 > 0.0) {
             double xc = s2 * C.lj_rc2inv;
             xc = xc * xc * xc;
-            lj = (r2 > C.pot.lj_rc2) ? 0.0 : lj - e4 * (xc * xc - xc);
+            const double uc = e4 * (xc * xc - xc);
+            ljn = (r2n > C.pot.lj_rc2) ? 0.0 : ljn - uc;
+            ljo = (r2o > C.pot.lj_rc2) ? 0.0 : ljo - uc;
         }
-        return e + lj;
+        return (en + ljn) - (eo + ljo);
     } else {
-        return e + pair_u64_t<ELEC_NONE, SR>(C.pot, tab, r2, aq, bq, ar, br, at, bt);
+        return (en + pair_u64_t<ELEC_NONE, SR>(C.pot, tab, r2n, aq, bq, ar, br, at, bt)) -
+               (eo + pair_u64_t<ELEC_NONE, SR>(C.pot, tab, r2o, aq, bq, ar, br, at, bt));
     }
 }
 
@@ -226,8 +246,7 @@ __device__ __forceinline__ double du_pair64_mixed(const ChainDev &C, const doubl
             const double r2o = rx * rx + (ry * ry + rz * rz);
             const int k = at * C.pot.ntypes + bt;
             const double qq = aq * bq * tab[TAB_QEQE * C.pot.ntypes2 + k] * C.pot.lB;
-            return u_ewald64<SR>(C, tab, cheb, r2n, qq, aq, bq, ar, br, at, bt) -
-                   u_ewald64<SR>(C, tab, cheb, r2o, qq, aq, bq, ar, br, at, bt);
+            return du_ewald64<SR>(C, tab, cheb, r2n, r2o, qq, aq, bq, ar, br, at, bt);
         }
     }
     return du_pair64<ELEC, SR>(C, tab, nx, ny, nz, ox, oy, oz, aq, ar, at, j);
@@ -296,7 +315,7 @@ __host__ __device__ constexpr bool at_fx_ok()
             (SR == SR_NONE || SR == SR_HS));
 }
 
-template <int ELEC, int SR, int U, bool FX>
+template <int ELEC, int SR, int U, bool FX, bool SCREEN_ONLY = false>
 __device__ __forceinline__ double pair_loop_mixed(const ChainDev &C, const float *__restrict__ tabf,
                                                   const double *__restrict__ tab, int i, int4 me,
                                                   float ar, int at, double aq64, double ar64,
@@ -437,7 +456,7 @@ __device__ __forceinline__ double pair_loop_mixed(const ChainDev &C, const float
             if (base + step < C.n) eval(mode, B, base + step);
         }
     };
-    if (FX && ELEC == ELEC_EWALD && C.lj_screen)
+    if constexpr (FX && ELEC == ELEC_EWALD && SCREEN_ONLY)
         run(std::integral_constant<int, 2>{});
     else if (FX && ELEC == ELEC_EWALD && C.lj_uniform && !C.lj_check)
         run(std::integral_constant<int, 1>{});
@@ -491,8 +510,7 @@ __device__ __forceinline__ double du_pair64_rec(const ChainDev &C, const double 
     if (ELEC == ELEC_EWALD && C.ew_ncheb > 0) {
         const int k = at * C.pot.ntypes + bt;
         const double qq = aq * bq * tab[TAB_QEQE * C.pot.ntypes2 + k] * C.pot.lB;
-        return u_ewald64<SR>(C, tab, cheb, r2n, qq, aq, bq, ar, 0.0, at, bt) -
-               u_ewald64<SR>(C, tab, cheb, r2o, qq, aq, bq, ar, 0.0, at, bt);
+        return du_ewald64<SR>(C, tab, cheb, r2n, r2o, qq, aq, bq, ar, 0.0, at, bt);
     }
     return pair_u64_t<ELEC, SR>(C.pot, tab, r2n, aq, bq, ar, 0.0, at, bt) -
            pair_u64_t<ELEC, SR>(C.pot, tab, r2o, aq, bq, ar, 0.0, at, bt);
@@ -683,6 +701,9 @@ __device__ __forceinline__ void kunpack(int v, int &nx, int &ny, int &nz)
 // A_k (double), packed n (int)
 __host__ __device__ inline size_t at_dyn_lds(int K) { return (size_t)K * (16 + 16 + 8 + 4); }
 
+// CELL: every pair term vanishes beyond the Ewald rc (CutShift LJ at rc); the
+// partner loop screens for partners inside rc, over the chain's cell list
+// when it has one and over all partners otherwise.
 template <int ELEC, int SR, bool MIX, int U, bool FX, bool CELL = false>
 __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restrict__ chains,
                                                          int ncalls)
@@ -943,7 +964,7 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
                     acc = cell_loop<ELEC, SR>(C, ltab, lcheb, lcnt, i, tr.pk, tr.q64, tr.r64, tr.ti,
                                               tr.nx, tr.ny, tr.nz, tr.ox, tr.oy, tr.oz);
                 else if (FX || lds_tab)
-                    acc = pair_loop_mixed<ELEC, SR, U, FX>(C, ltabf, ltab, i, tr.pk, tr.rf, tr.ti,
+                    acc = pair_loop_mixed<ELEC, SR, U, FX, CELL>(C, ltabf, ltab, i, tr.pk, tr.rf, tr.ti,
                                                            tr.q64, tr.r64, tr.nx, tr.ny, tr.nz,
                                                            tr.ox, tr.oy, tr.oz, dq, qcap, nq);
                 else
@@ -2126,7 +2147,8 @@ static void launch_at_u(hipStream_t st, const ChainDev *d_chains, int nchains, i
 {
     if constexpr (MIX && at_fx_ok<ELEC, SR>()) {
         if constexpr (ELEC == ELEC_EWALD && SR == SR_LJ && U == 4) {
-            // chains with a cell list (others in the batch keep the screen)
+            // pair terms cut at rc: chains with a cell list scan it, the
+            // others screen all partners
             if (fx && cell) {
                 launch_at_f<ELEC, SR, MIX, U, true, true>(st, d_chains, nchains, nthr, dyn, ncalls);
                 return;
@@ -2247,7 +2269,8 @@ static int launch_chains(std::vector<fgx_system *> &sys, const std::vector<int> 
     int nthr = 0;
     size_t dyn = 0;
     bool fx = true;   // fixed-unit form only when every chain has it
-    bool cell = false;   // some chain sweeps with a cell list
+    bool cell = false;   // every pair term cut at rc: the screening kernel
+    bool cells = false;  // some chain sweeps with a cell list
     const int unroll = s0->launch_unroll > 0 ? s0->launch_unroll : 4;
     // batch buffers, kept by the first system
     if (s0->batch_cap < nc) {
@@ -2307,10 +2330,14 @@ static int launch_chains(std::vector<fgx_system *> &sys, const std::vector<int> 
             dyn = std::max(dyn, chains[c].cell_nc > 0 ? at_dyn_lds_cells(s->ew.K, chains[c].cell_nc)
                                                       : at_dyn_lds(s->ew.K));
         if (!chains[c].fx) fx = false;
-        if (chains[c].cell_nc > 0) cell = true;
+        if (chains[c].cell_nc > 0) cells = true;
+        FGX_CHECK(chains[c].lj_screen == chains[0].lj_screen, FGX_EUNSUPPORTED,
+                  "batched chains must agree on a Lennard-Jones cut at the Ewald rc");
     }
+    // pair terms cut at rc: the screening kernel (cell lists where present)
+    if (!rc) cell = chains[0].lj_screen != 0;
     // the cell scan spreads 27 cells over waves 1..7 of a 512-thread block
-    if (cell) nthr = AT_THREADS;
+    if (cells) nthr = AT_THREADS;
     if (!rc) {
         // cell lists that no longer match the accepted state
         std::vector<ChainDev> stale;
