@@ -372,6 +372,11 @@ class System:
     def rng_seed(self, seed):
         _check(lib().fgx_rng_seed(self.h, seed))
 
+    def rng_set_state(self, words, pos):
+        """global engine state (624 words and the position)"""
+        w = (C.c_uint32 * 624)(*[int(x) & 0xFFFFFFFF for x in words])
+        _check(lib().fgx_rng_set_state(self.h, w, int(pos)))
+
     def rng_state(self):
         w = (C.c_uint32 * 624)()
         p = C.c_int()

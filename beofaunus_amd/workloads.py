@@ -346,6 +346,8 @@ def make_system(w, precision=None):
         ham["precision"] = precision
     s = fgx.System(w["box"], w["xyz"], w["charge"], w["radius"], w["mw"], w["types"],
                    w["groups"], **ham)
+    if w.get("move") is None:   # energies only
+        return s
     mv = dict(w["move"])
     kind = mv.pop("kind")
     molid = mv.pop("molid")

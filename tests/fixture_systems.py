@@ -14,7 +14,7 @@ def fgx_system_from_fixture(fx, precision=fgx.FP64, moves=True, ewald_override=N
     nb = fx["config"]["energy"]["nonbonded"]
     elec, sr, nbk = KINDS[fx["kind"]]
     kappa = 1.0 / nb["debyelength"] if "debyelength" in nb else 0.0
-    ew = dict(nb["ewald"]) if "ewald" in nb else None
+    ew = dict(nb["ewald"]) if "ewald" in nb and elec == "ewald" else None
     if ew and ewald_override:
         ew.update(ewald_override)
     s = fgx.System(fx["box"], p["xyz"], p["charge"], p["radius"], p["mw"], p["id"],

@@ -25,6 +25,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+REF = os.environ.get("FAUNUS_REF", "/root/reference")
 
 
 # --------------------------------------------------------------------------
@@ -264,6 +265,57 @@ def molecules(name, kind, nmol, nsite, nion, L, seed, ncalls, center_rotation=Tr
     return build_fixture(name, kind, cfg, [L, L, L], parts, groups, seed, ncalls, moves)
 
 
+def reference_example(name, kind, example):
+    """Energies the reference computes for one of its own example inputs
+    (src/examples/<example>.json + .state, read unchanged) under the pair
+    potential `kind`; the fixture keeps the input data and the results."""
+    ex = os.path.join(REF, "src", "examples", example)
+    with open(ex + ".json") as f:
+        cfg = json.load(f)
+    st = read_state_text(ex + ".state")
+    fx = {
+        "name": name,
+        "kind": kind,
+        "source": "src/examples/%s.json, src/examples/%s.state" % (example, example),
+        "box": st["box"],
+        "config": {k: cfg[k] for k in ("system", "atomlist", "moleculelist", "moves", "energy")
+                   if k in cfg},
+        "particles": {"xyz": st["xyz"], "charge": st["charge"], "radius": st["radius"],
+                      "mw": st["mw"], "id": st["id"]},
+        "groups": st["groups"],
+        "moves": [],
+        "rng": st["rng"],
+        "energy": run_harness(["energy", kind, ex + ".json", ex + ".state"]),
+    }
+    path = os.path.join(HERE, name + ".json")
+    with open(path, "w") as f:
+        json.dump(fx, f, separators=(",", ":"))
+    print(f"wrote {path} ({os.path.getsize(path) // 1024} KiB)")
+    return fx
+
+
+def read_state_text(path):
+    """Space::load text (space.h:807-880) as plain lists"""
+    tok = open(path).read().split()
+    box = [float(v) for v in tok[:3]]
+    n = int(tok[3])
+    rows = [tok[4 + 8 * i:12 + 8 * i] for i in range(n)]
+    pos = 4 + 8 * n
+    ng = int(tok[pos])
+    pos += 1
+    groups = []
+    for _ in range(ng):
+        g = tok[pos:pos + 7]
+        pos += 7
+        groups.append({"front": int(g[0]), "back": int(g[1]), "molid": int(g[2]),
+                       "molsize": int(g[3])})
+    rng = " ".join(tok[pos + 1:pos + 626]) if pos < len(tok) and tok[pos] == "randomstate" else None
+    return {"box": box, "xyz": [float(v) for r in rows for v in r[:3]],
+            "charge": [float(r[3]) for r in rows], "radius": [float(r[4]) for r in rows],
+            "mw": [float(r[5]) for r in rows], "id": [int(r[6]) for r in rows],
+            "groups": groups, "rng": rng}
+
+
 def rng_fixture():
     fx = {"seed_default": run_harness(["rng", "0", "2000"]),
           "seed_7": run_harness(["rng", "7", "500"])}
@@ -289,6 +341,8 @@ def main():
     molecules("mol_dhhs", "dhhs", 4, 12, 16, 60.0, 18, 12)
     molecules("mol_dhhs_offcenter", "dhhs", 3, 10, 10, 50.0, 19, 10, center_rotation=False)
     molecules("mol_dhhs_cut", "dhhs_cut", 5, 8, 10, 70.0, 20, 10, cutoff_g2g=25.0)
+    # the NaCl melt of the reference's bulk example (2304 ions, 1100 K), LJ part
+    reference_example("example_bulk_lj", "lj", "bulk")
 
 
 if __name__ == "__main__":

@@ -191,7 +191,8 @@ class OracleSystem:
                    type_sigma=e["type_sigma"], type_eps=e["type_eps"],
                    type_charge=e["type_charge"], type_radius=e["type_radius"],
                    type_dp=e["type_dp"], g2g_cut=nb.get("cutoff_g2g", 0.0),
-                   ewald=nb.get("ewald"), lj_cut=lj_cut_of(fx["kind"], nb))
+                   ewald=nb.get("ewald") if KINDS[fx["kind"]][0] == "ewald" else None,
+                   lj_cut=lj_cut_of(fx["kind"], nb))
         sys_.add_moves(fx)
         return sys_
 

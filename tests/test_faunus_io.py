@@ -1,0 +1,130 @@
+"""Reading the reference's own input files (beofaunus_amd.faunus_io): the JSON
+keys on the path and the Space::load text state.
+
+Each golden fixture holds a JSON input and a state that the compiled
+reference read, and the per-type tables, Bjerrum length and groups it derived
+from them.  Here the same files are written out, read back by the engine's
+reader, and compared with what the reference derived.  example_bulk_lj is the
+reference's own bulk example (src/examples/bulk.json + bulk.state, NaCl melt
+at 1100 K) carried unchanged.
+"""
+import glob
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+from oracle_py import GOLDEN, KINDS, load_fixture
+
+FIXTURES = sorted(os.path.basename(f)[:-5] for f in glob.glob(os.path.join(GOLDEN, "*.json"))
+                  if not f.endswith("rng.json"))
+
+
+def mt_state_text(seed):
+    x = [0] * 624
+    x[0] = seed & 0xFFFFFFFF
+    for i in range(1, 624):
+        x[i] = (1812433253 * (x[i - 1] ^ (x[i - 1] >> 30)) + i) & 0xFFFFFFFF
+    return " ".join(str(v) for v in x) + " 624"
+
+
+def kind_args(kind):
+    from beofaunus_amd import fgx
+    elec, sr, nb = KINDS[kind]
+    return dict(elec={"none": fgx.ELEC_NONE, "coulomb": fgx.ELEC_COULOMB, "dh": fgx.ELEC_DH,
+                      "ewald": fgx.ELEC_EWALD}[elec],
+                sr={"none": fgx.SR_NONE, "hs": fgx.SR_HS, "lj": fgx.SR_LJ}[sr],
+                nbkind={"plain": fgx.NB_PLAIN, "vector": fgx.NB_VECTOR,
+                        "cutg2g": fgx.NB_CUTG2G}[nb],
+                lj_cutshift=kind.endswith("ljcut"))
+
+
+def write_inputs(fx, d):
+    """the fixture's JSON input and state file, as the reference read them"""
+    import json
+    from beofaunus_amd import faunus_io
+    c = os.path.join(d, "in.json")
+    s = os.path.join(d, "state")
+    with open(c, "w") as f:
+        json.dump(fx["config"], f)
+    p = fx["particles"]
+    rng = fx.get("rng") or mt_state_text(fx["rng_seed"])
+    faunus_io.write_state(s, fx["box"], p["xyz"], p["charge"], p["radius"], p["mw"], p["id"],
+                          fx["groups"], rng=rng)
+    return c, s
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_reader_matches_reference_parameters(name):
+    from beofaunus_amd import faunus_io
+    fx = load_fixture(name)
+    e = fx.get("energy") or fx["mc"]
+    with tempfile.TemporaryDirectory() as d:
+        c, s = write_inputs(fx, d)
+        w = faunus_io.read_input(c, s, **kind_args(fx["kind"]))
+    p = fx["particles"]
+    assert list(w["box"]) == list(map(float, fx["box"]))
+    assert np.array_equal(w["xyz"].reshape(-1), np.asarray(p["xyz"], dtype=np.float64))
+    for k, v in (("charge", "charge"), ("radius", "radius"), ("mw", "mw")):
+        assert np.array_equal(w[k], np.asarray(p[v], dtype=np.float64))
+    assert np.array_equal(w["types"], np.asarray(p["id"]))
+    assert [{k: g[k] for k in ("front", "back", "molid", "molsize")} for g in fx["groups"]] == \
+        w["groups"]
+    # per-type tables and lB as the reference derived them (species.h, physconst.h)
+    for mine, ref in (("type_sigma", "type_sigma"), ("type_eps", "type_eps"),
+                      ("type_charge", "type_charge"), ("type_radius", "type_radius"),
+                      ("type_dp", "type_dp")):
+        np.testing.assert_allclose(w["ham"][mine], np.asarray(e[ref], dtype=np.float64),
+                                   rtol=1e-15, atol=0, err_msg=mine)
+    assert w["ham"]["lB"] == pytest.approx(e["lB"], rel=1e-15)
+    if "mc" not in fx:   # energies-only fixture
+        return
+    # moves: what the fixture's Propagator registered
+    mine = [{k: m[k] for k in ("type", "molid", "prob", "peratom", "permol")} for m in w["moves"]]
+    ref = [{k: m[k] for k in ("type", "molid", "prob", "peratom", "permol")} for m in fx["moves"]]
+    assert mine == ref
+    for m, r in zip(w["moves"], fx["moves"]):
+        if m["type"] == "TR":
+            assert (m["dp"], m["dprot"], m["center_rotation"]) == \
+                (r["dp"], r["dprot"], r["center_rotation"])
+
+
+def test_state_round_trip():
+    from beofaunus_amd import faunus_io
+    fx = load_fixture("mol_dhhs")
+    with tempfile.TemporaryDirectory() as d:
+        c, s = write_inputs(fx, d)
+        st = faunus_io.read_state(s)
+        s2 = os.path.join(d, "state2")
+        faunus_io.write_state(s2, st["box"], st["xyz"], st["charge"], st["radius"], st["mw"],
+                              st["id"], st["groups"], rng=st["rng"],
+                              hydrophobic=st["hydrophobic"])
+        assert open(s).read() == open(s2).read()
+        words, pos = faunus_io.mt_parse(st["rng"])
+        assert faunus_io.mt_text(words, pos) == st["rng"] == mt_state_text(fx["rng_seed"])
+
+
+def test_reference_example_state():
+    """bulk.state of the reference's examples, read as Space::load reads it"""
+    from beofaunus_amd import faunus_io
+    fx = load_fixture("example_bulk_lj")
+    with tempfile.TemporaryDirectory() as d:
+        c, s = write_inputs(fx, d)
+        w = faunus_io.read_input(c, s, **kind_args("lj"))
+    assert len(w["charge"]) == 2304 and w["box"] == [42.5, 42.5, 42.5]
+    assert sorted(set(w["charge"].tolist())) == [-1.0, 1.0]
+    assert w["atom_names"] == ["unk", "Cl", "Na"] and w["T"] == 1100
+    # its input registers an isobaric move beside atomtranslate: energies only
+    assert w["move"] is None and w["other_moves"] == ["isobaric"]
+    assert faunus_io.mt_parse(w["rng"])[1] <= 624
+
+
+def test_reader_refuses_bad_state():
+    from beofaunus_amd import faunus_io
+    with tempfile.TemporaryDirectory() as d:
+        s = os.path.join(d, "state")
+        with open(s, "w") as f:
+            f.write("10 10 10\n3\n0 0 0 1 1 1 1 0\n")
+        with pytest.raises(ValueError):
+            faunus_io.read_state(s)

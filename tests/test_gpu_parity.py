@@ -78,7 +78,7 @@ def test_energies_fp64(gpu, name):
     s.close()
 
 
-@pytest.mark.parametrize("name", FIXTURES)
+@pytest.mark.parametrize("name", [f for f in FIXTURES if "mc" in load_fixture(f)])
 def test_device_propagator_matches_reference(gpu, name):
     """fgx_propagate replays the reference's Propagator on the device: same
     RNG stream, same accept/reject sequence, same energy changes."""
