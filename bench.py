@@ -41,12 +41,13 @@ FLOPS_PAIR_DHHS = 27                           # Debye-Hueckel + hard sphere per
 # (24 B) plus S(k) read+write on an accepted move (32 B).
 BYTES_PARTNER = {"mixed": 20, "fp64": 40}
 BYTES_K_READ, BYTES_K_ACCEPT = 24, 32
-# HBM traffic per launch of the default Ewald step (256 replicas x N=20,000,
-# mixed, 512 threads, batch depth 4), from rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE of bench.py --steps 1, scaled by the counters' reading for a
-# 1 GiB copy (FETCH x2.0, WRITE x1.0):
-# 1,024,463,194 KiB x 2 x 1024 + 432,350 KiB x 1024 bytes.
-MEASURED_TRAFFIC = {("ewald", 20000, 256, "mixed"): 1024463194 * 2 * 1024 + 432350 * 1024}
+# HBM traffic per launch (256 replicas x N=20,000, mixed, 512 threads), from
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of one timed step, scaled by the
+# counters' reading for a 1 GiB copy (FETCH x2.0, WRITE x1.0), in bytes:
+# Ewald electrolyte (batch depth 4): 1,025,185,864 KiB fetched, 432,348 KiB
+# written; cut-LJ cell-list form: 64,251,548 KiB fetched, 564,152 KiB written.
+MEASURED_TRAFFIC = {("ewald", 20000, 256, "mixed"): 1025185864 * 2 * 1024 + 432348 * 1024,
+                    ("ewald-cells", 20000, 256, "mixed"): 64251548 * 2 * 1024 + 564152 * 1024}
 
 
 def cpu_baseline(w, seconds_target=12.0):
