@@ -149,6 +149,9 @@ struct fgx_system {
     int *d_cell_slot = nullptr;            // n: cell * cap + slot
     size_t cell_bytes = 0;
     bool cell_valid = false;               // cleared whenever the accepted state changes
+    // Energy::ExternalPressure term (1/A^3)
+    bool press_on = false;
+    double press = 0.0;
 };
 
 namespace fgx {

@@ -703,6 +703,10 @@ extern "C" int fgx_create(const fgx_hamiltonian *h, const double box[3], int n, 
               "fgx_create: lj_cut needs the Lennard-Jones short-range term");
     P.lj_rc2 = h->lj_cut > 0 ? std::pow(h->lj_cut, 2) : 0.0;
     P.lj_rc2f = (float)P.lj_rc2;
+    FGX_CHECK(!h->has_pressure || h->pressure >= 0, FGX_EARG,
+              "fgx_create: negative pressure (energy.h:1871-1872)");
+    s->press_on = h->has_pressure != 0;
+    s->press = h->pressure;
     // tables: LennardJonesMixed<LorentzBerthelot>::init (potentials.h:2010-2019)
     const int T = h->ntypes;
     s->tab.assign((size_t)TAB_COUNT * T * T, 0.0);
@@ -1163,26 +1167,49 @@ extern "C" int fgx_g_internal(fgx_system *s, int which, int g, double *u)
     return FGX_OK;
 }
 
+// Cuboid::_getVolume (src/geometry.cpp:293-300)
+static double box_volume(const Box &b) { return b.len[0] * b.len[1] * b.len[2]; }
+
+// NonbondedEwald::g_external (ewald.h:712-714), then ExternalPressure::
+// g_external = -N ln V for the group's N molecules (energy.h:1885-1893)
 extern "C" int fgx_g_external(fgx_system *s, int which, int g, double *u)
 {
     FGX_CHECK(check_which(s, which) && u && g >= 0 && g < s->ng, FGX_EARG,
               "fgx_g_external: bad arguments");
-    if (!s->has_ewald) { *u = 0.0; return FGX_OK; }
-    return ewald_self(s, which, s->groups[g].front, s->groups[g].back, u);
+    double e = 0.0;
+    if (s->has_ewald) {
+        int rc = ewald_self(s, which, s->groups[g].front, s->groups[g].back, &e);
+        if (rc) return rc;
+    }
+    if (s->press_on) {
+        const GroupDesc &G = s->groups[g];
+        const int N = G.back < G.front ? 0 : (G.back - G.front + 1) / G.molsize;
+        e += -N * std::log(box_volume(s->box));
+    }
+    *u = e;
+    return FGX_OK;
 }
 
-// NonbondedEwald::external (ewald.h:716-729)
+// NonbondedEwald::external (ewald.h:716-729), then ExternalPressure::external
+// = P V - ln V (energy.h:1878-1883), as CombinedEnergy adds them
 extern "C" int fgx_external(fgx_system *s, int which, double *u)
 {
     FGX_CHECK(check_which(s, which) && u, FGX_EARG, "fgx_external: bad arguments");
-    if (!s->has_ewald) { *u = 0.0; return FGX_OK; }
-    double surf, reci;
-    int rc = ewald_surface(s, which, s->ew.V, &surf);
-    if (rc) return rc;
-    if ((rc = ewald_reciprocal(s, which ? s->ew.Qt : s->ew.Q, s->ew.V, &reci))) return rc;
-    if (which) { s->ew.surf_trial = surf; s->ew.reci_trial = reci; }
-    else { s->ew.surf = surf; s->ew.reci = reci; }
-    *u = surf + reci;
+    double e = 0.0;
+    if (s->has_ewald) {
+        double surf, reci;
+        int rc = ewald_surface(s, which, s->ew.V, &surf);
+        if (rc) return rc;
+        if ((rc = ewald_reciprocal(s, which ? s->ew.Qt : s->ew.Q, s->ew.V, &reci))) return rc;
+        if (which) { s->ew.surf_trial = surf; s->ew.reci_trial = reci; }
+        else { s->ew.surf = surf; s->ew.reci = reci; }
+        e = surf + reci;
+    }
+    if (s->press_on) {
+        const double V = box_volume(s->box);
+        e += s->press * V - std::log(V);
+    }
+    *u = e;
     return FGX_OK;
 }
 
@@ -1197,7 +1224,7 @@ extern "C" int fgx_system_energy(fgx_system *s, int which, double *u)
     double ext = 0.0;
     if ((rc = fgx_external(s, which, &ext))) return rc;
     double gext = 0.0;
-    for (int g = 0; g < s->ng && s->has_ewald; ++g) {
+    for (int g = 0; g < s->ng && (s->has_ewald || s->press_on); ++g) {
         if (s->groups[g].back < s->groups[g].front) continue;
         double t;
         if ((rc = fgx_g_external(s, which, g, &t))) return rc;
@@ -1415,5 +1442,252 @@ extern "C" int fgx_commit(fgx_system *s, const fgx_change *c, int accepted, doub
     }
     FGX_HIP(hipStreamSynchronize(s->stream));
     if (drift) *drift = d;
+    return FGX_OK;
+}
+
+// ------------------------------------------------------------------------
+// Move::Isobaric (move.h:5424-5640) with Energy::ExternalPressure
+// ------------------------------------------------------------------------
+namespace {
+
+#pragma clang fp contract(off)
+
+// libstdc++ std::mt19937 over a host state (624 words, then the position),
+// and the draws the moves take from it (slump.h:24-126)
+struct HostMT {
+    uint32_t *x;
+    void twist()
+    {
+        for (int i = 0; i < 624; ++i) {
+            const uint32_t y = (x[i] & 0x80000000u) | (x[(i + 1) % 624] & 0x7fffffffu);
+            x[i] = x[(i + 397) % 624] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+        }
+    }
+    uint32_t next()
+    {
+        if (x[624] >= 624) {
+            twist();
+            x[624] = 0;
+        }
+        uint32_t z = x[x[624]++];
+        z ^= (z >> 11);
+        z ^= (z << 7) & 0x9d2c5680u;
+        z ^= (z << 15) & 0xefc60000u;
+        z ^= (z >> 18);
+        return z;
+    }
+    // generate_canonical<double,53>: (w1 + w2 2^32) / 2^64, clamped below 1
+    double canonical()
+    {
+        double sum = (double)next();
+        sum += (double)next() * 4294967296.0;
+        double r = sum / 18446744073709551616.0;
+        if (r >= 1.0) r = 0x1.fffffffffffffp-1;
+        return r;
+    }
+    // uniform_int_distribution<int>(a, b), Lemire's method
+    int range(int a, int b)
+    {
+        const uint32_t erange = (uint32_t)((uint32_t)b - (uint32_t)a) + 1u;
+        uint64_t prod = (uint64_t)next() * (uint64_t)erange;
+        uint32_t low = (uint32_t)prod;
+        if (low < erange) {
+            const uint32_t thr = (uint32_t)(0u - erange) % erange;
+            while (low < thr) {
+                prod = (uint64_t)next() * (uint64_t)erange;
+                low = (uint32_t)prod;
+            }
+        }
+        return (int)((uint32_t)(prod >> 32) + (uint32_t)a);
+    }
+};
+
+int anint_h(double x) { return (int)(x > 0. ? x + .5 : x - .5); }
+
+// Cuboid::boundary (geometry.h:539-547)
+void boundary_h(const double len[3], double *a)
+{
+    for (int k = 0; k < 3; ++k)
+        if (std::fabs(a[k]) > len[k] * 0.5) a[k] -= len[k] * anint_h(a[k] * (1 / len[k]));
+}
+
+void set_box(Box &b, const double len[3])
+{   // Cuboid::setlen (src/geometry.cpp:265-274)
+    for (int k = 0; k < 3; ++k) {
+        b.len[k] = len[k];
+        b.half[k] = len[k] * 0.5;
+        b.inv[k] = 1 / len[k];
+        b.lenf[k] = (float)len[k];
+        b.fscale[k] = (float)(len[k] / 4294967296.0);
+    }
+}
+
+#pragma clang fp contract(on)
+
+}  // namespace
+
+// energyChangeConfiguration (energy.h:3521-3572) for a change that moves every
+// group (atomic groups with all their indices, molecular groups whole):
+// g2g over all group pairs, external, each group's g_external, and the
+// internal pairs of the atomic groups.
+static int all_groups_config(fgx_system *s, int which, double *u)
+{
+    double t = 0.0, v;
+    int rc;
+    for (int a = 0; a < s->ng; ++a)
+        for (int b = a + 1; b < s->ng; ++b) {
+            if ((rc = fgx_g2g(s, which, a, b, &v))) return rc;
+            t += v;
+        }
+    if ((rc = fgx_external(s, which, &v))) return rc;
+    t += v;
+    for (int g = 0; g < s->ng; ++g) {
+        const GroupDesc &G = s->groups[g];
+        if (G.back < G.front) continue;
+        if ((rc = fgx_g_external(s, which, g, &v))) return rc;
+        t += v;
+        if (G.molsize == 1) {
+            if ((rc = fgx_g_internal(s, which, g, &v))) return rc;
+            t += v;
+        }
+    }
+    *u = t;
+    return FGX_OK;
+}
+
+extern "C" int fgx_get_box(const fgx_system *s, double len[3])
+{
+    FGX_CHECK(s && len, FGX_EARG, "fgx_get_box: bad arguments");
+    for (int k = 0; k < 3; ++k) len[k] = s->box.len[k];
+    return FGX_OK;
+}
+
+extern "C" int fgx_isobaric_propagate(fgx_system *s, double dp, double prob, int ncalls,
+                                      fgx_stats *st, double *du_out, int *acc_out, long cap)
+{
+    FGX_CHECK(s && ncalls >= 0 && (cap <= 0 || (du_out && acc_out)), FGX_EARG,
+              "fgx_isobaric_propagate: bad arguments");
+    FGX_CHECK(s->press_on, FGX_ESTATE,
+              "Isobaric needs the ExternalPressure term in the Hamiltonian (move.h:5466-5472)");
+    FGX_CHECK(!s->has_ewald, FGX_EUNSUPPORTED,
+              "Isobaric with Ewald: the k-vectors change with the box (not on the device path)");
+    for (int g = 0; g < s->ng; ++g) {
+        const GroupDesc &G = s->groups[g];
+        const int sz = G.back - G.front + 1;
+        FGX_CHECK(G.back < G.front || G.molsize == 1 || G.molsize == sz, FGX_EUNSUPPORTED,
+                  "Isobaric: a group holding several molecules (Group::isRange)");
+    }
+    if (st) std::memset(st, 0, sizeof(*st));
+    int rc = pull_device_state(s);
+    if (rc) return rc;
+    const int n = s->n, ng = s->ng;
+    std::vector<double> p(3 * (size_t)n), t(3 * (size_t)n);
+    if ((rc = fgx_get_positions(s, 0, p.data()))) return rc;
+    std::vector<double> cm(3 * (size_t)ng), cmt(3 * (size_t)ng);
+    HostMT G{s->rng_global}, I{s->rng_internal};
+    // Isobaric constructor: runfraction = prob, 0 when dp < 1e-6 (move.h:5462-5464)
+    const double runfraction = dp < 1e-6 ? 0.0 : prob;
+    long trials = 0, accepted = 0, inf = 0;
+    double utot = 0.0;
+    hipEvent_t e0, e1;
+    FGX_HIP(hipEventCreate(&e0));
+    FGX_HIP(hipEventCreate(&e1));
+    FGX_HIP(hipEventRecord(e0, s->stream));
+    for (int call = 0; call < ncalls && !rc; ++call) {
+        (void)I.range(0, 0);                       // Propagator::move (move.h:7887)
+        if (!(I.canonical() < runfraction)) continue;   // Movebase::run (move.h:575-581)
+        // ---- Isobaric::_trialMove (move.h:5529-5567)
+        double oldlen[3], newlen[3];
+        for (int k = 0; k < 3; ++k) oldlen[k] = s->box.len[k];
+        double newval, xyz;
+        {
+#pragma clang fp contract(off)
+            const double oldval = box_volume(s->box);
+            newval = std::exp(std::log(oldval) + (G.canonical() - 0.5) * dp);
+            xyz = std::cbrt(newval / oldval);
+            for (int k = 0; k < 3; ++k) newlen[k] = oldlen[k] * 1.0 * xyz;
+            t = p;
+            for (int g = 0; g < ng; ++g) {
+                const GroupDesc &Gd = s->groups[g];
+                double c[3];
+                mass_center_host(s, p, g, c);      // Group::setMassCenter (group.h:174-179)
+                for (int k = 0; k < 3; ++k) cm[3 * g + k] = cmt[3 * g + k] = c[k];
+                if (Gd.back < Gd.front) continue;
+                if (Gd.molsize == 1) {             // Group::scale, atomic (group.h:323-330)
+                    for (int k = 0; k < 3; ++k) cmt[3 * g + k] = c[k] * 1.0 * xyz;
+                    for (int i = Gd.front; i <= Gd.back; ++i)
+                        for (int k = 0; k < 3; ++k) t[3 * i + k] = t[3 * i + k] * 1.0 * xyz;
+                } else {                           // molecular (group.h:332-353)
+                    double ncm[3];
+                    for (int k = 0; k < 3; ++k) ncm[k] = c[k] * 1.0 * xyz;
+                    for (int i = Gd.front; i <= Gd.back; ++i) {
+                        for (int k = 0; k < 3; ++k) t[3 * i + k] += -c[k];
+                        boundary_h(oldlen, &t[3 * i]);
+                    }
+                    for (int i = Gd.front; i <= Gd.back; ++i) {
+                        for (int k = 0; k < 3; ++k) t[3 * i + k] += ncm[k];
+                        boundary_h(newlen, &t[3 * i]);
+                    }
+                    for (int k = 0; k < 3; ++k) cmt[3 * g + k] = ncm[k];
+                }
+            }
+        }
+        // ---- Energy::energyChange with the trial geometry (energy.h:3577-3621)
+        if ((rc = fgx_set_positions(s, 1, 0, n, t.data()))) break;
+        for (int g = 0; g < ng && !rc; ++g) rc = fgx_set_group_cm(s, g, nullptr, &cmt[3 * g]);
+        if (rc) break;
+        double du;
+        bool coll = false;
+        for (int i = 0; i < n && !coll; ++i)
+            for (int k = 0; k < 3; ++k)
+                if (std::fabs(t[3 * i + k]) > newlen[k] * 0.5) coll = true;
+        if (coll) {
+            du = HUGE_VAL;
+        } else {
+            const Box saved = s->box;
+            set_box(s->box, newlen);
+            double unew = 0.0, uold = 0.0;
+            rc = all_groups_config(s, 1, &unew);
+            s->box = saved;
+            if (!rc) rc = all_groups_config(s, 0, &uold);
+            if (rc) break;
+            du = unew - uold;
+        }
+        // ---- Movebase::metropolis (move.h:567-573)
+        const bool acc = !(G.canonical() > std::exp(-du));
+        if (acc) {   // Isobaric::_acceptMove (move.h:5570-5580)
+            set_box(s->box, newlen);
+            p = t;
+            rc = fgx_set_positions(s, 2, 0, n, p.data());
+            for (int g = 0; g < ng && !rc; ++g) rc = fgx_set_group_cm(s, g, &cmt[3 * g], &cmt[3 * g]);
+            utot += du;
+            accepted++;
+        } else {     // Isobaric::_rejectMove (move.h:5582-5592)
+            rc = fgx_set_positions(s, 1, 0, n, p.data());
+            for (int g = 0; g < ng && !rc; ++g) rc = fgx_set_group_cm(s, g, nullptr, &cm[3 * g]);
+            if (du == HUGE_VAL) inf++;
+        }
+        if (trials < cap) {
+            du_out[trials] = du;
+            acc_out[trials] = acc ? 1 : 0;
+        }
+        trials++;
+    }
+    FGX_HIP(hipEventRecord(e1, s->stream));
+    FGX_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    s->dusum += utot;
+    if (st) {
+        st->trials = trials;
+        st->accepted = accepted;
+        st->rejected_inf = inf;
+        st->utot = utot;
+        st->dusum = utot;
+        st->seconds = ms * 1e-3;
+    }
     return FGX_OK;
 }

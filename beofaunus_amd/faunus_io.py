@@ -189,17 +189,28 @@ def read_moves(moves, molnames):
                             "dp": float(d.get("dp", 0.0)), "dprot": float(d.get("dprot", 0.0)),
                             "dir": _vec3(d.get("dir")), "dir2": _vec3(d.get("dir2")),
                             "center_rotation": int(bool(m["center_rotation"]))})
+        elif key == "isobaric":
+            # Move::Isobaric (move.h:5455-5472): dp, pressure (mM), prob
+            out.append({"type": "NPT", "molid": -1, "prob": float(m.get("prob", 1.0)),
+                        "peratom": 0, "permol": 0, "dp": float(m["dp"]),
+                        "pressure_mM": float(m["pressure"])})
         elif key != "random":   # moves.random configures the engine (move.h:7783-7787)
             other.append(key)
     return out, other
 
 
+def mM_to_per_A3(c):
+    """operator "" _mM (physconst.h:170-173): particles per A^3"""
+    return c * (1.0e-3 * PC_NAV) / 1e27
+
+
 def read_input(json_path, state_path, *, elec, sr, nbkind=fgx.NB_PLAIN, precision=fgx.FP64,
-               lj_cutshift=False):
+               lj_cutshift=False, external_pressure=False):
     """A reference JSON input plus a state file as a workload dictionary.
     elec/sr/nbkind name the compile-time pair potential (FGX_ELEC_*, FGX_SR_*,
     FGX_NB_*); lj_cutshift makes LJ CutShift<LennardJonesLB> at
-    energy.nonbonded.cutoff."""
+    energy.nonbonded.cutoff; external_pressure adds Energy::ExternalPressure
+    at moves.isobaric.pressure (energy.h:1867-1873)."""
     with open(json_path) as f:
         cfg = json.load(f)
     T = float(cfg.get("system", {}).get("temperature", 298.15))
@@ -229,12 +240,14 @@ def read_input(json_path, state_path, *, elec, sr, nbkind=fgx.NB_PLAIN, precisio
                  "spherical_sum": bool(e.get("spherical_sum", True)),
                  "update_frequency": int(e.get("update_frequency", -1))}
     lj_cut = float(nb["cutoff"]) if lj_cutshift else 0.0
+    pressure = (mM_to_per_A3(float(cfg["moves"]["isobaric"]["pressure"]))
+                if external_pressure else None)
     moves, other = read_moves(cfg.get("moves", {}), molnames)
     # the device Propagator runs one registered move; with any other move
     # registered (even at prob 0, it takes part in Propagator's pick) the
     # system is built for energies only
     move = None
-    if len(moves) == 1 and not other:
+    if len(moves) == 1 and not other and moves[0]["type"] != "NPT":
         mv = moves[0]
         move = {"kind": fgx.MOVE_AT if mv["type"] == "AT" else fgx.MOVE_TR,
                 "molid": mv["molid"], "prob": mv["prob"], "peratom": mv["peratom"],
@@ -250,7 +263,7 @@ def read_input(json_path, state_path, *, elec, sr, nbkind=fgx.NB_PLAIN, precisio
         "ham": dict(elec=elec, sr=sr, nbkind=nbkind, precision=precision, lB=lB, kappa=kappa,
                     g2g_cut=g2g_cut, type_sigma=at["sigma"], type_eps=at["eps"],
                     type_charge=at["q"], type_radius=at["r"], type_dp=at["dp"], ewald=ewald,
-                    lj_cut=lj_cut),
+                    lj_cut=lj_cut, pressure=pressure),
         "move": move,
         "moves": moves,
         "other_moves": other,
@@ -262,16 +275,19 @@ def read_input(json_path, state_path, *, elec, sr, nbkind=fgx.NB_PLAIN, precisio
 
 
 def run_input(json_path, state_path, out_state, ncalls, *, elec, sr, nbkind=fgx.NB_PLAIN,
-              precision=fgx.FP64, lj_cutshift=False):
+              precision=fgx.FP64, lj_cutshift=False, external_pressure=False):
     """Load a reference input and state, seed the global engine from the
     state file (as Space::load does), construct the Propagator (its engine
     copied from the global one, move.h:7781), run `ncalls` Propagator::move()
-    calls on the device and write the final state (positions, mass centres,
-    the global engine) to `out_state`.  Returns the run statistics."""
+    calls on the device (its one registered move: atomtranslate, moltransrot
+    or isobaric) and write the final state (box, positions, mass centres, the
+    global engine) to `out_state`.  Returns the run statistics."""
     from . import workloads
     w = read_input(json_path, state_path, elec=elec, sr=sr, nbkind=nbkind,
-                   precision=precision, lj_cutshift=lj_cutshift)
-    if w["move"] is None:
+                   precision=precision, lj_cutshift=lj_cutshift,
+                   external_pressure=external_pressure)
+    npt = len(w["moves"]) == 1 and not w["other_moves"] and w["moves"][0]["type"] == "NPT"
+    if w["move"] is None and not npt:
         raise ValueError("%s: the device Propagator needs exactly one registered move "
                          "(atomtranslate or moltransrot), found %s and %s"
                          % (json_path, [m["type"] for m in w["moves"]], w["other_moves"]))
@@ -281,13 +297,17 @@ def run_input(json_path, state_path, out_state, ncalls, *, elec, sr, nbkind=fgx.
             words, pos = mt_parse(w["rng"])
             s.rng_set_state(words, pos)
         s.propagator_init()
-        st = s.propagate(ncalls)[0]
+        if npt:   # a Propagator holding one Isobaric move
+            mv = w["moves"][0]
+            st = s.isobaric_propagate(mv["dp"], mv["prob"], ncalls)[0]
+        else:
+            st = s.propagate(ncalls)[0]
         xyz = s.positions(0)
         words, pos = s.rng_state()
         cms = [s.group_cm(g)[0] for g in range(len(w["groups"]))]
         groups = [dict(g, cm=list(cm)) for g, cm in zip(w["groups"], cms)]
-        write_state(out_state, w["box"], xyz, w["charge"], w["radius"], w["mw"], w["types"],
-                    groups, rng=mt_text(words, pos))
+        write_state(out_state, list(s.box()), xyz, w["charge"], w["radius"], w["mw"],
+                    w["types"], groups, rng=mt_text(words, pos))
     finally:
         s.close()
     return st

@@ -37,6 +37,7 @@ class Hamiltonian(C.Structure):
         ("ewald_debyelength", C.c_double), ("ewald_eps_surf", C.c_double),
         ("ewald_spherical_sum", C.c_int), ("ewald_update_frequency", C.c_int),
         ("lj_cut", C.c_double),
+        ("has_pressure", C.c_int), ("pressure", C.c_double),
     ]
 
 
@@ -127,6 +128,8 @@ def lib():
         "fgx_set_launch": (I, [P, I, I]),
         "fgx_set_energy_matrix": (I, [P, I]),
         "fgx_set_cell_list": (I, [P, I]),
+        "fgx_isobaric_propagate": (I, [P, D, D, I, C.POINTER(Stats), _dp, _ip, C.c_long]),
+        "fgx_get_box": (I, [P, _dp]),
         "fgx_get_cell_list": (I, [P, _ip, _ip]),
         "fgx_chain_seconds": (I, [P, C.POINTER(C.c_double)]),
         "fgx_rdf": (I, [P, I, I, I, C.c_double, I, C.POINTER(C.c_ulonglong)]),
@@ -177,7 +180,7 @@ def device_info():
 
 def make_hamiltonian(arr, ntypes, *, elec, sr, nbkind=NB_PLAIN, precision=FP64, lB=7.0,
                      kappa=0.0, g2g_cut=0.0, type_sigma=None, type_eps=None, type_charge=None,
-                     type_radius=None, type_dp=None, ewald=None, lj_cut=0.0):
+                     type_radius=None, type_dp=None, ewald=None, lj_cut=0.0, pressure=None):
     """fgx_hamiltonian from keyword parameters; `arr(values, n)` must keep the
     per-type arrays alive for as long as the structure is used.  `lj_cut` > 0
     makes the LJ term CutShift<LennardJonesLB> at that cutoff."""
@@ -185,6 +188,8 @@ def make_hamiltonian(arr, ntypes, *, elec, sr, nbkind=NB_PLAIN, precision=FP64, 
     h.elec, h.sr, h.nbkind, h.precision = elec, sr, nbkind, precision
     h.lB, h.kappa, h.g2g_cut = lB, kappa, g2g_cut
     h.lj_cut = lj_cut or 0.0
+    if pressure is not None:   # Energy::ExternalPressure, 1/A^3
+        h.has_pressure, h.pressure = 1, float(pressure)
     h.ntypes = ntypes
     h.type_sigma = _d(arr(type_sigma, ntypes))
     h.type_eps = _d(arr(type_eps, ntypes))
@@ -208,7 +213,7 @@ class System:
     def __init__(self, box, xyz, charge, radius, mw, types, groups, *, elec, sr,
                  nbkind=NB_PLAIN, precision=FP64, lB=7.0, kappa=0.0, g2g_cut=0.0,
                  type_sigma=None, type_eps=None, type_charge=None, type_radius=None,
-                 type_dp=None, ewald=None, lj_cut=0.0):
+                 type_dp=None, ewald=None, lj_cut=0.0, pressure=None):
         L = lib()
         n = len(charge)
         ntypes = len(type_sigma) if type_sigma is not None else int(np.max(types)) + 1
@@ -224,7 +229,7 @@ class System:
                              lB=lB, kappa=kappa, g2g_cut=g2g_cut, type_sigma=type_sigma,
                              type_eps=type_eps, type_charge=type_charge,
                              type_radius=type_radius, type_dp=type_dp, ewald=ewald,
-                             lj_cut=lj_cut)
+                             lj_cut=lj_cut, pressure=pressure)
         self.ham = h
         self.n = n
         self.box_len = [float(v) for v in box]
@@ -346,6 +351,25 @@ class System:
     def set_energy_matrix(self, on):
         """Group-pair energy cache of the mixed TranslateRotate sweep."""
         _check(lib().fgx_set_energy_matrix(self.h, 1 if on else 0))
+
+    def isobaric_propagate(self, dp, prob, ncalls, record=0):
+        """Propagator::move() calls of a Propagator holding one Isobaric move
+        (volume trials; needs the ExternalPressure term)."""
+        st = Stats()
+        if record:
+            du = np.zeros(record)
+            acc = np.zeros(record, dtype=np.int32)
+            _check(lib().fgx_isobaric_propagate(self.h, dp, prob, ncalls, C.byref(st), _d(du),
+                                                _i(acc), record))
+            n = min(record, st.trials)
+            return st.as_dict(), du[:n], acc[:n]
+        _check(lib().fgx_isobaric_propagate(self.h, dp, prob, ncalls, C.byref(st), None, None, 0))
+        return st.as_dict(), None, None
+
+    def box(self):
+        b = np.zeros(3)
+        _check(lib().fgx_get_box(self.h, _d(b)))
+        return b
 
     def set_cell_list(self, mode):
         """Cell list of the mixed AtomicTranslation sweep: 0 off, 1 automatic,

@@ -74,6 +74,11 @@ typedef struct {
      * to ewald_rc every pair term vanishes beyond rc, and the mixed
      * AtomicTranslation sweep uses a cell list (fgx_set_cell_list). */
     double lj_cut;
+    /* Energy::ExternalPressure (energy.h:1817-1894) in the Hamiltonian:
+     * external() += P V - ln V, g_external() += -N ln V (N molecules of the
+     * group); pressure in 1/A^3 (the input's mM times 1.0_mM) */
+    int has_pressure;
+    double pressure;
 } fgx_hamiltonian;
 
 typedef struct {
@@ -164,6 +169,19 @@ int fgx_propagator_init(fgx_system *s);
  * record (dU and accept flag) up to `cap` trials. */
 int fgx_propagate(fgx_system *s, int ncalls, fgx_stats *st, double *du_out, int *acc_out,
                   long cap);
+
+/* Propagator::move() calls of a Propagator holding one Move::Isobaric
+ * (move.h:5424-5640): each call draws the move, its run fraction (prob; 0
+ * when dp < 1e-6), a volume trial V' = exp(ln V + dp (u - 1/2)) with every
+ * group scaled isotropically (Group::scale, group.h:318-355), the energy
+ * change at the trial geometry (energy.h:3577-3621) and the Metropolis
+ * decision, from the system's RNG engines.  Needs has_pressure; not with
+ * Ewald (its k-vectors would change with the box).  Optional per-trial
+ * record (dU, accept) up to `cap`. */
+int fgx_isobaric_propagate(fgx_system *s, double dp, double prob, int ncalls, fgx_stats *st,
+                           double *du_out, int *acc_out, long cap);
+/* Cuboid side lengths (Cuboid::setlen after an accepted volume move) */
+int fgx_get_box(const fgx_system *s, double len[3]);
 
 /* Launch schedule of the device sweeps: workgroup size (256 or 512 threads
  * per chain) and partner-loop batch depth (4 or 8) of the mixed path; 0

@@ -241,6 +241,9 @@ struct or_sys {
     int ntypes;
     double *s2, *eps4;   /* LennardJonesMixed tables (potentials.h:2010-2023) */
     double lj_rc2;       /* CutShift cutoff^2 (potentials.h:1785-1786); 0 = uncut */
+    int has_press;       /* ExternalPressure term present */
+    double press;        /* its pressure, 1/A^3 */
+    long iso_cnt;        /* Isobaric trial count (Movebase::cnt) */
     double *tdp;
     or_ewald *ew;
     /* RNG: global slump and the Movebase-internal _slump (move.h:356-360) */
@@ -726,7 +729,12 @@ double or_g_internal(or_sys *s, int which, int gi)
     return u;
 }
 
-double or_external(or_sys *s, int which)
+static double box_volume(const or_box *b)
+{   /* Cuboid::_getVolume (src/geometry.cpp:293-300) */
+    return b->len[0] * b->len[1] * b->len[2];
+}
+
+static double nonbonded_external(or_sys *s, int which)
 {   /* NonbondedEwald::external, ewald.h:716-729; Energybase default 0 */
     or_ewald *e = s->ew;
     if (!e) return 0.0;
@@ -740,10 +748,29 @@ double or_external(or_sys *s, int which)
     return e->surf + e->reci;
 }
 
+double or_external(or_sys *s, int which)
+{   /* CombinedEnergy (energy.h:646-749): the nonbonded term, then
+     * ExternalPressure::external = P V - ln V (energy.h:1878-1883) at the
+     * current geometry */
+    double u = nonbonded_external(s, which);
+    if (s->has_press) {
+        const double V = box_volume(&s->box);
+        u += s->press * V - log(V);
+    }
+    return u;
+}
+
 double or_g_external(or_sys *s, int which, int g)
-{   /* NonbondedEwald::g_external = getSelfEnergy over the group, ewald.h:712-714 */
-    if (!s->ew) return 0.0;
-    return ewald_self_range(s, vecp(s, which), s->g[g].front, s->g[g].back);
+{   /* NonbondedEwald::g_external = getSelfEnergy over the group, ewald.h:712-714;
+     * ExternalPressure::g_external = -N ln V, N molecules (energy.h:1885-1893) */
+    double u = s->ew ? ewald_self_range(s, vecp(s, which), s->g[g].front, s->g[g].back) : 0.0;
+    if (s->has_press) {
+        const or_group *gr = &s->g[g];
+        const int N = g_empty(gr) ? 0 : g_size(gr) / gr->molsize;
+        const double V = box_volume(&s->box);
+        u += -N * log(V);
+    }
+    return u;
 }
 
 /* Energy::systemEnergy (energy.h:3486-3491) -> setSpace + Energybase::
@@ -874,6 +901,8 @@ or_sys *or_create(const or_params *prm, const double box[3], int n,
         s->trial[i] = *q;
     }
     s->lj_rc2 = prm->lj_cut > 0 ? pow(prm->lj_cut, 2) : 0.0;   /* potentials.h:1786 */
+    s->has_press = prm->has_pressure;
+    s->press = prm->pressure;
     s->ng = ng;
     s->g = calloc((size_t)(ng > 0 ? ng : 1), sizeof(or_group));
     for (int k = 0; k < ng; ++k) {
@@ -1264,4 +1293,123 @@ void or_pt_exchange_seeded(int R, const double *u, int *slot, uint32_t ctl_seed,
     or_mt_seed(&glob, glob_seed);
     for (int t = 0; t < nsteps; ++t)
         or_pt_exchange(R, u, slot, &ctl, &glob, acc_out + (long)t * R, du_out + (long)t * R);
+}
+
+/* ===================================================================== */
+/* Move::Isobaric (move.h:5424-5640)                                       */
+/* ===================================================================== */
+
+void or_get_box(or_sys *s, double *len)
+{
+    memcpy(len, s->box.len, sizeof(double) * 3);
+}
+
+/* Group::scale (group.h:318-355) for a Cuboid scaled isotropically by xyz
+ * (Cuboid::scale, src/geometry.cpp:326-332), after setMassCenter
+ * (group.h:174-179); trial == p on entry. */
+static int iso_scale_group(or_sys *s, or_group *g, double xyz, const double oldlen[3],
+                           const double newlen[3])
+{
+    mass_center(s, s->p, g, g->cm);
+    memcpy(g->cm_trial, g->cm, sizeof g->cm);
+    if (g_empty(g)) return 0;
+    if (g_atomic(g)) {
+        for (int k = 0; k < 3; ++k) g->cm_trial[k] = g->cm[k] * 1.0 * xyz;
+        for (int i = g->front; i <= g->back; ++i)
+            for (int k = 0; k < 3; ++k) s->trial[i].r[k] = s->trial[i].r[k] * 1.0 * xyz;
+        return 0;
+    }
+    if (g_molecular(g)) {
+        double newcm[3], mcm[3];
+        for (int k = 0; k < 3; ++k) {
+            newcm[k] = g->cm[k] * 1.0 * xyz;
+            mcm[k] = -g->cm[k];
+        }
+        or_box ob, nb;
+        box_set(&ob, oldlen);
+        box_set(&nb, newlen);
+        /* translate(spc, -cm): Point::translate = += then boundary (point.h:95-100) */
+        for (int k = 0; k < 3; ++k) g->cm_trial[k] += mcm[k];
+        box_boundary(&ob, g->cm_trial);
+        for (int i = g->front; i <= g->back; ++i) {
+            for (int k = 0; k < 3; ++k) s->trial[i].r[k] += mcm[k];
+            box_boundary(&ob, s->trial[i].r);
+        }
+        for (int i = g->front; i <= g->back; ++i) {
+            for (int k = 0; k < 3; ++k) s->trial[i].r[k] += newcm[k];
+            box_boundary(&nb, s->trial[i].r);
+        }
+        memcpy(g->cm_trial, newcm, sizeof newcm);
+        return 0;
+    }
+    return -1;   /* several molecules in one group (isRange) */
+}
+
+long or_isobaric_propagate(or_sys *s, double dp, double prob, int ncalls, double *du_out,
+                           int *acc_out, long cap, double *utot)
+{
+    if (s->ew) return -1;
+    const int ng = s->ng;
+    int *mvg = malloc(sizeof(int) * (ng > 0 ? ng : 1));
+    int *off = malloc(sizeof(int) * (ng + 1));
+    int *idx = malloc(sizeof(int) * (s->n > 0 ? s->n : 1));
+    long nt = 0;
+    double ut = 0.0;
+    /* Isobaric constructor: runfraction = prob, 0 when dp < 1e-6 (move.h:5462-5464) */
+    const double runfraction = dp < 1e-6 ? 0.0 : prob;
+    for (int c = 0; c < ncalls; ++c) {
+        (void)or_range(&s->internal, 0, 0);                 /* Propagator::move, move.h:7887 */
+        if (!(or_canonical(&s->internal) < runfraction)) continue;   /* Movebase::run */
+        /* Isobaric::_trialMove (move.h:5529-5567) */
+        s->iso_cnt++;
+        double oldlen[3], newlen[3];
+        memcpy(oldlen, s->box.len, sizeof oldlen);
+        const double oldval = box_volume(&s->box);
+        const double newval = exp(log(oldval) + (or_canonical(&s->slump) - 0.5) * dp);
+        const double xyz = cbrt(newval / oldval);
+        for (int k = 0; k < 3; ++k) newlen[k] = oldlen[k] * 1.0 * xyz;
+        int nmv = 0, ni = 0, bad = 0;
+        off[0] = 0;
+        for (int g = 0; g < ng; ++g) {
+            bad |= iso_scale_group(s, &s->g[g], xyz, oldlen, newlen);
+            mvg[nmv] = g;
+            if (g_atomic(&s->g[g]) && !g_empty(&s->g[g]))
+                for (int i = s->g[g].front; i <= s->g[g].back; ++i) idx[ni++] = i;
+            off[++nmv] = ni;
+        }
+        if (bad) { nt = -1; break; }
+        /* Energy::energyChange with a geometry change (energy.h:3577-3621) */
+        double du;
+        box_set(&s->box, newlen);
+        int coll = 0;
+        for (int i = 0; i < s->n && !coll; ++i) coll = box_collision(&s->box, s->trial[i].r);
+        if (coll) {
+            box_set(&s->box, oldlen);
+            du = INFINITY;
+        } else {
+            const double dnew = change_config(s, 1, nmv, mvg, off, idx);
+            box_set(&s->box, oldlen);
+            const double dold = change_config(s, 0, nmv, mvg, off, idx);
+            du = dnew - dold;
+        }
+        /* Movebase::metropolis (move.h:567-573) */
+        const int acc = !(or_canonical(&s->slump) > exp(-du));
+        if (acc) {   /* Isobaric::_acceptMove (move.h:5570-5580) */
+            box_set(&s->box, newlen);
+            for (int i = 0; i < s->n; ++i) s->p[i] = s->trial[i];
+            for (int g = 0; g < ng; ++g) memcpy(s->g[g].cm, s->g[g].cm_trial, sizeof s->g[g].cm);
+            ut += du;
+        } else {     /* Isobaric::_rejectMove (move.h:5582-5592) */
+            for (int i = 0; i < s->n; ++i) s->trial[i] = s->p[i];
+            for (int g = 0; g < ng; ++g) memcpy(s->g[g].cm_trial, s->g[g].cm, sizeof s->g[g].cm);
+        }
+        if (nt < cap) {
+            if (du_out) du_out[nt] = du;
+            if (acc_out) acc_out[nt] = acc;
+        }
+        nt++;
+    }
+    free(mvg); free(off); free(idx);
+    if (utot) *utot = ut;
+    return nt;
 }

@@ -67,6 +67,10 @@ typedef struct {
     int spherical_sum, update_frequency;
     /* Potential::CutShift<LennardJonesLB> cutoff (A); <= 0 = uncut LJ */
     double lj_cut;
+    /* Energy::ExternalPressure in the Hamiltonian (energy.h:1817-1894):
+     * pressure in 1/A^3 (moves.isobaric.pressure in mM times 1.0_mM) */
+    int has_pressure;
+    double pressure;
 } or_params;
 
 typedef struct or_sys or_sys;
@@ -128,6 +132,15 @@ long or_propagate(or_sys *s, int ncalls, double *du_out, int *acc_out, long cap,
                   double *utot, double *uinit);
 /* per-trial extra record: moved index (AT) or group (TR) */
 long or_last_trials(or_sys *s, int *which_out, long cap);
+
+/* A Propagator holding one Move::Isobaric (move.h:5424-5640) with the
+ * ExternalPressure term in the Hamiltonian: `ncalls` Propagator::move()
+ * calls, each one volume trial.  Records dU and accept flags (up to cap);
+ * returns the number of trials, or -1 if the system is not supported
+ * (Ewald; groups holding several molecules). */
+long or_isobaric_propagate(or_sys *s, double dp, double prob, int ncalls, double *du_out,
+                           int *acc_out, long cap, double *utot);
+void or_get_box(or_sys *s, double *len);
 
 /* ParallelTempering (move.h:6520-6772) restated for R replicas in one
  * process.  u[r][c] is replica r's Hamiltonian evaluated on configuration c;

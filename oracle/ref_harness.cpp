@@ -12,7 +12,7 @@
 //   ref_harness energy <kind> <config.json> <statefile>
 //   ref_harness mc <kind> <config.json> <statefile> <ncalls>
 //   ref_harness mctime <kind> <config.json> <statefile> <ncalls>   (CPU baseline)
-// kind: dhhs | coulhs | lj | dh | ewaldlj | dhhs_cut | dhhs_vec | ljcut
+// kind: dhhs | coulhs | lj | dh | ewaldlj | dhhs_cut | dhhs_vec | ljcut | dhhs_npt
 // Output: one JSON object on stdout, doubles printed with %.17g.
 
 #include <faunus/faunus.h>
@@ -103,6 +103,25 @@ protected:
 };
 
 
+template<class Tspace>
+class RecIso : public Move::Isobaric<Tspace> {
+    typedef Move::Isobaric<Tspace> base;
+public:
+    Recorder *rec;
+    template<class Tenergy>
+    RecIso(Tenergy &e, Tspace &s, Tmjson &j, Recorder *r) : base(e, s, j), rec(r) {}
+protected:
+    double _energyChange() override {
+        double du = base::_energyChange();
+        rec->du.push_back(du);
+        rec->who.push_back(-1);
+        return du;
+    }
+    void _acceptMove() override { rec->acc.push_back(1); base::_acceptMove(); }
+    void _rejectMove() override { rec->acc.push_back(0); base::_rejectMove(); }
+};
+
+
 // Atom-type table and Bjerrum length as the reference derived them from the
 // JSON input (species.h:279-417 unit conversions, physconst.h:91-94).
 static void print_params(Tmjson &in)
@@ -118,16 +137,13 @@ static void print_params(Tmjson &in)
     jnum("lB", pc::lB(epsr));
 }
 
+// Every Energybase entry point of `pot` on the loaded space (prints the
+// JSON fields after the parameters).
 template<class Tenergy>
-static int run_energy(Tmjson &in, const char *state)
+static void print_energies(Tspace &spc, Tenergy &pot)
 {
-    Tspace spc(in);
-    Tenergy pot(in);
-    spc.load(state, Tspace::RESIZE);
     pot.setSpace(spc);
     int n = (int)spc.p.size();
-    std::printf("{\n");
-    print_params(in);
     double usys = Energy::systemEnergy(spc, pot, spc.p);
     jnum("system_energy", usys);
     jnum("external", pot.external(spc.p));
@@ -162,6 +178,17 @@ static int run_energy(Tmjson &in, const char *state)
     jivec("i2i_i", pi); jivec("i2i_j", pj); jvec("i2i", i2i);
     jivec("i2all_i", ai); jvec("i2all", i2all);
     jivec("i2g_g", gi); jvec("i2g", i2g, false);
+}
+
+template<class Tenergy>
+static int run_energy(Tmjson &in, const char *state)
+{
+    Tspace spc(in);
+    Tenergy pot(in);
+    spc.load(state, Tspace::RESIZE);
+    std::printf("{\n");
+    print_params(in);
+    print_energies(spc, pot);
     std::printf("\n}\n");
     return 0;
 }
@@ -251,6 +278,51 @@ static int run_mctime(Tmjson &in, const char *state, int ncalls)
 }
 
 typedef CombinedPairPotential<DebyeHuckel, HardSphere> TDHHS;
+
+// Nonbonded<DebyeHuckelHS> + ExternalPressure (energy.h:1817-1894) with a
+// Propagator holding one Isobaric move (move.h:5424-5640): energies, or a
+// recorded Metropolis run.
+static int run_npt(const std::string &mode, Tmjson &in, const char *state, int ncalls)
+{
+    Tspace spc(in);
+    auto pot = Energy::Nonbonded<Tspace, TDHHS>(in) + Energy::ExternalPressure<Tspace>(in);
+    spc.load(state, Tspace::RESIZE);
+    const double P = in["moves"]["isobaric"].at("pressure").get<double>() * 1.0_mM;
+    std::printf("{\n");
+    print_params(in);
+    jnum("pressure", P);
+    if (mode == "energy") {
+        print_energies(spc, pot);
+        std::printf("\n}\n");
+        return 0;
+    }
+    Recorder rec;
+    RecAT<Tspace>::internal().eng = slump.eng;   // Propagator constructor (move.h:7781)
+    RecIso<Tspace> mv(pot, spc, in["moves"]["isobaric"], &rec);
+    double uinit = Energy::systemEnergy(spc, pot, spc.p);
+    double utot = 0;
+    for (int c = 0; c < ncalls; ++c) {
+        (void)RecAT<Tspace>::internal().range(0, 0);
+        utot += mv.move();
+    }
+    double ufinal = Energy::systemEnergy(spc, pot, spc.p);
+    jnum("uinit", uinit);
+    jnum("ufinal", ufinal);
+    jnum("utot", utot);
+    jvec("du", rec.du);
+    jivec("acc", rec.acc);
+    std::vector<double> xyz;
+    for (auto &p : spc.p) { xyz.push_back(p.x()); xyz.push_back(p.y()); xyz.push_back(p.z()); }
+    jvec("final_xyz", xyz);
+    std::vector<double> cm;
+    for (auto g : spc.groupList()) { cm.push_back(g->cm.x()); cm.push_back(g->cm.y()); cm.push_back(g->cm.z()); }
+    jvec("final_cm", cm);
+    jvec("final_len", {spc.geo.len.x(), spc.geo.len.y(), spc.geo.len.z()});
+    std::ostringstream o;
+    o << slump.eng;
+    std::printf("\"final_rng\": \"%s\"\n}\n", o.str().c_str());
+    return 0;
+}
 typedef CombinedPairPotential<Coulomb, HardSphere> TCoulHS;
 
 int main(int argc, char **argv)
@@ -305,6 +377,7 @@ int main(int argc, char **argv)
     if (kind == "lj") { DISPATCH(E_lj); }
     if (kind == "ewaldlj") { DISPATCH(E_ewaldlj); }
     if (kind == "ljcut") { DISPATCH(E_ljcut); }
+    if (kind == "dhhs_npt") return run_npt(mode, in, state, ncalls);
     std::fprintf(stderr, "unknown kind %s\n", kind.c_str());
     return 2;
 }

@@ -23,9 +23,12 @@ def fgx_system_from_fixture(fx, precision=fgx.FP64, moves=True, ewald_override=N
                    g2g_cut=nb.get("cutoff_g2g", 0.0),
                    type_sigma=e["type_sigma"], type_eps=e["type_eps"],
                    type_charge=e["type_charge"], type_radius=e["type_radius"],
-                   type_dp=e["type_dp"], ewald=ew, lj_cut=lj_cut_of(fx["kind"], nb))
+                   type_dp=e["type_dp"], ewald=ew, lj_cut=lj_cut_of(fx["kind"], nb),
+                   pressure=e.get("pressure"))
     if moves:
         for mv in fx["moves"]:
+            if mv["type"] == "NPT":   # driven by isobaric_propagate
+                continue
             kind = fgx.MOVE_AT if mv["type"] == "AT" else fgx.MOVE_TR
             s.add_move(kind, mv["molid"], prob=mv.get("prob", 1.0),
                        peratom=mv.get("peratom", 0), permol=mv.get("permol", 0),

@@ -153,6 +153,30 @@ def salt(name, kind, n, L, seed, ncalls, elec_cfg, dp=2.0, r=2.0):
     return build_fixture(name, kind, cfg, [L, L, L], parts, groups, seed, ncalls, moves)
 
 
+def salt_npt(name, n, L, seed, ncalls, pressure_mM, dp, r=2.0):
+    """Salt with Debye-Hueckel + hard spheres and the ExternalPressure term,
+    a Propagator holding one Isobaric move (atomic salt group scaled)."""
+    rng = np.random.default_rng(seed)
+    radii = np.full(n, r)
+    xyz = rsa(rng, n, L, radii)
+    half = n // 2
+    ids = np.array([1] * half + [2] * (n - half))
+    charge = np.where(ids == 1, 1.0, -1.0)
+    cfg = {
+        "system": {"temperature": 298.15, "geometry": {"length": L}},
+        "atomlist": {"T1p": {"q": 1.0, "r": r, "dp": 2.0, "mw": 1.0},
+                     "T2m": {"q": -1.0, "r": r, "dp": 2.0, "mw": 1.0}},
+        "moleculelist": {"salt": {"atoms": "T1p T2m", "atomic": True, "Ninit": 0}},
+        "moves": {"isobaric": {"dp": dp, "pressure": pressure_mM, "prob": 1.0}},
+        "energy": {"nonbonded": {"epsr": lB_epsr(7.1), "debyelength": 10.0}},
+    }
+    parts = {"xyz": xyz, "charge": charge, "radius": radii, "mw": np.ones(n), "id": ids}
+    groups = [{"front": 0, "back": n - 1, "molid": 0, "molsize": 1}]
+    moves = [{"type": "NPT", "molid": -1, "prob": 1.0, "peratom": 0, "permol": 0, "dp": dp,
+              "pressure_mM": pressure_mM}]
+    return build_fixture(name, "dhhs_npt", cfg, [L, L, L], parts, groups, seed, ncalls, moves)
+
+
 def lj_mix(name, n, L, seed, ncalls, cutoff=None):
     rng = np.random.default_rng(seed)
     half = n // 2
@@ -336,6 +360,7 @@ def main():
     salt("salt_dh", "dh", 40, 20.0, 14, 8, dh)
     lj_mix("lj_mix", 50, 25.0, 15, 10)
     lj_mix("lj_cutshift", 80, 26.0, 21, 10, cutoff=8.0)
+    salt_npt("salt_npt", 60, 30.0, 22, 60, pressure_mM=300.0, dp=0.4)
     ewald("ewald_tinfoil", 40, 20.0, 16, 6, eps_surf=0.5, update_frequency=7)
     ewald("ewald_surface", 30, 18.0, 17, 4, eps_surf=1.0, update_frequency=-1, kc=3.0)
     molecules("mol_dhhs", "dhhs", 4, 12, 16, 60.0, 18, 12)

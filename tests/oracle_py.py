@@ -30,6 +30,8 @@ KINDS = {
     # LJ as Potential::CutShift<LennardJonesLB> at energy.nonbonded.cutoff
     "ljcut": ("none", "lj", "plain"),
     "ewaldljcut": ("ewald", "lj", "vector"),
+    # Nonbonded<DebyeHuckelHS> + ExternalPressure, Isobaric move
+    "dhhs_npt": ("dh", "hs", "plain"),
 }
 
 
@@ -50,6 +52,7 @@ class OrParams(C.Structure):
         ("debyelength", C.c_double), ("eps_surf", C.c_double),
         ("spherical_sum", C.c_int), ("update_frequency", C.c_int),
         ("lj_cut", C.c_double),
+        ("has_pressure", C.c_int), ("pressure", C.c_double),
     ]
 
 
@@ -87,6 +90,9 @@ def lib():
         L.or_ewald_get.argtypes = [P, I, dp, dp, dp, dp]
         L.or_ewald_update_change.restype = D; L.or_ewald_update_change.argtypes = [P, I, ip, ip, ip]
         L.or_ewald_update.restype = D; L.or_ewald_update.argtypes = [P, I]
+        L.or_isobaric_propagate.restype = C.c_long
+        L.or_isobaric_propagate.argtypes = [P, D, D, I, dp, ip, C.c_long, dp]
+        L.or_get_box.argtypes = [P, dp]
         for fn in ("or_ewald_self", "or_ewald_surface", "or_ewald_reciprocal"):
             getattr(L, fn).restype = D
             getattr(L, fn).argtypes = [P, I]
@@ -135,7 +141,7 @@ class OracleSystem:
 
     def __init__(self, kind, box, xyz, charge, radius, mw, ids, groups, lB, kappa=0.0,
                  type_sigma=None, type_eps=None, type_charge=None, type_radius=None,
-                 type_dp=None, g2g_cut=0.0, ewald=None, lj_cut=0.0):
+                 type_dp=None, g2g_cut=0.0, ewald=None, lj_cut=0.0, pressure=None):
         L = lib()
         elec, sr, nb = KINDS[kind]
         self._keep = []
@@ -150,6 +156,8 @@ class OracleSystem:
         prm.elec, prm.sr, prm.nbkind = ELEC[elec], SR[sr], NB[nb]
         prm.lB, prm.kappa, prm.g2g_cut = lB, kappa, g2g_cut
         prm.lj_cut = lj_cut
+        if pressure is not None:
+            prm.has_pressure, prm.pressure = 1, pressure
         prm.ntypes = T
         prm.sigma = dptr(arr(type_sigma))
         prm.eps = dptr(arr(type_eps))
@@ -192,13 +200,15 @@ class OracleSystem:
                    type_charge=e["type_charge"], type_radius=e["type_radius"],
                    type_dp=e["type_dp"], g2g_cut=nb.get("cutoff_g2g", 0.0),
                    ewald=nb.get("ewald") if KINDS[fx["kind"]][0] == "ewald" else None,
-                   lj_cut=lj_cut_of(fx["kind"], nb))
+                   lj_cut=lj_cut_of(fx["kind"], nb), pressure=e.get("pressure"))
         sys_.add_moves(fx)
         return sys_
 
     def add_moves(self, fx):
         L = lib()
         for mv in fx["moves"]:
+            if mv["type"] == "NPT":   # driven by isobaric_propagate
+                continue
             typ = 1 if mv["type"] == "AT" else 2
             L.or_add_move(self.h, typ, mv["molid"], mv.get("prob", 1.0), mv.get("peratom", 0),
                           mv.get("permol", 0), None, None, mv.get("dp", 0.0), mv.get("dprot", 0.0),
@@ -227,6 +237,21 @@ class OracleSystem:
     def set_pos(self, i, xyz, which=1):
         a = np.ascontiguousarray(np.asarray(xyz, dtype=np.float64))
         lib().or_set_pos(self.h, which, i, dptr(a))
+
+    def isobaric_propagate(self, dp, prob, ncalls, cap=1 << 16):
+        du = np.zeros(cap)
+        acc = np.zeros(cap, dtype=np.int32)
+        ut = C.c_double()
+        n = lib().or_isobaric_propagate(self.h, dp, prob, ncalls, dptr(du), iptr(acc), cap,
+                                        C.byref(ut))
+        if n < 0:
+            raise ValueError("oracle: isobaric move not supported for this system")
+        return du[:n], acc[:n], ut.value
+
+    def box(self):
+        b = np.zeros(3)
+        lib().or_get_box(self.h, dptr(b))
+        return b
 
     def energy_change(self, mv):
         """mv: list of (group, [indices]) pairs, ordered by group"""

@@ -37,7 +37,7 @@ def kind_args(kind):
                 sr={"none": fgx.SR_NONE, "hs": fgx.SR_HS, "lj": fgx.SR_LJ}[sr],
                 nbkind={"plain": fgx.NB_PLAIN, "vector": fgx.NB_VECTOR,
                         "cutg2g": fgx.NB_CUTG2G}[nb],
-                lj_cutshift=kind.endswith("ljcut"))
+                lj_cutshift=kind.endswith("ljcut"), external_pressure=kind.endswith("npt"))
 
 
 def write_inputs(fx, d):
@@ -78,6 +78,8 @@ def test_reader_matches_reference_parameters(name):
         np.testing.assert_allclose(w["ham"][mine], np.asarray(e[ref], dtype=np.float64),
                                    rtol=1e-15, atol=0, err_msg=mine)
     assert w["ham"]["lB"] == pytest.approx(e["lB"], rel=1e-15)
+    if "pressure" in e:
+        assert w["ham"]["pressure"] == pytest.approx(e["pressure"], rel=1e-15)
     if "mc" not in fx:   # energies-only fixture
         return
     # moves: what the fixture's Propagator registered
@@ -88,6 +90,8 @@ def test_reader_matches_reference_parameters(name):
         if m["type"] == "TR":
             assert (m["dp"], m["dprot"], m["center_rotation"]) == \
                 (r["dp"], r["dprot"], r["center_rotation"])
+        if m["type"] == "NPT":
+            assert (m["dp"], m["pressure_mM"]) == (r["dp"], r["pressure_mM"])
 
 
 def test_state_round_trip():
@@ -116,7 +120,7 @@ def test_reference_example_state():
     assert sorted(set(w["charge"].tolist())) == [-1.0, 1.0]
     assert w["atom_names"] == ["unk", "Cl", "Na"] and w["T"] == 1100
     # its input registers an isobaric move beside atomtranslate: energies only
-    assert w["move"] is None and w["other_moves"] == ["isobaric"]
+    assert w["move"] is None and [m["type"] for m in w["moves"]] == ["AT", "NPT"]
     assert faunus_io.mt_parse(w["rng"])[1] <= 624
 
 

@@ -34,7 +34,7 @@ def gpu():
 
 
 @pytest.mark.parametrize("name", ["salt_dhhs", "lj_cutshift", "ewald_tinfoil", "mol_dhhs",
-                                  "mol_dhhs_offcenter"])
+                                  "mol_dhhs_offcenter", "salt_npt"])
 def test_run_from_reference_files(gpu, name):
     from beofaunus_amd import faunus_io
     fx = load_fixture(name)
@@ -53,6 +53,8 @@ def test_run_from_reference_files(gpu, name):
     cm = np.array([g["cm"] for g in fin["groups"]])
     assert np.max(np.abs(cm - ref_cm)) <= 1e-9
     assert fin["rng"] == mc["final_rng"].strip()
+    if "final_len" in mc:
+        np.testing.assert_allclose(fin["box"], mc["final_len"], rtol=1e-12)
 
 
 def test_reference_example_energies(gpu):
