@@ -143,3 +143,30 @@ def test_full_size_energy_drift(gpu):
     assert cells[0] == 9
     assert st["trials"] == 20000
     assert abs(drift) <= 1e-6 * max(1.0, abs(u0))
+
+
+@pytest.mark.parametrize("case", ["lj_table", "three_cells"])
+def test_cell_list_variants(gpu, case):
+    """Per-type LJ tables (the scan reads partner types) and the smallest
+    grid of 3 x 3 x 3 cells: cell list and all-pairs screen agree."""
+    if case == "lj_table":
+        w = electrolyte(2000, seed=45)
+        w["ham"] = dict(w["ham"], type_sigma=[0.0, 4.0, 3.2], type_eps=[0.0, 0.1, 0.15])
+    else:
+        from beofaunus_amd import workloads
+        # side 40 A: 3 cells of 13.3 A per axis
+        w = workloads.ewald_electrolyte(n=740, L=40.0, seed=46, lj_cut=True)
+    say("[cells] %s, N=%d, L=%.1f" % (case, len(w["charge"]), w["box"][0]))
+    _, dp, ap, _ = run(gpu, w, gpu.MIXED, 21, 2, cell_mode=0)
+    _, dc, ac, cells = run(gpu, w, gpu.MIXED, 21, 2, cell_mode=1)
+    _, d64, a64, _ = run(gpu, w, gpu.FP64, 21, 2)
+    say("  cell list %s" % (cells,))
+    assert cells[0] >= 3
+    if case == "three_cells":
+        assert cells[0] == 3
+    assert np.array_equal(ac, ap)
+    assert np.max(np.abs(dc - dp)) <= 1e-9
+    mism = np.nonzero(ac != a64)[0]
+    upto = int(mism[0]) if len(mism) else len(a64)
+    assert upto >= 500
+    assert np.max(np.abs(dc[:upto] - d64[:upto])) <= TOL_MIXED
