@@ -71,6 +71,36 @@ def test_alone_equals_in_mix(gpu):
     assert abs(ra["utot"] - rm["utot"]) <= 1e-12
 
 
+def test_cell_list_job_alone_equals_in_mix(gpu):
+    """A cut-LJ Ewald job (cell-list sweep) gives the same result alone and
+    served beside jobs of other kinds, through per-tick trial budgets that
+    stop and resume its Propagator::move() calls."""
+    from beofaunus_amd import workloads
+    say("[server] cell-list job alone vs in mix")
+    a = workloads.ewald_electrolyte(n=2000, L=55.7, seed=24, lj_cut=True)
+    b = workloads.ewald_electrolyte(n=600, L=40.0, seed=25)
+    c = workloads.salt_dh(n=500, L=50.0, seed=26, lattice=True)
+    alone = gpu.Server(1 << 30, tick_seconds=0.5)
+    ja = alone.submit(a, seed=8, sweeps=3, keep_positions=True)
+    ra = drain(alone)[ja]
+    alone.close()
+    mix = gpu.Server(1 << 30, tick_seconds=0.002)
+    jb = mix.submit(b, seed=9, sweeps=2, keep_positions=True)
+    ja2 = mix.submit(a, seed=8, sweeps=3, keep_positions=True)
+    jc = mix.submit(c, seed=10, sweeps=4, keep_positions=True)
+    out = drain(mix)
+    mix.close()
+    rm = out[ja2]
+    say("  alone: trials %d utot %.12f; in mix: trials %d utot %.12f"
+        % (ra["trials"], ra["utot"], rm["trials"], rm["utot"]))
+    assert ra["status"] == 0 and rm["status"] == 0
+    assert out[jb]["status"] == 0 and out[jc]["status"] == 0
+    assert ra["trials"] == rm["trials"] == 3 * 2000
+    assert ra["accepted"] == rm["accepted"]
+    assert np.array_equal(ra["xyz"], rm["xyz"])
+    assert abs(ra["utot"] - rm["utot"]) <= 1e-12
+
+
 def test_budget_admission(gpu):
     """A budget that holds about one job at a time: every job completes, the
     peak resident footprint stays within the budget, and later jobs wait."""
