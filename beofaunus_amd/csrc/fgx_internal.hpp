@@ -106,6 +106,7 @@ struct fgx_system {
     bool has_ewald = false;
     fgx::EwaldDev ew{};
     std::vector<double> h_eff;
+    std::vector<int> h_kn;                 // k-vector index triples (3K)
     // moves and RNG
     std::vector<fgx::MoveDesc> moves;
     std::vector<long> move_cnt;
@@ -158,6 +159,7 @@ namespace fgx {
 // implemented in fgx_system.hip, used by fgx_sweep.hip
 int ensure_partials(fgx_system *s, int nblocks);
 int ewald_full_recompute(fgx_system *s, int which, double *Qdst);
+int ewald_set_space(fgx_system *s);
 int ewald_reciprocal(fgx_system *s, const double *Q, double V, double *out_host);
 void mt_seed_host(uint32_t *state625, uint32_t seed);
 int pull_device_state(fgx_system *s);

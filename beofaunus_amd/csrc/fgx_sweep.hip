@@ -2292,9 +2292,19 @@ static int launch_chains(std::vector<fgx_system *> &sys, const std::vector<int> 
                       s->pot.sr == s0->pot.sr,
                   FGX_EUNSUPPORTED, "batched chains must share move type and precision");
         // Propagator::move evaluates uinit = systemEnergy before its first move
-        // (move.h:7884-7885) only for drift reporting; S(k) is already the full
-        // sum over the accepted vector, as setSpace would leave it, so the
-        // O(N^2) energy is left to callers that ask for it (fgx_system_energy).
+        // (move.h:7884-7885).  The energy itself only serves drift reporting
+        // and is left to callers that ask for it (fgx_system_energy), but its
+        // setSpace recomputes S(k) from the accepted vector, which the first
+        // call does here too (S(k) may differ from that sum, e.g. after an
+        // accepted volume move).
+        if (!s->propagator_started && s->has_ewald) {
+            if ((rc = ewald_set_space(s))) break;
+            if (hipStreamSynchronize(s->stream) != hipSuccess) {
+                set_error("hipStreamSynchronize failed");
+                rc = FGX_EHIP;
+                break;
+            }
+        }
         s->propagator_started = true;
         if (s->move_cnt.empty()) s->move_cnt.assign(s->moves.size(), 0);
         if (s->move_cnt[0] == 0 && (rc = init_mass_centres(s))) break;

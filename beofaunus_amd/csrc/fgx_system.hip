@@ -546,6 +546,30 @@ static int upload(fgx_system *s, T *dst, const T *src, size_t count)
     return FGX_OK;
 }
 
+// k-vectors 2 pi n / L and A_k = f exp(-(k^2 + kappa^2) / 4 alpha^2) / (k^2 +
+// kappa^2) of the index triples h_kn for the box `L` (kVectorChange,
+// ewald.h:330-378).  Which triples enter depends on kc only: every nonzero
+// triple has k^2 above check_k2_zero = 0.1 (2 pi / max L)^2 for any box, so a
+// box change keeps the set and rescales the vectors and A_k.
+static void ewald_tables(const fgx_system *s, const double *L, std::vector<double> &kv,
+                         std::vector<double> &aks)
+{
+    const EwaldDev &e = s->ew;
+    const double pi = std::acos(-1.), twopi = 2 * pi;
+    const double alpha2 = e.alpha * e.alpha;
+    kv.resize(3 * (size_t)e.K);
+    aks.resize(e.K);
+    for (int k = 0; k < e.K; ++k) {
+        const int kx = s->h_kn[3 * k], ky = s->h_kn[3 * k + 1], kz = s->h_kn[3 * k + 2];
+        double factor = (kx > 0) ? 2.0 : 1.0;
+        double v[3] = {twopi * (kx / L[0]), twopi * (ky / L[1]), twopi * (kz / L[2])};
+        double k2 = v[0] * v[0] + (v[1] * v[1] + v[2] * v[2]);
+        for (int a = 0; a < 3; ++a) kv[3 * k + a] = v[a];
+        aks[k] = factor * std::exp(-(k2 + e.kappa * e.kappa) / (4.0 * alpha2)) /
+                 (k2 + e.kappa * e.kappa);
+    }
+}
+
 static int build_ewald(fgx_system *s, const fgx_hamiltonian *h)
 {   // NonbondedEwald ctor + kVectorChange (ewald.h:507-550, 330-378)
     EwaldDev &e = s->ew;
@@ -561,15 +585,12 @@ static int build_ewald(fgx_system *s, const fgx_hamiltonian *h)
     const double *L = s->box.len;
     double maxL = std::max(L[0], std::max(L[1], L[2]));
     double check_k2_zero = 0.1 * (4 * pi * pi) / (maxL * maxL);
-    std::vector<double> kv, aks;
     std::vector<int> kn;
-    const double alpha2 = e.alpha * e.alpha;
     for (int kx = 0; kx <= e.kcc; kx++) {
         double dkx2 = (double)(kx * kx);
         for (int ky = -e.kcc; ky <= e.kcc; ky++) {
             double dky2 = (double)(ky * ky);
             for (int kz = -e.kcc; kz <= e.kcc; kz++) {
-                double factor = (kx > 0) ? 2.0 : 1.0;
                 double dkz2 = (double)(kz * kz);
                 double twopi = 2 * pi;
                 double v[3] = {twopi * (kx / L[0]), twopi * (ky / L[1]), twopi * (kz / L[2])};
@@ -577,15 +598,15 @@ static int build_ewald(fgx_system *s, const fgx_hamiltonian *h)
                 if (k2 < check_k2_zero) continue;
                 if (h->ewald_spherical_sum)
                     if ((dkx2 / kc2) + (dky2 / kc2) + (dkz2 / kc2) > 1.0) continue;
-                kv.insert(kv.end(), v, v + 3);
                 kn.push_back(kx); kn.push_back(ky); kn.push_back(kz);
-                aks.push_back(factor * std::exp(-(k2 + e.kappa * e.kappa) / (4.0 * alpha2)) /
-                              (k2 + e.kappa * e.kappa));
             }
         }
     }
-    FGX_CHECK(!kv.empty(), FGX_EUNSUPPORTED, "Ewald: no k-vectors (cutoffK too small)");
-    e.K = (int)aks.size();
+    FGX_CHECK(!kn.empty(), FGX_EUNSUPPORTED, "Ewald: no k-vectors (cutoffK too small)");
+    e.K = (int)kn.size() / 3;
+    s->h_kn = kn;
+    std::vector<double> kv, aks;
+    ewald_tables(s, L, kv, aks);
     int rc;
     if ((rc = dalloc(s, &e.kv, kv.size()))) return rc;
     if ((rc = dalloc(s, &e.aks, aks.size()))) return rc;
@@ -602,13 +623,54 @@ static int build_ewald(fgx_system *s, const fgx_hamiltonian *h)
 }
 
 // NonbondedEwald::setSpace (ewald.h:750-757): recompute S(k) from p, trial = current
-static int ewald_set_space(fgx_system *s)
+int fgx::ewald_set_space(fgx_system *s)
 {
     if (!s->has_ewald) return FGX_OK;
     int rc = ewald_full_recompute(s, 0, s->ew.Q);
     if (rc) return rc;
     FGX_HIP(hipMemcpyAsync(s->ew.Qt, s->ew.Q, sizeof(double) * 2 * s->ew.K,
                            hipMemcpyDeviceToDevice, s->stream));
+    return FGX_OK;
+}
+
+// NonbondedEwald::setGeometry (ewald.h:731-746) for a box `len`: k-vectors,
+// A_k and V of that box.  S(k) are left as they are (setSpace recomputes them).
+static int ewald_rebox(fgx_system *s, const double len[3])
+{
+    if (!s->has_ewald) return FGX_OK;
+    std::vector<double> kv, aks;
+    ewald_tables(s, len, kv, aks);
+    int rc;
+    if ((rc = upload(s, s->ew.kv, kv.data(), kv.size()))) return rc;
+    if ((rc = upload(s, s->ew.aks, aks.data(), aks.size()))) return rc;
+    FGX_HIP(hipStreamSynchronize(s->stream));
+    s->ew.V = len[0] * len[1] * len[2];
+    return FGX_OK;
+}
+
+// NonbondedEwald::update(true) (ewald.h:626-648): the trial S(k) become the
+// current ones; every update_frequency accepted moves they are recomputed
+// from the (now accepted) trial vector and the change of the reciprocal
+// energy is returned in *drift
+static int ewald_accepted(fgx_system *s, double *drift)
+{
+    EwaldDev &e = s->ew;
+    *drift = 0.0;
+    if (++e.cnt_accepted > e.update_frequency - 1) {
+        double duB, duA;
+        int rc;
+        if ((rc = ewald_reciprocal(s, e.Qt, e.V, &duB))) return rc;
+        if ((rc = ewald_full_recompute(s, 1, e.Qt))) return rc;
+        if ((rc = ewald_reciprocal(s, e.Qt, e.V, &duA))) return rc;
+        FGX_HIP(hipMemcpyAsync(e.Q, e.Qt, sizeof(double) * 2 * e.K, hipMemcpyDeviceToDevice,
+                               s->stream));
+        e.cnt_accepted = 0;
+        e.update_drift += std::fabs(duA - duB);
+        *drift = duA - duB;
+    } else {
+        FGX_HIP(hipMemcpyAsync(e.Q, e.Qt, sizeof(double) * 2 * e.K, hipMemcpyDeviceToDevice,
+                               s->stream));
+    }
     return FGX_OK;
 }
 
@@ -1425,19 +1487,8 @@ extern "C" int fgx_commit(fgx_system *s, const fgx_change *c, int accepted, doub
         if (!accepted) {
             FGX_HIP(hipMemcpyAsync(e.Qt, e.Q, sizeof(double) * 2 * e.K, hipMemcpyDeviceToDevice,
                                    s->stream));
-        } else if (++e.cnt_accepted > e.update_frequency - 1) {
-            double duB, duA;
-            if ((rc = ewald_reciprocal(s, e.Qt, e.V, &duB))) return rc;
-            if ((rc = ewald_full_recompute(s, 1, e.Qt))) return rc;
-            if ((rc = ewald_reciprocal(s, e.Qt, e.V, &duA))) return rc;
-            FGX_HIP(hipMemcpyAsync(e.Q, e.Qt, sizeof(double) * 2 * e.K, hipMemcpyDeviceToDevice,
-                                   s->stream));
-            e.cnt_accepted = 0;
-            e.update_drift += std::fabs(duA - duB);
-            d = duA - duB;
-        } else {
-            FGX_HIP(hipMemcpyAsync(e.Q, e.Qt, sizeof(double) * 2 * e.K, hipMemcpyDeviceToDevice,
-                                   s->stream));
+        } else if ((rc = ewald_accepted(s, &d))) {
+            return rc;
         }
     }
     FGX_HIP(hipStreamSynchronize(s->stream));
@@ -1569,8 +1620,6 @@ extern "C" int fgx_isobaric_propagate(fgx_system *s, double dp, double prob, int
               "fgx_isobaric_propagate: bad arguments");
     FGX_CHECK(s->press_on, FGX_ESTATE,
               "Isobaric needs the ExternalPressure term in the Hamiltonian (move.h:5466-5472)");
-    FGX_CHECK(!s->has_ewald, FGX_EUNSUPPORTED,
-              "Isobaric with Ewald: the k-vectors change with the box (not on the device path)");
     for (int g = 0; g < s->ng; ++g) {
         const GroupDesc &G = s->groups[g];
         const int sz = G.back - G.front + 1;
@@ -1588,12 +1637,18 @@ extern "C" int fgx_isobaric_propagate(fgx_system *s, double dp, double prob, int
     // Isobaric constructor: runfraction = prob, 0 when dp < 1e-6 (move.h:5462-5464)
     const double runfraction = dp < 1e-6 ? 0.0 : prob;
     long trials = 0, accepted = 0, inf = 0;
-    double utot = 0.0;
+    double utot = 0.0;   // Movebase::move: accepted dU plus update() drift terms
     hipEvent_t e0, e1;
     FGX_HIP(hipEventCreate(&e0));
     FGX_HIP(hipEventCreate(&e1));
     FGX_HIP(hipEventRecord(e0, s->stream));
     for (int call = 0; call < ncalls && !rc; ++call) {
+        // Propagator::move evaluates uinit = systemEnergy before its first
+        // move (move.h:7884-7885), whose setSpace recomputes S(k)
+        if (!s->propagator_started) {
+            if ((rc = ewald_set_space(s))) break;
+            s->propagator_started = true;
+        }
         (void)I.range(0, 0);                       // Propagator::move (move.h:7887)
         if (!(I.canonical() < runfraction)) continue;   // Movebase::run (move.h:575-581)
         // ---- Isobaric::_trialMove (move.h:5529-5567)
@@ -1632,7 +1687,14 @@ extern "C" int fgx_isobaric_propagate(fgx_system *s, double dp, double prob, int
                 }
             }
         }
-        // ---- Energy::energyChange with the trial geometry (energy.h:3577-3621)
+        // ---- Energy::energyChange with the trial geometry (energy.h:3577-3621).
+        // Each setSpace there (and in _acceptMove / _rejectMove) makes
+        // NonbondedEwald take the k-vectors, A_k and V of the Space's box and
+        // recompute S(k) from the accepted coordinates (ewald.h:731-757); the
+        // trial configuration's reciprocal energy therefore uses the S(k) of
+        // the unscaled coordinates under the trial box's k-vectors.  (On a
+        // collision the trial geometry stays set until _rejectMove, which
+        // restores it; nothing is computed with it in between.)
         if ((rc = fgx_set_positions(s, 1, 0, n, t.data()))) break;
         for (int g = 0; g < ng && !rc; ++g) rc = fgx_set_group_cm(s, g, nullptr, &cmt[3 * g]);
         if (rc) break;
@@ -1644,26 +1706,35 @@ extern "C" int fgx_isobaric_propagate(fgx_system *s, double dp, double prob, int
         if (coll) {
             du = HUGE_VAL;
         } else {
-            const Box saved = s->box;
             set_box(s->box, newlen);
             double unew = 0.0, uold = 0.0;
-            rc = all_groups_config(s, 1, &unew);
-            s->box = saved;
-            if (!rc) rc = all_groups_config(s, 0, &uold);
+            if (!(rc = ewald_rebox(s, newlen)) && !(rc = ewald_set_space(s)))
+                rc = all_groups_config(s, 1, &unew);
+            set_box(s->box, oldlen);
+            if (!rc && !(rc = ewald_rebox(s, oldlen)) && !(rc = ewald_set_space(s)))
+                rc = all_groups_config(s, 0, &uold);
             if (rc) break;
             du = unew - uold;
         }
         // ---- Movebase::metropolis (move.h:567-573)
         const bool acc = !(G.canonical() > std::exp(-du));
-        if (acc) {   // Isobaric::_acceptMove (move.h:5570-5580)
+        double drift = 0.0;
+        if (acc) {   // Isobaric::_acceptMove (move.h:5570-5580): setSpace with the
+                     // new box while p still holds the unscaled coordinates, then
+                     // the groups take their trial coordinates
             set_box(s->box, newlen);
+            if (!(rc = ewald_rebox(s, newlen))) rc = ewald_set_space(s);
             p = t;
-            rc = fgx_set_positions(s, 2, 0, n, p.data());
+            if (!rc) rc = fgx_set_positions(s, 2, 0, n, p.data());
             for (int g = 0; g < ng && !rc; ++g) rc = fgx_set_group_cm(s, g, &cmt[3 * g], &cmt[3 * g]);
+            // Movebase::move adds pot->update(true) (move.h:547-548)
+            if (!rc && s->has_ewald) rc = ewald_accepted(s, &drift);
             utot += du;
+            utot += drift;
             accepted++;
-        } else {     // Isobaric::_rejectMove (move.h:5582-5592)
-            rc = fgx_set_positions(s, 1, 0, n, p.data());
+        } else {     // Isobaric::_rejectMove (move.h:5582-5592), then update(false)
+            if (s->has_ewald && !(rc = ewald_rebox(s, oldlen))) rc = ewald_set_space(s);
+            if (!rc) rc = fgx_set_positions(s, 1, 0, n, p.data());
             for (int g = 0; g < ng && !rc; ++g) rc = fgx_set_group_cm(s, g, nullptr, &cm[3 * g]);
             if (du == HUGE_VAL) inf++;
         }
@@ -1680,7 +1751,7 @@ extern "C" int fgx_isobaric_propagate(fgx_system *s, double dp, double prob, int
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
-    s->dusum += utot;
+    s->dusum += utot;   // Propagator::move adds what Movebase::move returns
     if (st) {
         st->trials = trials;
         st->accepted = accepted;

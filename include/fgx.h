@@ -175,9 +175,14 @@ int fgx_propagate(fgx_system *s, int ncalls, fgx_stats *st, double *du_out, int 
  * when dp < 1e-6), a volume trial V' = exp(ln V + dp (u - 1/2)) with every
  * group scaled isotropically (Group::scale, group.h:318-355), the energy
  * change at the trial geometry (energy.h:3577-3621) and the Metropolis
- * decision, from the system's RNG engines.  Needs has_pressure; not with
- * Ewald (its k-vectors would change with the box).  Optional per-trial
- * record (dU, accept) up to `cap`. */
+ * decision, from the system's RNG engines.  Needs has_pressure.  With Ewald
+ * every setSpace the reference makes around a volume trial is followed
+ * (ewald.h:731-757): k-vectors, A_k and V of the trial box, S(k) recomputed
+ * from the accepted coordinates; an accepted trial leaves the S(k) of the
+ * unscaled coordinates under the new k-vectors, and the Ewald accept counter
+ * and refresh (ewald.h:613-648) advance.  The first call of a Propagator
+ * (here or in fgx_propagate) recomputes S(k), as its uinit evaluation does.
+ * Optional per-trial record (dU, accept) up to `cap`. */
 int fgx_isobaric_propagate(fgx_system *s, double dp, double prob, int ncalls, fgx_stats *st,
                            double *du_out, int *acc_out, long cap);
 /* Cuboid side lengths (Cuboid::setlen after an accepted volume move) */

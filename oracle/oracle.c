@@ -1348,7 +1348,12 @@ static int iso_scale_group(or_sys *s, or_group *g, double xyz, const double oldl
 long or_isobaric_propagate(or_sys *s, double dp, double prob, int ncalls, double *du_out,
                            int *acc_out, long cap, double *utot)
 {
-    if (s->ew) return -1;
+    /* With NonbondedEwald every setSpace call (energyChange with the trial
+     * geometry and back, _acceptMove, _rejectMove) regenerates the k-vectors
+     * and A_k for the Space's current box and recomputes S(k) from the
+     * accepted coordinates (ewald.h:731-757), then copies them to the trial
+     * set (undo).  updateChange's own geometry branch (ewald.h:653-658) is
+     * overwritten by the first of these calls and leaves nothing behind. */
     const int ng = s->ng;
     int *mvg = malloc(sizeof(int) * (ng > 0 ? ng : 1));
     int *off = malloc(sizeof(int) * (ng + 1));
@@ -1358,6 +1363,10 @@ long or_isobaric_propagate(or_sys *s, double dp, double prob, int ncalls, double
     /* Isobaric constructor: runfraction = prob, 0 when dp < 1e-6 (move.h:5462-5464) */
     const double runfraction = dp < 1e-6 ? 0.0 : prob;
     for (int c = 0; c < ncalls; ++c) {
+        if (!s->propagator_started) {   /* Propagator::move: uinit (move.h:7884-7885) */
+            s->uinit = or_system_energy(s, 0);
+            s->propagator_started = 1;
+        }
         (void)or_range(&s->internal, 0, 0);                 /* Propagator::move, move.h:7887 */
         if (!(or_canonical(&s->internal) < runfraction)) continue;   /* Movebase::run */
         /* Isobaric::_trialMove (move.h:5529-5567) */
@@ -1378,9 +1387,13 @@ long or_isobaric_propagate(or_sys *s, double dp, double prob, int ncalls, double
             off[++nmv] = ni;
         }
         if (bad) { nt = -1; break; }
-        /* Energy::energyChange with a geometry change (energy.h:3577-3621) */
+        /* Energy::energyChange with a geometry change (energy.h:3577-3621):
+         * the trial geometry and setSpace, the collision test (an early
+         * return that leaves the trial geometry in place until _rejectMove),
+         * the trial configuration, then the old geometry and setSpace */
         double du;
         box_set(&s->box, newlen);
+        if (s->ew) or_ewald_set_space(s);
         int coll = 0;
         for (int i = 0; i < s->n && !coll; ++i) coll = box_collision(&s->box, s->trial[i].r);
         if (coll) {
@@ -1389,20 +1402,27 @@ long or_isobaric_propagate(or_sys *s, double dp, double prob, int ncalls, double
         } else {
             const double dnew = change_config(s, 1, nmv, mvg, off, idx);
             box_set(&s->box, oldlen);
+            if (s->ew) or_ewald_set_space(s);
             const double dold = change_config(s, 0, nmv, mvg, off, idx);
             du = dnew - dold;
         }
         /* Movebase::metropolis (move.h:567-573) */
         const int acc = !(or_canonical(&s->slump) > exp(-du));
-        if (acc) {   /* Isobaric::_acceptMove (move.h:5570-5580) */
+        if (acc) {   /* Isobaric::_acceptMove (move.h:5570-5580): setSpace with the
+                      * new box before the groups copy their trial coordinates */
             box_set(&s->box, newlen);
+            if (s->ew) or_ewald_set_space(s);
             for (int i = 0; i < s->n; ++i) s->p[i] = s->trial[i];
             for (int g = 0; g < ng; ++g) memcpy(s->g[g].cm, s->g[g].cm_trial, sizeof s->g[g].cm);
             ut += du;
         } else {     /* Isobaric::_rejectMove (move.h:5582-5592) */
+            if (s->ew) or_ewald_set_space(s);
             for (int i = 0; i < s->n; ++i) s->trial[i] = s->p[i];
             for (int g = 0; g < ng; ++g) memcpy(s->g[g].cm_trial, s->g[g].cm, sizeof s->g[g].cm);
         }
+        /* Movebase::move adds pot->update(accepted) (move.h:547-548): the
+         * Ewald accept counter and refresh (ewald.h:613-648) */
+        ut += or_ewald_update(s, acc);
         if (nt < cap) {
             if (du_out) du_out[nt] = du;
             if (acc_out) acc_out[nt] = acc;

@@ -12,7 +12,9 @@
 //   ref_harness energy <kind> <config.json> <statefile>
 //   ref_harness mc <kind> <config.json> <statefile> <ncalls>
 //   ref_harness mctime <kind> <config.json> <statefile> <ncalls>   (CPU baseline)
-// kind: dhhs | coulhs | lj | dh | ewaldlj | dhhs_cut | dhhs_vec | ljcut | dhhs_npt
+//   ref_harness mc <kind>_npt <config.json> <statefile> <ncalls> [<ncalls_at>]
+// kind: dhhs | coulhs | lj | dh | ewaldlj | dhhs_cut | dhhs_vec | ljcut | dhhs_npt |
+//       ewaldlj_npt
 // Output: one JSON object on stdout, doubles printed with %.17g.
 
 #include <faunus/faunus.h>
@@ -279,13 +281,18 @@ static int run_mctime(Tmjson &in, const char *state, int ncalls)
 
 typedef CombinedPairPotential<DebyeHuckel, HardSphere> TDHHS;
 
-// Nonbonded<DebyeHuckelHS> + ExternalPressure (energy.h:1817-1894) with a
+// A pair Hamiltonian + ExternalPressure (energy.h:1817-1894) with a
 // Propagator holding one Isobaric move (move.h:5424-5640): energies, or a
-// recorded Metropolis run.
-static int run_npt(const std::string &mode, Tmjson &in, const char *state, int ncalls)
+// recorded Metropolis run of `ncalls` calls.  With ncalls_at > 0 the
+// Propagator also holds the input's AtomicTranslation move and the run goes on
+// with `ncalls_at` calls of that move (so the structure factors an accepted
+// volume move leaves behind reach the translation trials that follow).
+template<class Tpair>
+static int run_npt(const std::string &mode, Tmjson &in, const char *state, int ncalls,
+                   int ncalls_at)
 {
     Tspace spc(in);
-    auto pot = Energy::Nonbonded<Tspace, TDHHS>(in) + Energy::ExternalPressure<Tspace>(in);
+    auto pot = Tpair(in) + Energy::ExternalPressure<Tspace>(in);
     spc.load(state, Tspace::RESIZE);
     const double P = in["moves"]["isobaric"].at("pressure").get<double>() * 1.0_mM;
     std::printf("{\n");
@@ -297,7 +304,15 @@ static int run_npt(const std::string &mode, Tmjson &in, const char *state, int n
         return 0;
     }
     Recorder rec;
-    RecAT<Tspace>::internal().eng = slump.eng;   // Propagator constructor (move.h:7781)
+    // Propagator constructor (move.h:7758-7858): every move is built before
+    // the first call (and each Movebase constructor calls setSpace,
+    // move.h:403-406), the internal engine seeded from the global one
+    std::unique_ptr<RecAT<Tspace>> at;
+    if (ncalls_at > 0) {
+        RecAT<Tspace>::internal().eng = slump.eng;
+        at.reset(new RecAT<Tspace>(pot, spc, in["moves"]["atomtranslate"], &rec));
+    }
+    RecAT<Tspace>::internal().eng = slump.eng;   // move.h:7781
     RecIso<Tspace> mv(pot, spc, in["moves"]["isobaric"], &rec);
     double uinit = Energy::systemEnergy(spc, pot, spc.p);
     double utot = 0;
@@ -305,12 +320,21 @@ static int run_npt(const std::string &mode, Tmjson &in, const char *state, int n
         (void)RecAT<Tspace>::internal().range(0, 0);
         utot += mv.move();
     }
+    const size_t nvol = rec.du.size();
+    double utot_vol = utot;
+    for (int c = 0; c < ncalls_at; ++c) {
+        (void)RecAT<Tspace>::internal().range(0, 0);
+        utot += at->move();
+    }
     double ufinal = Energy::systemEnergy(spc, pot, spc.p);
     jnum("uinit", uinit);
     jnum("ufinal", ufinal);
     jnum("utot", utot);
+    jnum("utot_vol", utot_vol);
+    jnum("nvol", (double)nvol);
     jvec("du", rec.du);
     jivec("acc", rec.acc);
+    jivec("who", rec.who);
     std::vector<double> xyz;
     for (auto &p : spc.p) { xyz.push_back(p.x()); xyz.push_back(p.y()); xyz.push_back(p.z()); }
     jvec("final_xyz", xyz);
@@ -377,7 +401,9 @@ int main(int argc, char **argv)
     if (kind == "lj") { DISPATCH(E_lj); }
     if (kind == "ewaldlj") { DISPATCH(E_ewaldlj); }
     if (kind == "ljcut") { DISPATCH(E_ljcut); }
-    if (kind == "dhhs_npt") return run_npt(mode, in, state, ncalls);
+    const int ncalls_at = argc > 6 ? std::atoi(argv[6]) : 0;
+    if (kind == "dhhs_npt") return run_npt<E_dhhs>(mode, in, state, ncalls, ncalls_at);
+    if (kind == "ewaldlj_npt") return run_npt<E_ewaldlj>(mode, in, state, ncalls, ncalls_at);
     std::fprintf(stderr, "unknown kind %s\n", kind.c_str());
     return 2;
 }

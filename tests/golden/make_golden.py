@@ -94,7 +94,8 @@ def run_harness(args):
     return json.loads(out)
 
 
-def build_fixture(name, kind, cfg, L, parts, groups, seed, ncalls, moves_meta, energy=True):
+def build_fixture(name, kind, cfg, L, parts, groups, seed, ncalls, moves_meta, energy=True,
+                  mc_extra=()):
     with tempfile.TemporaryDirectory() as td:
         cpath = os.path.join(td, "in.json")
         spath = os.path.join(td, "state")
@@ -115,8 +116,11 @@ def build_fixture(name, kind, cfg, L, parts, groups, seed, ncalls, moves_meta, e
         if energy:
             fx["energy"] = run_harness(["energy", kind, cpath, spath])
         if ncalls > 0:
-            fx["mc"] = run_harness(["mc", kind, cpath, spath, str(ncalls)])
+            fx["mc"] = run_harness(["mc", kind, cpath, spath, str(ncalls)] +
+                                   [str(a) for a in mc_extra])
             fx["mc"]["ncalls"] = ncalls
+            if mc_extra:
+                fx["mc"]["ncalls_at"] = int(mc_extra[0])
     path = os.path.join(HERE, name + ".json")
     with open(path, "w") as f:
         json.dump(fx, f, separators=(",", ":"))
@@ -175,6 +179,38 @@ def salt_npt(name, n, L, seed, ncalls, pressure_mM, dp, r=2.0):
     moves = [{"type": "NPT", "molid": -1, "prob": 1.0, "peratom": 0, "permol": 0, "dp": dp,
               "pressure_mM": pressure_mM}]
     return build_fixture(name, "dhhs_npt", cfg, [L, L, L], parts, groups, seed, ncalls, moves)
+
+
+def ewald_npt(name, n, L, seed, ncalls, ncalls_at, pressure_mM, dp, eps_surf,
+              update_frequency, alpha=0.35, rc=7.0, kc=4.0):
+    """Ewald salt (NonbondedEwald<LennardJonesLB>) with the ExternalPressure
+    term: `ncalls` calls of a Propagator holding one Isobaric move, then
+    `ncalls_at` calls of one holding an AtomicTranslation move."""
+    rng = np.random.default_rng(seed)
+    half = n // 2
+    radii = np.full(n, 1.5)
+    xyz = rsa(rng, n, L, radii)
+    ids = np.array([1] * half + [2] * (n - half))
+    charge = np.where(ids == 1, 1.0, -1.0)
+    cfg = {
+        "system": {"temperature": 298.15, "geometry": {"length": L}},
+        "atomlist": {"T1p": {"q": 1.0, "sigma": 3.0, "eps": 0.2479, "dp": 1.0, "mw": 1.0},
+                     "T2m": {"q": -1.0, "sigma": 3.0, "eps": 0.2479, "dp": 1.0, "mw": 1.0}},
+        "moleculelist": {"salt": {"atoms": "T1p T2m", "atomic": True, "Ninit": 0}},
+        "moves": {"isobaric": {"dp": dp, "pressure": pressure_mM, "prob": 1.0},
+                  "atomtranslate": {"salt": {"peratom": True}}},
+        "energy": {"nonbonded": {"epsr": lB_epsr(7.1), "ewald": {
+            "alpha": alpha, "cutoff": rc, "cutoffK": kc, "debyelength": 1e12,
+            "eps_surf": eps_surf, "spherical_sum": True, "update_frequency": update_frequency}}},
+    }
+    parts = {"xyz": xyz, "charge": charge, "radius": radii, "mw": np.ones(n), "id": ids}
+    groups = [{"front": 0, "back": n - 1, "molid": 0, "molsize": 1}]
+    moves = [{"type": "AT", "molid": 0, "prob": 1.0, "peratom": 1, "permol": 0},
+             {"type": "NPT", "molid": -1, "prob": 1.0, "peratom": 0, "permol": 0, "dp": dp,
+              "pressure_mM": pressure_mM}]
+    fx = build_fixture(name, "ewaldlj_npt", cfg, [L, L, L], parts, groups, seed, ncalls, moves,
+                       mc_extra=(ncalls_at,))
+    return fx
 
 
 def lj_mix(name, n, L, seed, ncalls, cutoff=None):
@@ -349,9 +385,14 @@ def rng_fixture():
     print(f"wrote {path}")
 
 
-def main():
+def main(only=()):
+    """Writes every fixture, or only those named on the command line."""
     if not os.path.exists(HARNESS):
         sys.exit("build oracle/_ref/ref_harness first: make -C oracle ref")
+    if only:
+        for f in only:
+            ONLY[f]()
+        return
     rng_fixture()
     dh = {"epsr": lB_epsr(7.1), "debyelength": 10.0}
     salt("salt_dhhs", "dhhs", 80, 30.0, 11, 25, dh)
@@ -368,7 +409,20 @@ def main():
     molecules("mol_dhhs_cut", "dhhs_cut", 5, 8, 10, 70.0, 20, 10, cutoff_g2g=25.0)
     # the NaCl melt of the reference's bulk example (2304 ions, 1100 K), LJ part
     reference_example("example_bulk_lj", "lj", "bulk")
+    for f in ONLY.values():
+        f()
+
+
+# Ewald with volume moves: tin-foil, and a finite surface dielectric (the
+# surface term changes with V); refreshes of S(k) every 3 accepted moves
+ONLY = {
+    "ewald_npt": lambda: ewald_npt("ewald_npt", 40, 20.0, 23, 30, 4, pressure_mM=6000.0,
+                                   dp=0.4, eps_surf=0.5, update_frequency=3),
+    "ewald_npt_surface": lambda: ewald_npt("ewald_npt_surface", 30, 18.0, 24, 24, 3,
+                                           pressure_mM=6000.0, dp=0.3, eps_surf=1.0,
+                                           update_frequency=3, kc=3.0),
+}
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

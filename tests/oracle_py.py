@@ -32,6 +32,9 @@ KINDS = {
     "ewaldljcut": ("ewald", "lj", "vector"),
     # Nonbonded<DebyeHuckelHS> + ExternalPressure, Isobaric move
     "dhhs_npt": ("dh", "hs", "plain"),
+    # NonbondedEwald<LennardJonesLB> + ExternalPressure, Isobaric move (then
+    # AtomicTranslation)
+    "ewaldlj_npt": ("ewald", "lj", "vector"),
 }
 
 

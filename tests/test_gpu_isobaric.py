@@ -1,8 +1,16 @@
 """Move::Isobaric with Energy::ExternalPressure on the engine
-(fgx_isobaric_propagate) against the compiled reference's run of the same
-system (fixture salt_npt: 60 ions, Debye-Hueckel + hard spheres, 300 mM,
-60 volume trials): the same accept/reject sequence, energy changes to 1e-10,
-the same final box, coordinates and global engine state."""
+(fgx_isobaric_propagate) against the compiled reference's runs of the same
+systems: the same accept/reject sequence, energy changes to 1e-10, the same
+final box, coordinates and global engine state.
+
+* salt_npt: 60 ions, Debye-Hueckel + hard spheres, 300 mM, 60 volume trials.
+* ewald_npt / ewald_npt_surface: Ewald salt (NonbondedEwald<LennardJonesLB>,
+  tin-foil and a finite surface dielectric), volume trials followed by
+  AtomicTranslation calls of the same Propagator on the device sweep.  Every
+  volume trial regenerates the k-vectors and A_k for the trial box; an
+  accepted one leaves the structure factors of the unscaled coordinates
+  under the new k-vectors (as the reference's setSpace order does), which
+  the translation trials that follow start from."""
 import sys
 
 import numpy as np
@@ -27,17 +35,27 @@ def gpu():
     return fgx
 
 
-def test_isobaric_matches_reference(gpu):
+@pytest.mark.parametrize("name", ["salt_npt", "ewald_npt", "ewald_npt_surface"])
+def test_isobaric_matches_reference(gpu, name):
     from fixture_systems import fgx_system_from_fixture
-    fx = load_fixture("salt_npt")
+    fx = load_fixture(name)
     mc = fx["mc"]
-    mv = fx["moves"][0]
-    say("[isobaric] salt_npt: %d volume trials" % mc["ncalls"])
+    mv = [m for m in fx["moves"] if m["type"] == "NPT"][0]
+    nat = mc.get("ncalls_at", 0)
+    say("[isobaric] %s: %d volume trials, then %d AtomicTranslation calls"
+        % (name, mc["ncalls"], nat))
     s = fgx_system_from_fixture(fx)
     try:
         s.rng_seed(fx["rng_seed"])
         s.propagator_init()
         st, du, acc = s.isobaric_propagate(mv["dp"], mv["prob"], mc["ncalls"], record=1000)
+        utot = st["utot"]
+        if nat:
+            assert len(du) == mc["nvol"]
+            assert utot == pytest.approx(mc["utot_vol"], rel=1e-9, abs=1e-9)
+            st2, du2, acc2 = s.propagate(nat, record=100000)
+            du, acc = np.concatenate([du, du2]), np.concatenate([acc, acc2])
+            utot += st2["utot"]
         box = s.box()
         xyz = s.positions(0).reshape(-1)
         words, pos = s.rng_state()
@@ -56,21 +74,16 @@ def test_isobaric_matches_reference(gpu):
     np.testing.assert_allclose(box, farr(mc["final_len"]), rtol=1e-12)
     assert np.max(np.abs(xyz - farr(mc["final_xyz"]))) <= 1e-9
     assert " ".join(str(w) for w in words) + " %d" % pos == mc["final_rng"].strip()
-    assert st["utot"] == pytest.approx(mc["utot"], rel=1e-9, abs=1e-9)
+    assert utot == pytest.approx(mc["utot"], rel=1e-9, abs=1e-9)
     assert u1 == pytest.approx(mc["ufinal"], rel=1e-10)
 
 
-def test_isobaric_refuses_ewald_and_missing_pressure(gpu):
+def test_isobaric_refuses_missing_pressure(gpu):
     from fixture_systems import fgx_system_from_fixture
-    s = fgx_system_from_fixture(load_fixture("salt_dhhs"))
-    try:
-        with pytest.raises(gpu.FgxError):
-            s.isobaric_propagate(0.1, 1.0, 1)
-    finally:
-        s.close()
-    s = fgx_system_from_fixture(load_fixture("ewald_tinfoil"))
-    try:
-        with pytest.raises(gpu.FgxError):
-            s.isobaric_propagate(0.1, 1.0, 1)
-    finally:
-        s.close()
+    for name in ("salt_dhhs", "ewald_tinfoil"):
+        s = fgx_system_from_fixture(load_fixture(name))
+        try:
+            with pytest.raises(gpu.FgxError):
+                s.isobaric_propagate(0.1, 1.0, 1)
+        finally:
+            s.close()

@@ -79,7 +79,7 @@ def test_energies_fp64(gpu, name):
 
 
 @pytest.mark.parametrize("name", [f for f in FIXTURES if "mc" in load_fixture(f) and
-                                  load_fixture(f)["moves"][0]["type"] in ("AT", "TR")])
+                                  all(m["type"] in ("AT", "TR") for m in load_fixture(f)["moves"])])
 def test_device_propagator_matches_reference(gpu, name):
     """fgx_propagate replays the reference's Propagator on the device: same
     RNG stream, same accept/reject sequence, same energy changes."""
