@@ -1613,13 +1613,24 @@ extern "C" int fgx_get_box(const fgx_system *s, double len[3])
     return FGX_OK;
 }
 
-extern "C" int fgx_isobaric_propagate(fgx_system *s, double dp, double prob, int ncalls,
-                                      fgx_stats *st, double *du_out, int *acc_out, long cap)
+// Propagator::move() calls (move.h:7877-7891) of a Propagator holding one
+// Isobaric move (npt_index < 0), or the system's one registered move and an
+// Isobaric move at position npt_index (0 or 1) of its move list.  The pick is
+// drawn on the host from a copy of the internal engine; a pick of the
+// registered move leaves that draw to the device sweep, whose own pick over
+// its one move (range(0, 0)) takes exactly one engine output as range(0, 1)
+// does, and runs that one call there.
+static int propagate_with_isobaric(fgx_system *s, double dp, double prob, int npt_index,
+                                   int ncalls, fgx_stats *st, double *du_out, int *acc_out,
+                                   long cap)
 {
     FGX_CHECK(s && ncalls >= 0 && (cap <= 0 || (du_out && acc_out)), FGX_EARG,
               "fgx_isobaric_propagate: bad arguments");
     FGX_CHECK(s->press_on, FGX_ESTATE,
               "Isobaric needs the ExternalPressure term in the Hamiltonian (move.h:5466-5472)");
+    FGX_CHECK(npt_index < 0 || ((npt_index == 0 || npt_index == 1) && s->moves.size() == 1),
+              FGX_EUNSUPPORTED,
+              "Propagator with Isobaric: exactly one registered move besides it, npt_index 0 or 1");
     for (int g = 0; g < s->ng; ++g) {
         const GroupDesc &G = s->groups[g];
         const int sz = G.back - G.front + 1;
@@ -1627,30 +1638,20 @@ extern "C" int fgx_isobaric_propagate(fgx_system *s, double dp, double prob, int
                   "Isobaric: a group holding several molecules (Group::isRange)");
     }
     if (st) std::memset(st, 0, sizeof(*st));
-    int rc = pull_device_state(s);
-    if (rc) return rc;
+    if (cap < 0) cap = 0;
+    int rc = FGX_OK;
     const int n = s->n, ng = s->ng;
     std::vector<double> p(3 * (size_t)n), t(3 * (size_t)n);
-    if ((rc = fgx_get_positions(s, 0, p.data()))) return rc;
     std::vector<double> cm(3 * (size_t)ng), cmt(3 * (size_t)ng);
-    HostMT G{s->rng_global}, I{s->rng_internal};
+    bool have_p = false;   // p holds the accepted positions
     // Isobaric constructor: runfraction = prob, 0 when dp < 1e-6 (move.h:5462-5464)
     const double runfraction = dp < 1e-6 ? 0.0 : prob;
-    long trials = 0, accepted = 0, inf = 0;
+    long trials = 0, accepted = 0, inf = 0, refresh = 0;
     double utot = 0.0;   // Movebase::move: accepted dU plus update() drift terms
-    hipEvent_t e0, e1;
-    FGX_HIP(hipEventCreate(&e0));
-    FGX_HIP(hipEventCreate(&e1));
-    FGX_HIP(hipEventRecord(e0, s->stream));
-    for (int call = 0; call < ncalls && !rc; ++call) {
-        // Propagator::move evaluates uinit = systemEnergy before its first
-        // move (move.h:7884-7885), whose setSpace recomputes S(k)
-        if (!s->propagator_started) {
-            if ((rc = ewald_set_space(s))) break;
-            s->propagator_started = true;
-        }
-        (void)I.range(0, 0);                       // Propagator::move (move.h:7887)
-        if (!(I.canonical() < runfraction)) continue;   // Movebase::run (move.h:575-581)
+    // one Movebase::move of the Isobaric move after the pick; adds the call's
+    // utot (accepted dU, then pot->update) to `ucall`
+    auto iso_call = [&](HostMT &G, HostMT &I, double &ucall) -> int {
+        if (!(I.canonical() < runfraction)) return FGX_OK;   // Movebase::run (move.h:575-581)
         // ---- Isobaric::_trialMove (move.h:5529-5567)
         double oldlen[3], newlen[3];
         for (int k = 0; k < 3; ++k) oldlen[k] = s->box.len[k];
@@ -1695,9 +1696,9 @@ extern "C" int fgx_isobaric_propagate(fgx_system *s, double dp, double prob, int
         // the unscaled coordinates under the trial box's k-vectors.  (On a
         // collision the trial geometry stays set until _rejectMove, which
         // restores it; nothing is computed with it in between.)
-        if ((rc = fgx_set_positions(s, 1, 0, n, t.data()))) break;
+        if ((rc = fgx_set_positions(s, 1, 0, n, t.data()))) return rc;
         for (int g = 0; g < ng && !rc; ++g) rc = fgx_set_group_cm(s, g, nullptr, &cmt[3 * g]);
-        if (rc) break;
+        if (rc) return rc;
         double du;
         bool coll = false;
         for (int i = 0; i < n && !coll; ++i)
@@ -1713,7 +1714,7 @@ extern "C" int fgx_isobaric_propagate(fgx_system *s, double dp, double prob, int
             set_box(s->box, oldlen);
             if (!rc && !(rc = ewald_rebox(s, oldlen)) && !(rc = ewald_set_space(s)))
                 rc = all_groups_config(s, 0, &uold);
-            if (rc) break;
+            if (rc) return rc;
             du = unew - uold;
         }
         // ---- Movebase::metropolis (move.h:567-573)
@@ -1729,8 +1730,8 @@ extern "C" int fgx_isobaric_propagate(fgx_system *s, double dp, double prob, int
             for (int g = 0; g < ng && !rc; ++g) rc = fgx_set_group_cm(s, g, &cmt[3 * g], &cmt[3 * g]);
             // Movebase::move adds pot->update(true) (move.h:547-548)
             if (!rc && s->has_ewald) rc = ewald_accepted(s, &drift);
-            utot += du;
-            utot += drift;
+            ucall += du;
+            ucall += drift;
             accepted++;
         } else {     // Isobaric::_rejectMove (move.h:5582-5592), then update(false)
             if (s->has_ewald && !(rc = ewald_rebox(s, oldlen))) rc = ewald_set_space(s);
@@ -1743,6 +1744,48 @@ extern "C" int fgx_isobaric_propagate(fgx_system *s, double dp, double prob, int
             acc_out[trials] = acc ? 1 : 0;
         }
         trials++;
+        return rc;
+    };
+    hipEvent_t e0, e1;
+    FGX_HIP(hipEventCreate(&e0));
+    FGX_HIP(hipEventCreate(&e1));
+    FGX_HIP(hipEventRecord(e0, s->stream));
+    for (int call = 0; call < ncalls && !rc; ++call) {
+        if ((rc = pull_device_state(s))) break;
+        // Propagator::move evaluates uinit = systemEnergy before its first
+        // move (move.h:7884-7885), whose setSpace recomputes S(k)
+        if (!s->propagator_started) {
+            if ((rc = ewald_set_space(s))) break;
+            s->propagator_started = true;
+        }
+        int k = 0;
+        if (npt_index >= 0) {
+            uint32_t peek[625];
+            std::memcpy(peek, s->rng_internal, sizeof(peek));
+            HostMT P{peek};
+            k = P.range(0, 1);
+        }
+        if (npt_index < 0 || k == npt_index) {
+            HostMT G{s->rng_global}, I{s->rng_internal};
+            (void)I.range(0, npt_index < 0 ? 0 : 1);   // Propagator::move (move.h:7887)
+            if (!have_p && (rc = fgx_get_positions(s, 0, p.data()))) break;
+            have_p = true;
+            double ucall = 0.0;
+            rc = iso_call(G, I, ucall);
+            utot += ucall;
+        } else {
+            fgx_stats s1;
+            const long room = cap > trials ? cap - trials : 0;
+            rc = fgx_propagate(s, 1, &s1, room ? du_out + trials : nullptr,
+                               room ? acc_out + trials : nullptr, room);
+            if (rc) break;
+            trials += s1.trials;
+            accepted += s1.accepted;
+            inf += s1.rejected_inf;
+            refresh += s1.ewald_refresh;
+            utot += s1.utot;
+            have_p = false;
+        }
     }
     FGX_HIP(hipEventRecord(e1, s->stream));
     FGX_HIP(hipEventSynchronize(e1));
@@ -1756,9 +1799,24 @@ extern "C" int fgx_isobaric_propagate(fgx_system *s, double dp, double prob, int
         st->trials = trials;
         st->accepted = accepted;
         st->rejected_inf = inf;
+        st->ewald_refresh = refresh;
         st->utot = utot;
         st->dusum = utot;
         st->seconds = ms * 1e-3;
     }
     return FGX_OK;
+}
+
+extern "C" int fgx_isobaric_propagate(fgx_system *s, double dp, double prob, int ncalls,
+                                      fgx_stats *st, double *du_out, int *acc_out, long cap)
+{
+    return propagate_with_isobaric(s, dp, prob, -1, ncalls, st, du_out, acc_out, cap);
+}
+
+extern "C" int fgx_propagate_npt(fgx_system *s, double dp, double prob, int npt_index,
+                                 int ncalls, fgx_stats *st, double *du_out, int *acc_out,
+                                 long cap)
+{
+    FGX_CHECK(npt_index == 0 || npt_index == 1, FGX_EARG, "fgx_propagate_npt: npt_index 0 or 1");
+    return propagate_with_isobaric(s, dp, prob, npt_index, ncalls, st, du_out, acc_out, cap);
 }

@@ -129,6 +129,7 @@ def lib():
         "fgx_set_energy_matrix": (I, [P, I]),
         "fgx_set_cell_list": (I, [P, I]),
         "fgx_isobaric_propagate": (I, [P, D, D, I, C.POINTER(Stats), _dp, _ip, C.c_long]),
+        "fgx_propagate_npt": (I, [P, D, D, I, I, C.POINTER(Stats), _dp, _ip, C.c_long]),
         "fgx_get_box": (I, [P, _dp]),
         "fgx_get_cell_list": (I, [P, _ip, _ip]),
         "fgx_chain_seconds": (I, [P, C.POINTER(C.c_double)]),
@@ -364,6 +365,22 @@ class System:
             n = min(record, st.trials)
             return st.as_dict(), du[:n], acc[:n]
         _check(lib().fgx_isobaric_propagate(self.h, dp, prob, ncalls, C.byref(st), None, None, 0))
+        return st.as_dict(), None, None
+
+    def propagate_npt(self, dp, prob, npt_index, ncalls, record=0):
+        """Propagator::move() calls of a Propagator holding the registered
+        move and one Isobaric move at position npt_index of its move list
+        (the input's move keys in sorted order)."""
+        st = Stats()
+        if record:
+            du = np.zeros(record)
+            acc = np.zeros(record, dtype=np.int32)
+            _check(lib().fgx_propagate_npt(self.h, dp, prob, npt_index, ncalls, C.byref(st),
+                                           _d(du), _i(acc), record))
+            n = min(record, st.trials)
+            return st.as_dict(), du[:n], acc[:n]
+        _check(lib().fgx_propagate_npt(self.h, dp, prob, npt_index, ncalls, C.byref(st), None,
+                                       None, 0))
         return st.as_dict(), None, None
 
     def box(self):

@@ -185,6 +185,15 @@ int fgx_propagate(fgx_system *s, int ncalls, fgx_stats *st, double *du_out, int 
  * Optional per-trial record (dU, accept) up to `cap`. */
 int fgx_isobaric_propagate(fgx_system *s, double dp, double prob, int ncalls, fgx_stats *st,
                            double *du_out, int *acc_out, long cap);
+/* Propagator::move() calls (move.h:7877-7891) of a Propagator holding the
+ * system's one registered move (fgx_add_move) and one Move::Isobaric at
+ * position npt_index (0 or 1) of its move list, i.e. the input's move keys in
+ * sorted order ("atomtranslate" < "isobaric" < "moltransrot").  Each call
+ * picks one of the two with the internal engine; volume trials run as in
+ * fgx_isobaric_propagate, the other move's calls on the device sweep.
+ * Records every trial (dU, accept) up to `cap`. */
+int fgx_propagate_npt(fgx_system *s, double dp, double prob, int npt_index, int ncalls,
+                      fgx_stats *st, double *du_out, int *acc_out, long cap);
 /* Cuboid side lengths (Cuboid::setlen after an accepted volume move) */
 int fgx_get_box(const fgx_system *s, double len[3]);
 

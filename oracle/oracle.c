@@ -1345,21 +1345,101 @@ static int iso_scale_group(or_sys *s, or_group *g, double xyz, const double oldl
     return -1;   /* several molecules in one group (isRange) */
 }
 
-long or_isobaric_propagate(or_sys *s, double dp, double prob, int ncalls, double *du_out,
-                           int *acc_out, long cap, double *utot)
+/* One Movebase::move (move.h:515-555) of an Isobaric move with the given
+ * run fraction, after the Propagator's pick: returns the call's utot
+ * (accepted dU plus pot->update), records the trial at *nt, and sets *bad when
+ * a group cannot be scaled.  mvg/off/idx are scratch of ng, ng+1, n ints.
+ *
+ * With NonbondedEwald every setSpace call (energyChange with the trial
+ * geometry and back, _acceptMove, _rejectMove) regenerates the k-vectors and
+ * A_k for the Space's current box and recomputes S(k) from the accepted
+ * coordinates (ewald.h:731-757), then copies them to the trial set (undo).
+ * updateChange's own geometry branch (ewald.h:653-658) is overwritten by the
+ * first of these calls and leaves nothing behind. */
+static double iso_move(or_sys *s, double dp, double runfraction, int *mvg, int *off, int *idx,
+                       double *du_out, int *acc_out, long cap, long *nt, int *bad_out)
 {
-    /* With NonbondedEwald every setSpace call (energyChange with the trial
-     * geometry and back, _acceptMove, _rejectMove) regenerates the k-vectors
-     * and A_k for the Space's current box and recomputes S(k) from the
-     * accepted coordinates (ewald.h:731-757), then copies them to the trial
-     * set (undo).  updateChange's own geometry branch (ewald.h:653-658) is
-     * overwritten by the first of these calls and leaves nothing behind. */
+    const int ng = s->ng;
+    double utot = 0.0;
+    *bad_out = 0;
+    if (!(or_canonical(&s->internal) < runfraction)) return 0.0;   /* Movebase::run */
+    /* Isobaric::_trialMove (move.h:5529-5567) */
+    s->iso_cnt++;
+    double oldlen[3], newlen[3];
+    memcpy(oldlen, s->box.len, sizeof oldlen);
+    const double oldval = box_volume(&s->box);
+    const double newval = exp(log(oldval) + (or_canonical(&s->slump) - 0.5) * dp);
+    const double xyz = cbrt(newval / oldval);
+    for (int k = 0; k < 3; ++k) newlen[k] = oldlen[k] * 1.0 * xyz;
+    int nmv = 0, ni = 0, bad = 0;
+    off[0] = 0;
+    for (int g = 0; g < ng; ++g) {
+        bad |= iso_scale_group(s, &s->g[g], xyz, oldlen, newlen);
+        mvg[nmv] = g;
+        if (g_atomic(&s->g[g]) && !g_empty(&s->g[g]))
+            for (int i = s->g[g].front; i <= s->g[g].back; ++i) idx[ni++] = i;
+        off[++nmv] = ni;
+    }
+    if (bad) { *bad_out = 1; return 0.0; }
+    /* Energy::energyChange with a geometry change (energy.h:3577-3621): the
+     * trial geometry and setSpace, the collision test (an early return that
+     * leaves the trial geometry in place until _rejectMove), the trial
+     * configuration, then the old geometry and setSpace */
+    double du;
+    box_set(&s->box, newlen);
+    if (s->ew) or_ewald_set_space(s);
+    int coll = 0;
+    for (int i = 0; i < s->n && !coll; ++i) coll = box_collision(&s->box, s->trial[i].r);
+    if (coll) {
+        box_set(&s->box, oldlen);
+        du = INFINITY;
+    } else {
+        const double dnew = change_config(s, 1, nmv, mvg, off, idx);
+        box_set(&s->box, oldlen);
+        if (s->ew) or_ewald_set_space(s);
+        const double dold = change_config(s, 0, nmv, mvg, off, idx);
+        du = dnew - dold;
+    }
+    /* Movebase::metropolis (move.h:567-573) */
+    const int acc = !(or_canonical(&s->slump) > exp(-du));
+    if (acc) {   /* Isobaric::_acceptMove (move.h:5570-5580): setSpace with the new
+                  * box before the groups copy their trial coordinates */
+        box_set(&s->box, newlen);
+        if (s->ew) or_ewald_set_space(s);
+        for (int i = 0; i < s->n; ++i) s->p[i] = s->trial[i];
+        for (int g = 0; g < ng; ++g) memcpy(s->g[g].cm, s->g[g].cm_trial, sizeof s->g[g].cm);
+        utot += du;
+    } else {     /* Isobaric::_rejectMove (move.h:5582-5592) */
+        if (s->ew) or_ewald_set_space(s);
+        for (int i = 0; i < s->n; ++i) s->trial[i] = s->p[i];
+        for (int g = 0; g < ng; ++g) memcpy(s->g[g].cm_trial, s->g[g].cm, sizeof s->g[g].cm);
+    }
+    /* pot->update(accepted): the Ewald accept counter and refresh (ewald.h:613-648) */
+    utot += or_ewald_update(s, acc);
+    record_trial(s, -1);
+    if (*nt < cap) {
+        if (du_out) du_out[*nt] = du;
+        if (acc_out) acc_out[*nt] = acc;
+    }
+    (*nt)++;
+    return utot;
+}
+
+/* Propagator::move calls (move.h:7877-7891) of a Propagator whose moves are
+ * the registered ones with one Isobaric move inserted at position npt_index
+ * (the input's key order); npt_index < 0 means the Isobaric move alone. */
+static long propagate_with_isobaric(or_sys *s, double dp, double prob, int npt_index,
+                                    int ncalls, double *du_out, int *acc_out, long cap,
+                                    double *utot)
+{
     const int ng = s->ng;
     int *mvg = malloc(sizeof(int) * (ng > 0 ? ng : 1));
     int *off = malloc(sizeof(int) * (ng + 1));
     int *idx = malloc(sizeof(int) * (s->n > 0 ? s->n : 1));
     long nt = 0;
     double ut = 0.0;
+    const int nm = npt_index < 0 ? 1 : s->nmoves + 1;
+    if (npt_index < 0) npt_index = 0;
     /* Isobaric constructor: runfraction = prob, 0 when dp < 1e-6 (move.h:5462-5464) */
     const double runfraction = dp < 1e-6 ? 0.0 : prob;
     for (int c = 0; c < ncalls; ++c) {
@@ -1367,69 +1447,33 @@ long or_isobaric_propagate(or_sys *s, double dp, double prob, int ncalls, double
             s->uinit = or_system_energy(s, 0);
             s->propagator_started = 1;
         }
-        (void)or_range(&s->internal, 0, 0);                 /* Propagator::move, move.h:7887 */
-        if (!(or_canonical(&s->internal) < runfraction)) continue;   /* Movebase::run */
-        /* Isobaric::_trialMove (move.h:5529-5567) */
-        s->iso_cnt++;
-        double oldlen[3], newlen[3];
-        memcpy(oldlen, s->box.len, sizeof oldlen);
-        const double oldval = box_volume(&s->box);
-        const double newval = exp(log(oldval) + (or_canonical(&s->slump) - 0.5) * dp);
-        const double xyz = cbrt(newval / oldval);
-        for (int k = 0; k < 3; ++k) newlen[k] = oldlen[k] * 1.0 * xyz;
-        int nmv = 0, ni = 0, bad = 0;
-        off[0] = 0;
-        for (int g = 0; g < ng; ++g) {
-            bad |= iso_scale_group(s, &s->g[g], xyz, oldlen, newlen);
-            mvg[nmv] = g;
-            if (g_atomic(&s->g[g]) && !g_empty(&s->g[g]))
-                for (int i = s->g[g].front; i <= s->g[g].back; ++i) idx[ni++] = i;
-            off[++nmv] = ni;
-        }
-        if (bad) { nt = -1; break; }
-        /* Energy::energyChange with a geometry change (energy.h:3577-3621):
-         * the trial geometry and setSpace, the collision test (an early
-         * return that leaves the trial geometry in place until _rejectMove),
-         * the trial configuration, then the old geometry and setSpace */
+        const int k = or_range(&s->internal, 0, nm - 1);   /* move.h:7887 */
         double du;
-        box_set(&s->box, newlen);
-        if (s->ew) or_ewald_set_space(s);
-        int coll = 0;
-        for (int i = 0; i < s->n && !coll; ++i) coll = box_collision(&s->box, s->trial[i].r);
-        if (coll) {
-            box_set(&s->box, oldlen);
-            du = INFINITY;
+        if (k == npt_index) {
+            int bad;
+            du = iso_move(s, dp, runfraction, mvg, off, idx, du_out, acc_out, cap, &nt, &bad);
+            if (bad) { nt = -1; break; }
         } else {
-            const double dnew = change_config(s, 1, nmv, mvg, off, idx);
-            box_set(&s->box, oldlen);
-            if (s->ew) or_ewald_set_space(s);
-            const double dold = change_config(s, 0, nmv, mvg, off, idx);
-            du = dnew - dold;
+            du = movebase_move(s, &s->moves[k < npt_index ? k : k - 1], du_out, acc_out, cap,
+                               &nt);
         }
-        /* Movebase::metropolis (move.h:567-573) */
-        const int acc = !(or_canonical(&s->slump) > exp(-du));
-        if (acc) {   /* Isobaric::_acceptMove (move.h:5570-5580): setSpace with the
-                      * new box before the groups copy their trial coordinates */
-            box_set(&s->box, newlen);
-            if (s->ew) or_ewald_set_space(s);
-            for (int i = 0; i < s->n; ++i) s->p[i] = s->trial[i];
-            for (int g = 0; g < ng; ++g) memcpy(s->g[g].cm, s->g[g].cm_trial, sizeof s->g[g].cm);
-            ut += du;
-        } else {     /* Isobaric::_rejectMove (move.h:5582-5592) */
-            if (s->ew) or_ewald_set_space(s);
-            for (int i = 0; i < s->n; ++i) s->trial[i] = s->p[i];
-            for (int g = 0; g < ng; ++g) memcpy(s->g[g].cm_trial, s->g[g].cm, sizeof s->g[g].cm);
-        }
-        /* Movebase::move adds pot->update(accepted) (move.h:547-548): the
-         * Ewald accept counter and refresh (ewald.h:613-648) */
-        ut += or_ewald_update(s, acc);
-        if (nt < cap) {
-            if (du_out) du_out[nt] = du;
-            if (acc_out) acc_out[nt] = acc;
-        }
-        nt++;
+        s->dusum += du;
+        ut += du;
     }
     free(mvg); free(off); free(idx);
     if (utot) *utot = ut;
     return nt;
+}
+
+long or_isobaric_propagate(or_sys *s, double dp, double prob, int ncalls, double *du_out,
+                           int *acc_out, long cap, double *utot)
+{
+    return propagate_with_isobaric(s, dp, prob, -1, ncalls, du_out, acc_out, cap, utot);
+}
+
+long or_propagate_npt(or_sys *s, double dp, double prob, int npt_index, int ncalls,
+                      double *du_out, int *acc_out, long cap, double *utot)
+{
+    if (npt_index < 0 || npt_index > s->nmoves) return -1;
+    return propagate_with_isobaric(s, dp, prob, npt_index, ncalls, du_out, acc_out, cap, utot);
 }

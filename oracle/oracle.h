@@ -140,6 +140,10 @@ long or_last_trials(or_sys *s, int *which_out, long cap);
  * (Ewald; groups holding several molecules). */
 long or_isobaric_propagate(or_sys *s, double dp, double prob, int ncalls, double *du_out,
                            int *acc_out, long cap, double *utot);
+/* Propagator::move calls of a Propagator holding the registered moves and one
+ * Isobaric move at position npt_index of its move list (move.h:7877-7891) */
+long or_propagate_npt(or_sys *s, double dp, double prob, int npt_index, int ncalls,
+                      double *du_out, int *acc_out, long cap, double *utot);
 void or_get_box(or_sys *s, double *len);
 
 /* ParallelTempering (move.h:6520-6772) restated for R replicas in one

@@ -13,6 +13,7 @@
 //   ref_harness mc <kind> <config.json> <statefile> <ncalls>
 //   ref_harness mctime <kind> <config.json> <statefile> <ncalls>   (CPU baseline)
 //   ref_harness mc <kind>_npt <config.json> <statefile> <ncalls> [<ncalls_at>]
+//   ref_harness mcprop <kind>_npt <config.json> <statefile> <ncalls>  (one Propagator, all moves)
 // kind: dhhs | coulhs | lj | dh | ewaldlj | dhhs_cut | dhhs_vec | ljcut | dhhs_npt |
 //       ewaldlj_npt
 // Output: one JSON object on stdout, doubles printed with %.17g.
@@ -347,6 +348,59 @@ static int run_npt(const std::string &mode, Tmjson &in, const char *state, int n
     std::printf("\"final_rng\": \"%s\"\n}\n", o.str().c_str());
     return 0;
 }
+// A pair Hamiltonian + ExternalPressure with one Propagator holding every
+// move of the input that this path covers (atomtranslate, isobaric,
+// moltransrot), built in the input's key order (nlohmann objects iterate
+// sorted keys) and picked per call as Propagator::move picks them
+// (move.h:7758-7858, 7877-7891: element() = begin + range(0, n - 1)).
+template<class Tpair>
+static int run_npt_prop(Tmjson &in, const char *state, int ncalls)
+{
+    Tspace spc(in);
+    auto pot = Tpair(in) + Energy::ExternalPressure<Tspace>(in);
+    spc.load(state, Tspace::RESIZE);
+    Recorder rec;
+    std::vector<std::unique_ptr<Move::Movebase<Tspace>>> mv;
+    auto m = in.at("moves");
+    for (auto i = m.begin(); i != m.end(); ++i) {
+        RecAT<Tspace>::internal().eng = slump.eng;   // move.h:7781
+        if (i.key() == "atomtranslate") mv.emplace_back(new RecAT<Tspace>(pot, spc, i.value(), &rec));
+        if (i.key() == "isobaric") mv.emplace_back(new RecIso<Tspace>(pot, spc, i.value(), &rec));
+        if (i.key() == "moltransrot") mv.emplace_back(new RecTR<Tspace>(pot, spc, i.value(), &rec));
+    }
+    if (mv.empty()) return 2;
+    std::vector<int> picks;
+    double uinit = Energy::systemEnergy(spc, pot, spc.p);
+    double utot = 0;
+    for (int c = 0; c < ncalls; ++c) {
+        int k = RecAT<Tspace>::internal().range(0, (int)mv.size() - 1);
+        picks.push_back(k);
+        utot += mv[k]->move();
+    }
+    double ufinal = Energy::systemEnergy(spc, pot, spc.p);
+    std::printf("{\n");
+    print_params(in);
+    jnum("pressure", in["moves"]["isobaric"].at("pressure").get<double>() * 1.0_mM);
+    jnum("uinit", uinit);
+    jnum("ufinal", ufinal);
+    jnum("utot", utot);
+    jivec("picks", picks);
+    jvec("du", rec.du);
+    jivec("acc", rec.acc);
+    jivec("who", rec.who);
+    std::vector<double> xyz;
+    for (auto &p : spc.p) { xyz.push_back(p.x()); xyz.push_back(p.y()); xyz.push_back(p.z()); }
+    jvec("final_xyz", xyz);
+    std::vector<double> cm;
+    for (auto g : spc.groupList()) { cm.push_back(g->cm.x()); cm.push_back(g->cm.y()); cm.push_back(g->cm.z()); }
+    jvec("final_cm", cm);
+    jvec("final_len", {spc.geo.len.x(), spc.geo.len.y(), spc.geo.len.z()});
+    std::ostringstream o;
+    o << slump.eng;
+    std::printf("\"final_rng\": \"%s\"\n}\n", o.str().c_str());
+    return 0;
+}
+
 typedef CombinedPairPotential<Coulomb, HardSphere> TCoulHS;
 
 int main(int argc, char **argv)
@@ -402,6 +456,8 @@ int main(int argc, char **argv)
     if (kind == "ewaldlj") { DISPATCH(E_ewaldlj); }
     if (kind == "ljcut") { DISPATCH(E_ljcut); }
     const int ncalls_at = argc > 6 ? std::atoi(argv[6]) : 0;
+    if (mode == "mcprop" && kind == "dhhs_npt") return run_npt_prop<E_dhhs>(in, state, ncalls);
+    if (mode == "mcprop" && kind == "ewaldlj_npt") return run_npt_prop<E_ewaldlj>(in, state, ncalls);
     if (kind == "dhhs_npt") return run_npt<E_dhhs>(mode, in, state, ncalls, ncalls_at);
     if (kind == "ewaldlj_npt") return run_npt<E_ewaldlj>(mode, in, state, ncalls, ncalls_at);
     std::fprintf(stderr, "unknown kind %s\n", kind.c_str());

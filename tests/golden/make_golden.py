@@ -95,7 +95,7 @@ def run_harness(args):
 
 
 def build_fixture(name, kind, cfg, L, parts, groups, seed, ncalls, moves_meta, energy=True,
-                  mc_extra=()):
+                  mc_extra=(), mc_mode="mc"):
     with tempfile.TemporaryDirectory() as td:
         cpath = os.path.join(td, "in.json")
         spath = os.path.join(td, "state")
@@ -116,9 +116,10 @@ def build_fixture(name, kind, cfg, L, parts, groups, seed, ncalls, moves_meta, e
         if energy:
             fx["energy"] = run_harness(["energy", kind, cpath, spath])
         if ncalls > 0:
-            fx["mc"] = run_harness(["mc", kind, cpath, spath, str(ncalls)] +
+            fx["mc"] = run_harness([mc_mode, kind, cpath, spath, str(ncalls)] +
                                    [str(a) for a in mc_extra])
             fx["mc"]["ncalls"] = ncalls
+            fx["mc"]["mode"] = mc_mode
             if mc_extra:
                 fx["mc"]["ncalls_at"] = int(mc_extra[0])
     path = os.path.join(HERE, name + ".json")
@@ -182,10 +183,11 @@ def salt_npt(name, n, L, seed, ncalls, pressure_mM, dp, r=2.0):
 
 
 def ewald_npt(name, n, L, seed, ncalls, ncalls_at, pressure_mM, dp, eps_surf,
-              update_frequency, alpha=0.35, rc=7.0, kc=4.0):
+              update_frequency, alpha=0.35, rc=7.0, kc=4.0, mixed=False):
     """Ewald salt (NonbondedEwald<LennardJonesLB>) with the ExternalPressure
-    term: `ncalls` calls of a Propagator holding one Isobaric move, then
-    `ncalls_at` calls of one holding an AtomicTranslation move."""
+    term and a Propagator holding an AtomicTranslation and an Isobaric move:
+    `ncalls` volume trials, then `ncalls_at` AtomicTranslation calls; or
+    (mixed) `ncalls` calls picking either move as Propagator::move does."""
     rng = np.random.default_rng(seed)
     half = n // 2
     radii = np.full(n, 1.5)
@@ -208,9 +210,11 @@ def ewald_npt(name, n, L, seed, ncalls, ncalls_at, pressure_mM, dp, eps_surf,
     moves = [{"type": "AT", "molid": 0, "prob": 1.0, "peratom": 1, "permol": 0},
              {"type": "NPT", "molid": -1, "prob": 1.0, "peratom": 0, "permol": 0, "dp": dp,
               "pressure_mM": pressure_mM}]
-    fx = build_fixture(name, "ewaldlj_npt", cfg, [L, L, L], parts, groups, seed, ncalls, moves,
-                       mc_extra=(ncalls_at,))
-    return fx
+    if mixed:
+        return build_fixture(name, "ewaldlj_npt", cfg, [L, L, L], parts, groups, seed, ncalls,
+                             moves, mc_mode="mcprop")
+    return build_fixture(name, "ewaldlj_npt", cfg, [L, L, L], parts, groups, seed, ncalls, moves,
+                         mc_extra=(ncalls_at,))
 
 
 def lj_mix(name, n, L, seed, ncalls, cutoff=None):
@@ -421,6 +425,10 @@ ONLY = {
     "ewald_npt_surface": lambda: ewald_npt("ewald_npt_surface", 30, 18.0, 24, 24, 3,
                                            pressure_mM=6000.0, dp=0.3, eps_surf=1.0,
                                            update_frequency=3, kc=3.0),
+    # one Propagator picking AtomicTranslation or Isobaric per call
+    "ewald_npt_mixed": lambda: ewald_npt("ewald_npt_mixed", 40, 20.0, 25, 40, 0,
+                                         pressure_mM=6000.0, dp=0.4, eps_surf=0.5,
+                                         update_frequency=5, mixed=True),
 }
 
 

@@ -95,6 +95,8 @@ def lib():
         L.or_ewald_update.restype = D; L.or_ewald_update.argtypes = [P, I]
         L.or_isobaric_propagate.restype = C.c_long
         L.or_isobaric_propagate.argtypes = [P, D, D, I, dp, ip, C.c_long, dp]
+        L.or_propagate_npt.restype = C.c_long
+        L.or_propagate_npt.argtypes = [P, D, D, I, I, dp, ip, C.c_long, dp]
         L.or_get_box.argtypes = [P, dp]
         for fn in ("or_ewald_self", "or_ewald_surface", "or_ewald_reciprocal"):
             getattr(L, fn).restype = D
@@ -250,6 +252,24 @@ class OracleSystem:
         if n < 0:
             raise ValueError("oracle: isobaric move not supported for this system")
         return du[:n], acc[:n], ut.value
+
+    def propagate_npt(self, dp, prob, npt_index, ncalls, cap=1 << 16):
+        """Propagator::move calls of a Propagator holding the registered
+        moves and one Isobaric move at position npt_index; the trial record
+        includes who (-1 for volume trials)."""
+        L = lib()
+        du = np.zeros(cap)
+        acc = np.zeros(cap, dtype=np.int32)
+        who = np.full(cap, -1, dtype=np.int32)
+        L.or_last_trials(self.h, iptr(who), cap)
+        ut = C.c_double()
+        n = L.or_propagate_npt(self.h, dp, prob, npt_index, ncalls, dptr(du), iptr(acc), cap,
+                               C.byref(ut))
+        L.or_last_trials(self.h, None, 0)
+        self._who = who
+        if n < 0:
+            raise ValueError("oracle: isobaric move not supported for this system")
+        return du[:n], acc[:n], who[:n], ut.value
 
     def box(self):
         b = np.zeros(3)

@@ -87,3 +87,43 @@ def test_isobaric_refuses_missing_pressure(gpu):
                 s.isobaric_propagate(0.1, 1.0, 1)
         finally:
             s.close()
+
+
+def test_mixed_propagator_matches_reference(gpu):
+    """One Propagator holding AtomicTranslation and Isobaric (fixture
+    ewald_npt_mixed, Ewald salt): each call picks one of the two as the
+    reference's Propagator::move does; volume trials run host-driven,
+    translation calls on the device sweep.  Same trials, accept sequence and
+    final state as the compiled reference."""
+    from fixture_systems import fgx_system_from_fixture
+    fx = load_fixture("ewald_npt_mixed")
+    mc = fx["mc"]
+    mv = [m for m in fx["moves"] if m["type"] == "NPT"][0]
+    keys = sorted(k for k in fx["config"]["moves"])
+    say("[isobaric] ewald_npt_mixed: %d Propagator calls over %s" % (mc["ncalls"], keys))
+    s = fgx_system_from_fixture(fx)
+    try:
+        s.rng_seed(fx["rng_seed"])
+        s.propagator_init()
+        st, du, acc = s.propagate_npt(mv["dp"], mv["prob"], keys.index("isobaric"),
+                                      mc["ncalls"], record=100000)
+        box = s.box()
+        xyz = s.positions(0).reshape(-1)
+        words, pos = s.rng_state()
+        u1 = s.system_energy()
+    finally:
+        s.close()
+    ref_du = farr(mc["du"])
+    assert st["trials"] == len(ref_du) == len(du)
+    assert np.array_equal(acc, np.array(mc["acc"], dtype=np.int32))
+    fin = np.isfinite(ref_du)
+    assert np.array_equal(np.isfinite(du), fin)
+    err = np.abs(du[fin] - ref_du[fin]) / np.maximum(1.0, np.abs(ref_du[fin]))
+    say("  trials %d (volume %d), accepted %d, max rel dU err %.2e, box %.6f"
+        % (len(du), sum(1 for w in mc["who"] if w == -1), acc.sum(), err.max(), box[0]))
+    assert err.max() <= 1e-10
+    np.testing.assert_allclose(box, farr(mc["final_len"]), rtol=1e-12)
+    assert np.max(np.abs(xyz - farr(mc["final_xyz"]))) <= 1e-9
+    assert " ".join(str(w) for w in words) + " %d" % pos == mc["final_rng"].strip()
+    assert st["utot"] == pytest.approx(mc["utot"], rel=1e-9, abs=1e-9)
+    assert u1 == pytest.approx(mc["ufinal"], rel=1e-10)

@@ -126,7 +126,40 @@ def test_metropolis_sequence_matches_reference(name):
     assert uinit == float(mc["uinit"])
 
 
-@pytest.mark.parametrize("name", [f for f in FIXTURES if is_npt(load_fixture(f))])
+def npt_index(fx):
+    """position of the Isobaric move in the Propagator's move list: the
+    input's move keys in nlohmann's (sorted) order"""
+    keys = sorted(k for k in fx["config"]["moves"] if k in ("atomtranslate", "isobaric",
+                                                               "moltransrot"))
+    return keys.index("isobaric")
+
+
+@pytest.mark.parametrize("name", [f for f in FIXTURES if is_npt(load_fixture(f))
+                                  and load_fixture(f)["mc"].get("mode") == "mcprop"])
+def test_mixed_propagator_matches_reference(name):
+    """One Propagator holding AtomicTranslation and Isobaric, picked per call
+    (move.h:7877-7891): the picks, every trial's dU and accept flag, the
+    final box, coordinates and engine state as the compiled reference left
+    them."""
+    fx = load_fixture(name)
+    mc = fx["mc"]
+    mv = [m for m in fx["moves"] if m["type"] == "NPT"][0]
+    s = OracleSystem.from_fixture(fx)
+    s.seed(fx["rng_seed"])
+    lib().or_propagator_init(s.h)
+    du, acc, who, utot = s.propagate_npt(mv["dp"], mv["prob"], npt_index(fx), mc["ncalls"])
+    assert (who == -1).sum() > 0 and (who >= 0).sum() > 0
+    assert np.array_equal(who, np.array(mc["who"], dtype=np.int32))
+    assert np.array_equal(acc, np.array(mc["acc"], dtype=np.int32))
+    assert np.array_equal(du, farr(mc["du"]))
+    assert np.array_equal(s.box(), farr(mc["final_len"]))
+    assert np.array_equal(s.xyz().reshape(-1), farr(mc["final_xyz"]))
+    assert utot == mc["utot"]
+    assert s.system_energy() == pytest.approx(mc["ufinal"], rel=1e-12)
+
+
+@pytest.mark.parametrize("name", [f for f in FIXTURES if is_npt(load_fixture(f))
+                                  and load_fixture(f)["mc"].get("mode") != "mcprop"])
 def test_isobaric_matches_reference(name):
     """Move::Isobaric with ExternalPressure (move.h:5424-5640, energy.h:1817-1894):
     volume trials, accept sequence, final box and coordinates as the compiled
