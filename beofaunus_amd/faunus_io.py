@@ -243,12 +243,16 @@ def read_input(json_path, state_path, *, elec, sr, nbkind=fgx.NB_PLAIN, precisio
     pressure = (mM_to_per_A3(float(cfg["moves"]["isobaric"]["pressure"]))
                 if external_pressure else None)
     moves, other = read_moves(cfg.get("moves", {}), molnames)
-    # the device Propagator runs one registered move; with any other move
-    # registered (even at prob 0, it takes part in Propagator's pick) the
-    # system is built for energies only
+    # the device Propagator runs one registered move, alone or beside one
+    # isobaric move; with any other move registered (even at prob 0, it takes
+    # part in Propagator's pick) the system is built for energies only
     move = None
-    if len(moves) == 1 and not other and moves[0]["type"] != "NPT":
-        mv = moves[0]
+    path = [m for m in moves if m["type"] != "NPT"]
+    keys = sorted(k for k in cfg.get("moves", {}) if k in ("atomtranslate", "isobaric",
+                                                            "moltransrot"))
+    npt_index = keys.index("isobaric") if "isobaric" in keys else None
+    if len(path) == 1 and not other:
+        mv = path[0]
         move = {"kind": fgx.MOVE_AT if mv["type"] == "AT" else fgx.MOVE_TR,
                 "molid": mv["molid"], "prob": mv["prob"], "peratom": mv["peratom"],
                 "permol": mv["permol"], "dir": mv["dir"]}
@@ -266,6 +270,9 @@ def read_input(json_path, state_path, *, elec, sr, nbkind=fgx.NB_PLAIN, precisio
                     lj_cut=lj_cut, pressure=pressure),
         "move": move,
         "moves": moves,
+        # position of the isobaric move in the Propagator's move list (its
+        # keys in sorted order), None without one
+        "npt_index": npt_index,
         "other_moves": other,
         "atom_names": names,
         "molecule_names": molnames,
@@ -279,27 +286,32 @@ def run_input(json_path, state_path, out_state, ncalls, *, elec, sr, nbkind=fgx.
     """Load a reference input and state, seed the global engine from the
     state file (as Space::load does), construct the Propagator (its engine
     copied from the global one, move.h:7781), run `ncalls` Propagator::move()
-    calls on the device (its one registered move: atomtranslate, moltransrot
-    or isobaric) and write the final state (box, positions, mass centres, the
-    global engine) to `out_state`.  Returns the run statistics."""
+    calls on the device (its registered moves: atomtranslate or moltransrot,
+    isobaric, or one of the first two with isobaric) and write the final
+    state (box, positions, mass centres, the global engine) to `out_state`.
+    Returns the run statistics."""
     from . import workloads
     w = read_input(json_path, state_path, elec=elec, sr=sr, nbkind=nbkind,
                    precision=precision, lj_cutshift=lj_cutshift,
                    external_pressure=external_pressure)
-    npt = len(w["moves"]) == 1 and not w["other_moves"] and w["moves"][0]["type"] == "NPT"
-    if w["move"] is None and not npt:
-        raise ValueError("%s: the device Propagator needs exactly one registered move "
-                         "(atomtranslate or moltransrot), found %s and %s"
-                         % (json_path, [m["type"] for m in w["moves"]], w["other_moves"]))
+    npt_only = (len(w["moves"]) == 1 and not w["other_moves"]
+                and w["moves"][0]["type"] == "NPT")
+    if w["move"] is None and not npt_only:
+        raise ValueError("%s: the device Propagator needs one registered move (atomtranslate "
+                         "or moltransrot for one molecule), optionally with isobaric; found %s "
+                         "and %s" % (json_path, [m["type"] for m in w["moves"]],
+                                     w["other_moves"]))
+    iso = [m for m in w["moves"] if m["type"] == "NPT"]
     s = workloads.make_system(w)
     try:
         if w["rng"] is not None:
             words, pos = mt_parse(w["rng"])
             s.rng_set_state(words, pos)
         s.propagator_init()
-        if npt:   # a Propagator holding one Isobaric move
-            mv = w["moves"][0]
-            st = s.isobaric_propagate(mv["dp"], mv["prob"], ncalls)[0]
+        if npt_only:   # a Propagator holding one Isobaric move
+            st = s.isobaric_propagate(iso[0]["dp"], iso[0]["prob"], ncalls)[0]
+        elif iso:      # the registered move and an Isobaric move
+            st = s.propagate_npt(iso[0]["dp"], iso[0]["prob"], w["npt_index"], ncalls)[0]
         else:
             st = s.propagate(ncalls)[0]
         xyz = s.positions(0)

@@ -111,7 +111,7 @@ def test_state_round_trip():
 
 def test_reference_example_state():
     """bulk.state of the reference's examples, read as Space::load reads it"""
-    from beofaunus_amd import faunus_io
+    from beofaunus_amd import faunus_io, fgx
     fx = load_fixture("example_bulk_lj")
     with tempfile.TemporaryDirectory() as d:
         c, s = write_inputs(fx, d)
@@ -119,8 +119,10 @@ def test_reference_example_state():
     assert len(w["charge"]) == 2304 and w["box"] == [42.5, 42.5, 42.5]
     assert sorted(set(w["charge"].tolist())) == [-1.0, 1.0]
     assert w["atom_names"] == ["unk", "Cl", "Na"] and w["T"] == 1100
-    # its input registers an isobaric move beside atomtranslate: energies only
-    assert w["move"] is None and [m["type"] for m in w["moves"]] == ["AT", "NPT"]
+    # its input registers an isobaric move beside atomtranslate (the
+    # Propagator's list in key order: atomtranslate, isobaric)
+    assert w["move"]["kind"] == fgx.MOVE_AT and [m["type"] for m in w["moves"]] == ["AT", "NPT"]
+    assert w["npt_index"] == 1
     assert faunus_io.mt_parse(w["rng"])[1] <= 624
 
 
