@@ -1606,6 +1606,77 @@ static int all_groups_config(fgx_system *s, int which, double *u)
     return FGX_OK;
 }
 
+// Energybase::setSpace (energy.h:86-90) with the Space's box (Cuboid::setlen,
+// src/geometry.cpp:265-274): the box, and for NonbondedEwald::setSpace
+// (ewald.h:731-757) the k-vectors, A_k and V of that box and S(k) recomputed
+// from the accepted vector (trial S(k) = current)
+extern "C" int fgx_set_space(fgx_system *s, const double len[3])
+{
+    FGX_CHECK(s && len && len[0] > 0 && len[1] > 0 && len[2] > 0, FGX_EARG,
+              "fgx_set_space: bad arguments");
+    int rc;
+    if (len[0] != s->box.len[0] || len[1] != s->box.len[1] || len[2] != s->box.len[2]) {
+        set_box(s->box, len);
+        if ((rc = ewald_rebox(s, len))) return rc;
+        // the fixed-point copies, cell lists and group-pair cache depend on the box
+        hipLaunchKernelGGL(k_fixed_all, dim3((s->n + 255) / 256), dim3(256), 0, s->stream,
+                           s->v[0].x, s->v[0].y, s->v[0].z, s->d_qf, s->n, s->box, s->d_pk);
+        FGX_HIP(hipGetLastError());
+        s->emat_valid = s->cell_valid = false;
+    }
+    if ((rc = ewald_set_space(s))) return rc;
+    FGX_HIP(hipStreamSynchronize(s->stream));
+    return FGX_OK;
+}
+
+// energyChangeConfiguration (energy.h:3521-3572) of a change that moves every
+// group (Isobaric::_trialMove, move.h:5548-5565), for `which` at the current
+// box: g2g over all group pairs, external, each group's g_external and the
+// internal pairs of the atomic groups
+extern "C" int fgx_config_energy(fgx_system *s, int which, double *u)
+{
+    FGX_CHECK(check_which(s, which) && u, FGX_EARG, "fgx_config_energy: bad arguments");
+    return all_groups_config(s, which, u);
+}
+
+// accept / reject of a change that moved every group: Isobaric::_acceptMove /
+// _rejectMove copy every group (group.h:371-389), then NonbondedEwald::update
+// (ewald.h:613-648) with its accept counter and refresh
+extern "C" int fgx_commit_all(fgx_system *s, int accepted, double *drift)
+{
+    FGX_CHECK(s, FGX_EARG, "fgx_commit_all: null system");
+    if (int prc = pull_device_state(s)) return prc;
+    s->state_on_dev = false;
+    s->emat_valid = s->cell_valid = false;
+    Vec64 dst = accepted ? s->v[0] : s->v[1];
+    Vec64 src = accepted ? s->v[1] : s->v[0];
+    if (s->n > 0) {
+        hipLaunchKernelGGL(k_copy_moved, dim3((s->n + 255) / 256), dim3(256), 0, s->stream, dst,
+                           src, (const int *)nullptr, s->n, 0, s->d_pk, s->d_qf, s->box,
+                           accepted ? 1 : 0);
+        FGX_HIP(hipGetLastError());
+    }
+    int rc;
+    if ((rc = sync_cm_from_device(s))) return rc;
+    for (size_t k = 0; k < s->cm.size(); ++k) {
+        if (accepted) s->cm[k] = s->cm_trial[k];
+        else s->cm_trial[k] = s->cm[k];
+    }
+    if ((rc = push_cm(s))) return rc;
+    double d = 0.0;
+    if (s->has_ewald) {
+        if (!accepted) {
+            FGX_HIP(hipMemcpyAsync(s->ew.Qt, s->ew.Q, sizeof(double) * 2 * s->ew.K,
+                                   hipMemcpyDeviceToDevice, s->stream));
+        } else if ((rc = ewald_accepted(s, &d))) {
+            return rc;
+        }
+    }
+    FGX_HIP(hipStreamSynchronize(s->stream));
+    if (drift) *drift = d;
+    return FGX_OK;
+}
+
 extern "C" int fgx_get_box(const fgx_system *s, double len[3])
 {
     FGX_CHECK(s && len, FGX_EARG, "fgx_get_box: bad arguments");

@@ -130,6 +130,9 @@ def lib():
         "fgx_set_cell_list": (I, [P, I]),
         "fgx_isobaric_propagate": (I, [P, D, D, I, C.POINTER(Stats), _dp, _ip, C.c_long]),
         "fgx_propagate_npt": (I, [P, D, D, I, I, C.POINTER(Stats), _dp, _ip, C.c_long]),
+        "fgx_set_space": (I, [P, _dp]),
+        "fgx_config_energy": (I, [P, I, _dp]),
+        "fgx_commit_all": (I, [P, I, _dp]),
         "fgx_get_box": (I, [P, _dp]),
         "fgx_get_cell_list": (I, [P, _ip, _ip]),
         "fgx_chain_seconds": (I, [P, C.POINTER(C.c_double)]),

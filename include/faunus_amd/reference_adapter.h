@@ -16,6 +16,9 @@
  *       -- the reference moves (move.h:730-967, 1505-1801) with
  *       _energyChange() replaced by one device call per trial (instead of
  *       2(N-1) virtual i2i calls) and accept/reject mirrored on the device.
+ *   Move::GPUIsobaric<Tspace> -- the reference's Isobaric move (move.h:
+ *       5424-5640) with its energy change in two device calls; the setSpace
+ *       calls around it keep the box and Ewald's k-vectors on the device.
  *   Move::GPUPropagator<Tspace> -- Propagator::move (move.h:7877-7891) run
  *       entirely on the device: reads the global and move-internal engines
  *       (slump.h:126, move.h:356-360), writes the coordinates and engines
@@ -52,14 +55,22 @@ struct GPUParams {
     double alpha = 0, rc = 0, kc = 0, debyelength = 0, eps_surf = 0;
     int spherical_sum = 1, update_frequency = -1;
     double lj_cut = 0;   /* CutShift<LennardJonesLB> cutoff; 0 = uncut */
+    int has_pressure = 0;   /* + Energy::ExternalPressure */
+    double pressure = 0;    /* 1/A^3 */
 
     /* energy.nonbonded section as read by Potential::Coulomb / DebyeHuckel
      * (src/potentials.cpp:357-363, 770-782), NonbondedCutg2g
      * (energy.h:1363-1369) and NonbondedEwald (ewald.h:507-527) */
     static GPUParams fromJson(Tmjson &j, int elec, int sr, int nbkind, int precision = FGX_FP64,
-                              bool lj_cutshift = false)
+                              bool lj_cutshift = false, bool external_pressure = false)
     {
         GPUParams p;
+        /* Energy::ExternalPressure: moves.isobaric.pressure in mM
+         * (energy.h:1866-1871) */
+        if (external_pressure) {
+            p.has_pressure = 1;
+            p.pressure = j["moves"]["isobaric"].at("pressure").get<double>() * 1.0_mM;
+        }
         /* Potential::CutShift<LennardJonesLB>: energy.nonbonded.cutoff
          * (potentials.h:1785-1786) */
         if (lj_cutshift) p.lj_cut = j["energy"]["nonbonded"].at("cutoff");
@@ -122,6 +133,8 @@ fgx_system *createSystem(Tspace &spc, const GPUParams &prm)
     h.ewald_debyelength = prm.debyelength; h.ewald_eps_surf = prm.eps_surf;
     h.ewald_spherical_sum = prm.spherical_sum; h.ewald_update_frequency = prm.update_frequency;
     h.lj_cut = prm.lj_cut;
+    h.has_pressure = prm.has_pressure;
+    h.pressure = prm.pressure;
     double box[3] = {spc.geo.len.x(), spc.geo.len.y(), spc.geo.len.z()};
     fgx_system *s = nullptr;
     check(fgx_create(&h, box, n, xyz.data(), q.data(), r.data(), mw.data(), t.data(),
@@ -129,6 +142,22 @@ fgx_system *createSystem(Tspace &spc, const GPUParams &prm)
     return s;
 }
 
+}  // namespace AMD
+
+namespace Energy {
+template <class Tspace> class GPUNonbonded;
+}
+
+namespace AMD {
+/* the GPUNonbonded term of a Hamiltonian (or the term itself) */
+template <class Tspace, class Tenergy>
+Energy::GPUNonbonded<Tspace> &gpuTerm(Tenergy &e)
+{
+    auto t = e.tuple();   // pointers to the energy terms
+    auto g = TupleFindType::get<Energy::GPUNonbonded<Tspace> *>(t);
+    if (g == nullptr) throw std::runtime_error("libfgx: the Hamiltonian has no GPUNonbonded term");
+    return **g;
+}
 }  // namespace AMD
 
 namespace Energy {
@@ -146,9 +175,17 @@ class GPUNonbonded : public Energybase<Tspace> {
 
 public:
     explicit GPUNonbonded(const AMD::GPUParams &p) : prm(p) { base::name = "Nonbonded (libfgx, MI355X)"; }
+    /* a copy (CombinedEnergy holds its terms by value, energy.h:646-661,
+     * 749) creates its own device system at its first setSpace */
+    GPUNonbonded(const GPUNonbonded &o) : base(o), prm(o.prm) {}
+    GPUNonbonded &operator=(const GPUNonbonded &) = delete;
     ~GPUNonbonded() { if (h) fgx_destroy(h); }
 
+    /* the energy terms for TupleFindType lookups (as Nonbonded, energy.h:809) */
+    auto tuple() -> decltype(std::make_tuple(this)) { return std::make_tuple(this); }
+
     fgx_system *handle() { return h; }
+    bool hasPressure() const { return prm.has_pressure != 0; }
 
     static std::string device()
     {
@@ -161,11 +198,15 @@ public:
         return o.str();
     }
 
+    /* Energybase::setSpace; with Ewald NonbondedEwald::setSpace
+     * (ewald.h:731-757): k-vectors of the Space's box, S(k) from p */
     void setSpace(Tspace &s) override
     {
         base::setSpace(s);
         if (!h) h = AMD::createSystem(s, prm);
         else resync();
+        double len[3] = {s.geo.len.x(), s.geo.len.y(), s.geo.len.z()};
+        AMD::check(fgx_set_space(h, len));
     }
 
     /* full upload of both vectors and mass centres, after host code outside
@@ -226,11 +267,21 @@ public:
 
     int index(Group &g) { return base::spc->findIndex(&g); }
 
+    /* energyChangeConfiguration for a change moving every group (one call
+     * instead of the per-pair i2i loop over the atomic groups) */
+    double configEnergy(const Tpvec &p)
+    {
+        double u; upload(p); AMD::check(fgx_config_energy(h, which(p), &u)); return u;
+    }
+
     /* Energybase::updateChange (ewald.h:651-705): stage the trial
      * coordinates and mass centre of the moved group and the trial S(k) */
     double updateChange(const typename Tspace::Change &c) override
     {
         pending = c;
+        /* geometry change: what NonbondedEwald computes here (ewald.h:653-658)
+         * is replaced by the setSpace calls around the trial */
+        if (c.geometryChange) return 0.0;
         if (c.mvGroup.size() != 1)
             throw std::runtime_error("libfgx: a change must move exactly one group");
         auto &m = *c.mvGroup.begin();
@@ -272,8 +323,13 @@ public:
     double update(bool accepted) override
     {
         if (pending.mvGroup.empty()) return 0.0;
-        fgx_change fc = change(pending);
         double drift = 0.0;
+        if (pending.geometryChange) {
+            AMD::check(fgx_commit_all(h, accepted ? 1 : 0, &drift));
+            pending.clear();
+            return drift;
+        }
+        fgx_change fc = change(pending);
         AMD::check(fgx_commit(h, &fc, accepted ? 1 : 0, &drift));
         pending.clear();
         return drift;
@@ -302,8 +358,10 @@ class GPUAtomicTranslation : public AtomicTranslation<Tspace> {
     Energy::GPUNonbonded<Tspace> &gpu;
 
 public:
-    GPUAtomicTranslation(Energy::GPUNonbonded<Tspace> &e, Tspace &s, Tmjson &j)
-        : base(e, s, j), gpu(e) {}
+    /* e: a GPUNonbonded, or a Hamiltonian holding one */
+    template <class Tenergy>
+    GPUAtomicTranslation(Tenergy &e, Tspace &s, Tmjson &j)
+        : base(e, s, j), gpu(AMD::gpuTerm<Tspace>(e)) {}
 
 protected:
     double _energyChange() override
@@ -320,14 +378,69 @@ class GPUTranslateRotate : public TranslateRotate<Tspace> {
     Energy::GPUNonbonded<Tspace> &gpu;
 
 public:
-    GPUTranslateRotate(Energy::GPUNonbonded<Tspace> &e, Tspace &s, Tmjson &j)
-        : base(e, s, j), gpu(e) {}
+    template <class Tenergy>
+    GPUTranslateRotate(Tenergy &e, Tspace &s, Tmjson &j)
+        : base(e, s, j), gpu(AMD::gpuTerm<Tspace>(e)) {}
 
 protected:
     double _energyChange() override
     {
         if (this->dp_rot < 1e-6 && this->dp_trans < 1e-6) return 0;
         return gpu.deltaU();
+    }
+};
+
+/* Isobaric (move.h:5424-5640) with Energy::energyChange (energy.h:3577-3621)
+ * evaluated by two device calls (trial configuration at the trial box, the
+ * accepted one at the current box); the setSpace calls around them follow the
+ * reference's order, so Ewald's k-vectors and S(k) evolve as there.  The
+ * Hamiltonian's ExternalPressure term comes from GPUParams::has_pressure. */
+template <class Tspace>
+class GPUIsobaric : public Isobaric<Tspace> {
+    typedef Isobaric<Tspace> base;
+    Energy::GPUNonbonded<Tspace> &gpu;
+    Energy::ExternalPressure<Tspace> *press;
+
+    /* the ExternalPressure terms of energyChangeConfiguration: external and
+     * each group's g_external (energy.h:1876-1893) */
+    double pressureTerms(const typename Tspace::ParticleVector &p)
+    {
+        double u = press->external(p);
+        for (auto g : this->spc->groupList()) u += press->g_external(p, *g);
+        return u;
+    }
+
+public:
+    /* e: the Hamiltonian, holding a GPUNonbonded and an ExternalPressure
+     * term (which Isobaric's constructor looks up, move.h:5467-5472) */
+    template <class Tenergy>
+    GPUIsobaric(Tenergy &e, Tspace &s, Tmjson &j)
+        : base(e, s, j), gpu(AMD::gpuTerm<Tspace>(e))
+    {
+        auto t = e.tuple();
+        press = *TupleFindType::get<Energy::ExternalPressure<Tspace> *>(t);   // base checked it
+        if (gpu.hasPressure())
+            throw std::runtime_error("GPUIsobaric: pressure both in GPUParams and in the "
+                                     "Hamiltonian's ExternalPressure term");
+    }
+
+protected:
+    double _energyChange() override
+    {
+        Tspace &s = *this->spc;
+        auto backup = s.geo;
+        s.geo = s.geo_trial;
+        this->pot->setSpace(s);
+        for (auto g : s.groupList())
+            for (auto j : *g)
+                if (s.geo.collision(s.trial[j], s.trial[j].radius,
+                                    Geometry::Geometrybase::BOUNDARY))
+                    return pc::infty;   // the trial geometry stays until _rejectMove
+        double unew = gpu.configEnergy(s.trial) + pressureTerms(s.trial);
+        s.geo = backup;
+        this->pot->setSpace(s);
+        double uold = gpu.configEnergy(s.p) + pressureTerms(s.p);
+        return unew - uold;
     }
 };
 
@@ -342,14 +455,27 @@ class GPUPropagator {
 
 public:
     fgx_stats last{};
+    /* an Isobaric move in the list: its position (npt_index, -1 without
+     * one), dp and run fraction (move.h:5455-5464) */
+    int npt_index = -1, nregistered = 0;
+    double npt_dp = 0, npt_prob = 1;
 
     GPUPropagator(Tmjson &in, Energy::GPUNonbonded<Tspace> &e, Tspace &s) : gpu(e), spc(s)
     {
         gpu.setSpace(s);
         auto m = in.at("moves");
+        int pos = 0;
         for (auto i = m.begin(); i != m.end(); ++i) {
             bool at = i.key() == "atomtranslate", tr = i.key() == "moltransrot";
+            if (i.key() == "isobaric") {
+                npt_index = pos++;
+                npt_dp = i.value().at("dp");
+                npt_prob = i.value().value("prob", 1.0);
+                continue;
+            }
             if (!at && !tr) continue;
+            ++pos;
+            ++nregistered;
             auto js = i.value();
             bool center = tr ? (bool)js["center_rotation"] : true;
             for (auto it = js.begin(); it != js.end(); ++it) {
@@ -397,7 +523,14 @@ public:
     /* n Propagator::move() calls; returns the summed energy change */
     double move(int n = 1)
     {
-        AMD::check(fgx_propagate(gpu.handle(), n, &last, nullptr, nullptr, 0));
+        if (npt_index < 0)
+            AMD::check(fgx_propagate(gpu.handle(), n, &last, nullptr, nullptr, 0));
+        else if (nregistered == 0)
+            AMD::check(fgx_isobaric_propagate(gpu.handle(), npt_dp, npt_prob, n, &last, nullptr,
+                                              nullptr, 0));
+        else
+            AMD::check(fgx_propagate_npt(gpu.handle(), npt_dp, npt_prob, npt_index, n, &last,
+                                         nullptr, nullptr, 0));
         std::vector<double> xyz(3 * spc.p.size());
         AMD::check(fgx_get_positions(gpu.handle(), 0, xyz.data()));
         for (size_t i = 0; i < spc.p.size(); ++i) {
@@ -410,6 +543,12 @@ public:
             AMD::check(fgx_get_group_cm(gpu.handle(), gi++, cm, ct));
             g->cm = Point(cm[0], cm[1], cm[2]);
             g->cm_trial = g->cm;
+        }
+        if (npt_index >= 0) {   // the box after accepted volume moves
+            double len[3];
+            AMD::check(fgx_get_box(gpu.handle(), len));
+            spc.geo.setlen(Point(len[0], len[1], len[2]));
+            spc.geo_trial = spc.geo;
         }
         pullGlobalEngine();
         return last.utot;

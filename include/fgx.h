@@ -194,6 +194,20 @@ int fgx_isobaric_propagate(fgx_system *s, double dp, double prob, int ncalls, fg
  * Records every trial (dU, accept) up to `cap`. */
 int fgx_propagate_npt(fgx_system *s, double dp, double prob, int npt_index, int ncalls,
                       fgx_stats *st, double *du_out, int *acc_out, long cap);
+
+/* Energybase::setSpace (energy.h:86-90) with the Space's box `len`
+ * (Cuboid::setlen): sets the box, and for NonbondedEwald::setSpace
+ * (ewald.h:731-757) the k-vectors, A_k and V of that box, with S(k)
+ * recomputed from the accepted vector (trial S(k) = current).  The host
+ * binding calls it wherever the reference calls setSpace. */
+int fgx_set_space(fgx_system *s, const double len[3]);
+/* energyChangeConfiguration (energy.h:3521-3572) of `which` for a change that
+ * moves every group (Isobaric, move.h:5548-5565), at the current box */
+int fgx_config_energy(fgx_system *s, int which, double *u);
+/* accept/reject of a change that moved every group (Isobaric::_acceptMove /
+ * _rejectMove, group.h:371-389: every group copied) and NonbondedEwald::update
+ * (ewald.h:613-648); *drift = the refresh's energy correction */
+int fgx_commit_all(fgx_system *s, int accepted, double *drift);
 /* Cuboid side lengths (Cuboid::setlen after an accepted volume move) */
 int fgx_get_box(const fgx_system *s, double len[3]);
 

@@ -12,6 +12,8 @@
 // adapter), output in oracle/_ref/, run by tests/test_gpu_adapter.py.
 //
 // Usage: adapter_test <kind> <config.json> <statefile> <ncalls>
+// (kinds ending in _npt: ExternalPressure and a Propagator holding every move
+// of the input, Isobaric among them)
 
 #include <faunus/faunus.h>
 #include <faunus/ewald.h>
@@ -152,6 +154,93 @@ static int run(const std::string &kind, Tmjson &in, const char *state, int ncall
     return 0;
 }
 
+// The moves of the input in its key order (the Propagator's list,
+// move.h:7789-7843), the engine seeded as the Propagator's constructor does.
+template <class Tat, class Tiso, class Ttr, class Tenergy>
+static std::vector<std::unique_ptr<Move::Movebase<Tspace>>> make_moves(Tmjson &in, Tenergy &pot,
+                                                                       Tspace &spc, Rec *rec)
+{
+    std::vector<std::unique_ptr<Move::Movebase<Tspace>>> mv;
+    auto m = in.at("moves");
+    for (auto i = m.begin(); i != m.end(); ++i) {
+        Recording<Move::AtomicTranslation<Tspace>>::internal().eng = slump.eng;
+        if (i.key() == "atomtranslate") mv.emplace_back(new Tat(pot, spc, i.value(), rec));
+        if (i.key() == "isobaric") mv.emplace_back(new Tiso(pot, spc, i.value(), rec));
+        if (i.key() == "moltransrot") mv.emplace_back(new Ttr(pot, spc, i.value(), rec));
+    }
+    return mv;
+}
+
+static std::vector<double> box_of(Tspace &s)
+{
+    return {s.geo.len.x(), s.geo.len.y(), s.geo.len.z()};
+}
+
+// A Hamiltonian with Energy::ExternalPressure and a Propagator holding every
+// move of the input (Isobaric among them), run the three ways
+template <class Eref>
+static int run_npt(Tmjson &in, const char *state, int ncalls, int elec, int sr, int nbkind)
+{
+    std::printf("{\n");
+    // ---- A: reference CPU
+    {
+        Tspace spc(in);
+        auto pot = Eref(in) + Energy::ExternalPressure<Tspace>(in);
+        spc.load(state, Tspace::RESIZE);
+        Rec rec;
+        auto mv = make_moves<Recording<Move::AtomicTranslation<Tspace>>,
+                             Recording<Move::Isobaric<Tspace>>,
+                             Recording<Move::TranslateRotate<Tspace>>>(in, pot, spc, &rec);
+        (void)Energy::systemEnergy(spc, pot, spc.p);   // Propagator's uinit
+        for (int c = 0; c < ncalls; ++c) {
+            int k = Recording<Move::AtomicTranslation<Tspace>>::internal().range(0, (int)mv.size() - 1);
+            mv[k]->move();
+        }
+        pvec("A_du", rec.du);
+        pivec("A_acc", rec.acc);
+        pvec("A_xyz", coords(spc));
+        pvec("A_len", box_of(spc));
+        std::printf("\"A_rng\": \"%s\",\n", engine_text().c_str());
+    }
+    // ---- B: reference moves, GPU energy (the reference's ExternalPressure
+    // term beside it in the Hamiltonian)
+    {
+        Tspace spc(in);
+        spc.load(state, Tspace::RESIZE);
+        auto prm = AMD::GPUParams::fromJson(in, elec, sr, nbkind, FGX_FP64);
+        auto &pot = Energy::GPUNonbonded<Tspace>(prm) + Energy::ExternalPressure<Tspace>(in);
+        Rec rec;
+        auto mv = make_moves<Recording<Move::GPUAtomicTranslation<Tspace>>,
+                             Recording<Move::GPUIsobaric<Tspace>>,
+                             Recording<Move::GPUTranslateRotate<Tspace>>>(in, pot, spc, &rec);
+        (void)Energy::systemEnergy(spc, pot, spc.p);
+        for (int c = 0; c < ncalls; ++c) {
+            int k = Recording<Move::AtomicTranslation<Tspace>>::internal().range(0, (int)mv.size() - 1);
+            mv[k]->move();
+        }
+        pvec("B_du", rec.du);
+        pivec("B_acc", rec.acc);
+        pvec("B_xyz", coords(spc));
+        pvec("B_len", box_of(spc));
+        std::printf("\"B_rng\": \"%s\",\n", engine_text().c_str());
+    }
+    // ---- C: device-resident Propagator (ExternalPressure on the device)
+    {
+        Tspace spc(in);
+        spc.load(state, Tspace::RESIZE);
+        auto prm = AMD::GPUParams::fromJson(in, elec, sr, nbkind, FGX_FP64, false, true);
+        Energy::GPUNonbonded<Tspace> pot(prm);
+        Move::GPUPropagator<Tspace> prop(in, pot, spc);
+        prop.move(ncalls);
+        std::printf("\"C_trials\": %ld,\n\"C_accepted\": %ld,\n", prop.last.trials,
+                    prop.last.accepted);
+        pvec("C_xyz", coords(spc));
+        pvec("C_len", box_of(spc));
+        std::printf("\"C_rng\": \"%s\"\n}\n", engine_text().c_str());
+    }
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     std::cout.setstate(std::ios_base::failbit);
@@ -171,6 +260,10 @@ int main(int argc, char **argv)
     if (kind == "lj") return run<E_lj>(kind, in, argv[3], ncalls, FGX_ELEC_NONE, FGX_SR_LJ, FGX_NB_PLAIN);
     if (kind == "ewaldlj") return run<E_ewaldlj>(kind, in, argv[3], ncalls, FGX_ELEC_EWALD, FGX_SR_LJ, FGX_NB_VECTOR);
     if (kind == "ljcut") return run<E_ljcut>(kind, in, argv[3], ncalls, FGX_ELEC_NONE, FGX_SR_LJ, FGX_NB_PLAIN, true);
+    if (kind == "dhhs_npt")
+        return run_npt<E_dhhs>(in, argv[3], ncalls, FGX_ELEC_DH, FGX_SR_HS, FGX_NB_PLAIN);
+    if (kind == "ewaldlj_npt")
+        return run_npt<E_ewaldlj>(in, argv[3], ncalls, FGX_ELEC_EWALD, FGX_SR_LJ, FGX_NB_VECTOR);
     std::fprintf(stderr, "unknown kind\n");
     return 2;
 }

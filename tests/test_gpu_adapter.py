@@ -69,3 +69,43 @@ def test_reference_moves_on_gpu_energy(name):
     a = np.array(r["A_xyz"])
     assert np.allclose(np.array(r["B_xyz"]), a, rtol=0, atol=1e-9)
     assert np.allclose(np.array(r["C_xyz"]), a, rtol=0, atol=1e-9)
+
+
+@pytest.mark.parametrize("name", ["salt_npt", "ewald_npt_mixed"])
+def test_reference_isobaric_on_gpu_energy(name):
+    """The reference's Isobaric (and AtomicTranslation) moves in one
+    Propagator: A the reference itself (equal to the fixture), B the
+    reference moves on GPUNonbonded + the reference's ExternalPressure term
+    with Move::GPUIsobaric / GPUAtomicTranslation, C GPUPropagator with the
+    pressure term on the device.  Same trials, accepts, box, coordinates and
+    engine state."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    if not os.path.exists(BIN):
+        pytest.skip("oracle/_ref/adapter_test not built (needs the reference tree at build time)")
+    print("[adapter] %s (isobaric)" % name, flush=True)
+    fx = load_fixture(name)
+    mc = fx["mc"]
+    with tempfile.TemporaryDirectory() as d:
+        c, s = write_inputs(fx, d)
+        out = subprocess.run([BIN, fx["kind"], c, s, str(mc["ncalls"])], capture_output=True,
+                             text=True, timeout=300)
+        assert out.returncode == 0, out.stderr[-2000:]
+        r = json.loads(out.stdout)
+    A_du, B_du = farr(r["A_du"]), farr(r["B_du"])
+    assert np.array_equal(A_du, farr(mc["du"]))
+    assert r["A_acc"] == r["B_acc"] == mc["acc"]
+    fin = np.isfinite(A_du)
+    assert np.array_equal(np.isinf(A_du), np.isinf(B_du))
+    err = np.abs(A_du[fin] - B_du[fin]) / np.maximum(1.0, np.abs(A_du[fin]))
+    print("  %d trials, max rel dU err %.2e" % (len(A_du), err.max()), flush=True)
+    assert err.max() <= 1e-10
+    assert r["A_rng"] == r["B_rng"] == r["C_rng"] == mc["final_rng"].strip()
+    assert r["C_trials"] == len(A_du) and r["C_accepted"] == sum(r["A_acc"])
+    np.testing.assert_allclose(r["A_len"], mc["final_len"], rtol=0, atol=0)
+    np.testing.assert_allclose(r["B_len"], r["A_len"], rtol=1e-12)
+    np.testing.assert_allclose(r["C_len"], r["A_len"], rtol=1e-12)
+    a = np.array(r["A_xyz"])
+    assert np.allclose(np.array(r["B_xyz"]), a, rtol=0, atol=1e-9)
+    assert np.allclose(np.array(r["C_xyz"]), a, rtol=0, atol=1e-9)
