@@ -217,6 +217,34 @@ def ewald_npt(name, n, L, seed, ncalls, ncalls_at, pressure_mM, dp, eps_surf,
                          mc_extra=(ncalls_at,))
 
 
+def ewald_neutral(name, n, n0, L, seed, ncalls, alpha=0.35, rc=7.0, kc=4.0):
+    """Ewald salt with n0 particles of a neutral atom type.  The reference's
+    effective-charge ratio of a q = 0 type is 0/0 = NaN (ewald.h:535-545), so
+    S(k) and every energy change are NaN, and Movebase::metropolis accepts a
+    NaN change (move.h:567-573)."""
+    rng = np.random.default_rng(seed)
+    half = n // 2
+    radii = np.full(n + n0, 1.5)
+    xyz = rsa(rng, n + n0, L, radii)
+    ids = np.array([1] * half + [2] * (n - half) + [3] * n0)
+    charge = np.where(ids == 1, 1.0, np.where(ids == 2, -1.0, 0.0))
+    cfg = {
+        "system": {"temperature": 298.15, "geometry": {"length": L}},
+        "atomlist": {"T1p": {"q": 1.0, "sigma": 3.0, "eps": 0.2479, "dp": 1.0, "mw": 1.0},
+                     "T2m": {"q": -1.0, "sigma": 3.0, "eps": 0.2479, "dp": 1.0, "mw": 1.0},
+                     "T3n": {"q": 0.0, "sigma": 3.0, "eps": 0.2479, "dp": 1.0, "mw": 1.0}},
+        "moleculelist": {"salt": {"atoms": "T1p T2m T3n", "atomic": True, "Ninit": 0}},
+        "moves": {"atomtranslate": {"salt": {"peratom": True}}},
+        "energy": {"nonbonded": {"epsr": lB_epsr(7.1), "ewald": {
+            "alpha": alpha, "cutoff": rc, "cutoffK": kc, "debyelength": 1e12,
+            "eps_surf": 0.5, "spherical_sum": True, "update_frequency": 5}}},
+    }
+    parts = {"xyz": xyz, "charge": charge, "radius": radii, "mw": np.ones(n + n0), "id": ids}
+    groups = [{"front": 0, "back": n + n0 - 1, "molid": 0, "molsize": 1}]
+    moves = [{"type": "AT", "molid": 0, "prob": 1.0, "peratom": 1, "permol": 0}]
+    return build_fixture(name, "ewaldlj", cfg, [L, L, L], parts, groups, seed, ncalls, moves)
+
+
 def lj_mix(name, n, L, seed, ncalls, cutoff=None):
     rng = np.random.default_rng(seed)
     half = n // 2
@@ -425,6 +453,8 @@ ONLY = {
     "ewald_npt_surface": lambda: ewald_npt("ewald_npt_surface", 30, 18.0, 24, 24, 3,
                                            pressure_mM=6000.0, dp=0.3, eps_surf=1.0,
                                            update_frequency=3, kc=3.0),
+    # a neutral atom type under Ewald: NaN energies, every trial accepted
+    "ewald_neutral": lambda: ewald_neutral("ewald_neutral", 30, 4, 18.0, 26, 3),
     # one Propagator picking AtomicTranslation or Isobaric per call
     "ewald_npt_mixed": lambda: ewald_npt("ewald_npt_mixed", 40, 20.0, 25, 40, 0,
                                          pressure_mM=6000.0, dp=0.4, eps_surf=0.5,

@@ -6,6 +6,8 @@
 * a group list holding an empty group (front = back + 1, as Group leaves
   one without particles): energies and a Propagator run on the other group;
 * zero Propagator calls: nothing changes;
+* a neutral atom type under Ewald: NaN energy changes, all accepted, as the
+  compiled reference does (fixture ewald_neutral);
 * arguments the library refuses (indices out of range, a box of zero length,
   a Propagator without moves) raise errors instead of reaching the device.
 """
@@ -160,3 +162,27 @@ def test_refused_arguments(gpu):
             s.propagate(1)
     finally:
         s.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_neutral_type_under_ewald(gpu, prec):
+    """A particle of a q = 0 atom type under Ewald: the effective-charge ratio
+    0/0 = NaN (ewald.h:535-545) makes S(k) and every energy change NaN, and
+    Movebase::metropolis accepts NaN (move.h:567-573).  Fixture ewald_neutral
+    is the compiled reference's run (102 trials, all NaN, all accepted)."""
+    from fixture_systems import fgx_system_from_fixture
+    from oracle_py import farr, load_fixture
+    fx = load_fixture("ewald_neutral")
+    mc = fx["mc"]
+    say("[edges] neutral type under Ewald, %s" % prec)
+    s = fgx_system_from_fixture(fx, precision=gpu.FP64 if prec == "fp64" else gpu.MIXED)
+    try:
+        s.rng_seed(fx["rng_seed"])
+        s.propagator_init()
+        st, du, acc = s.propagate(mc["ncalls"], record=100000)
+        x = s.positions().reshape(-1)
+    finally:
+        s.close()
+    assert len(du) == len(mc["du"]) and np.all(np.isnan(farr(mc["du"])))
+    assert np.all(np.isnan(du)) and np.all(acc == 1) and st["accepted"] == len(du)
+    assert np.allclose(x, farr(mc["final_xyz"]), rtol=0, atol=1e-9)
