@@ -97,6 +97,7 @@ struct fgx_system {
     double *d_type_dp = nullptr;
     unsigned char *d_mask = nullptr;       // 2 * ng keep masks (new, old)
     double *d_partial = nullptr;           // reduction scratch
+    double *d_stage = nullptr;             // 3n: coordinates as xyz triples (host transfers)
     int partial_cap = 0;
     double *d_out = nullptr;               // 8 result slots
     double *h_out = nullptr;               // pinned mirror

@@ -82,6 +82,7 @@ size_t estimate_bytes(const JobSpec &j)
     b += (size_t)PARTNER_PAD * (16 + 4 + 4);
     b += (size_t)j.ng * (16 + 48 + 4) + 4 * (size_t)j.h.ntypes * j.h.ntypes * 12;
     b += 2 * 625 * 4 + 64 * 1024;   // RNG, reduction scratch and small buffers
+    b += n * 3 * 8;                  // staging of coordinate transfers (final positions)
     if (j.h.elec == FGX_ELEC_EWALD) {
         const double kc = j.h.ewald_kc;
         const size_t K = (size_t)(2.5 * kc * kc * kc + 64);

@@ -213,6 +213,21 @@ __global__ void k_copy_moved(Vec64 dst, Vec64 src, const int *idx, int nm, int f
     if (update_fixed) pk[m] = pack_particle(src.x[m], src.y[m], src.z[m], qf[m], b);
 }
 
+// coordinates between the host's xyz triples and the device's per-axis arrays
+__global__ void k_to_triples(Vec64 v, int n, double *out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[3 * i] = v.x[i]; out[3 * i + 1] = v.y[i]; out[3 * i + 2] = v.z[i];
+}
+
+__global__ void k_from_triples(Vec64 v, int first, int count, const double *in)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    v.x[first + i] = in[3 * i]; v.y[first + i] = in[3 * i + 1]; v.z[first + i] = in[3 * i + 2];
+}
+
 __global__ void k_fixed_all(const double *x, const double *y, const double *z, const float *qf,
                             int n, Box b, int4 *pk)
 {
@@ -925,7 +940,7 @@ extern "C" int fgx_destroy(fgx_system *s)
                     s->ew.kv, s->ew.aks, s->ew.kn, s->ew.Q, s->ew.Qt, s->ew.eff,
                     s->ew.cnt_dev, s->d_emat, s->ew.cheb, s->d_gvec, s->d_batch,
                     s->d_bstat_i, s->d_bstat_d, s->d_resume, s->d_cell_cnt, s->d_cell_rec,
-                    s->d_cell_idx, s->d_cell_slot};
+                    s->d_cell_idx, s->d_cell_slot, s->d_stage};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (s->h_out) (void)hipHostFree(s->h_out);
@@ -936,21 +951,31 @@ extern "C" int fgx_destroy(fgx_system *s)
 
 extern "C" size_t fgx_device_bytes(const fgx_system *s) { return s ? s->device_bytes : 0; }
 
+// staging for host transfers of coordinates, 3n doubles, kept until destroy
+static int ensure_stage(fgx_system *s)
+{
+    if (s->d_stage || s->n == 0) return FGX_OK;
+    FGX_HIP(hipMalloc(&s->d_stage, sizeof(double) * 3 * (size_t)s->n));
+    s->device_bytes += sizeof(double) * 3 * (size_t)s->n;
+    return FGX_OK;
+}
+
 extern "C" int fgx_set_positions(fgx_system *s, int which, int first, int count, const double *xyz)
 {
     FGX_CHECK(s && xyz && which >= 0 && which <= 2 && first >= 0 && count >= 0 &&
                   first + count <= s->n,
               FGX_EARG, "fgx_set_positions: bad arguments");
     if (which != 1) s->emat_valid = s->cell_valid = false;
-    std::vector<double> t(count);
-    for (int w = 0; w < 2; ++w) {
-        if (which != 2 && which != w) continue;
-        for (int k = 0; k < 3; ++k) {
-            for (int i = 0; i < count; ++i) t[i] = xyz[3 * i + k];
-            double *d = k == 0 ? s->v[w].x : k == 1 ? s->v[w].y : s->v[w].z;
-            int rc = upload(s, d + first, t.data(), count);
-            if (rc) return rc;
-            FGX_HIP(hipStreamSynchronize(s->stream));
+    int rc;
+    if (count > 0 && (rc = ensure_stage(s))) return rc;
+    if (count > 0) {   // one copy of the triples, then a scatter per vector
+        FGX_HIP(hipMemcpyAsync(s->d_stage, xyz, sizeof(double) * 3 * (size_t)count,
+                               hipMemcpyHostToDevice, s->stream));
+        for (int w = 0; w < 2; ++w) {
+            if (which != 2 && which != w) continue;
+            hipLaunchKernelGGL(k_from_triples, dim3((count + 255) / 256), dim3(256), 0, s->stream,
+                               s->v[w], first, count, (const double *)s->d_stage);
+            FGX_HIP(hipGetLastError());
         }
     }
     if (which != 1 && count > 0) {
@@ -965,14 +990,15 @@ extern "C" int fgx_set_positions(fgx_system *s, int which, int first, int count,
 extern "C" int fgx_get_positions(fgx_system *s, int which, double *xyz)
 {
     FGX_CHECK(s && xyz && (which == 0 || which == 1), FGX_EARG, "fgx_get_positions: bad arguments");
-    std::vector<double> t(s->n);
-    for (int k = 0; k < 3; ++k) {
-        const double *d = k == 0 ? s->v[which].x : k == 1 ? s->v[which].y : s->v[which].z;
-        FGX_HIP(hipMemcpyAsync(t.data(), d, sizeof(double) * s->n, hipMemcpyDeviceToHost,
-                               s->stream));
-        FGX_HIP(hipStreamSynchronize(s->stream));
-        for (int i = 0; i < s->n; ++i) xyz[3 * i + k] = t[i];
-    }
+    if (s->n == 0) return FGX_OK;
+    int rc = ensure_stage(s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_to_triples, dim3((s->n + 255) / 256), dim3(256), 0, s->stream,
+                       s->v[which], s->n, s->d_stage);
+    FGX_HIP(hipGetLastError());
+    FGX_HIP(hipMemcpyAsync(xyz, s->d_stage, sizeof(double) * 3 * (size_t)s->n,
+                           hipMemcpyDeviceToHost, s->stream));
+    FGX_HIP(hipStreamSynchronize(s->stream));
     return FGX_OK;
 }
 

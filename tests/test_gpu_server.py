@@ -68,7 +68,8 @@ def test_alone_equals_in_mix(gpu):
     assert ra["trials"] == rm["trials"] == 6 * 300
     assert ra["accepted"] == rm["accepted"]
     assert np.array_equal(ra["xyz"], rm["xyz"])
-    assert abs(ra["utot"] - rm["utot"]) <= 1e-12
+    # the sum over a job's launches groups differently with the tick boundaries
+    assert abs(ra["utot"] - rm["utot"]) <= 1e-12 * max(1.0, abs(ra["utot"]))
 
 
 def test_cell_list_job_alone_equals_in_mix(gpu):
@@ -98,7 +99,8 @@ def test_cell_list_job_alone_equals_in_mix(gpu):
     assert ra["trials"] == rm["trials"] == 3 * 2000
     assert ra["accepted"] == rm["accepted"]
     assert np.array_equal(ra["xyz"], rm["xyz"])
-    assert abs(ra["utot"] - rm["utot"]) <= 1e-12
+    # the sum over a job's launches groups differently with the tick boundaries
+    assert abs(ra["utot"] - rm["utot"]) <= 1e-12 * max(1.0, abs(ra["utot"]))
 
 
 def test_budget_admission(gpu):
