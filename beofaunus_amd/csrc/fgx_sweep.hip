@@ -349,6 +349,10 @@ __device__ __forceinline__ double pair_loop_mixed(const ChainDev &C, const float
     auto eval = [&](auto mode, const PartnerBatch<U, SR == SR_HS, NEED_T> &X, int base) {
         constexpr bool FAST = decltype(mode)::value >= 1;
         constexpr bool SCREEN = decltype(mode)::value == 2;
+        // FAST Ewald: beyond rc only the LJ tail remains (|u| < 1e-2 kT per
+        // pair), so a batch's U differences are summed in FP32 and widened once
+        constexpr bool BATCH32 = FX && FAST && !SCREEN && ELEC == ELEC_EWALD;
+        float bsum = 0.f;
         unsigned dm = 0;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -398,8 +402,12 @@ __device__ __forceinline__ double pair_loop_mixed(const ChainDev &C, const float
                 }
                 const bool take = ok && !bad;
                 dm |= ((ok && bad) ? 1u : 0u) << u;
-                // select on the FP32 bits, then widen: one select, not two
-                acc += (double)__int_as_float(__float_as_int(d) & (take ? -1 : 0));
+                if constexpr (BATCH32) {
+                    bsum += take ? d : 0.f;
+                } else {
+                    // select on the FP32 bits, then widen: one select, not two
+                    acc += (double)__int_as_float(__float_as_int(d) & (take ? -1 : 0));
+                }
                 continue;
             }
             float dx = (float)(int32_t)(nx - P.x) * sx, dy = (float)(int32_t)(ny - P.y) * sy,
@@ -426,6 +434,7 @@ __device__ __forceinline__ double pair_loop_mixed(const ChainDev &C, const float
             dm |= ((ok && bad) ? 1u : 0u) << u;
             acc += (double)((ok && !bad) ? un - uo : 0.f);
         }
+        if constexpr (BATCH32) acc += (double)bsum;
         if (__ballot(dm != 0)) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
