@@ -46,7 +46,7 @@ BYTES_K_READ, BYTES_K_ACCEPT = 24, 32
 # counters' reading for a 1 GiB copy (FETCH x2.0, WRITE x1.0), in bytes:
 # Ewald electrolyte (batch depth 4): 1,025,185,864 KiB fetched, 432,348 KiB
 # written; cut-LJ cell-list form: 64,251,548 KiB fetched, 564,152 KiB written.
-MEASURED_TRAFFIC = {("ewald", 20000, 256, "mixed"): 1025185864 * 2 * 1024 + 432348 * 1024,
+MEASURED_TRAFFIC = {("ewald", 20000, 256, "mixed"): 1024940990 * 2 * 1024 + 432348 * 1024,
                     ("ewald-cells", 20000, 256, "mixed"): 64251548 * 2 * 1024 + 564152 * 1024}
 
 
