@@ -1258,6 +1258,22 @@ extern "C" int fgx_g_internal(fgx_system *s, int which, int g, double *u)
 // Cuboid::_getVolume (src/geometry.cpp:293-300)
 static double box_volume(const Box &b) { return b.len[0] * b.len[1] * b.len[2]; }
 
+// NonbondedEwald::i_external (ewald.h:707-710): the self energy of particle i
+// (Energybase's default 0 without Ewald, energy.h:128-129; ExternalPressure
+// has no per-particle term)
+extern "C" int fgx_i_external(fgx_system *s, int which, int i, double *u)
+{
+    FGX_CHECK(check_which(s, which) && u && i >= 0 && i < s->n, FGX_EARG,
+              "fgx_i_external: bad arguments");
+    double e = 0.0;
+    if (s->has_ewald) {
+        int rc = ewald_self(s, which, i, i, &e);
+        if (rc) return rc;
+    }
+    *u = e;
+    return FGX_OK;
+}
+
 // NonbondedEwald::g_external (ewald.h:712-714), then ExternalPressure::
 // g_external = -N ln V for the group's N molecules (energy.h:1885-1893)
 extern "C" int fgx_g_external(fgx_system *s, int which, int g, double *u)

@@ -131,6 +131,7 @@ def lib():
         "fgx_isobaric_propagate": (I, [P, D, D, I, C.POINTER(Stats), _dp, _ip, C.c_long]),
         "fgx_propagate_npt": (I, [P, D, D, I, I, C.POINTER(Stats), _dp, _ip, C.c_long]),
         "fgx_set_space": (I, [P, _dp]),
+        "fgx_i_external": (I, [P, I, I, _dp]),
         "fgx_config_energy": (I, [P, I, _dp]),
         "fgx_commit_all": (I, [P, I, _dp]),
         "fgx_get_box": (I, [P, _dp]),
@@ -276,6 +277,7 @@ class System:
     def g2g(self, a, b, which=0): return self._e("fgx_g2g", which, a, b)
     def g_internal(self, g, which=0): return self._e("fgx_g_internal", which, g)
     def g_external(self, g, which=0): return self._e("fgx_g_external", which, g)
+    def i_external(self, i, which=0): return self._e("fgx_i_external", which, i)
     def external(self, which=0): return self._e("fgx_external", which)
     def system_energy(self, which=0): return self._e("fgx_system_energy", which)
 

@@ -260,6 +260,21 @@ public:
     {
         double u; upload(p); AMD::check(fgx_external(h, which(p), &u)); return u;
     }
+    double i_external(const Tpvec &p, int i) override
+    {
+        double u; upload(p); AMD::check(fgx_i_external(h, which(p), i, &u)); return u;
+    }
+    /* Nonbonded::g1g2 (energy.h:921-929): every pair of g1 x g2, which is g2g
+     * for disjoint groups of one vector; the engine holds no pair kernel
+     * across the accepted and trial vectors */
+    double g1g2(const Tpvec &p1, Group &g1, const Tpvec &p2, Group &g2) override
+    {
+        if (&p1 != &p2)
+            throw std::runtime_error("libfgx: g1g2 across two particle vectors");
+        if (!g1.empty() && !g2.empty() && g1.front() <= g2.back() && g2.front() <= g1.back())
+            throw std::runtime_error("libfgx: g1g2 of overlapping groups");
+        return g2g(p1, g1, g2);
+    }
     double systemEnergy(const Tpvec &p) override
     {
         double u; upload(p); AMD::check(fgx_system_energy(h, which(p), &u)); return u;

@@ -118,6 +118,7 @@ int fgx_i2all(fgx_system *s, int which, int i, double *u);             /* energy
 int fgx_g2g(fgx_system *s, int which, int g1, int g2, double *u);      /* energy.h:879 */
 int fgx_g_internal(fgx_system *s, int which, int g, double *u);        /* energy.h:930 */
 int fgx_g_external(fgx_system *s, int which, int g, double *u);        /* ewald.h:712 */
+int fgx_i_external(fgx_system *s, int which, int i, double *u);        /* ewald.h:707 */
 int fgx_external(fgx_system *s, int which, double *u);                 /* ewald.h:716 */
 /* Energy::systemEnergy (energy.h:3486-3491 -> 169-180) */
 int fgx_system_energy(fgx_system *s, int which, double *u);

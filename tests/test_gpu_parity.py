@@ -20,7 +20,7 @@ import sys
 import numpy as np
 import pytest
 
-from oracle_py import GOLDEN, OracleSystem, farr, fnum, lib as olib, load_fixture
+from oracle_py import GOLDEN, KINDS, OracleSystem, farr, fnum, lib as olib, load_fixture
 
 pytestmark = pytest.mark.gpu
 
@@ -224,3 +224,26 @@ def test_mixed_precision_error(gpu, name):
     assert len(err) > 0
     assert err.max() <= 1e-5
     s.close()
+
+
+@pytest.mark.parametrize("name", ["ewald_tinfoil", "ewald_surface", "salt_dhhs"])
+def test_i_external(gpu, name):
+    """NonbondedEwald::i_external (ewald.h:707-710): each particle's self
+    energy; their sum is the whole group's g_external, and the oracle's
+    self-energy sum (getSelfEnergy over all particles).  Without Ewald: 0."""
+    from fixture_systems import fgx_system_from_fixture
+    fx = load_fixture(name)
+    s = fgx_system_from_fixture(fx, moves=False)
+    o = OracleSystem.from_fixture(fx)
+    try:
+        n = len(fx["particles"]["charge"])
+        ui = np.array([s.i_external(i) for i in range(n)])
+        if KINDS[fx["kind"]][0] != "ewald":
+            assert np.all(ui == 0.0)
+            return
+        assert np.all(ui < 0.0)
+        total = olib().or_ewald_self(o.h, 0)
+        assert ui.sum() == pytest.approx(total, rel=1e-12)
+        assert ui.sum() == pytest.approx(s.g_external(0), rel=1e-12)
+    finally:
+        s.close()
