@@ -134,3 +134,40 @@ def test_reader_refuses_bad_state():
             f.write("10 10 10\n3\n0 0 0 1 1 1 1 0\n")
         with pytest.raises(ValueError):
             faunus_io.read_state(s)
+
+
+def test_moves_outside_the_path():
+    """A move the device Propagator does not run (here atomrotate) takes part
+    in Propagator's pick, so the system is built for energies only and
+    run_input refuses to run it; moltransrot needs center_rotation, as
+    TranslateRotate's constructor does (move.h:1589)."""
+    import json
+    from beofaunus_amd import faunus_io, fgx
+    fx = load_fixture("salt_dhhs")
+    with tempfile.TemporaryDirectory() as d:
+        c, s = write_inputs(fx, d)
+        cfg = dict(fx["config"])
+        cfg["moves"] = dict(cfg["moves"], atomrotate={"salt": {"peratom": True}})
+        with open(c, "w") as f:
+            json.dump(cfg, f)
+        w = faunus_io.read_input(c, s, **kind_args(fx["kind"]))
+        assert w["move"] is None and w["other_moves"] == ["atomrotate"]
+        with pytest.raises(ValueError):
+            faunus_io.run_input(c, s, os.path.join(d, "out"), 1, **kind_args(fx["kind"]))
+        cfg["moves"] = {"moltransrot": {"salt": {"dp": 1.0, "dprot": 0.5}}}
+        with open(c, "w") as f:
+            json.dump(cfg, f)
+        with pytest.raises(ValueError):
+            faunus_io.read_input(c, s, **kind_args(fx["kind"]))
+
+
+def test_isobaric_position_in_the_propagator():
+    """The Propagator lists its moves in the input's (sorted) key order:
+    isobaric comes after atomtranslate and before moltransrot."""
+    from beofaunus_amd import faunus_io
+    fx = load_fixture("ewald_npt_mixed")
+    with tempfile.TemporaryDirectory() as d:
+        c, s = write_inputs(fx, d)
+        w = faunus_io.read_input(c, s, **kind_args(fx["kind"]))
+    assert w["npt_index"] == 1 and w["move"] is not None
+    assert [m["type"] for m in w["moves"]] == ["AT", "NPT"]
