@@ -127,3 +127,31 @@ def test_mixed_propagator_matches_reference(gpu):
     assert " ".join(str(w) for w in words) + " %d" % pos == mc["final_rng"].strip()
     assert st["utot"] == pytest.approx(mc["utot"], rel=1e-9, abs=1e-9)
     assert u1 == pytest.approx(mc["ufinal"], rel=1e-10)
+
+
+def test_mixed_propagator_mixed_precision(gpu):
+    """The same Propagator (fixture ewald_npt_mixed) with the mixed path for
+    the translation calls: |dU - dU_ref| <= 1e-5 kT while the accept
+    sequences agree (they agree throughout in practice)."""
+    from fixture_systems import fgx_system_from_fixture
+    fx = load_fixture("ewald_npt_mixed")
+    mc = fx["mc"]
+    mv = [m for m in fx["moves"] if m["type"] == "NPT"][0]
+    keys = sorted(k for k in fx["config"]["moves"])
+    s = fgx_system_from_fixture(fx, precision=gpu.MIXED)
+    try:
+        s.rng_seed(fx["rng_seed"])
+        s.propagator_init()
+        st, du, acc = s.propagate_npt(mv["dp"], mv["prob"], keys.index("isobaric"),
+                                      mc["ncalls"], record=100000)
+    finally:
+        s.close()
+    ref_du = farr(mc["du"])
+    ref_acc = np.array(mc["acc"], dtype=np.int32)
+    mism = np.nonzero(acc[:len(ref_acc)] != ref_acc[:len(acc)])[0]
+    upto = int(mism[0]) if len(mism) else min(len(acc), len(ref_acc))
+    err = np.abs(du[:upto] - ref_du[:upto])
+    say("[isobaric] mixed precision: %d trials compared, max |dU err| %.2e kT, first mismatch %s"
+        % (upto, err.max(), mism[0] if len(mism) else None))
+    assert upto >= len(ref_acc) // 2
+    assert err.max() <= 1e-5
