@@ -1047,6 +1047,9 @@ extern "C" int fgx_set_group_cm(fgx_system *s, int g, const double cm[3], const 
 {
     FGX_CHECK(s && g >= 0 && g < s->ng, FGX_EARG, "fgx_set_group_cm: bad group");
     if (cm) s->emat_valid = false;
+    // the device sweeps move mass centres on the device: start from those, so
+    // the value not given here is kept as it is
+    if (int rc = sync_cm_from_device(s)) return rc;
     for (int k = 0; k < 3; ++k) {
         if (cm) s->cm[3 * g + k] = cm[k];
         if (cm_trial) s->cm_trial[3 * g + k] = cm_trial[k];
@@ -1810,7 +1813,8 @@ static int propagate_with_isobaric(fgx_system *s, double dp, double prob, int np
         // collision the trial geometry stays set until _rejectMove, which
         // restores it; nothing is computed with it in between.)
         if ((rc = fgx_set_positions(s, 1, 0, n, t.data()))) return rc;
-        for (int g = 0; g < ng && !rc; ++g) rc = fgx_set_group_cm(s, g, nullptr, &cmt[3 * g]);
+        // Group::setMassCenter set cm to the recomputed mass centre, scale set cm_trial
+        for (int g = 0; g < ng && !rc; ++g) rc = fgx_set_group_cm(s, g, &cm[3 * g], &cmt[3 * g]);
         if (rc) return rc;
         double du;
         bool coll = false;
@@ -1846,10 +1850,12 @@ static int propagate_with_isobaric(fgx_system *s, double dp, double prob, int np
             ucall += du;
             ucall += drift;
             accepted++;
-        } else {     // Isobaric::_rejectMove (move.h:5582-5592), then update(false)
+        } else {     // Isobaric::_rejectMove (move.h:5582-5592), then update(false);
+                     // _trialMove's setMassCenter left cm = cm_trial = the mass
+                     // centre of the accepted coordinates (group.h:174-179)
             if (s->has_ewald && !(rc = ewald_rebox(s, oldlen))) rc = ewald_set_space(s);
             if (!rc) rc = fgx_set_positions(s, 1, 0, n, p.data());
-            for (int g = 0; g < ng && !rc; ++g) rc = fgx_set_group_cm(s, g, nullptr, &cm[3 * g]);
+            for (int g = 0; g < ng && !rc; ++g) rc = fgx_set_group_cm(s, g, &cm[3 * g], &cm[3 * g]);
             if (du == HUGE_VAL) inf++;
         }
         if (trials < cap) {

@@ -290,7 +290,7 @@ def ewald(name, n, L, seed, ncalls, eps_surf, update_frequency, alpha=0.35, rc=7
     return build_fixture(name, "ewaldlj", cfg, [L, L, L], parts, groups, seed, ncalls, moves)
 
 
-def molecules(name, kind, nmol, nsite, nion, L, seed, ncalls, center_rotation=True,
+def molecules(name, kind, nmol, nsite, nion, L, seed, ncalls, center_rotation=True, isobaric=None,
               cutoff_g2g=None, R=9.0):
     """Rigid molecules (sites packed in a sphere of radius R, charges from
     {-1,0,+1} with p = 0.2/0.6/0.2, radii U[1.5,3]) plus monovalent ions."""
@@ -354,7 +354,16 @@ def molecules(name, kind, nmol, nsite, nion, L, seed, ncalls, center_rotation=Tr
              "mw": np.ones(len(xyz)), "id": np.array(ids)}
     moves = [{"type": "TR", "molid": 0, "prob": 1.0, "peratom": 0, "permol": 1,
               "dp": 5.0, "dprot": 0.5, "center_rotation": int(center_rotation)}]
-    return build_fixture(name, kind, cfg, [L, L, L], parts, groups, seed, ncalls, moves)
+    if isobaric is None:
+        return build_fixture(name, kind, cfg, [L, L, L], parts, groups, seed, ncalls, moves)
+    # one Propagator holding isobaric and moltransrot (sorted key order):
+    # volume moves scale the molecules' mass centres and keep them rigid
+    cfg["moves"]["isobaric"] = {"dp": isobaric["dp"], "pressure": isobaric["pressure"],
+                                "prob": 1.0}
+    moves = [{"type": "NPT", "molid": -1, "prob": 1.0, "peratom": 0, "permol": 0,
+              "dp": isobaric["dp"], "pressure_mM": isobaric["pressure"]}] + moves
+    return build_fixture(name, kind, cfg, [L, L, L], parts, groups, seed, ncalls, moves,
+                         mc_mode="mcprop")
 
 
 def reference_example(name, kind, example):
@@ -455,6 +464,9 @@ ONLY = {
                                            update_frequency=3, kc=3.0),
     # a neutral atom type under Ewald: NaN energies, every trial accepted
     "ewald_neutral": lambda: ewald_neutral("ewald_neutral", 30, 4, 18.0, 26, 3),
+    # rigid molecules and ions: one Propagator picking TranslateRotate or Isobaric
+    "mol_npt_mixed": lambda: molecules("mol_npt_mixed", "dhhs_npt", 4, 12, 16, 60.0, 27, 30,
+                                       isobaric={"dp": 0.3, "pressure": 150.0}),
     # one Propagator picking AtomicTranslation or Isobaric per call
     "ewald_npt_mixed": lambda: ewald_npt("ewald_npt_mixed", 40, 20.0, 25, 40, 0,
                                          pressure_mM=6000.0, dp=0.4, eps_surf=0.5,

@@ -89,18 +89,21 @@ def test_isobaric_refuses_missing_pressure(gpu):
             s.close()
 
 
-def test_mixed_propagator_matches_reference(gpu):
-    """One Propagator holding AtomicTranslation and Isobaric (fixture
-    ewald_npt_mixed, Ewald salt): each call picks one of the two as the
-    reference's Propagator::move does; volume trials run host-driven,
-    translation calls on the device sweep.  Same trials, accept sequence and
-    final state as the compiled reference."""
+@pytest.mark.parametrize("name", ["ewald_npt_mixed", "mol_npt_mixed"])
+def test_mixed_propagator_matches_reference(gpu, name):
+    """One Propagator holding a translation move and Isobaric: AtomicTranslation
+    on an Ewald salt (ewald_npt_mixed), or TranslateRotate of rigid molecules
+    beside ions, Debye-Hueckel (mol_npt_mixed, whose volume moves scale the
+    molecules' mass centres).  Each call picks one move as the reference's
+    Propagator::move does; volume trials run host-driven, the other calls on
+    the device sweep.  Same trials, accept sequence and final state as the
+    compiled reference."""
     from fixture_systems import fgx_system_from_fixture
-    fx = load_fixture("ewald_npt_mixed")
+    fx = load_fixture(name)
     mc = fx["mc"]
     mv = [m for m in fx["moves"] if m["type"] == "NPT"][0]
     keys = sorted(k for k in fx["config"]["moves"])
-    say("[isobaric] ewald_npt_mixed: %d Propagator calls over %s" % (mc["ncalls"], keys))
+    say("[isobaric] %s: %d Propagator calls over %s" % (name, mc["ncalls"], keys))
     s = fgx_system_from_fixture(fx)
     try:
         s.rng_seed(fx["rng_seed"])
