@@ -108,7 +108,8 @@ int fgx_set_positions(fgx_system *s, int which, int first, int count, const doub
 int fgx_get_positions(fgx_system *s, int which, double *xyz);
 int fgx_get_group_cm(fgx_system *s, int g, double cm[3], double cm_trial[3]);
 /* Group::cm / Group::cm_trial (group.h:55-56), e.g. after a host-side
- * Group::translate / newrotate of the trial vector */
+ * Group::translate / newrotate of the trial vector.  A null pointer keeps
+ * that centre at its current value (as the device sweeps left it). */
 int fgx_set_group_cm(fgx_system *s, int g, const double cm[3], const double cm_trial[3]);
 
 /* Energy::Energybase entry points (energy.h:54-224) */
