@@ -106,3 +106,26 @@ def test_every_schedule_small_system(gpu, threads, unroll):
     assert np.array_equal(fin, np.isfinite(dmx[:upto]))
     assert upto >= 200
     assert np.abs(dmx[:upto][fin] - d64[:upto][fin]).max() <= TOL_DU
+
+
+def test_full_size_headline_energy_bookkeeping(gpu):
+    """The Ewald electrolyte at its full size (N = 20,000, uncut LJ as the
+    reference evaluates it, mixed path): over one sweep, the accepted dU plus
+    the Ewald drift terms equal the change of the FP64 system energy."""
+    from beofaunus_amd import workloads
+    w = workloads.ewald_electrolyte(n=20000, L=120.0, seed=61)
+    print("[mixed] N=20000 headline Hamiltonian, one sweep", flush=True)
+    s = workloads.make_system(w, precision=gpu.MIXED)
+    try:
+        s.rng_seed(29)
+        s.propagator_init()
+        u0 = s.system_energy()
+        st = s.propagate(1)[0]
+        u1 = s.system_energy()
+    finally:
+        s.close()
+    drift = (u1 - u0) - st["utot"]
+    print("  trials %d, accepted %d, U0 %.6f, U1 %.6f, sum dU %.6f, drift %.2e"
+          % (st["trials"], st["accepted"], u0, u1, st["utot"], drift), flush=True)
+    assert st["trials"] == 20000 and st["accepted"] > 10000
+    assert abs(drift) <= 1e-6 * max(1.0, abs(u0))
