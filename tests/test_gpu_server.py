@@ -103,6 +103,33 @@ def test_cell_list_job_alone_equals_in_mix(gpu):
     assert abs(ra["utot"] - rm["utot"]) <= 1e-12 * max(1.0, abs(ra["utot"]))
 
 
+def test_rigid_molecule_job_alone_equals_in_mix(gpu):
+    """A TranslateRotate job (rigid molecules beside ions, mixed path with the
+    group-pair energy cache) gives the same result alone and beside an
+    AtomicTranslation job, through per-tick trial budgets."""
+    from beofaunus_amd import workloads
+    say("[server] TranslateRotate job alone vs in mix")
+    a = workloads.rigid_molecules(nmol=6, nsite=40, nion=300, L=90.0, seed=33)
+    b = workloads.salt_dh(n=800, L=60.0, seed=34, lattice=True)
+    alone = gpu.Server(1 << 30, tick_seconds=0.5)
+    ja = alone.submit(a, seed=11, sweeps=20, keep_positions=True)
+    ra = drain(alone)[ja]
+    alone.close()
+    mix = gpu.Server(1 << 30, tick_seconds=0.002)
+    jb = mix.submit(b, seed=12, sweeps=3, keep_positions=True)
+    ja2 = mix.submit(a, seed=11, sweeps=20, keep_positions=True)
+    out = drain(mix)
+    mix.close()
+    rm = out[ja2]
+    say("  alone: trials %d utot %.12f; in mix: trials %d utot %.12f"
+        % (ra["trials"], ra["utot"], rm["trials"], rm["utot"]))
+    assert ra["status"] == 0 and rm["status"] == 0 and out[jb]["status"] == 0
+    assert ra["trials"] == rm["trials"] > 0
+    assert ra["accepted"] == rm["accepted"]
+    assert np.array_equal(ra["xyz"], rm["xyz"])
+    assert abs(ra["utot"] - rm["utot"]) <= 1e-12 * max(1.0, abs(ra["utot"]))
+
+
 def test_budget_admission(gpu):
     """A budget that holds about one job at a time: every job completes, the
     peak resident footprint stays within the budget, and later jobs wait."""
