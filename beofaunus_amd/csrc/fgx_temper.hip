@@ -74,6 +74,23 @@ extern "C" int fgx_temper_create(fgx_system **replicas, int R, uint32_t ctl_seed
         // the reference aborts when partners differ in size (move.h:6686-6687)
         FGX_CHECK(replicas[r]->n == replicas[0]->n, FGX_EARG,
                   "fgx_temper_create: replicas must have the same particle count");
+        // an exchange evaluates each configuration with the partner's Bjerrum
+        // length only; replicas whose Hamiltonians differ otherwise (e.g. LJ
+        // eps in kT converted at their own temperatures, species.h:293) are
+        // outside that
+        const Pot &a = replicas[r]->pot, &b = replicas[0]->pot;
+        FGX_CHECK(a.elec == b.elec && a.sr == b.sr && a.nbkind == b.nbkind &&
+                      a.ntypes == b.ntypes && a.kappa == b.kappa &&
+                      replicas[r]->has_ewald == replicas[0]->has_ewald,
+                  FGX_EUNSUPPORTED, "fgx_temper_create: replicas differ in more than lB");
+        if (a.sr == SR_LJ) {
+            const size_t T2 = (size_t)a.ntypes * a.ntypes;
+            bool same = true;
+            for (size_t k = 0; k < 2 * T2; ++k)   // sigma^2 and 4 eps
+                same = same && replicas[r]->tab[k] == replicas[0]->tab[k];
+            FGX_CHECK(same, FGX_EUNSUPPORTED,
+                      "fgx_temper_create: replicas with different LJ tables (only lB is swapped)");
+        }
     }
     fgx_temper *t = new fgx_temper();
     t->cfg.assign(replicas, replicas + R);

@@ -87,3 +87,67 @@ def test_batched_ladders_drive():
         t.close()
     for s in systems:
         s.close()
+
+
+def test_temper_ewald_matches_oracle():
+    """The same exchanges for replicas of an Ewald LJ + Coulomb electrolyte
+    whose Hamiltonians differ only in lB (LJ eps given in kT): the oracle's
+    U_r(x_c) includes the reciprocal, self and real-space terms at lB_r."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from beofaunus_amd import fgx, workloads
+    print("[temper] R=4 Ewald replicas", flush=True)
+    R, nsteps = 4, 16
+    T = 280.0 * 1.5 ** (np.arange(R) / (R - 1))
+    lBs = 7.1 * 298.15 / T
+    ws = [workloads.ewald_electrolyte(n=200, L=26.0, seed=200 + r, lB=lBs[r], kc=5.0,
+                                      precision=fgx.FP64) for r in range(R)]
+    gpu = [workloads.make_system(w) for w in ws]
+    t = fgx.Temper(gpu, ctl_seed=31, glob_seed=32)
+    g_acc, g_du = [], []
+    for _ in range(nsteps):
+        a, d = t.exchange()
+        g_acc.append(a.copy())
+        g_du.append(d.copy())
+    g_slots = t.slots()
+    U = np.zeros((R, R))
+    for r in range(R):
+        for c in range(R):
+            w = ws[c]
+            h = w["ham"]
+            o = OracleSystem(w["ref_kind"], w["box"], w["xyz"], w["charge"], w["radius"],
+                             w["mw"], w["types"], w["groups"], lBs[r],
+                             type_sigma=h["type_sigma"], type_eps=h["type_eps"],
+                             type_charge=h["type_charge"], type_radius=h["type_radius"],
+                             type_dp=h["type_dp"], ewald=h["ewald"])
+            U[r, c] = o.system_energy()
+    slot = np.arange(R, dtype=np.int32)
+    acc = np.zeros(R * nsteps, dtype=np.int32)
+    du = np.zeros(R * nsteps)
+    uu = np.ascontiguousarray(U.reshape(-1))
+    olib().or_pt_exchange_seeded(R, dptr(uu), iptr(slot), 31, 32, nsteps, iptr(acc), dptr(du))
+    acc = acc.reshape(nsteps, R)
+    du = du.reshape(nsteps, R)
+    assert np.array_equal(np.array(g_acc), acc)
+    assert np.allclose(np.array(g_du), du, rtol=1e-9, atol=1e-9)
+    assert np.array_equal(g_slots, slot)
+    t.close()
+    for s in gpu:
+        s.close()
+
+
+def test_temper_refuses_replicas_differing_beyond_lB():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from beofaunus_amd import fgx, workloads
+    ws = [workloads.ewald_electrolyte(n=200, L=26.0, seed=210 + r, eps_kt=0.1 * (1 + r),
+                                      kc=5.0, precision=fgx.FP64) for r in range(2)]
+    gpu = [workloads.make_system(w) for w in ws]
+    try:
+        with pytest.raises(fgx.FgxError):
+            fgx.Temper(gpu)
+    finally:
+        for s in gpu:
+            s.close()
