@@ -170,3 +170,34 @@ def test_cell_list_variants(gpu, case):
     upto = int(mism[0]) if len(mism) else len(a64)
     assert upto >= 500
     assert np.max(np.abs(dc[:upto] - d64[:upto])) <= TOL_MIXED
+
+
+def test_cell_list_across_volume_moves(gpu):
+    """A Propagator holding AtomicTranslation and Isobaric on the cut-LJ
+    electrolyte: after every accepted volume move the cell grid, the records
+    and the fixed-point copies follow the new box.  Cell list and all-pairs
+    screen give the same accepts and dU to rounding."""
+    from beofaunus_amd import workloads, faunus_io
+    w = electrolyte(2000, seed=47)
+    w["ham"] = dict(w["ham"], pressure=faunus_io.mM_to_per_A3(20000.0))
+    say("[cells] N=2000 with volume moves, L=%.2f" % w["box"][0])
+    out = {}
+    for mode in (1, 0):
+        s = workloads.make_system(w, precision=gpu.MIXED)
+        try:
+            s.set_cell_list(mode)
+            s.rng_seed(27)
+            s.propagator_init()
+            st, du, acc = s.propagate_npt(0.05, 1.0, 1, 12, record=100000)
+            out[mode] = (du, acc, s.box()[0], s.cell_list())
+        finally:
+            s.close()
+    (dc, ac, Lc, cells), (dp, ap, Lp, _) = out[1], out[0]
+    nvol = int((np.abs(dc[np.isfinite(dc)]) > 0).sum())
+    say("  cell list %s, trials %d, final L %.4f / %.4f" % (cells, len(dc), Lc, Lp))
+    assert cells[0] >= 3
+    assert np.array_equal(ac, ap) and Lc == Lp
+    fin = np.isfinite(dp)
+    assert np.array_equal(np.isfinite(dc), fin)
+    assert np.max(np.abs(dc[fin] - dp[fin])) <= 1e-9
+    assert len(dc) > 2000   # translation sweeps and volume trials both ran
