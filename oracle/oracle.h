@@ -20,9 +20,10 @@
  *     pinned by golden vectors of the compiled reference (oracle/_ref),
  *     by closed forms (lB known answers, 27-image brute force) and by the
  *     Ewald 4-ion known answer of doi:10.1063/1.481216.
- *   - ParallelTempering exchange: the reference needs MPI, which is absent,
- *     so it is pinned only by invariants (symmetric decision, detailed
- *     balance on a 2-state toy, swap of identical replicas has dU = 0).
+ *   - ParallelTempering exchange: parity unpinned against a reference run.
+ *     The reference needs MPI, which is absent, so it is pinned only by
+ *     invariants (symmetric decision, detailed balance on a 2-state toy,
+ *     swap of identical replicas has dU = 0).
  */
 #ifndef FGX_ORACLE_H
 #define FGX_ORACLE_H
@@ -147,7 +148,7 @@ long or_propagate_npt(or_sys *s, double dp, double prob, int npt_index, int ncal
 void or_get_box(or_sys *s, double *len);
 
 /* ParallelTempering (move.h:6520-6772) restated for R replicas in one
- * process.  u[r][c] is replica r's Hamiltonian evaluated on configuration c;
+ * process (parity unpinned against a reference run; see the header note).  u[r][c] is replica r's Hamiltonian evaluated on configuration c;
  * slot[r] is the configuration currently held by replica r.  One exchange
  * attempt draws the partner direction from `ctl` (MPIController::random)
  * and each pair's acceptance from `glob` (global slump, lock-step).
