@@ -12,7 +12,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "libfgx.so")
-SOURCES = ["fgx_system.hip", "fgx_sweep.hip", "fgx_temper.hip", "fgx_server.hip"]
+SOURCES = ["fgx_sweep_at_ewald_mixed.hip", "fgx_sweep_at_dh.hip", "fgx_sweep_at_plain.hip",
+           "fgx_sweep_at_ewald64.hip", "fgx_sweep_tr.hip", "fgx_sweep.hip", "fgx_system.hip",
+           "fgx_temper.hip", "fgx_server.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
@@ -45,7 +47,7 @@ def build(verbose=False):
             sys.stderr.write(r.stderr)
         return obj
 
-    with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
+    with ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 4)) as ex:
         objs = list(ex.map(one, SOURCES))
     tmp = LIB + ".tmp"
     cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
