@@ -1251,34 +1251,71 @@ long or_last_trials(or_sys *s, int *which_out, long cap)
 }
 
 /* ===================================================================== */
-/* ParallelTempering (move.h:6520-6772) restated for R replicas            */
+/* ParallelTempering (move.h:6520-6772) restated for R ranks in one process */
 /* ===================================================================== */
 
+/* The decision of one exchange attempt, shared by both forms below.
+ * findPartner (move.h:6621-6633): rank r pairs with r + dr (even r) or r - dr
+ * (odd r); a partner outside [0, R) is no partner and _energyChange returns
+ * +inf (move.h:6703-6704).  _energyChange returns (unew - uold) +
+ * du_partner (move.h:6701-6723), i.e. du[r] = du_self[r] + du_self[partner].
+ * metropolis (move.h:567-573) rejects iff u > exp(-du); each rank draws its
+ * own u (uni[r]).  The reference takes the decision on both ranks
+ * separately and relies on their global engines being in lock-step (the
+ * precondition of move.h:557-566), so that both see the same u; here the
+ * pair takes the lower rank's u, which is the same decision under that
+ * precondition.  part[r] gets the partner (-1: none). */
+#define OR_PT_MAXR 256
+
+static void pt_decide(int R, int dr, const double *du_self, const double *uni, int *part,
+                      int *acc, double *du)
+{
+    for (int r = 0; r < R; ++r) {
+        int p = (r % 2 == 0) ? r + dr : r - dr;
+        part[r] = (p >= 0 && p < R && p != r) ? p : -1;
+    }
+    for (int r = 0; r < R; ++r) {
+        acc[r] = 0;
+        if (part[r] < 0) {
+            du[r] = INFINITY;
+            continue;
+        }
+        du[r] = du_self[r] + du_self[part[r]];
+    }
+    for (int r = 0; r < R; ++r) {
+        int p = part[r];
+        if (p < 0 || p < r) continue;
+        int a = !(uni[r] > exp(-du[r]));
+        acc[r] = acc[p] = a;
+    }
+}
+
+/* Energy-matrix form: u[r*R + c] is rank r's Hamiltonian on configuration
+ * c, slot[r] the configuration rank r holds; du_self[r] = U_r(x_partner) -
+ * U_r(x_own).  Every rank's global engine is the same (lock-step), so one
+ * draw of `glob` is every rank's Metropolis uniform.  Returns the accepted
+ * swaps; acc_out/du_out per rank (du = 0 for a rank without partner). */
 int or_pt_exchange(int R, const double *u, int *slot, or_mt *ctl, or_mt *glob, int *acc_out,
                    double *du_out)
 {
-    /* every rank draws the same MPIController::random value (default seed,
-     * mpi.h:48), so the direction is shared: findPartner, move.h:6621-6633 */
+    int part[OR_PT_MAXR];
+    double du_self[OR_PT_MAXR], uni[OR_PT_MAXR];
+    if (R < 1 || R > OR_PT_MAXR) return -2;
+    /* MPIController::random (mpi.h:48): default-seeded on every rank, so
+     * the direction is shared */
     int dr = (or_canonical(ctl) > 0.5) ? 1 : -1;
-    /* every rank then calls metropolis() once (move.h:536-537, 567-573);
-     * with the global engines in lock-step (the precondition stated at
-     * move.h:557-566) all ranks draw the same uniform */
-    double uni = or_canonical(glob);
-    int nacc = 0;
-    for (int r = 0; r < R; ++r) { acc_out[r] = 0; du_out[r] = 0; }
+    double g = or_canonical(glob);
     for (int r = 0; r < R; ++r) {
-        int partner = (r % 2 == 0) ? r + dr : r - dr;
-        if (partner < 0 || partner >= R || partner < r) continue;
-        int a = slot[r], b = slot[partner];
-        /* _energyChange, move.h:6701-6723: (unew-uold) + du_partner */
-        double du_r = u[r * R + b] - u[r * R + a];
-        double du_p = u[partner * R + a] - u[partner * R + b];
-        double du = du_r + du_p;
-        int accept = !(uni > exp(-du));
-        du_out[r] = du; du_out[partner] = du;
-        if (accept) {
-            slot[r] = b; slot[partner] = a;
-            acc_out[r] = acc_out[partner] = 1;
+        int p = (r % 2 == 0) ? r + dr : r - dr;
+        du_self[r] = (p >= 0 && p < R) ? u[r * R + slot[p]] - u[r * R + slot[r]] : 0.0;
+        uni[r] = g;
+    }
+    pt_decide(R, dr, du_self, uni, part, acc_out, du_out);
+    int nacc = 0;
+    for (int r = 0; r < R; ++r) {
+        if (part[r] < 0) { du_out[r] = 0.0; continue; }
+        if (part[r] > r && acc_out[r]) {
+            int t = slot[r]; slot[r] = slot[part[r]]; slot[part[r]] = t;
             nacc++;
         }
     }
@@ -1293,6 +1330,103 @@ void or_pt_exchange_seeded(int R, const double *u, int *slot, uint32_t ctl_seed,
     or_mt_seed(&glob, glob_seed);
     for (int t = 0; t < nsteps; ++t)
         or_pt_exchange(R, u, slot, &ctl, &glob, acc_out + (long)t * R, du_out + (long)t * R);
+}
+
+or_mt *or_mt_new(uint32_t seed)
+{
+    or_mt *m = (or_mt *)malloc(sizeof(or_mt));
+    if (m) or_mt_seed(m, seed);
+    return m;
+}
+
+void or_mt_free(or_mt *m) { free(m); }
+
+/* One ParallelTempering move on each of R ranks, as Movebase::move
+ * (move.h:515-555) runs it when every rank calls it together.  Rank r is
+ * the system rk[r]: its Hamiltonian (lB), its global and internal engines
+ * and its Ewald state stay with it; configurations move between ranks
+ * (ParticleTransmitter, mpi.h:238-331, format XYZQI).
+ *   run(): one canonical of the rank's internal engine against prob
+ *     (move.h:575-581; the move has no molecule list, so no randomMolId);
+ *   _trialMove: findPartner from `ctl`; the partner's positions arrive in
+ *     the trial vector and every group's cm_trial is its mass centre there
+ *     (move.h:6648-6669);
+ *   _energyChange: uold = systemEnergy(p), unew = systemEnergy(trial),
+ *     both through Energy::systemEnergy, whose setSpace recomputes the
+ *     Ewald S(k) from the rank's own p -- so unew's reciprocal term uses the
+ *     rank's own S(k) (ewald.h:716-729, 731-757);
+ *   metropolis on the rank's global engine; _acceptMove copies trial to p
+ *     and cm_trial to cm; _rejectMove calls setSpace (S(k) recomputed from p)
+ *     and copies p to trial; then pot->update(accepted) (ewald.h:613-648).
+ * Volumes must agree (no Isobaric with tempering here) and so must the
+ * particle compositions.  ret_out[r] = what Movebase::move returns for rank r
+ * (unew - uold when accepted, move.h:6719-6722, plus the Ewald update drift).
+ * Returns the number of accepted swaps, -1 if the ranks' run() draws
+ * disagree (the reference would block), -2 for mismatched systems. */
+int or_pt_step(int R, or_sys **rk, or_mt *ctl, double prob, int *acc_out, double *du_out,
+               double *ret_out)
+{
+    int run0 = -1, part[OR_PT_MAXR];
+    double du_self[OR_PT_MAXR], uni[OR_PT_MAXR];
+    if (R < 1 || R > OR_PT_MAXR) return -2;
+    for (int r = 0; r < R; ++r) {
+        acc_out[r] = 0; du_out[r] = 0.0; ret_out[r] = 0.0;
+        if (rk[r]->n != rk[0]->n) return -2;
+        for (int k = 0; k < 3; ++k)
+            if (rk[r]->box.len[k] != rk[0]->box.len[k]) return -2;
+        for (int i = 0; i < rk[r]->n; ++i)
+            if (rk[r]->p[i].charge != rk[0]->p[i].charge || rk[r]->p[i].id != rk[0]->p[i].id)
+                return -2;
+    }
+    for (int r = 0; r < R; ++r) {
+        int run = or_canonical(&rk[r]->internal) < prob;
+        if (run0 < 0) run0 = run;
+        else if (run != run0) return -1;
+    }
+    if (!run0) return 0;
+    int dr = (or_canonical(ctl) > 0.5) ? 1 : -1;
+    for (int r = 0; r < R; ++r) {
+        int p = (r % 2 == 0) ? r + dr : r - dr;
+        part[r] = (p >= 0 && p < R) ? p : -1;
+    }
+    /* every rank sends its accepted positions; each receives its partner's */
+    for (int r = 0; r < R; ++r) {
+        if (part[r] < 0) continue;
+        const or_sys *src = rk[part[r]];
+        for (int i = 0; i < rk[r]->n; ++i)
+            memcpy(rk[r]->trial[i].r, src->p[i].r, sizeof(double) * 3);
+        for (int g = 0; g < rk[r]->ng; ++g)
+            mass_center(rk[r], rk[r]->trial, &rk[r]->g[g], rk[r]->g[g].cm_trial);
+    }
+    for (int r = 0; r < R; ++r) {
+        du_self[r] = 0.0;
+        if (part[r] < 0) continue;
+        double uold = or_system_energy(rk[r], 0);
+        double unew = or_system_energy(rk[r], 1);
+        du_self[r] = unew - uold;
+    }
+    for (int r = 0; r < R; ++r) uni[r] = or_canonical(&rk[r]->slump);
+    pt_decide(R, dr, du_self, uni, part, acc_out, du_out);
+    int nacc = 0;
+    for (int r = 0; r < R; ++r) {
+        or_sys *s = rk[r];
+        if (part[r] < 0) {
+            ret_out[r] = or_ewald_update(s, 0);
+            continue;
+        }
+        if (acc_out[r]) {
+            for (int i = 0; i < s->n; ++i) memcpy(s->p[i].r, s->trial[i].r, sizeof(double) * 3);
+            for (int g = 0; g < s->ng; ++g) memcpy(s->g[g].cm, s->g[g].cm_trial, sizeof(double) * 3);
+            ret_out[r] = du_self[r];
+            if (part[r] > r) nacc++;
+        } else {
+            if (s->ew) or_ewald_set_space(s);
+            for (int i = 0; i < s->n; ++i) memcpy(s->trial[i].r, s->p[i].r, sizeof(double) * 3);
+            for (int g = 0; g < s->ng; ++g) memcpy(s->g[g].cm_trial, s->g[g].cm, sizeof(double) * 3);
+        }
+        ret_out[r] += or_ewald_update(s, acc_out[r]);
+    }
+    return nacc;
 }
 
 /* ===================================================================== */

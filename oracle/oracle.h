@@ -20,10 +20,10 @@
  *     pinned by golden vectors of the compiled reference (oracle/_ref),
  *     by closed forms (lB known answers, 27-image brute force) and by the
  *     Ewald 4-ion known answer of doi:10.1063/1.481216.
- *   - ParallelTempering exchange: parity unpinned against a reference run.
- *     The reference needs MPI, which is absent, so it is pinned only by
- *     invariants (symmetric decision, detailed balance on a 2-state toy,
- *     swap of identical replicas has dU = 0).
+ *   - ParallelTempering exchange: no reference run (the reference needs
+ *     MPI, which is absent).  Pinned by hand values of _energyChange's sign
+ *     and partner terms, by the stationary occupancy of a two-state toy
+ *     (which a negated dU fails), and by swaps of identical replicas.
  */
 #ifndef FGX_ORACLE_H
 #define FGX_ORACLE_H
@@ -147,16 +147,31 @@ long or_propagate_npt(or_sys *s, double dp, double prob, int npt_index, int ncal
                       double *du_out, int *acc_out, long cap, double *utot);
 void or_get_box(or_sys *s, double *len);
 
-/* ParallelTempering (move.h:6520-6772) restated for R replicas in one
- * process (parity unpinned against a reference run; see the header note).  u[r][c] is replica r's Hamiltonian evaluated on configuration c;
- * slot[r] is the configuration currently held by replica r.  One exchange
- * attempt draws the partner direction from `ctl` (MPIController::random)
- * and each pair's acceptance from `glob` (global slump, lock-step).
- * Returns the number of accepted swaps; acc_out[r] = 1 if r swapped. */
+/* ParallelTempering (move.h:6520-6772) restated for R ranks in one process.
+ * Pinned by hand values of the exchange energy and the stationary occupancy
+ * of a two-state toy (tests/test_oracle_temper.py); no reference run (the
+ * reference's ParallelTempering needs MPI, absent here).
+ *
+ * Energy-matrix form: u[r][c] is rank r's Hamiltonian on configuration c;
+ * slot[r] is the configuration rank r holds.  One exchange attempt draws the
+ * partner direction from `ctl` (MPIController::random) and one Metropolis
+ * uniform from `glob`, standing for every rank's global engine in lock-step.
+ * Returns the accepted swaps; acc_out[r] = 1 if r swapped, du_out[r] the
+ * pair's energy change (0 without partner). */
 int  or_pt_exchange(int R, const double *u, int *slot, or_mt *ctl, or_mt *glob, int *acc_out,
                     double *du_out);
 void or_pt_exchange_seeded(int R, const double *u, int *slot, uint32_t ctl_seed,
                            uint32_t glob_seed, int nsteps, int *acc_out, double *du_out);
+or_mt *or_mt_new(uint32_t seed);
+void   or_mt_free(or_mt *m);
+/* System form: one ParallelTempering move on every rank, rank r being rk[r]
+ * with its own engines, Hamiltonian and Ewald state; configurations move
+ * between ranks.  du_out[r] = +inf without partner.  ret_out[r] = the energy
+ * Movebase::move returns on rank r.  Returns the accepted swaps, -1 if the
+ * ranks' run() draws disagree, -2 for systems of different size, box or
+ * composition. */
+int  or_pt_step(int R, or_sys **rk, or_mt *ctl, double prob, int *acc_out, double *du_out,
+                double *ret_out);
 
 #ifdef __cplusplus
 }

@@ -65,7 +65,9 @@ _lib = None
 def lib():
     global _lib
     if _lib is None:
-        path = os.path.join(ORACLE_DIR, "liboracle.so")
+        # ORACLE_LIB names another build of oracle.c (e.g. one with a planted
+        # mistake, to show which tests catch it)
+        path = os.environ.get("ORACLE_LIB") or os.path.join(ORACLE_DIR, "liboracle.so")
         if not os.path.exists(path):
             subprocess.run(["make", "-C", ORACLE_DIR, "oracle"], check=True, capture_output=True)
         L = C.CDLL(path)
@@ -113,6 +115,14 @@ def lib():
         L.or_last_trials.argtypes = [P, ip, C.c_long]
         L.or_rng_draws.argtypes = [C.c_uint32, I, I, I, I, dp]
         L.or_pt_exchange_seeded.argtypes = [I, dp, ip, C.c_uint32, C.c_uint32, I, ip, dp]
+        L.or_pt_exchange.restype = I
+        L.or_pt_exchange.argtypes = [I, dp, ip, P, P, ip, dp]
+        L.or_mt_new.restype = P
+        L.or_mt_new.argtypes = [C.c_uint32]
+        L.or_mt_free.argtypes = [P]
+        L.or_pt_step.restype = I
+        L.or_pt_step.argtypes = [I, C.POINTER(P), P, D, ip, dp, dp]
+        L.or_rng_draws.restype = None
         _lib = L
     return _lib
 
@@ -310,3 +320,29 @@ class OracleSystem:
         L.or_last_trials(self.h, None, 0)
         self._who = who
         return du[:n], acc[:n], who[:n], ut.value, ui.value
+
+
+class OracleEngine:
+    """An or_mt engine (libstdc++ mt19937) owned by Python."""
+
+    def __init__(self, seed=5489):
+        self.h = lib().or_mt_new(seed)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_mt_free(self.h)
+            self.h = None
+
+
+def pt_step(ranks, ctl, prob=1.0):
+    """One ParallelTempering move on every rank (or_pt_step): returns
+    (acc, du, ret) per rank."""
+    R = len(ranks)
+    arr = (C.c_void_p * R)(*[r.h for r in ranks])
+    acc = np.zeros(R, dtype=np.int32)
+    du = np.zeros(R)
+    ret = np.zeros(R)
+    rc = lib().or_pt_step(R, arr, ctl.h, prob, iptr(acc), dptr(du), dptr(ret))
+    if rc < 0:
+        raise ValueError("or_pt_step refused the ranks (%d)" % rc)
+    return acc, du, ret
