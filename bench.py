@@ -418,7 +418,7 @@ def bench_temper(args, fgx, workloads):
             s.propagator_init()
             reps.append(s)
         systems += reps
-        tempers.append(fgx.Temper(reps, ctl_seed=11 + l, glob_seed=7 + l))
+        tempers.append(fgx.Temper(reps, ctl_seed=11 + l))
     t_build = time.perf_counter() - t_build
     sweeps = args.exchange_every
     tuned = fgx.autotune_batch(systems, 1) if args.warmup > 0 else None

@@ -161,6 +161,10 @@ namespace fgx {
 int ensure_partials(fgx_system *s, int nblocks);
 int ewald_full_recompute(fgx_system *s, int which, double *Qdst);
 int ewald_set_space(fgx_system *s);
+// NonbondedEwald::update(true) (ewald.h:613-648) on the host-side counter:
+// advances cnt_accepted, refreshes S(k) from the trial vector when due
+// (*drift = the reciprocal-energy correction), and makes Q = Q_trial
+int ewald_accepted(fgx_system *s, double *drift);
 int ewald_reciprocal(fgx_system *s, const double *Q, double V, double *out_host);
 void mt_seed_host(uint32_t *state625, uint32_t seed);
 int pull_device_state(fgx_system *s);

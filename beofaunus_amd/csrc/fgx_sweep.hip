@@ -412,20 +412,16 @@ struct Launch {
     int rc = FGX_OK;
 };
 
-// Enqueues one launch advancing chain c by ncalls[c] Propagator::move()
-// calls.  All transfers go on the first system's stream; RNG states and
-// counters stay on the device between launches.  Nothing waits for the
-// sweep itself, so launches of different Hamiltonians on different first
-// systems overlap on the device.
-static int launch_chains(std::vector<fgx_system *> &sys, const std::vector<int> &ncalls,
-                         const std::vector<long> *max_trials, double *du_rec, int *acc_rec,
-                         long cap, Launch &L)
+// Everything of a batched launch up to the kernel: per-chain state on the
+// device, cell lists and energy caches rebuilt where stale, the ChainDev
+// array copied to the first system's batch buffer.
+int prepare_batch(std::vector<fgx_system *> &sys, const std::vector<int> &ncalls,
+                  const std::vector<long> *max_trials, double *du_rec, int *acc_rec, long cap,
+                  BatchPlan &B)
 {
     fgx_system *s0 = sys[0];
     const int nc = (int)sys.size();
     hipStream_t st = s0->stream;
-    L.sys = sys;
-    L.st = st;
     std::vector<ChainDev> chains(nc);
     int rc = FGX_OK;
     int nthr = 0;
@@ -568,32 +564,72 @@ static int launch_chains(std::vector<fgx_system *> &sys, const std::vector<int> 
         ChainDev *d_chains = (ChainDev *)s0->d_batch;
         FGX_HIP(hipMemcpyAsync(d_chains, chains.data(), sizeof(ChainDev) * nc, hipMemcpyHostToDevice,
                                st));
+        B.d_chains = d_chains;
+        B.nchains = nc;
+        B.nthr = nthr;
+        B.dyn = dyn;
+        B.ncalls = ncalls.empty() ? 0 : ncalls[0];
+        B.unroll = unroll;
+        B.fx = fx;
+        B.cell = cell;
+        B.pot = s0->pot;
+        B.move_type = s0->moves[0].type;
+    }
+    return rc;
+}
+
+// The sweep kernel of a prepared batch on stream st (nothing else: graph
+// capture records exactly this launch).
+int enqueue_batch(const BatchPlan &B, hipStream_t st)
+{
+    if (B.move_type == MOVE_AT)
+        return B.pot.precision == FGX_MIXED
+                   ? launch_at<true>(B.pot, st, B.d_chains, B.nchains, B.nthr, B.dyn, B.ncalls,
+                                     B.unroll, B.fx, B.cell)
+                   : launch_at<false>(B.pot, st, B.d_chains, B.nchains, B.nthr, B.dyn, B.ncalls,
+                                      B.unroll, B.fx, false);
+    return launch_tr(B.pot.precision == FGX_MIXED, B.pot, st, B.d_chains, B.nchains, B.nthr,
+                     B.ncalls);
+}
+
+// Trial vector and trial S(k) mirror the accepted state after device sweeps
+// (undo/accept leave p == trial between moves).
+int mirror_trial(std::vector<fgx_system *> &sys, hipStream_t st)
+{
+    for (fgx_system *s : sys) {
+        if (s->has_ewald)
+            FGX_HIP(hipMemcpyAsync(s->ew.Qt, s->ew.Q, sizeof(double) * 2 * s->ew.K,
+                                   hipMemcpyDeviceToDevice, st));
+        for (int k = 0; k < 3; ++k) {
+            double *src = k == 0 ? s->v[0].x : k == 1 ? s->v[0].y : s->v[0].z;
+            double *dst = k == 0 ? s->v[1].x : k == 1 ? s->v[1].y : s->v[1].z;
+            FGX_HIP(hipMemcpyAsync(dst, src, sizeof(double) * s->n, hipMemcpyDeviceToDevice, st));
+        }
+    }
+    return FGX_OK;
+}
+
+// Enqueues one launch advancing chain c by ncalls[c] Propagator::move()
+// calls.  All transfers go on the first system's stream; RNG states and
+// counters stay on the device between launches.  Nothing waits for the
+// sweep itself, so launches of different Hamiltonians on different first
+// systems overlap on the device.
+static int launch_chains(std::vector<fgx_system *> &sys, const std::vector<int> &ncalls,
+                         const std::vector<long> *max_trials, double *du_rec, int *acc_rec,
+                         long cap, Launch &L)
+{
+    hipStream_t st = sys[0]->stream;
+    L.sys = sys;
+    L.st = st;
+    BatchPlan B;
+    int rc = prepare_batch(sys, ncalls, max_trials, du_rec, acc_rec, cap, B);
+    if (!rc) {
         FGX_HIP(hipEventCreate(&L.e0));
         FGX_HIP(hipEventCreate(&L.e1));
         FGX_HIP(hipEventRecord(L.e0, st));
-        const int ncall0 = ncalls.empty() ? 0 : ncalls[0];
-        if (s0->moves[0].type == MOVE_AT) {
-            rc = s0->pot.precision == FGX_MIXED
-                     ? launch_at<true>(s0->pot, st, d_chains, nc, nthr, dyn, ncall0, unroll, fx, cell)
-                     : launch_at<false>(s0->pot, st, d_chains, nc, nthr, dyn, ncall0, unroll, fx,
-                                        false);
-        } else {
-            rc = launch_tr(s0->pot.precision == FGX_MIXED, s0->pot, st, d_chains, nc, nthr, ncall0);
-        }
+        rc = enqueue_batch(B, st);
         FGX_HIP(hipEventRecord(L.e1, st));
-        // trial vector, trial S(k) and mass centres mirror the accepted state
-        // (undo/accept leave p == trial between moves)
-        for (int c = 0; c < nc && !rc; ++c) {
-            fgx_system *s = sys[c];
-            if (s->has_ewald)
-                FGX_HIP(hipMemcpyAsync(s->ew.Qt, s->ew.Q, sizeof(double) * 2 * s->ew.K,
-                                       hipMemcpyDeviceToDevice, st));
-            for (int k = 0; k < 3; ++k) {
-                double *src = k == 0 ? s->v[0].x : k == 1 ? s->v[0].y : s->v[0].z;
-                double *dst = k == 0 ? s->v[1].x : k == 1 ? s->v[1].y : s->v[1].z;
-                FGX_HIP(hipMemcpyAsync(dst, src, sizeof(double) * s->n, hipMemcpyDeviceToDevice, st));
-            }
-        }
+        if (!rc) rc = mirror_trial(sys, st);
     }
     L.rc = rc;
     return rc;

@@ -1837,6 +1837,21 @@ static void launch_at_k(hipStream_t st, const ChainDev *d_chains, int nchains, i
         launch_at_u<ELEC, SR, MIX, 8>(st, d_chains, nchains, nthr, dyn, ncalls, fx, cell);
 }
 
+// A prepared batched launch (fgx_sweep.hip): the ChainDev array on the device
+// and the launch shape of its sweep kernel.
+struct BatchPlan {
+    ChainDev *d_chains = nullptr;
+    int nchains = 0, nthr = 0, ncalls = 0, unroll = 4, move_type = 0;
+    size_t dyn = 0;
+    bool fx = false, cell = false;
+    Pot pot{};
+};
+int prepare_batch(std::vector<fgx_system *> &sys, const std::vector<int> &ncalls,
+                  const std::vector<long> *max_trials, double *du_rec, int *acc_rec, long cap,
+                  BatchPlan &B);
+int enqueue_batch(const BatchPlan &B, hipStream_t st);
+int mirror_trial(std::vector<fgx_system *> &sys, hipStream_t st);
+
 // Per-family launchers (fgx_sweep_at_*.hip, fgx_sweep_tr.hip).  Each returns
 // FGX_EUNSUPPORTED without an error message for a Hamiltonian outside its
 // family, so the caller can try the next one.

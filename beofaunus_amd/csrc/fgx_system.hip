@@ -667,7 +667,7 @@ static int ewald_rebox(fgx_system *s, const double len[3])
 // current ones; every update_frequency accepted moves they are recomputed
 // from the (now accepted) trial vector and the change of the reciprocal
 // energy is returned in *drift
-static int ewald_accepted(fgx_system *s, double *drift)
+int fgx::ewald_accepted(fgx_system *s, double *drift)
 {
     EwaldDev &e = s->ew;
     *drift = 0.0;

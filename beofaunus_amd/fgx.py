@@ -148,7 +148,9 @@ def lib():
         "fgx_get_launch": (I, [P, _ip, _ip]),
         "fgx_autotune": (I, [P, I, _ip, _ip, _dp]),
         "fgx_autotune_batch": (I, [pp, I, I, _ip, _ip, _dp]),
-        "fgx_temper_create": (I, [pp, I, C.c_uint32, C.c_uint32, pp]),
+        "fgx_temper_create": (I, [pp, I, C.c_uint32, D, pp]),
+        "fgx_temper_run": (I, [pp, I, I, I, C.POINTER(Stats), _dp]),
+        "fgx_temper_records": (I, [P, _ip, _ip, _dp, _ip, _dp]),
         "fgx_temper_exchange": (I, [P, _ip, _dp]),
         "fgx_temper_slots": (I, [P, _ip]),
         "fgx_temper_destroy": (I, [P]),
@@ -476,24 +478,40 @@ def propagate_batch_record(systems, ncalls, cap):
 
 
 class Temper:
-    """ParallelTempering over replicas resident on one GPU (fgx_temper_*)."""
+    """Move::ParallelTempering over replicas resident on one GPU
+    (fgx_temper_*): rank r is replica r's Bjerrum length, and the rank's
+    engines and Ewald counter travel with it on a swap."""
 
-    def __init__(self, replicas, ctl_seed=5489, glob_seed=5489):
+    def __init__(self, replicas, ctl_seed=5489, prob=1.0):
         self.replicas = list(replicas)
-        arr = (C.c_void_p * len(self.replicas))(*[r.h for r in self.replicas])
+        self.R = len(self.replicas)
+        arr = (C.c_void_p * self.R)(*[r.h for r in self.replicas])
         h = C.c_void_p()
-        _check(lib().fgx_temper_create(arr, len(self.replicas), ctl_seed, glob_seed, C.byref(h)))
+        _check(lib().fgx_temper_create(arr, self.R, ctl_seed, float(prob), C.byref(h)))
         self.h = h
 
     def exchange(self):
-        R = len(self.replicas)
-        acc = np.zeros(R, dtype=np.int32)
-        du = np.zeros(R)
+        """One exchange attempt: (acc, du) per rank (du = +inf without partner)."""
+        acc = np.zeros(self.R, dtype=np.int32)
+        du = np.zeros(self.R)
         _check(lib().fgx_temper_exchange(self.h, _i(acc), _d(du)))
         return acc, du
 
+    def records(self):
+        """Records of the last exchange / run: acc, du, slot (rounds x R) and
+        the configurations' sweep seconds per round."""
+        n = C.c_int()
+        _check(lib().fgx_temper_records(self.h, C.byref(n), None, None, None, None))
+        k = n.value
+        acc = np.zeros((k, self.R), dtype=np.int32)
+        du = np.zeros((k, self.R))
+        slot = np.zeros((k, self.R), dtype=np.int32)
+        sec = np.zeros((k, self.R))
+        _check(lib().fgx_temper_records(self.h, C.byref(n), _i(acc), _d(du), _i(slot), _d(sec)))
+        return acc, du, slot, sec
+
     def slots(self):
-        s = np.zeros(len(self.replicas), dtype=np.int32)
+        s = np.zeros(self.R, dtype=np.int32)
         _check(lib().fgx_temper_slots(self.h, _i(s)))
         return s
 
@@ -507,6 +525,18 @@ class Temper:
             self.close()
         except Exception:
             pass
+
+
+def temper_run(tempers, rounds, calls):
+    """fgx_temper_run: `rounds` x (`calls` Propagator::move() calls of every
+    configuration, then one exchange per temper), as one replayed HIP graph.
+    Returns (stats, device seconds)."""
+    arr = (C.c_void_p * len(tempers))(*[t.h for t in tempers])
+    st = Stats()
+    sec = C.c_double()
+    _check(lib().fgx_temper_run(arr, len(tempers), int(rounds), int(calls), C.byref(st),
+                                C.byref(sec)))
+    return st.as_dict(), sec.value
 
 
 class Server:

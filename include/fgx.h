@@ -303,21 +303,44 @@ int fgx_server_info(const fgx_server *sv, size_t *resident, size_t *peak, int *a
 int fgx_server_destroy(fgx_server *sv);
 
 /* ------------------------------------------------------------------------
- * ParallelTempering (move.h:6520-6772) for R replicas on one device.  The
- * replicas differ in Bjerrum length (temperature enters through lB,
- * physconst.h:91-94).  An exchange draws the partner direction from a
- * shared controller engine (MPIController::random, mpi.h:48) and one
- * Metropolis uniform from the shared global engine; pair energies are
- * evaluated on the device (Energy::systemEnergy).  Configurations are
- * swapped by exchanging replica labels, not coordinates.
+ * Move::ParallelTempering (move.h:6520-6772) for R ranks on one device.  Rank
+ * r is replica r's Hamiltonian (the replicas differ in Bjerrum length only:
+ * temperature enters through lB, physconst.h:91-94), and what belongs to a
+ * rank in the reference -- its lB, its global and internal engines
+ * (slump.h:126, move.h:356-360) and its Ewald accept counter -- travels with
+ * it; configurations stay in their handles.  An exchange attempt is one
+ * ParallelTempering move on every rank (Movebase::move, move.h:515-555): the
+ * run() draw of each rank's internal engine, the partner direction from a
+ * controller engine shared by all ranks (MPIController::random, mpi.h:48),
+ * du = (U_r(x_p) - U_r(x_r)) + (U_p(x_r) - U_p(x_p)) (_energyChange,
+ * move.h:6701-6723) and the Metropolis draw of each rank's global engine; a
+ * pair decides with its lower rank's draw, the reference's own decision when
+ * the ranks' engines are in lock-step (move.h:557-566).  With Ewald the
+ * partner energy's reciprocal term uses the rank's own S(k), as the
+ * reference's systemEnergy does after setSpace (ewald.h:716-757).
  * ------------------------------------------------------------------------ */
 typedef struct fgx_temper fgx_temper;
-int fgx_temper_create(fgx_system **replicas, int R, uint32_t ctl_seed, uint32_t glob_seed,
+/* ParallelTempering constructor (move.h:6592-6613); prob = its runfraction
+ * ("prob" key).  Replicas must agree in everything but lB (FGX_EUNSUPPORTED
+ * otherwise; ExternalPressure and NonbondedCutg2g are not supported). */
+int fgx_temper_create(fgx_system **replicas, int R, uint32_t ctl_seed, double prob,
                       fgx_temper **out);
-/* one exchange attempt; acc[r] = 1 if replica r swapped; du[r] = pair dU */
+/* one exchange attempt (one ParallelTempering::move on every rank);
+ * acc[r] = 1 if rank r swapped; du[r] = the pair's energy change, +inf for a
+ * rank without partner.  FGX_ESTATE if the ranks' run() draws disagree. */
 int fgx_temper_exchange(fgx_temper *t, int *acc, double *du);
-/* slot[r] = index of the configuration (system) replica r's Hamiltonian
- * currently holds */
+/* `rounds` times: `calls` Propagator::move() calls of every configuration's
+ * registered move (one batched sweep of all tempers' configurations), then
+ * one exchange attempt in each temper; the round is captured once as a HIP
+ * graph and replayed.  Not for Ewald replicas.  *seconds = device time. */
+int fgx_temper_run(fgx_temper **tempers, int nt, int rounds, int calls, fgx_stats *st,
+                   double *seconds);
+/* records of the last exchange / run call: *rounds, then per round and rank
+ * acc, du and slot (after the round), and per round and configuration its
+ * sweep time in seconds (run only).  Any output may be null. */
+int fgx_temper_records(fgx_temper *t, int *rounds, int *acc, double *du, int *slot,
+                       double *sweep_seconds);
+/* slot[r] = index of the configuration (replica handle) rank r holds */
 int fgx_temper_slots(fgx_temper *t, int *slot);
 int fgx_temper_destroy(fgx_temper *t);
 
