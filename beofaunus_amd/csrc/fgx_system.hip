@@ -1722,6 +1722,16 @@ extern "C" int fgx_commit_all(fgx_system *s, int accepted, double *drift)
     return FGX_OK;
 }
 
+// NonbondedEwald's k-vectors in use for the current box (kVectorChange,
+// ewald.h:330-378; the count its info() reports as "Wavefunctions",
+// ewald.h:466), 0 without Ewald
+extern "C" int fgx_ewald_k(const fgx_system *s, int *K)
+{
+    FGX_CHECK(s && K, FGX_EARG, "fgx_ewald_k: bad arguments");
+    *K = s->has_ewald ? s->ew.K : 0;
+    return FGX_OK;
+}
+
 extern "C" int fgx_get_box(const fgx_system *s, double len[3])
 {
     FGX_CHECK(s && len, FGX_EARG, "fgx_get_box: bad arguments");

@@ -135,6 +135,7 @@ def lib():
         "fgx_config_energy": (I, [P, I, _dp]),
         "fgx_commit_all": (I, [P, I, _dp]),
         "fgx_get_box": (I, [P, _dp]),
+        "fgx_ewald_k": (I, [P, _ip]),
         "fgx_get_cell_list": (I, [P, _ip, _ip]),
         "fgx_chain_seconds": (I, [P, C.POINTER(C.c_double)]),
         "fgx_rdf": (I, [P, I, I, I, C.c_double, I, C.POINTER(C.c_ulonglong)]),
@@ -394,6 +395,11 @@ class System:
         b = np.zeros(3)
         _check(lib().fgx_get_box(self.h, _d(b)))
         return b
+
+    def ewald_k(self):
+        k = np.zeros(1, dtype=np.int32)
+        _check(lib().fgx_ewald_k(self.h, _i(k)))
+        return int(k[0])
 
     def set_cell_list(self, mode):
         """Cell list of the mixed AtomicTranslation sweep: 0 off, 1 automatic,

@@ -212,6 +212,9 @@ int fgx_config_energy(fgx_system *s, int which, double *u);
 int fgx_commit_all(fgx_system *s, int accepted, double *drift);
 /* Cuboid side lengths (Cuboid::setlen after an accepted volume move) */
 int fgx_get_box(const fgx_system *s, double len[3]);
+/* NonbondedEwald's k-vectors in use (kVectorChange, ewald.h:330-378; the
+ * "Wavefunctions" count of its info(), ewald.h:466); 0 without Ewald */
+int fgx_ewald_k(const fgx_system *s, int *K);
 
 /* Launch schedule of the device sweeps: workgroup size (256 or 512 threads
  * per chain) and partner-loop batch depth (4 or 8) of the mixed path; 0

@@ -140,6 +140,19 @@ static void print_params(Tmjson &in)
     jnum("lB", pc::lB(epsr));
 }
 
+// Number of k-vectors in use as the energy's own info() reports it
+// (NonbondedEwald::_info, ewald.h:417-470: "Wavefunctions  K (represented)");
+// -1 for energies without k-space.
+template<class Tenergy>
+static int info_kvectors(Tenergy &pot)
+{
+    std::string s = pot.info();
+    size_t k = s.find("Wavefunctions");
+    if (k == std::string::npos) return -1;
+    k = s.find_first_of("0123456789", k);
+    return k == std::string::npos ? -1 : std::atoi(s.c_str() + k);
+}
+
 // Every Energybase entry point of `pot` on the loaded space (prints the
 // JSON fields after the parameters).
 template<class Tenergy>
@@ -149,6 +162,7 @@ static void print_energies(Tspace &spc, Tenergy &pot)
     int n = (int)spc.p.size();
     double usys = Energy::systemEnergy(spc, pot, spc.p);
     jnum("system_energy", usys);
+    jnum("ewald_nk", info_kvectors(pot));
     jnum("external", pot.external(spc.p));
     std::vector<double> gext, gint;
     for (auto g : spc.groupList()) {
