@@ -176,6 +176,9 @@ int batch_launch(std::vector<fgx_system *> &sys, const std::vector<int> &ncalls,
                  const std::vector<long> &max_trials, void **handle);
 int batch_finish(void *handle, double *seconds, std::vector<long> *stats_i,
                  std::vector<double> *stats_d);
+// the device sweeps' checks of a system and its registered move (the ones a
+// launch makes), without a launch: FGX_OK or the error the launch would give
+int check_chain(const fgx_system *s);
 }  // namespace fgx
 // folds one chain's launch statistics (8 longs, 8 doubles) into the system
 // and into *st (may be null)

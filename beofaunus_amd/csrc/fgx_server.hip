@@ -14,7 +14,8 @@
 // unfinished Propagator::move() call), a small job runs several sweeps per
 // tick, and arrivals join at the next tick (continuous batching).  A job's result does not
 // depend on the other jobs it shares launches with: chains are independent
-// workgroups and every reduction has a fixed order.
+// workgroups and every reduction has a fixed order.  A job the device
+// sweeps cannot run fails at admission, alone.
 //
 // The reference runs one simulation per process (src/examples/bulk.cpp) and
 // has no counterpart to this component.
@@ -128,6 +129,9 @@ static int admit(fgx_server *sv, Job &J)
     // one fixed schedule for every job, so a result does not depend on what
     // else is resident
     if (!rc) rc = fgx_set_launch(s, 512, 4);
+    // a job the device sweeps cannot run fails here, alone, instead of
+    // failing the shared launch of its kind
+    if (!rc) rc = check_chain(s);
     if (rc) {
         if (s) fgx_destroy(s);
         return rc;
@@ -283,6 +287,21 @@ extern "C" int fgx_server_step(fgx_server *sv, int *n_admitted, int *n_finished)
             sys.push_back(J->sys);
         }
         f.rc = batch_launch(sys, f.ncalls, f.budget, &f.h);
+        if (f.rc && f.jobs.size() > 1) {
+            // a shared launch that could not be prepared: run its jobs one
+            // by one, so only the one that fails is retired
+            (void)batch_finish(f.h, nullptr, nullptr, nullptr);
+            for (size_t c = 0; c < f.jobs.size(); ++c) {
+                InFlight g;
+                g.jobs = {f.jobs[c]};
+                g.ncalls = {f.ncalls[c]};
+                g.budget = {f.budget[c]};
+                std::vector<fgx_system *> one = {sys[c]};
+                g.rc = batch_launch(one, g.ncalls, g.budget, &g.h);
+                fl.push_back(std::move(g));
+            }
+            continue;
+        }
         fl.push_back(std::move(f));
     }
     int rc_all = FGX_OK;
@@ -312,7 +331,8 @@ extern "C" int fgx_server_step(fgx_server *sv, int *n_admitted, int *n_finished)
                 ++fin;
             }
         }
-        if (rc && !rc_all) rc_all = rc;
+        // a job's own failure is in its result; a runtime error is everyone's
+        if (rc == FGX_EHIP && !rc_all) rc_all = rc;
     }
     ++sv->ticks;
     if (n_admitted) *n_admitted = adm;

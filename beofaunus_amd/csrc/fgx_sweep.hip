@@ -181,19 +181,18 @@ static int setup_cells(fgx_system *s, ChainDev &C, const MoveDesc &mv)
     return FGX_OK;
 }
 
-static int prepare_chain(fgx_system *s, ChainDev &C, hipStream_t st, double *du_rec, int *acc_rec,
-                         long cap)
+// What the device sweeps support for a system's registered move, checked
+// without touching the device: the move's groups and, per move, the limits of
+// the kernels' LDS stages.  `gvec` gets the move's groups.
+static int chain_limits(const fgx_system *s, std::vector<int> &gvec)
 {
     FGX_CHECK(s->moves.size() == 1, FGX_EUNSUPPORTED,
               "device Propagator: exactly one registered move is supported");
     const MoveDesc &mv = s->moves[0];
-    std::vector<int> gvec;
+    gvec.clear();
     for (int g = 0; g < s->ng; ++g)
         if (s->groups[g].molid == mv.molid) gvec.push_back(g);
     FGX_CHECK(!gvec.empty(), FGX_ESTATE, "device Propagator: no group carries the move's molid");
-    int repeat = 1;
-    if (mv.permol) repeat *= (int)gvec.size();
-    if (mv.peratom) repeat *= s->groups[gvec[0]].back - s->groups[gvec[0]].front + 1;
     if (mv.type == MOVE_AT) {
         for (int g : gvec)
             FGX_CHECK(s->groups[g].molsize == 1, FGX_EUNSUPPORTED,
@@ -209,6 +208,24 @@ static int prepare_chain(fgx_system *s, ChainDev &C, hipStream_t st, double *du_
             FGX_CHECK(s->groups[g].back - s->groups[g].front + 1 <= MAX_GROUP_SITES,
                       FGX_EUNSUPPORTED, "device TranslateRotate: group larger than LDS stage");
     }
+    return FGX_OK;
+}
+
+int check_chain(const fgx_system *s)
+{
+    std::vector<int> gvec;
+    return chain_limits(s, gvec);
+}
+
+static int prepare_chain(fgx_system *s, ChainDev &C, hipStream_t st, double *du_rec, int *acc_rec,
+                         long cap)
+{
+    std::vector<int> gvec;
+    if (int rc = chain_limits(s, gvec)) return rc;
+    const MoveDesc &mv = s->moves[0];
+    int repeat = 1;
+    if (mv.permol) repeat *= (int)gvec.size();
+    if (mv.peratom) repeat *= s->groups[gvec[0]].back - s->groups[gvec[0]].front + 1;
     if (!s->d_gvec || s->gvec_n != (int)gvec.size()) {
         if (s->d_gvec) (void)hipFree(s->d_gvec);
         s->d_gvec = nullptr;

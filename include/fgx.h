@@ -297,7 +297,9 @@ typedef struct {
 } fgx_job_result;
 int fgx_server_create(size_t hbm_budget, int max_active, double tick_seconds, fgx_server **out);
 int fgx_server_submit(fgx_server *sv, const fgx_job *job, int *id);
-/* admit what fits, advance every resident job by one tick, retire finished */
+/* admit what fits, advance every resident job by one tick, retire finished.
+ * A job the device sweeps cannot run is retired at admission with its error
+ * (fgx_server_result status); the call itself fails only on runtime errors. */
 int fgx_server_step(fgx_server *sv, int *n_admitted, int *n_finished);
 int fgx_server_poll(fgx_server *sv, int *ids, int cap, int *count);
 int fgx_server_result(fgx_server *sv, int id, fgx_job_result *r, double *xyz);

@@ -30,17 +30,30 @@ def gpu():
     return fgx
 
 
+def gpu_fgx():
+    from beofaunus_amd import fgx
+    return fgx
+
+
 def drain(sv, positions=True):
+    def res(j):
+        r = sv.result(j)
+        if positions and r["status"] == 0:
+            try:
+                r = sv.result(j, positions=True)
+            except gpu_fgx().FgxError:   # positions not kept for this job
+                pass
+        return r
     out = {}
-    for _ in range(10000):
+    for _ in range(100000):
         inf = sv.info()
         if not inf["active"] and not inf["queued"]:
             break
         sv.step()
         for j in sv.poll():
-            out[j] = sv.result(j, positions=positions)
+            out[j] = res(j)
     for j in sv.poll():
-        out[j] = sv.result(j, positions=positions)
+        out[j] = res(j)
     return out
 
 
@@ -150,3 +163,97 @@ def test_budget_admission(gpu):
     assert all(out[j]["status"] == 0 and out[j]["sweeps"] == 3 for j in ids)
     assert peak <= budget
     assert max(out[j]["queue_seconds"] for j in ids) > 0
+
+
+def test_unsupported_job_fails_alone(gpu):
+    """A job the device sweeps cannot run (an AtomicTranslation of molecular
+    groups) is retired at admission with its own error; the jobs beside it
+    finish as they do alone."""
+    from beofaunus_amd import workloads
+    say("[server] one unsupported job beside good ones")
+    a = workloads.salt_dh(n=300, L=40.0, seed=41, lattice=True)
+    b = workloads.salt_dh(n=500, L=50.0, seed=42, lattice=True)
+    bad = workloads.rigid_molecules(nmol=3, nsite=20, nion=100, L=60.0, R=8.0, seed=43)
+    bad["move"] = dict(kind=gpu.MOVE_AT, molid=0, peratom=True)
+    alone = {}
+    for w, sd in ((a, 1), (b, 2)):
+        sv = gpu.Server(1 << 30, tick_seconds=0.01)
+        j = sv.submit(w, seed=sd, sweeps=3, keep_positions=True)
+        alone[sd] = drain(sv)[j]
+        sv.close()
+    sv = gpu.Server(1 << 30, tick_seconds=0.01)
+    ja = sv.submit(a, seed=1, sweeps=3, keep_positions=True)
+    jx = sv.submit(bad, seed=3, sweeps=3, keep_positions=True)
+    jb = sv.submit(b, seed=2, sweeps=3, keep_positions=True)
+    out = drain(sv)
+    sv.close()
+    assert out[jx]["status"] == -5   # FGX_EUNSUPPORTED
+    for j, sd in ((ja, 1), (jb, 2)):
+        assert out[j]["status"] == 0
+        assert out[j]["trials"] == alone[sd]["trials"]
+        assert np.array_equal(out[j]["xyz"], alone[sd]["xyz"])
+
+
+def _oracle_of(w):
+    from oracle_py import OracleSystem, lib as olib
+    h = w["ham"]
+    kw = dict(type_sigma=h["type_sigma"], type_eps=h["type_eps"], type_charge=h["type_charge"],
+              type_radius=h["type_radius"], type_dp=h["type_dp"])
+    if h.get("ewald"):
+        kw["ewald"] = h["ewald"]
+    else:
+        kw["kappa"] = h["kappa"]
+    o = OracleSystem(w["ref_kind"], w["box"], w["xyz"], w["charge"], w["radius"], w["mw"],
+                     w["types"], w["groups"], h["lB"], **kw)
+    olib().or_add_move(o.h, 1, 0, 1.0, 0, 0, None, None, 0.0, 0.0, 1)
+    return o
+
+
+def test_stream_sizes_big_jobs(gpu):
+    """The served stream's largest sizes: a 50,000-ion Debye-Hueckel job and a
+    20,000-ion Ewald job (FP64, 2,000 one-trial Propagator calls each) served
+    beside small mixed-precision jobs.  Each big job ends as it does alone,
+    and as the oracle's run of the same 2,000 trials (same accepts, final
+    positions within 1e-9 A)."""
+    from beofaunus_amd import workloads
+    from oracle_py import lib as olib
+    say("[server] N=50000 DH and N=20000 Ewald jobs beside small jobs")
+    rho = 0.01
+    big_dh = workloads.salt_dh(n=50000, L=(50000 / rho) ** (1 / 3), seed=51, lattice=True)
+    big_ew = workloads.ewald_electrolyte(n=20000, L=(20000 / rho) ** (1 / 3), seed=52)
+    for w in (big_dh, big_ew):
+        w["move"] = dict(kind=gpu.MOVE_AT, molid=0, peratom=False)
+    small = [workloads.salt_dh(n=500, L=(500 / rho) ** (1 / 3), seed=53, lattice=True),
+             workloads.ewald_electrolyte(n=800, L=(800 / rho) ** (1 / 3), seed=54),
+             workloads.salt_dh(n=2000, L=(2000 / rho) ** (1 / 3), seed=55, lattice=True)]
+    calls = 2000
+    alone = {}
+    for w, sd in ((big_dh, 61), (big_ew, 62)):
+        sv = gpu.Server(1 << 33, tick_seconds=0.05)
+        j = sv.submit(w, seed=sd, sweeps=calls, precision=gpu.FP64, keep_positions=True)
+        alone[sd] = drain(sv)[j]
+        sv.close()
+        say("  alone N=%d: trials %d accepted %d" % (len(w["charge"]), alone[sd]["trials"],
+                                                    alone[sd]["accepted"]))
+    sv = gpu.Server(1 << 33, tick_seconds=0.02)
+    ids = {}
+    ids[61] = sv.submit(big_dh, seed=61, sweeps=calls, precision=gpu.FP64, keep_positions=True)
+    for k, w in enumerate(small):
+        sv.submit(w, seed=70 + k, sweeps=2)
+    ids[62] = sv.submit(big_ew, seed=62, sweeps=calls, precision=gpu.FP64, keep_positions=True)
+    out = drain(sv)
+    sv.close()
+    assert all(r["status"] == 0 for r in out.values())
+    for sd, w in ((61, big_dh), (62, big_ew)):
+        r, a = out[ids[sd]], alone[sd]
+        assert r["trials"] == a["trials"] == calls
+        assert r["accepted"] == a["accepted"]
+        assert np.array_equal(r["xyz"], a["xyz"])
+        o = _oracle_of(w)
+        o.seed(sd)
+        olib().or_propagator_init(o.h)
+        du, acc, who, utot, uinit = o.propagate(calls, cap=calls + 1)
+        assert int(acc.sum()) == r["accepted"]
+        assert np.max(np.abs(o.xyz(0) - r["xyz"])) <= 1e-9
+        say("  N=%d served in the mix: matches alone and the oracle (%d accepted)"
+            % (len(w["charge"]), r["accepted"]))
