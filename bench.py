@@ -41,19 +41,49 @@ FLOPS_PAIR_DHHS = 27                           # Debye-Hueckel + hard sphere per
 # (24 B) plus S(k) read+write on an accepted move (32 B).
 BYTES_PARTNER = {"mixed": 20, "fp64": 40}
 BYTES_K_READ, BYTES_K_ACCEPT = 24, 32
-# HBM traffic per launch (256 replicas x N=20,000, mixed, 512 threads), from
+# HBM-side traffic per launch (256 replicas x N=20,000, mixed path), from
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of one timed step, scaled by the
-# counters' reading for a 1 GiB copy (FETCH x2.0, WRITE x1.0), in bytes:
-# Ewald electrolyte (batch depth 4): 1,025,185,864 KiB fetched, 432,348 KiB
-# written; cut-LJ cell-list form: 64,251,548 KiB fetched, 564,152 KiB written.
-MEASURED_TRAFFIC = {("ewald", 20000, 256, "mixed"): 1024940990 * 2 * 1024 + 432348 * 1024,
-                    ("ewald-cells", 20000, 256, "mixed"): 64251548 * 2 * 1024 + 564152 * 1024}
+# counters' reading for a 1 GiB copy (FETCH x2.0, WRITE x1.0).  Valid only for
+# the kernel variant and schedule it was measured on; other schedules report
+# no traffic.
+MEASURED_TRAFFIC = {
+    ("ewald", 20000, 256, "mixed"): {
+        "bytes": 1024940990 * 2 * 1024 + 432348 * 1024,
+        "kernel": "k_sweep_at<3,2,true,4,true,false>", "threads": 512, "unroll": 4,
+        "source": "profiles/k_sweep_at_traffic_ewald_electrolyte_n20000.md"},
+    ("ewald-cells", 20000, 256, "mixed"): {
+        "bytes": 64251548 * 2 * 1024 + 564152 * 1024,
+        "kernel": "k_sweep_at<3,2,true,4,true,true>", "threads": 512, "unroll": 4,
+        "source": "profiles/k_sweep_at_traffic_ewald_electrolyte_n20000.md"},
+}
 
 
-def cpu_baseline(w, seconds_target=12.0):
+def traffic_for(key, schedule):
+    """measured traffic of a launch, when its schedule is the measured one"""
+    m = MEASURED_TRAFFIC.get(key)
+    if not m or not schedule or schedule.get("threads") != m["threads"] or \
+            schedule.get("unroll") != m["unroll"]:
+        return None, None
+    return m["bytes"], {k: m[k] for k in ("kernel", "threads", "unroll", "source")}
+
+
+def host_cores():
+    """host cores to use: the process's CPU set, at most 16 (the box's CPU
+    share per GPU)"""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(w, seconds_target=8.0, all_cores=True):
     """Time the reference (or its C restatement) on a bounded sample: single
-    trials (one trial per move() call) of the same system, one thread."""
+    trials (one trial per move() call) of the same system, one thread; then
+    C independent chains of the same sample, one per host core (pinned with
+    taskset where available), as their aggregate rate."""
     from beofaunus_amd import workloads
+    import shutil
     harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
     npart = len(w["charge"])
     what = "single-group TranslateRotate" if w["move"]["kind"] == 2 else "single-particle"
@@ -72,11 +102,30 @@ def cpu_baseline(w, seconds_target=12.0):
             out = json.loads(subprocess.run([harness, "mctime", w["ref_kind"], cfg, state,
                                              str(ncalls)], check=True, capture_output=True,
                                             text=True).stdout)
-            return {"value": out["trials_per_s"], "unit": "trial dU evals/s", "cores": 1,
-                    "kind": "reference",
-                    "sample": "%d %s trials of the same N=%d system "
-                              "(reference C++ hot path, g++ -O2, 1 thread; timed loop only)"
-                              % (int(out["trials"]), what, npart)}
+            res = {"value": out["trials_per_s"], "unit": "trial dU evals/s", "cores": 1,
+                   "kind": "reference",
+                   "sample": "%d %s trials of the same N=%d system (reference C++ hot path "
+                             "compiled with its Release flags -O3 -DNDEBUG -funroll-loops, 1 "
+                             "thread; timed loop only)" % (int(out["trials"]), what, npart)}
+            if all_cores:
+                C = host_cores()
+                ts = shutil.which("taskset")
+                cores = sorted(os.sched_getaffinity(0))[:C] if hasattr(os, "sched_getaffinity") \
+                    else list(range(C))
+                t0 = time.perf_counter()
+                procs = [subprocess.Popen(([ts, "-c", str(c)] if ts else []) +
+                                          [harness, "mctime", w["ref_kind"], cfg, state,
+                                           str(ncalls)], stdout=subprocess.PIPE, text=True)
+                         for c in cores]
+                outs = [json.loads(p.communicate()[0]) for p in procs]
+                wall = time.perf_counter() - t0
+                tr = sum(o["trials"] for o in outs)
+                res["all_cores"] = {
+                    "value": tr / max(o["seconds"] for o in outs), "cores": len(cores),
+                    "pinned": bool(ts), "wall_s": wall,
+                    "sample": "%d independent chains of the same sample, one per core"
+                              % len(cores)}
+            return res
     if w["move"]["kind"] != 1:
         return {"error": "reference binary absent; the C restatement baseline covers "
                          "AtomicTranslation workloads only"}
@@ -270,6 +319,7 @@ def bench_ewald(args, fgx, workloads, cut=False):
     bytes_trial = (BYTES_PARTNER[args.precision] * (partners + 2) + BYTES_K_READ * k
                    + acc_rate * BYTES_K_ACCEPT * k)
     gbs = bytes_trial * trials_per_launch / launch_s / 1e9
+    traffic = traffic_for(("ewald-cells" if cut else "ewald", n, R, args.precision), tuned)
     line = base_line(args, t["value"], t["wall"],
                      "f32 pairs + f64 accumulate" if prec == fgx.MIXED else "f64",
                      {"workload": "Ewald LJ+Coulomb electrolyte: N=%d, L=%.1f A, +-1e 50/50, LJ-LB "
@@ -287,8 +337,7 @@ def bench_ewald(args, fgx, workloads, cut=False):
         # algorithmic bytes; the flop rate against the vector peak beside it
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gbs / HBM_PEAK_GBS,
-                     "traffic": MEASURED_TRAFFIC.get(("ewald-cells" if cut else "ewald", n, R,
-                                                      args.precision)),
+                     "traffic": traffic[0], "traffic_measured_on": traffic[1],
                      "kernel": "k_sweep_at<%s>" % ("mixed" if prec == fgx.MIXED else "fp64"),
                      "bytes_per_trial": bytes_trial, "trials_per_launch": int(trials_per_launch),
                      "launch_ms": 1e3 * launch_s,
@@ -327,7 +376,91 @@ def bench_ewald(args, fgx, workloads, cut=False):
             if k in c:
                 line["cut_lj_cell_list"][k] = c[k]
         line["cut_lj_cell_list"]["launch_ms"] = c["roofline"]["launch_ms"]
+    if not cut and args.config == "ewald" and prec == fgx.MIXED and not args.no_sub:
+        headline_subconfigs(args, fgx, workloads, line)
     return line
+
+
+def headline_subconfigs(args, fgx, workloads, line):
+    """The other configurations' rates beside the headline, a few steps each:
+    the same electrolyte on the FP64 path, the rigid-group system (mixed with
+    and without the group-pair cache, FP64), 48-replica parallel tempering
+    (mixed, FP64) and a served job stream; then the mixed-over-FP64 target
+    under both readings of the configuration numbering."""
+    def sub(**kw):
+        a = argparse.Namespace(**vars(args))
+        a.no_cpu_baseline = True
+        a.no_sub = True
+        for k, v in kw.items():
+            setattr(a, k, v)
+        return a
+
+    def guarded(key, fn):
+        t0 = time.perf_counter()
+        try:
+            line[key] = fn()
+        except Exception as e:  # reported, never hidden
+            line[key] = {"error": str(e)}
+        line[key]["bench_seconds"] = time.perf_counter() - t0
+        print("[bench] %s done in %.1f s" % (key, line[key]["bench_seconds"]),
+              file=sys.stderr, flush=True)
+
+    def fp64_line():
+        f = bench_ewald(sub(precision="fp64", steps=2, warmup=0, no_error_check=True), fgx,
+                        workloads)
+        return {"value": f["value"], "sweep_latency_ms": f["sweep_latency_ms"],
+                "launch_ms": f["roofline"]["launch_ms"], "pairs_per_s": f["pairs_per_s"],
+                "steps": 2, "mixed_over_fp64": line["value"] / f["value"]}
+
+    def groups_line():
+        g = bench_groups(sub(steps=2, warmup=1, replicas=256), fgx, workloads)
+        out = {k: g[k] for k in ("value", "sweep_latency_ms", "pairs_per_s", "fp64",
+                                 "mixed_uncached", "speedup_vs_fp64", "du_error_vs_fp64")
+               if k in g}
+        out["workload"] = g["config"]["workload"]
+        out["roofline"] = g["roofline"]
+        if "mixed_uncached" in g:
+            out["uncached_over_fp64"] = g["mixed_uncached"]["value"] / g["fp64"]["value"]
+        return out
+
+    def temper_line():
+        t = bench_temper(sub(steps=2, warmup=1, exchange_every=10, ladders=6, rungs=8), fgx,
+                         workloads)
+        out = {k: t[k] for k in ("value", "pairs_per_s", "sweep_latency_ms", "exchange_acceptance",
+                                 "fp64", "speedup_vs_fp64", "ms_per_step")}
+        out["workload"] = t["config"]["workload"]
+        return out
+
+    def serve_line():
+        v = bench_serve(sub(jobs=64, job_sweeps=2, rate=50.0, hbm_mb=256.0, tick_ms=50.0,
+                            seed=5), fgx, workloads)
+        out = {k: v[k] for k in ("value", "pairs_per_s", "job_latency_s", "sweep_latency_ms",
+                                 "queue_wait_s", "peak_resident_mb", "failed_jobs", "wall_s")}
+        out["workload"] = v["config"]["workload"]
+        out["note"] = ("64 jobs x 2 sweeps each, not the 256 x 1,000 of the served-stream "
+                       "configuration (that stream is hours of device time)")
+        return out
+
+    guarded("fp64_same_workload", fp64_line)
+    guarded("rigid_groups", groups_line)
+    guarded("parallel_tempering", temper_line)
+    guarded("served_stream", serve_line)
+    tgt = {}
+    g, t = line.get("rigid_groups", {}), line.get("parallel_tempering", {})
+    if "speedup_vs_fp64" in g:
+        err = (g.get("du_error_vs_fp64") or {}).get("max_abs_kT")
+        tgt["rigid_groups (configs counted from 1: config 3)"] = {
+            "mixed_over_fp64": g["speedup_vs_fp64"], "meets_5x": g["speedup_vs_fp64"] >= 5.0,
+            "uncached_over_fp64": g.get("uncached_over_fp64"), "max_abs_du_err_kT": err,
+            "p99_sweep_ms_under_load": (t.get("sweep_latency_ms") or {}).get("p99"),
+            "load": "parallel tempering (config 4 counted from 1)"}
+    if "speedup_vs_fp64" in t:
+        tgt["parallel_tempering (configs counted from 0: config 3)"] = {
+            "mixed_over_fp64": t["speedup_vs_fp64"], "meets_5x": t["speedup_vs_fp64"] >= 5.0,
+            "p99_sweep_ms_under_load": (line.get("served_stream", {}).get("sweep_latency_ms")
+                                        or {}).get("p99"),
+            "load": "served stream (config 4 counted from 0)"}
+    line["target_5x_vs_fp64"] = tgt
 
 
 def bench_groups(args, fgx, workloads):
@@ -403,11 +536,13 @@ def bench_groups(args, fgx, workloads):
     return line
 
 
-def bench_temper(args, fgx, workloads):
-    """Batched parallel tempering: 48 replicas in 6 equal-N ladders, all
-    advanced by one launch of 10 sweeps, then replica exchange inside each
-    ladder.  Reports throughput and per-replica sweep latency."""
-    ladders = workloads.pt_ladders(nladder=args.ladders, rungs=args.rungs)
+def temper_run_stats(args, fgx, workloads, precision, rounds, sweeps):
+    """48 replicas (6 equal-N ladders of 8) advanced by fgx_temper_run: each
+    round is `sweeps` sweeps of every replica in one launch, then one exchange
+    attempt per ladder, captured once as a HIP graph and replayed."""
+    import torch
+    prec = fgx.MIXED if precision == "mixed" else fgx.FP64
+    ladders = workloads.pt_ladders(nladder=args.ladders, rungs=args.rungs, precision=prec)
     systems, tempers = [], []
     t_build = time.perf_counter()
     for l, lad in enumerate(ladders):
@@ -420,58 +555,62 @@ def bench_temper(args, fgx, workloads):
         systems += reps
         tempers.append(fgx.Temper(reps, ctl_seed=11 + l))
     t_build = time.perf_counter() - t_build
-    sweeps = args.exchange_every
     tuned = fgx.autotune_batch(systems, 1) if args.warmup > 0 else None
-    for _ in range(max(0, args.warmup - 1)):
-        fgx.propagate_batch(systems, sweeps)
-    import torch
+    fgx.temper_run(tempers, 1, 1)               # warm-up round (graph capture path)
     torch.cuda.synchronize()
-    lat, dev = [], []
-    trials = accepted = pairs = 0
-    ex_att = ex_acc = 0
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        st = fgx.propagate_batch(systems, sweeps)
-        dev.append(st["seconds"])
-        trials += st["trials"]
-        accepted += st["accepted"]
-        for s in systems:
-            lat.append(s.chain_seconds() / sweeps)
-        for t in tempers:
-            acc, _ = t.exchange()
-            ex_acc += int(acc.sum()) // 2
-            ex_att += len(acc) // 2
+    st, dev_s = fgx.temper_run(tempers, rounds, sweeps)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    for lad in ladders:
-        pairs += 2 * (lad["n"] - 1) * lad["n"] * len(lad["replicas"]) * sweeps * args.steps
-    line = base_line(args, trials / wall, wall, "f32 pairs + f64 accumulate",
-                     {"workload": "batched parallel tempering: %d ladders x %d replicas, N "
-                                  "log-uniform 1000-16000 (%s), salt at 500 ions / (60 A)^3, "
-                                  "Debye-Hueckel 10 A + hard spheres 2 A, lB 7.1 A at 298.15 K "
-                                  "scaled 1/T, T geometric 280-420 K, AtomicTranslation dp 2 A, "
-                                  "exchange every %d sweeps"
-                                  % (len(ladders), args.rungs,
-                                     ",".join(str(l["n"]) for l in ladders), sweeps),
-                      "replicas": len(systems), "sweeps_per_step": sweeps,
-                      "precision": "mixed"})
-    line.update({
-        "pairs_per_s": pairs / wall,
-        "acceptance": accepted / max(1, trials),
-        "sweep_latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)),
-                             "p99": 1e3 * float(np.percentile(lat, 99)),
-                             "max": 1e3 * float(np.max(lat))},
-        "launch_ms": {"p50": 1e3 * float(np.percentile(dev, 50)),
-                      "p99": 1e3 * float(np.percentile(dev, 99))},
-        "exchange_acceptance": ex_acc / max(1, ex_att),
-        "exchange_ms_per_step": 1e3 * (wall - float(np.sum(dev))) / args.steps,
-        "build_seconds": t_build,
-        "schedule": tuned,
-    })
+    lat, ex_att, ex_acc = [], 0, 0
+    for t in tempers:
+        acc, du, slot, ssec = t.records()
+        lat.extend((ssec / sweeps).reshape(-1).tolist())
+        for row_a, row_d in zip(acc, du):
+            ex_att += int(np.isfinite(row_d).sum()) // 2
+            ex_acc += int(row_a.sum()) // 2
+    pairs = sum(2 * (lad["n"] - 1) * lad["n"] * len(lad["replicas"]) * sweeps * rounds
+                for lad in ladders)
     for t in tempers:
         t.close()
     for s in systems:
         s.close()
+    return {"value": st["trials"] / wall, "wall": wall, "device_s": dev_s,
+            "trials": st["trials"], "accepted": st["accepted"], "pairs_per_s": pairs / wall,
+            "sweep_latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)),
+                                 "p99": 1e3 * float(np.percentile(lat, 99)),
+                                 "max": 1e3 * float(np.max(lat))},
+            "round_ms": 1e3 * wall / rounds,
+            "exchange_acceptance": ex_acc / max(1, ex_att), "exchange_attempts": ex_att,
+            "build_seconds": t_build, "schedule": tuned,
+            "ladders": [l["n"] for l in ladders]}
+
+
+def bench_temper(args, fgx, workloads):
+    """Batched parallel tempering: 48 replicas in 6 equal-N ladders, each
+    round one launch of `exchange_every` sweeps of all replicas and an
+    exchange attempt in every ladder, replayed as one HIP graph; the mixed
+    path, and the FP64 path for the same rounds."""
+    sweeps = args.exchange_every
+    m = temper_run_stats(args, fgx, workloads, "mixed", args.steps, sweeps)
+    f = temper_run_stats(args, fgx, workloads, "fp64", max(1, args.steps // 2), sweeps)
+    line = base_line(args, m["value"], m["wall"], "f32 pairs + f64 accumulate",
+                     {"workload": "batched parallel tempering: %d ladders x %d replicas, N "
+                                  "log-uniform 1000-16000 (%s), salt at 500 ions / (60 A)^3, "
+                                  "Debye-Hueckel 10 A + hard spheres 2 A, lB 7.1 A at 298.15 K "
+                                  "scaled 1/T, T geometric 280-420 K, AtomicTranslation dp 2 A, "
+                                  "exchange every %d sweeps, rounds replayed as one HIP graph"
+                                  % (args.ladders, args.rungs,
+                                     ",".join(str(n) for n in m["ladders"]), sweeps),
+                      "replicas": args.ladders * args.rungs, "sweeps_per_step": sweeps,
+                      "precision": "mixed"})
+    line["ms_per_step"] = m["round_ms"]
+    line.update({k: m[k] for k in ("pairs_per_s", "sweep_latency_ms", "exchange_acceptance",
+                                   "exchange_attempts", "build_seconds", "schedule")})
+    line["acceptance"] = m["accepted"] / max(1, m["trials"])
+    line["fp64"] = {k: f[k] for k in ("value", "pairs_per_s", "sweep_latency_ms", "round_ms",
+                                      "exchange_acceptance")}
+    line["speedup_vs_fp64"] = m["value"] / f["value"]
     return line
 
 
@@ -645,6 +784,8 @@ def main():
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error-check", action="store_true")
+    ap.add_argument("--no-sub", action="store_true",
+                    help="headline only: skip the other configurations' sub-results")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; replicas of the Markov chain are batched "
