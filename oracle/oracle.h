@@ -18,8 +18,12 @@
  *   - Cuboid geometry, pair potentials, Nonbonded sums, energyChange,
  *     AtomicTranslation / TranslateRotate trajectories, NonbondedEwald:
  *     pinned by golden vectors of the compiled reference (oracle/_ref),
- *     by closed forms (lB known answers, 27-image brute force) and by the
- *     Ewald 4-ion known answer of doi:10.1063/1.481216.
+ *     including runs at the benchmark configurations' own sizes, by closed
+ *     forms (lB known answers; the minimum image against the brute-force
+ *     minimum over 27 images, tests/test_oracle_known_answers.py) and by
+ *     the Ewald ion pair of the reference's unit test
+ *     (src/examples/unittests.cpp:131-175, doi:10.1063/1.481216) at
+ *     kappa -> 0.
  *   - ParallelTempering exchange: no reference run (the reference needs
  *     MPI, which is absent).  Pinned by hand values of _energyChange's sign
  *     and partner terms, by the stationary occupancy of a two-state toy
