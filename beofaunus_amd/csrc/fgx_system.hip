@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "fgx_device.hpp"
@@ -359,12 +360,35 @@ static int ensure_idx(fgx_system *s, int n)
     return FGX_OK;
 }
 
-// run a sum kernel configuration into out slot, optionally fetch to host
-static int finish_sum(fgx_system *s, int nblocks, int slot)
+// fixed-order sum of `nblocks` partials into out slot
+static int finish_sum(fgx_system *s, int nblocks, int slot, const double *partial = nullptr)
 {
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(RB), 0, s->stream, s->d_partial, nblocks, s->d_out,
-                       slot);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(RB), 0, s->stream,
+                       partial ? partial : s->d_partial, nblocks, s->d_out, slot);
     FGX_HIP(hipGetLastError());
+    return FGX_OK;
+}
+
+// Where a pair kernel writes its per-block partial sums: the system's own
+// buffer, grown on demand, or a fixed buffer of `cap` doubles that is never
+// grown (the allocation-free entry point's scratch).
+struct FixedPartials {
+    double *buf;
+    long cap;
+};
+
+static int partials_for(fgx_system *s, int nblocks, const FixedPartials *fixed, double **out)
+{
+    if (fixed) {
+        FGX_CHECK(nblocks <= fixed->cap, FGX_ENOMEM,
+                  "allocation-free call: the system needs more partial sums than the fixed "
+                  "scratch holds (particle count too large)");
+        *out = fixed->buf;
+        return FGX_OK;
+    }
+    int rc = ensure_partials(s, nblocks);
+    if (rc) return rc;
+    *out = s->d_partial;
     return FGX_OK;
 }
 
@@ -396,7 +420,8 @@ static int row_sum(fgx_system *s, int which, int ia, int j0, int j1, int skip, i
 }
 
 static int block_sum_launch(fgx_system *s, int which, int a0, int a1, int b0, int b1, int tri,
-                            const unsigned char *mask, int slot)
+                            const unsigned char *mask, int slot,
+                            const FixedPartials *fixed = nullptr)
 {
     if (a1 <= a0 || b1 <= b0) {
         FGX_HIP(hipMemsetAsync(s->d_out + slot, 0, sizeof(double), s->stream));
@@ -404,12 +429,13 @@ static int block_sum_launch(fgx_system *s, int which, int a0, int a1, int b0, in
     }
     dim3 g((b1 - b0 + RB - 1) / RB, (a1 - a0 + TILE - 1) / TILE);
     int nb = g.x * g.y;
-    int rc = ensure_partials(s, nb);
+    double *partial = nullptr;
+    int rc = partials_for(s, nb, fixed, &partial);
     if (rc) return rc;
     hipLaunchKernelGGL(k_block, g, dim3(RB), 0, s->stream, s->box, s->pot, s->d_tab,
-                       pv(s, which), a0, a1, b0, b1, tri, mask, s->ng, s->d_partial);
+                       pv(s, which), a0, a1, b0, b1, tri, mask, s->ng, partial);
     FGX_HIP(hipGetLastError());
-    return finish_sum(s, nb, slot);
+    return finish_sum(s, nb, slot, partial);
 }
 
 int ewald_full_recompute(fgx_system *s, int which, double *Qdst)
@@ -1322,10 +1348,12 @@ extern "C" int fgx_external(fgx_system *s, int which, double *u)
 
 // Energy::systemEnergy (energy.h:3486-3491): setSpace, then Energybase::
 // systemEnergy (energy.h:169-180) = external + sum_g (g_external + g_internal)
-// + sum_{g<h} g2g.  Pair sums run as one masked i<j sweep.
-extern "C" int fgx_system_energy(fgx_system *s, int which, double *u)
+// + sum_{g<h} g2g.  Pair sums run as one masked i<j sweep.  `fixed` and
+// `mask_buf` (mask_cap bytes): the allocation-free form's scratch; null
+// pointers take the system's own buffers.
+static int system_energy_impl(fgx_system *s, int which, double *u, const FixedPartials *fixed,
+                              unsigned char *mask_buf, size_t mask_cap)
 {
-    FGX_CHECK(check_which(s, which) && u, FGX_EARG, "fgx_system_energy: bad arguments");
     int rc;
     if (s->has_ewald && (rc = ewald_set_space(s))) return rc;
     double ext = 0.0;
@@ -1346,16 +1374,82 @@ extern "C" int fgx_system_energy(fgx_system *s, int which, double *u)
         for (int a = 0; a < s->ng; ++a)
             for (int b = 0; b < s->ng; ++b)
                 if (a != b && cut_host(s, a, which, b, which)) hm[(size_t)a * s->ng + b] = 0;
-        FGX_HIP(hipMalloc(&dm, hm.size()));
-        FGX_HIP(hipMemcpyAsync(dm, hm.data(), hm.size(), hipMemcpyHostToDevice, s->stream));
-        mask = dm;
+        if (fixed) {
+            FGX_CHECK(hm.size() <= mask_cap, FGX_ENOMEM,
+                      "allocation-free call: too many groups for the fixed group-pair mask");
+            FGX_HIP(hipMemcpyAsync(mask_buf, hm.data(), hm.size(), hipMemcpyHostToDevice,
+                                   s->stream));
+            mask = mask_buf;
+        } else {
+            FGX_HIP(hipMalloc(&dm, hm.size()));
+            FGX_HIP(hipMemcpyAsync(dm, hm.data(), hm.size(), hipMemcpyHostToDevice, s->stream));
+            mask = dm;
+        }
     }
-    rc = block_sum_launch(s, which, 0, s->n, 0, s->n, 1, mask, 0);
+    rc = block_sum_launch(s, which, 0, s->n, 0, s->n, 1, mask, 0, fixed);
     if (!rc) rc = fetch(s, 1);
     if (dm) (void)hipFree(dm);
     if (rc) return rc;
     *u = ext + gext + s->h_out[0];
     return FGX_OK;
+}
+
+extern "C" int fgx_system_energy(fgx_system *s, int which, double *u)
+{
+    FGX_CHECK(check_which(s, which) && u, FGX_EARG, "fgx_system_energy: bad arguments");
+    return system_energy_impl(s, which, u, nullptr, nullptr, 0);
+}
+
+// ------------------------------------------------------------------------
+// Allocation-free Energy::systemEnergy.  Its pair kernels write their
+// partial sums, and NonbondedCutg2g its group-pair mask, into this file-scope
+// device array instead of buffers the library allocates.  Every GPU holds its
+// own copy for the life of the process; its address on each device is looked
+// up once (hipGetSymbolAddress with that device current) and only that
+// address is handed to kernels and copies.  Calls on one GPU share the array:
+// calls that overlap on one GPU race.  Everything a call reads from it is
+// written by that call first.
+// ------------------------------------------------------------------------
+static constexpr long NOALLOC_PARTIALS = 1L << 18;   // FP64 partial sums (2 MiB)
+static constexpr size_t NOALLOC_MASK = 1u << 16;     // group-pair mask bytes (ng <= 256)
+static __device__ __align__(256) unsigned char scratch[NOALLOC_PARTIALS * 8 + NOALLOC_MASK];
+
+static constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// the array's device address on `device` (looked up once per device)
+static int scratch_on(int device, unsigned char **out)
+{
+    FGX_CHECK(device >= 0 && device < NOALLOC_MAX_DEVICES, FGX_EARG,
+              "allocation-free call: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        int cur = 0;
+        FGX_HIP(hipGetDevice(&cur));
+        if (cur != device) FGX_HIP(hipSetDevice(device));
+        void *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (cur != device) FGX_HIP(hipSetDevice(cur));
+        if (e != hipSuccess || !ptr) {
+            set_error(std::string("allocation-free call: hipGetSymbolAddress failed: ") +
+                      hipGetErrorString(e));
+            return FGX_EHIP;
+        }
+        g_scratch_ptr[device] = ptr;
+    }
+    *out = (unsigned char *)g_scratch_ptr[device];
+    return FGX_OK;
+}
+
+extern "C" int fgx_system_energy_noalloc(fgx_system *s, int which, double *u)
+{
+    FGX_CHECK(check_which(s, which) && u, FGX_EARG, "fgx_system_energy_noalloc: bad arguments");
+    unsigned char *base = nullptr;
+    int rc = scratch_on(s->device, &base);
+    if (rc) return rc;
+    const FixedPartials fixed{(double *)base, NOALLOC_PARTIALS};
+    return system_energy_impl(s, which, u, &fixed, base + NOALLOC_PARTIALS * 8, NOALLOC_MASK);
 }
 
 // ------------------------------------------------------------- Delta U

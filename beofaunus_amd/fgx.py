@@ -113,6 +113,7 @@ def lib():
         "fgx_g_external": (I, [P, I, I, _dp]),
         "fgx_external": (I, [P, I, _dp]),
         "fgx_system_energy": (I, [P, I, _dp]),
+        "fgx_system_energy_noalloc": (I, [P, I, _dp]),
         "fgx_update_change": (I, [P, C.POINTER(Change)]),
         "fgx_energy_change": (I, [P, C.POINTER(Change), _dp]),
         "fgx_commit": (I, [P, C.POINTER(Change), I, _dp]),
@@ -283,6 +284,7 @@ class System:
     def i_external(self, i, which=0): return self._e("fgx_i_external", which, i)
     def external(self, which=0): return self._e("fgx_external", which)
     def system_energy(self, which=0): return self._e("fgx_system_energy", which)
+    def system_energy_noalloc(self, which=0): return self._e("fgx_system_energy_noalloc", which)
 
     @property
     def device_bytes(self):

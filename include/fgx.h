@@ -123,6 +123,14 @@ int fgx_i_external(fgx_system *s, int which, int i, double *u);        /* ewald.
 int fgx_external(fgx_system *s, int which, double *u);                 /* ewald.h:716 */
 /* Energy::systemEnergy (energy.h:3486-3491 -> 169-180) */
 int fgx_system_energy(fgx_system *s, int which, double *u);
+/* fgx_system_energy without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops).  No
+ * counterpart in the reference.  Its kernels' scratch (partial sums and the
+ * NonbondedCutg2g group-pair mask) is one static device array per GPU of
+ * 2 MiB + 64 KiB: systems needing more than 2^18 partial sums (about 65,000
+ * particles) or more than 256 groups under NonbondedCutg2g get FGX_ENOMEM.
+ * Calls on one GPU share the array, so calls that overlap on one GPU race. */
+int fgx_system_energy_noalloc(fgx_system *s, int which, double *u);
 
 /* Analysis::AtomRDF sampling (analysis.h:2117-2148): pair counts of types
  * {type_a, type_b} over i < j at bin int(r/dr + 0.5), r the minimum-image
