@@ -194,9 +194,21 @@ def read_moves(moves, molnames):
             out.append({"type": "NPT", "molid": -1, "prob": float(m.get("prob", 1.0)),
                         "peratom": 0, "permol": 0, "dp": float(m["dp"]),
                         "pressure_mM": float(m["pressure"])})
-        elif key != "random":   # moves.random configures the engine (move.h:7783-7787)
+        elif key not in ("random", "_jsonfile"):   # moves.random: the engine (below)
             other.append(key)
     return out, other
+
+
+def move_engine_random(moves_cfg):
+    """moves.random (move.h:7783-7787): None without it; else whether it asks
+    for a hardware seed.  Without one the Propagator's move engine starts
+    from mt19937's default seed instead of a copy of the global engine (the
+    supported move keys all sort before "random", so no later key copies the
+    global engine back)."""
+    r = moves_cfg.get("random")
+    if not isinstance(r, dict):
+        return None
+    return {"hardware": bool(r.get("hardware", False))}
 
 
 def mM_to_per_A3(c):
@@ -274,6 +286,7 @@ def read_input(json_path, state_path, *, elec, sr, nbkind=fgx.NB_PLAIN, precisio
         # keys in sorted order), None without one
         "npt_index": npt_index,
         "other_moves": other,
+        "move_random": move_engine_random(cfg.get("moves", {})),
         "atom_names": names,
         "molecule_names": molnames,
         "rng": st["rng"],
@@ -308,6 +321,17 @@ def run_input(json_path, state_path, out_state, ncalls, *, elec, sr, nbkind=fgx.
             words, pos = mt_parse(w["rng"])
             s.rng_set_state(words, pos)
         s.propagator_init()
+        if w["move_random"] is not None:
+            if w["move_random"]["hardware"]:
+                raise ValueError("%s: moves.random.hardware asks for a non-deterministic seed "
+                                 "(slump.h:50-57); a replayable device run needs a fixed one"
+                                 % json_path)
+            # the move engine from mt19937's default seed, the global engine
+            # as the state file left it
+            gw, gp = s.rng_state()
+            s.rng_seed(5489)
+            s.propagator_init()
+            s.rng_set_state(gw, gp)
         if npt_only:   # a Propagator holding one Isobaric move
             st = s.isobaric_propagate(iso[0]["dp"], iso[0]["prob"], ncalls)[0]
         elif iso:      # the registered move and an Isobaric move

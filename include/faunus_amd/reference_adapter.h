@@ -480,8 +480,24 @@ public:
         gpu.setSpace(s);
         auto m = in.at("moves");
         int pos = 0;
+        bool reseed = false;   // moves.random (move.h:7783-7787)
         for (auto i = m.begin(); i != m.end(); ++i) {
             bool at = i.key() == "atomtranslate", tr = i.key() == "moltransrot";
+            if (i.key() == "_jsonfile") continue;
+            if (i.key() == "random") {
+                if (!i.value().is_object()) continue;
+                if (i.value().value("hardware", false))
+                    throw std::runtime_error("GPUPropagator: moves.random.hardware asks for a "
+                                             "non-deterministic seed; the device run replays "
+                                             "fixed engines only");
+                reseed = true;
+                continue;
+            }
+            // every other key the reference registers takes part in the
+            // Propagator's pick (move.h:7788-7843); the device runs none of them
+            if (!at && !tr && i.key() != "isobaric")
+                throw std::runtime_error("GPUPropagator: move '" + i.key() +
+                                         "' has no device implementation");
             if (i.key() == "isobaric") {
                 npt_index = pos++;
                 npt_dp = i.value().at("dp");
@@ -509,6 +525,15 @@ public:
         }
         pushGlobalEngine();
         AMD::check(fgx_propagator_init(gpu.handle()));
+        if (reseed) {
+            // the move engine from mt19937's default seed (RandomTwister(json)
+            // without "hardware"); the supported keys sort before "random", so
+            // no later key copies the global engine back; then the global
+            // engine as the host holds it
+            AMD::check(fgx_rng_seed(gpu.handle(), 5489u));
+            AMD::check(fgx_propagator_init(gpu.handle()));
+            pushGlobalEngine();
+        }
     }
 
     void pushGlobalEngine()

@@ -164,6 +164,10 @@ static std::vector<std::unique_ptr<Move::Movebase<Tspace>>> make_moves(Tmjson &i
     auto m = in.at("moves");
     for (auto i = m.begin(); i != m.end(); ++i) {
         Recording<Move::AtomicTranslation<Tspace>>::internal().eng = slump.eng;
+        if (i.key() == "random" && i.value().is_object()) {   // move.h:7783-7787
+            auto v = i.value();
+            Recording<Move::AtomicTranslation<Tspace>>::internal() = RandomTwister<>(v);
+        }
         if (i.key() == "atomtranslate") mv.emplace_back(new Tat(pot, spc, i.value(), rec));
         if (i.key() == "isobaric") mv.emplace_back(new Tiso(pot, spc, i.value(), rec));
         if (i.key() == "moltransrot") mv.emplace_back(new Ttr(pot, spc, i.value(), rec));

@@ -378,6 +378,10 @@ static int run_npt_prop(Tmjson &in, const char *state, int ncalls)
     auto m = in.at("moves");
     for (auto i = m.begin(); i != m.end(); ++i) {
         RecAT<Tspace>::internal().eng = slump.eng;   // move.h:7781
+        if (i.key() == "random" && i.value().is_object()) {   // move.h:7783-7787
+            auto v = i.value();
+            RecAT<Tspace>::internal() = RandomTwister<>(v);
+        }
         if (i.key() == "atomtranslate") mv.emplace_back(new RecAT<Tspace>(pot, spc, i.value(), &rec));
         if (i.key() == "isobaric") mv.emplace_back(new RecIso<Tspace>(pot, spc, i.value(), &rec));
         if (i.key() == "moltransrot") mv.emplace_back(new RecTR<Tspace>(pot, spc, i.value(), &rec));

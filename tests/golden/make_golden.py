@@ -158,9 +158,12 @@ def salt(name, kind, n, L, seed, ncalls, elec_cfg, dp=2.0, r=2.0):
     return build_fixture(name, kind, cfg, [L, L, L], parts, groups, seed, ncalls, moves)
 
 
-def salt_npt(name, n, L, seed, ncalls, pressure_mM, dp, r=2.0):
+def salt_npt(name, n, L, seed, ncalls, pressure_mM, dp, r=2.0, with_at_and_random=False):
     """Salt with Debye-Hueckel + hard spheres and the ExternalPressure term,
-    a Propagator holding one Isobaric move (atomic salt group scaled)."""
+    a Propagator holding one Isobaric move (atomic salt group scaled); or,
+    with_at_and_random, a Propagator holding AtomicTranslation and Isobaric
+    whose move engine the input's moves.random object reseeds
+    (move.h:7783-7787: a default-seeded mt19937 without "hardware")."""
     rng = np.random.default_rng(seed)
     radii = np.full(n, r)
     xyz = rsa(rng, n, L, radii)
@@ -179,6 +182,12 @@ def salt_npt(name, n, L, seed, ncalls, pressure_mM, dp, r=2.0):
     groups = [{"front": 0, "back": n - 1, "molid": 0, "molsize": 1}]
     moves = [{"type": "NPT", "molid": -1, "prob": 1.0, "peratom": 0, "permol": 0, "dp": dp,
               "pressure_mM": pressure_mM}]
+    if with_at_and_random:
+        cfg["moves"]["atomtranslate"] = {"salt": {"peratom": True}}
+        cfg["moves"]["random"] = {}
+        moves = [{"type": "AT", "molid": 0, "prob": 1.0, "peratom": 1, "permol": 0}] + moves
+        return build_fixture(name, "dhhs_npt", cfg, [L, L, L], parts, groups, seed, ncalls,
+                             moves, mc_mode="mcprop")
     return build_fixture(name, "dhhs_npt", cfg, [L, L, L], parts, groups, seed, ncalls, moves)
 
 
@@ -467,6 +476,9 @@ ONLY = {
     # rigid molecules and ions: one Propagator picking TranslateRotate or Isobaric
     "mol_npt_mixed": lambda: molecules("mol_npt_mixed", "dhhs_npt", 4, 12, 16, 60.0, 27, 30,
                                        isobaric={"dp": 0.3, "pressure": 150.0}),
+    # AtomicTranslation + Isobaric with moves.random reseeding the move engine
+    "salt_npt_random": lambda: salt_npt("salt_npt_random", 50, 30.0, 28, 30, pressure_mM=300.0,
+                                        dp=0.4, with_at_and_random=True),
     # one Propagator picking AtomicTranslation or Isobaric per call
     "ewald_npt_mixed": lambda: ewald_npt("ewald_npt_mixed", 40, 20.0, 25, 40, 0,
                                          pressure_mM=6000.0, dp=0.4, eps_surf=0.5,

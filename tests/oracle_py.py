@@ -135,6 +135,26 @@ def iptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_int))
 
 
+def random_reseeds_move_engine(cfg):
+    """True when the input's moves.random object reseeds the Propagator's move
+    engine (move.h:7783-7787): a default-seeded mt19937 unless "hardware",
+    which no fixture uses"""
+    r = cfg.get("moves", {}).get("random")
+    return isinstance(r, dict) and not r.get("hardware", False)
+
+
+def init_propagator(seed_fn, init_fn, fx):
+    """Propagator construction for a fixture: the move engine copies the
+    global one (move.h:7781), or, with moves.random, starts from the default
+    seed while the global engine keeps the fixture's seed"""
+    seed_fn(fx["rng_seed"])
+    init_fn()
+    if random_reseeds_move_engine(fx["config"]):
+        seed_fn(5489)
+        init_fn()
+        seed_fn(fx["rng_seed"])
+
+
 def load_fixture(name):
     with open(os.path.join(GOLDEN, name + ".json")) as f:
         return json.load(f)

@@ -71,7 +71,8 @@ def test_reference_moves_on_gpu_energy(name):
     assert np.allclose(np.array(r["C_xyz"]), a, rtol=0, atol=1e-9)
 
 
-@pytest.mark.parametrize("name", ["salt_npt", "ewald_npt_mixed", "mol_npt_mixed"])
+@pytest.mark.parametrize("name", ["salt_npt", "ewald_npt_mixed", "mol_npt_mixed",
+                                  "salt_npt_random"])
 def test_reference_isobaric_on_gpu_energy(name):
     """The reference's Isobaric (and AtomicTranslation) moves in one
     Propagator: A the reference itself (equal to the fixture), B the

@@ -174,13 +174,17 @@ def test_cell_list_variants(gpu, case):
 
 def test_cell_list_across_volume_moves(gpu):
     """A Propagator holding AtomicTranslation and Isobaric on the cut-LJ
-    electrolyte: after every accepted volume move the cell grid, the records
-    and the fixed-point copies follow the new box.  Cell list and all-pairs
-    screen give the same accepts and dU to rounding."""
+    electrolyte, then translation sweeps in the box the volume moves left:
+    after accepted volume moves the cell grid, the records and the
+    fixed-point copies follow the new box.  Cell list and all-pairs screen
+    give the same accepts and dU to rounding in both phases; the box did
+    change and volume trials did run."""
     from beofaunus_amd import workloads, faunus_io
     w = electrolyte(2000, seed=47)
+    L0 = w["box"][0]
     w["ham"] = dict(w["ham"], pressure=faunus_io.mM_to_per_A3(20000.0))
-    say("[cells] N=2000 with volume moves, L=%.2f" % w["box"][0])
+    say("[cells] N=2000 with volume moves, L=%.2f" % L0)
+    ncalls, n = 12, len(w["charge"])
     out = {}
     for mode in (1, 0):
         s = workloads.make_system(w, precision=gpu.MIXED)
@@ -188,16 +192,23 @@ def test_cell_list_across_volume_moves(gpu):
             s.set_cell_list(mode)
             s.rng_seed(27)
             s.propagator_init()
-            st, du, acc = s.propagate_npt(0.05, 1.0, 1, 12, record=100000)
-            out[mode] = (du, acc, s.box()[0], s.cell_list())
+            st, du, acc = s.propagate_npt(0.05, 1.0, 1, ncalls, record=100000)
+            L1 = s.box()[0]
+            st2, du2, acc2 = s.propagate(2, record=2 * n)
+            out[mode] = (du, acc, L1, du2, acc2, s.cell_list())
         finally:
             s.close()
-    (dc, ac, Lc, cells), (dp, ap, Lp, _) = out[1], out[0]
-    nvol = int((np.abs(dc[np.isfinite(dc)]) > 0).sum())
-    say("  cell list %s, trials %d, final L %.4f / %.4f" % (cells, len(dc), Lc, Lp))
+    (dc, ac, Lc, dc2, ac2, cells), (dp, ap, Lp, dp2, ap2, _) = out[1], out[0]
+    # each call is one volume trial or one sweep of n translation trials
+    nvol = (ncalls * n - len(dc)) // (n - 1)
+    say("  cell list %s, trials %d (%d volume), L %.4f -> %.4f" % (cells, len(dc), nvol, L0, Lc))
+    assert 1 <= nvol < ncalls and len(dc) == (ncalls - nvol) * n + nvol
+    assert Lc != L0              # a volume move was accepted
     assert cells[0] >= 3
     assert np.array_equal(ac, ap) and Lc == Lp
     fin = np.isfinite(dp)
     assert np.array_equal(np.isfinite(dc), fin)
     assert np.max(np.abs(dc[fin] - dp[fin])) <= 1e-9
-    assert len(dc) > 2000   # translation sweeps and volume trials both ran
+    # translation sweeps in the changed box: the rebuilt list against the screen
+    assert len(dc2) == 2 * n and np.array_equal(ac2, ap2)
+    assert np.max(np.abs(dc2 - dp2)) <= 1e-9

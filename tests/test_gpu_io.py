@@ -35,7 +35,7 @@ def gpu():
 
 @pytest.mark.parametrize("name", ["salt_dhhs", "lj_cutshift", "ewald_tinfoil", "mol_dhhs",
                                   "mol_dhhs_offcenter", "salt_npt", "ewald_npt_mixed",
-                                  "mol_npt_mixed"])
+                                  "mol_npt_mixed", "salt_npt_random"])
 def test_run_from_reference_files(gpu, name):
     from beofaunus_amd import faunus_io
     fx = load_fixture(name)

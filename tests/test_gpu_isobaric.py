@@ -16,7 +16,7 @@ import sys
 import numpy as np
 import pytest
 
-from oracle_py import farr, load_fixture
+from oracle_py import init_propagator, farr, load_fixture
 
 pytestmark = pytest.mark.gpu
 
@@ -89,7 +89,7 @@ def test_isobaric_refuses_missing_pressure(gpu):
             s.close()
 
 
-@pytest.mark.parametrize("name", ["ewald_npt_mixed", "mol_npt_mixed"])
+@pytest.mark.parametrize("name", ["ewald_npt_mixed", "mol_npt_mixed", "salt_npt_random"])
 def test_mixed_propagator_matches_reference(gpu, name):
     """One Propagator holding a translation move and Isobaric: AtomicTranslation
     on an Ewald salt (ewald_npt_mixed), or TranslateRotate of rigid molecules
@@ -106,8 +106,7 @@ def test_mixed_propagator_matches_reference(gpu, name):
     say("[isobaric] %s: %d Propagator calls over %s" % (name, mc["ncalls"], keys))
     s = fgx_system_from_fixture(fx)
     try:
-        s.rng_seed(fx["rng_seed"])
-        s.propagator_init()
+        init_propagator(s.rng_seed, s.propagator_init, fx)
         st, du, acc = s.propagate_npt(mv["dp"], mv["prob"], keys.index("isobaric"),
                                       mc["ncalls"], record=100000)
         box = s.box()
@@ -132,7 +131,7 @@ def test_mixed_propagator_matches_reference(gpu, name):
     assert u1 == pytest.approx(mc["ufinal"], rel=1e-10)
 
 
-@pytest.mark.parametrize("name", ["ewald_npt_mixed", "mol_npt_mixed"])
+@pytest.mark.parametrize("name", ["ewald_npt_mixed", "mol_npt_mixed", "salt_npt_random"])
 def test_mixed_propagator_mixed_precision(gpu, name):
     """The same Propagators with the mixed path for the translation calls
     (for the molecules: the TranslateRotate sweep with its group-pair energy
@@ -145,8 +144,7 @@ def test_mixed_propagator_mixed_precision(gpu, name):
     keys = sorted(k for k in fx["config"]["moves"])
     s = fgx_system_from_fixture(fx, precision=gpu.MIXED)
     try:
-        s.rng_seed(fx["rng_seed"])
-        s.propagator_init()
+        init_propagator(s.rng_seed, s.propagator_init, fx)
         st, du, acc = s.propagate_npt(mv["dp"], mv["prob"], keys.index("isobaric"),
                                       mc["ncalls"], record=100000)
     finally:

@@ -11,7 +11,7 @@ import os
 import numpy as np
 import pytest
 
-from oracle_py import fnum, GOLDEN, OracleSystem, farr, lib, load_fixture, dptr
+from oracle_py import init_propagator, fnum, GOLDEN, OracleSystem, farr, lib, load_fixture, dptr
 
 FIXTURES = sorted(os.path.basename(f)[:-5] for f in glob.glob(os.path.join(GOLDEN, "*.json"))
                   if not f.endswith("rng.json"))
@@ -146,8 +146,7 @@ def test_mixed_propagator_matches_reference(name):
     mc = fx["mc"]
     mv = [m for m in fx["moves"] if m["type"] == "NPT"][0]
     s = OracleSystem.from_fixture(fx)
-    s.seed(fx["rng_seed"])
-    lib().or_propagator_init(s.h)
+    init_propagator(s.seed, lambda: lib().or_propagator_init(s.h), fx)
     du, acc, who, utot = s.propagate_npt(mv["dp"], mv["prob"], npt_index(fx), mc["ncalls"])
     assert (who == -1).sum() > 0 and (who >= 0).sum() > 0
     assert np.array_equal(who, np.array(mc["who"], dtype=np.int32))
