@@ -55,6 +55,10 @@ struct EwaldDev {
     double *cheb = nullptr;
     int ncheb = 0;
     double cheb_xmax = 0;
+    // device sweeps with more k-vectors than LDS holds keep the trial's
+    // dS(k) (2K doubles) and the packed n-triples (K ints) here
+    double *sk_dS = nullptr;
+    int *sk_kid = nullptr;
 };
 
 static constexpr int EW_CHEB_MAX = 48;

@@ -72,12 +72,13 @@ __global__ __launch_bounds__(256) void k_cell_build(const ChainDev *__restrict__
 // AtomicTranslation sweep launch for the chains' Hamiltonian
 template <bool MIX>
 static int launch_at(const Pot &P, hipStream_t st, const ChainDev *d_chains, int nchains, int nthr,
-                     size_t dyn, int ncalls, int unroll, bool fx, bool cell)
+                     size_t dyn, int ncalls, int unroll, bool fx, bool cell, bool skg)
 {
     int rc;
     if (P.elec == ELEC_EWALD)
-        rc = MIX ? launch_at_ewald_mixed(P, st, d_chains, nchains, nthr, dyn, ncalls, unroll, fx, cell)
-                 : launch_at_ewald_fp64(P, st, d_chains, nchains, nthr, dyn, ncalls);
+        rc = MIX ? launch_at_ewald_mixed(P, st, d_chains, nchains, nthr, dyn, ncalls, unroll, fx, cell,
+                                         skg)
+                 : launch_at_ewald_fp64(P, st, d_chains, nchains, nthr, dyn, ncalls, skg);
     else if (P.elec == ELEC_DH)
         rc = launch_at_dh(MIX, P, st, d_chains, nchains, nthr, dyn, ncalls, unroll, fx, cell);
     else
@@ -105,10 +106,20 @@ struct Prepared {
     int *d_gvec = nullptr;
 };
 
-// LDS bytes of the Ewald state of a mixed AtomicTranslation chain with a
-// cell list of nc^3 cells
-static size_t at_dyn_lds_cells(int K, int nc) { return at_dyn_lds(K) + 4 * (size_t)nc * nc * nc; }
 static constexpr size_t AT_DYN_LDS_MAX = 110 * 1024;
+// dynamic LDS for the Ewald state of one chain (the static LDS of the
+// mixed sweep leaves this much of the CU's 160 KiB); more k-vectors than
+// this holds keep S(k), dS(k), A_k and n in device memory instead
+static constexpr size_t AT_DYN_LDS_EWALD_MAX = 110 * 1024;
+static bool ewald_in_hbm(int K) { return at_dyn_lds(K) > AT_DYN_LDS_EWALD_MAX; }
+// LDS bytes of the Ewald state of an AtomicTranslation chain (none when it
+// lives in device memory)
+static size_t at_dyn_lds_ewald(int K) { return ewald_in_hbm(K) ? 0 : at_dyn_lds(K); }
+// ... and with a cell list of nc^3 cells
+static size_t at_dyn_lds_cells(int K, int nc)
+{
+    return at_dyn_lds_ewald(K) + 4 * (size_t)nc * nc * nc;
+}
 
 // Cell list geometry and storage of a chain (cubic box, every pair term cut
 // at rc).  Cells of side >= rc + the largest single-trial displacement per
@@ -199,8 +210,6 @@ static int chain_limits(const fgx_system *s, std::vector<int> &gvec)
                       "device AtomicTranslation: moved groups must be atomic");
         FGX_CHECK(!s->has_ewald || s->ew.kcc < MAXK_AXIS, FGX_EUNSUPPORTED,
                   "device Ewald: cutoffK too large");
-        FGX_CHECK(!s->has_ewald || at_dyn_lds(s->ew.K) <= 120 * 1024, FGX_EUNSUPPORTED,
-                  "device Ewald: too many k-vectors for the LDS-resident S(k)");
     } else {
         FGX_CHECK(!s->has_ewald, FGX_EUNSUPPORTED, "device TranslateRotate with Ewald");
         FGX_CHECK(s->ng <= MAX_GROUPS_LDS, FGX_EUNSUPPORTED, "device TranslateRotate: too many groups");
@@ -315,6 +324,8 @@ static int prepare_chain(fgx_system *s, ChainDev &C, hipStream_t st, double *du_
     C.cell_rec = nullptr;
     C.cell_idx = nullptr;
     C.cell_slot = nullptr;
+    C.sk_dS = nullptr;
+    C.sk_kid = nullptr;
     C.cell_type = 1;
     if (cubic && (dh_like || ew_lj)) {
         const double unit = s->box.len[0] / 4294967296.0;
@@ -513,7 +524,7 @@ int prepare_batch(std::vector<fgx_system *> &sys, const std::vector<int> &ncalls
                   "device sweep: block size exceeds the partner-array padding");
         if (s->has_ewald)
             dyn = std::max(dyn, chains[c].cell_nc > 0 ? at_dyn_lds_cells(s->ew.K, chains[c].cell_nc)
-                                                      : at_dyn_lds(s->ew.K));
+                                                      : at_dyn_lds_ewald(s->ew.K));
         if (!chains[c].fx) fx = false;
         if (chains[c].cell_nc > 0) cells = true;
         FGX_CHECK(chains[c].lj_screen == chains[0].lj_screen, FGX_EUNSUPPORTED,
@@ -521,6 +532,34 @@ int prepare_batch(std::vector<fgx_system *> &sys, const std::vector<int> &ncalls
     }
     // pair terms cut at rc: the screening kernel (cell lists where present)
     if (!rc) cell = chains[0].lj_screen != 0;
+    // more k-vectors than the LDS-resident Ewald state holds: the batch's
+    // Ewald chains keep S(k), dS(k), A_k and n in device memory
+    bool skg = false;
+    for (int c = 0; c < nc && !rc; ++c)
+        skg = skg || (sys[c]->has_ewald && ewald_in_hbm(sys[c]->ew.K));
+    if (!rc && skg) {
+        // dynamic LDS then holds only the cell counts
+        dyn = 0;
+        for (int c = 0; c < nc; ++c)
+            if (chains[c].cell_nc > 0)
+                dyn = std::max(dyn, 4 * (size_t)chains[c].cell_nc * chains[c].cell_nc * chains[c].cell_nc);
+        for (int c = 0; c < nc; ++c) {
+            fgx_system *s = sys[c];
+            if (!s->has_ewald) continue;
+            if (!s->ew.sk_dS) {
+                FGX_HIP(hipMalloc(&s->ew.sk_dS, sizeof(double) * 2 * s->ew.K));
+                FGX_HIP(hipMalloc(&s->ew.sk_kid, sizeof(int) * s->ew.K));
+                std::vector<int> kid(s->ew.K);
+                for (int k = 0; k < s->ew.K; ++k)
+                    kid[k] = kpack(s->h_kn[3 * k], s->h_kn[3 * k + 1], s->h_kn[3 * k + 2]);
+                FGX_HIP(hipMemcpy(s->ew.sk_kid, kid.data(), sizeof(int) * s->ew.K,
+                                  hipMemcpyHostToDevice));
+                s->device_bytes += (16 + 4) * (size_t)s->ew.K;
+            }
+            chains[c].sk_dS = (double2 *)s->ew.sk_dS;
+            chains[c].sk_kid = s->ew.sk_kid;
+        }
+    }
     // the cell scan spreads 27 cells over waves 1..7 of a 512-thread block
     if (cells) nthr = AT_THREADS;
     if (!rc) {
@@ -589,6 +628,7 @@ int prepare_batch(std::vector<fgx_system *> &sys, const std::vector<int> &ncalls
         B.unroll = unroll;
         B.fx = fx;
         B.cell = cell;
+        B.skg = skg;
         B.pot = s0->pot;
         B.move_type = s0->moves[0].type;
     }
@@ -602,9 +642,9 @@ int enqueue_batch(const BatchPlan &B, hipStream_t st)
     if (B.move_type == MOVE_AT)
         return B.pot.precision == FGX_MIXED
                    ? launch_at<true>(B.pot, st, B.d_chains, B.nchains, B.nthr, B.dyn, B.ncalls,
-                                     B.unroll, B.fx, B.cell)
+                                     B.unroll, B.fx, B.cell, B.skg)
                    : launch_at<false>(B.pot, st, B.d_chains, B.nchains, B.nthr, B.dyn, B.ncalls,
-                                      B.unroll, B.fx, false);
+                                      B.unroll, B.fx, false, B.skg);
     return launch_tr(B.pot.precision == FGX_MIXED, B.pot, st, B.d_chains, B.nchains, B.nthr,
                      B.ncalls);
 }

@@ -7,9 +7,17 @@
 namespace fgx {
 
 int launch_at_ewald_mixed(const Pot &P, hipStream_t st, const ChainDev *d_chains, int nchains,
-                          int nthr, size_t dyn, int ncalls, int unroll, bool fx, bool cell)
+                          int nthr, size_t dyn, int ncalls, int unroll, bool fx, bool cell,
+                          bool skg)
 {
     if (P.elec != ELEC_EWALD) return FGX_EUNSUPPORTED;
+    if (skg) {
+        if (P.sr == SR_NONE) launch_at_skg<SR_NONE, true>(st, d_chains, nchains, nthr, dyn, ncalls, fx, cell);
+        else if (P.sr == SR_HS) launch_at_skg<SR_HS, true>(st, d_chains, nchains, nthr, dyn, ncalls, fx, cell);
+        else if (P.sr == SR_LJ) launch_at_skg<SR_LJ, true>(st, d_chains, nchains, nthr, dyn, ncalls, fx, cell);
+        else return FGX_EUNSUPPORTED;
+        return FGX_OK;
+    }
 #define FGX_AT(S) \
     if (P.sr == S) { launch_at_k<ELEC_EWALD, S, true>(st, d_chains, nchains, nthr, dyn, ncalls, unroll, fx, cell); return FGX_OK; }
     FGX_AT(SR_NONE)

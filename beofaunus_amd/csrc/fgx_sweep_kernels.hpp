@@ -100,6 +100,10 @@ struct ChainDev {
     double4 *cell_rec;            // FP64 {x, y, z, charge} per slot
     int *cell_idx;                // particle per slot
     int *cell_slot;               // per particle: cell * cap + slot
+    // Ewald state in device memory (more k-vectors than LDS holds): the
+    // trial's dS(k) and the packed n-triples; S(k) is Q itself, A_k aks
+    double2 *sk_dS;
+    int *sk_kid;
 };
 
 // ------------------------------------------------------------------------
@@ -646,7 +650,7 @@ static constexpr int AT_THREADS = 512;
 static constexpr int REFRESH_CHUNK = 16;   // particles per LDS table chunk
 static constexpr int REFRESH_KAX = 16;     // kcc + 1 limit for the table refresh
 
-__device__ __forceinline__ int kpack(int nx, int ny, int nz)
+__host__ __device__ __forceinline__ int kpack(int nx, int ny, int nz)
 {
     return (nx & 0xff) | ((ny + 128) << 8) | ((nz + 128) << 16);
 }
@@ -664,7 +668,9 @@ __host__ __device__ inline size_t at_dyn_lds(int K) { return (size_t)K * (16 + 1
 // CELL: every pair term vanishes beyond the Ewald rc (CutShift LJ at rc); the
 // partner loop screens for partners inside rc, over the chain's cell list
 // when it has one and over all partners otherwise.
-template <int ELEC, int SR, bool MIX, int U, bool FX, bool CELL = false>
+// SKG: the Ewald state (S(k), the trial's dS(k), A_k, packed n) stays in the
+// chain's device memory instead of LDS, for more k-vectors than LDS holds.
+template <int ELEC, int SR, bool MIX, int U, bool FX, bool CELL = false, bool SKG = false>
 __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restrict__ chains,
                                                          int ncalls)
 {
@@ -693,12 +699,23 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
     __shared__ double sst[5];
     __shared__ double lcheb[EW && MIX ? EW_CHEB_MAX : 1];
     extern __shared__ double2 dyn[];
-    double2 *S = dyn;
-    double2 *dS = S + (EW ? C.K : 0);
-    double *Ak = (double *)(dS + (EW ? C.K : 0));
-    int *kid = (int *)(Ak + (EW ? C.K : 0));
-    // cell counts (cell-list sweeps): LDS-resident for the launch
-    int *lcnt = kid + (EW ? C.K : 0);
+    double2 *S, *dS;
+    double *Ak;
+    int *kid, *lcnt;
+    if constexpr (SKG) {
+        S = (double2 *)C.Q;
+        dS = C.sk_dS;
+        Ak = (double *)C.aks;
+        kid = C.sk_kid;
+        lcnt = (int *)dyn;
+    } else {
+        S = dyn;
+        dS = S + (EW ? C.K : 0);
+        Ak = (double *)(dS + (EW ? C.K : 0));
+        kid = (int *)(Ak + (EW ? C.K : 0));
+        // cell counts (cell-list sweeps): LDS-resident for the launch
+        lcnt = kid + (EW ? C.K : 0);
+    }
     __shared__ int lcell_ok;
 
     for (int k = tid; k < 624; k += nthr) {
@@ -765,7 +782,7 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
     const double *tab = lds_tab ? ltab : C.tab;
     const float *tabf = (lds_tab && MIX) ? ltabf : C.tabf;
     const int kcc1 = C.kcc + 1;
-    if constexpr (EW) {
+    if constexpr (EW && !SKG) {
         for (int k = tid; k < C.K; k += nthr) {
             S[k] = make_double2(C.Q[2 * k], C.Q[2 * k + 1]);
             Ak[k] = C.aks[k];
@@ -1170,7 +1187,7 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
         C.stats_d[1] = sst[4];
         if (EW) *C.ew_cnt = ew_cnt;
     }
-    if constexpr (EW) {
+    if constexpr (EW && !SKG) {
         for (int k = tid; k < C.K; k += nthr) { C.Q[2 * k] = S[k].x; C.Q[2 * k + 1] = S[k].y; }
     }
     if constexpr (CELL) {
@@ -1792,15 +1809,36 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_tr(const ChainDev *__restr
     }
 }
 
-template <int ELEC, int SR, bool MIX, int U, bool FX, bool CELL = false>
+template <int ELEC, int SR, bool MIX, int U, bool FX, bool CELL = false, bool SKG = false>
 static void launch_at_f(hipStream_t st, const ChainDev *d_chains, int nchains, int nthr,
                         size_t dyn, int ncalls)
 {
     if (dyn > 0)
-        (void)hipFuncSetAttribute((const void *)k_sweep_at<ELEC, SR, MIX, U, FX, CELL>,
+        (void)hipFuncSetAttribute((const void *)k_sweep_at<ELEC, SR, MIX, U, FX, CELL, SKG>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
-    hipLaunchKernelGGL((k_sweep_at<ELEC, SR, MIX, U, FX, CELL>), dim3(nchains), dim3(nthr), dyn,
-                       st, d_chains, ncalls);
+    hipLaunchKernelGGL((k_sweep_at<ELEC, SR, MIX, U, FX, CELL, SKG>), dim3(nchains), dim3(nthr),
+                       dyn, st, d_chains, ncalls);
+}
+
+// Ewald chains whose S(k) lives in device memory: batch depth 4, no cell list
+template <int SR, bool MIX>
+static void launch_at_skg(hipStream_t st, const ChainDev *d_chains, int nchains, int nthr,
+                          size_t dyn, int ncalls, bool fx, bool cell)
+{
+    if constexpr (MIX && SR == SR_LJ && at_fx_ok<ELEC_EWALD, SR>())
+        if (fx && cell) {
+            // pair terms cut at rc: cell lists where present, else the screen
+            launch_at_f<ELEC_EWALD, SR, MIX, 4, true, true, true>(st, d_chains, nchains, nthr, dyn,
+                                                                  ncalls);
+            return;
+        }
+    if constexpr (MIX && at_fx_ok<ELEC_EWALD, SR>())
+        if (fx) {
+            launch_at_f<ELEC_EWALD, SR, MIX, 4, true, false, true>(st, d_chains, nchains, nthr, 0,
+                                                                   ncalls);
+            return;
+        }
+    launch_at_f<ELEC_EWALD, SR, MIX, 4, false, false, true>(st, d_chains, nchains, nthr, 0, ncalls);
 }
 
 template <int ELEC, int SR, bool MIX, int U>
@@ -1844,6 +1882,7 @@ struct BatchPlan {
     int nchains = 0, nthr = 0, ncalls = 0, unroll = 4, move_type = 0;
     size_t dyn = 0;
     bool fx = false, cell = false;
+    bool skg = false;   // Ewald S(k) in device memory
     Pot pot{};
 };
 int prepare_batch(std::vector<fgx_system *> &sys, const std::vector<int> &ncalls,
@@ -1860,9 +1899,10 @@ int launch_at_plain(bool mix, const Pot &P, hipStream_t st, const ChainDev *d_ch
 int launch_at_dh(bool mix, const Pot &P, hipStream_t st, const ChainDev *d_chains, int nchains,
                  int nthr, size_t dyn, int ncalls, int unroll, bool fx, bool cell);
 int launch_at_ewald_mixed(const Pot &P, hipStream_t st, const ChainDev *d_chains, int nchains,
-                          int nthr, size_t dyn, int ncalls, int unroll, bool fx, bool cell);
+                          int nthr, size_t dyn, int ncalls, int unroll, bool fx, bool cell,
+                          bool skg);
 int launch_at_ewald_fp64(const Pot &P, hipStream_t st, const ChainDev *d_chains, int nchains,
-                         int nthr, size_t dyn, int ncalls);
+                         int nthr, size_t dyn, int ncalls, bool skg);
 int launch_tr(bool mix, const Pot &P, hipStream_t st, const ChainDev *d_chains, int nchains,
               int nthr, int ncalls);
 int launch_emat_init(const Pot &P, hipStream_t st, const ChainDev *d_chains, int nchains,

@@ -964,7 +964,8 @@ extern "C" int fgx_destroy(fgx_system *s)
                     s->d_rf, s->d_groups, s->d_cm, s->d_tab, s->d_tabf, s->d_type_dp, s->d_mask,
                     s->d_partial, s->d_out, s->d_idx, s->d_rng, s->d_stats_i, s->d_stats_d,
                     s->ew.kv, s->ew.aks, s->ew.kn, s->ew.Q, s->ew.Qt, s->ew.eff,
-                    s->ew.cnt_dev, s->d_emat, s->ew.cheb, s->d_gvec, s->d_batch,
+                    s->ew.cnt_dev, s->d_emat, s->ew.cheb, s->ew.sk_dS, s->ew.sk_kid,
+                    s->d_gvec, s->d_batch,
                     s->d_bstat_i, s->d_bstat_d, s->d_resume, s->d_cell_cnt, s->d_cell_rec,
                     s->d_cell_idx, s->d_cell_slot, s->d_stage};
     for (void *p : ptrs)

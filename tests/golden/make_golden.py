@@ -476,6 +476,9 @@ ONLY = {
     # rigid molecules and ions: one Propagator picking TranslateRotate or Isobaric
     "mol_npt_mixed": lambda: molecules("mol_npt_mixed", "dhhs_npt", 4, 12, 16, 60.0, 27, 30,
                                        isobaric={"dp": 0.3, "pressure": 150.0}),
+    # |n| <= 12: more k-vectors than the LDS-resident device S(k) holds
+    "ewald_kc12": lambda: ewald("ewald_kc12", 40, 20.0, 29, 6, eps_surf=0.5, update_frequency=7,
+                                kc=12.0),
     # AtomicTranslation + Isobaric with moves.random reseeding the move engine
     "salt_npt_random": lambda: salt_npt("salt_npt_random", 50, 30.0, 28, 30, pressure_mM=300.0,
                                         dp=0.4, with_at_and_random=True),

@@ -14,6 +14,9 @@ cell-list form of the mixed AtomicTranslation sweep.
   all-pairs screen and the results do not change.
 * At N = 20,000 (the full electrolyte): the accepted dU summed over a sweep
   equals the change of the FP64 system energy.
+* |n| <= 12 (more k-vectors than the LDS-resident S(k) holds, so S(k),
+  dS(k), A_k and n stay in device memory): FP64 against the oracle, cell
+  list against the all-pairs screen and against FP64.
 """
 import sys
 
@@ -170,6 +173,40 @@ def test_cell_list_variants(gpu, case):
     upto = int(mism[0]) if len(mism) else len(a64)
     assert upto >= 500
     assert np.max(np.abs(dc[:upto] - d64[:upto])) <= TOL_MIXED
+
+
+def test_device_memory_sk_with_cells(gpu):
+    from beofaunus_amd import workloads
+    n = 2000
+    L = 120.0 * (n / 20000.0) ** (1.0 / 3.0)
+    w = workloads.ewald_electrolyte(n=n, L=L, seed=47, lj_cut=True, kc=12.0)
+    o = oracle_of(w)
+    K = olib().or_ewald_nk(o.h)
+    say("[cells] |n| <= 12, K = %d, N=%d, L=%.2f" % (K, n, L))
+    # 44 bytes per k-vector beyond the 110 KiB of dynamic LDS
+    assert 44 * K > 110 * 1024
+    o.seed(23)
+    olib().or_propagator_init(o.h)
+    odu, oacc, _, _, _ = o.propagate(2)
+    st, d64, a64, _ = run(gpu, w, gpu.FP64, 23, 2)
+    assert np.array_equal(a64, oacc)
+    fin = np.isfinite(odu)
+    assert np.array_equal(np.isfinite(d64), fin)
+    err = np.abs(d64[fin] - odu[fin]) / np.maximum(1.0, np.abs(odu[fin]))
+    say("  FP64 vs oracle: max rel dU err %.2e, accepted %d" % (err.max(), oacc.sum()))
+    assert err.max() <= REL
+    _, dp, ap, nocells = run(gpu, w, gpu.MIXED, 23, 2, cell_mode=0)
+    _, dc, ac, cells = run(gpu, w, gpu.MIXED, 23, 2, cell_mode=1)
+    say("  cell list %s, all-pairs %s" % (cells, nocells))
+    assert cells[0] >= 3 and nocells == (0, 0)
+    assert np.array_equal(ac, ap)
+    assert np.max(np.abs(dc - dp)) <= 1e-9
+    mism = np.nonzero(ac != a64)[0]
+    upto = int(mism[0]) if len(mism) else len(a64)
+    e = float(np.max(np.abs(dc[:upto] - d64[:upto])))
+    say("  mixed vs FP64: compared %d trials, max |err| %.2e kT" % (upto, e))
+    assert upto >= 1000
+    assert e <= TOL_MIXED
 
 
 def test_cell_list_across_volume_moves(gpu):

@@ -199,7 +199,7 @@ def _oracle_update_change(o, mv):
 
 
 @pytest.mark.parametrize("name", ["salt_dhhs", "salt_coulhs", "lj_mix", "lj_cutshift",
-                                  "ewald_tinfoil", "mol_dhhs", "mol_dhhs_cut"])
+                                  "ewald_tinfoil", "ewald_kc12", "mol_dhhs", "mol_dhhs_cut"])
 def test_mixed_precision_error(gpu, name):
     """Mixed path (FP32 pairs on fixed-point coordinates, FP64 accumulation,
     FP64 S(k) and Metropolis): |dU - dU_FP64| <= 1e-5 kT while the two
