@@ -251,6 +251,11 @@ __host__ __device__ __forceinline__ int32_t to_fixed(double x, double inv_len)
 struct WaveMT {
     uint32_t *x;  // LDS, 624 words
     int p;        // wave-uniform position
+    // window: lane l holds the tempered word x[wb + l] of the current state,
+    // for draws wb <= p < wb + 64 (wb < 0: empty), so a draw is a lane read
+    // instead of a dependent LDS load
+    int wb = -1;
+    uint32_t win = 0;
 };
 
 __device__ __forceinline__ void wave_sync_lds()
@@ -283,17 +288,30 @@ __device__ __forceinline__ void mt_twist_wave(uint32_t *x)
     }
 }
 
+__device__ __forceinline__ uint32_t mt_temper(uint32_t z)
+{
+    z ^= (z >> 11);
+    z ^= (z << 7) & 0x9d2c5680u;
+    z ^= (z << 15) & 0xefc60000u;
+    z ^= (z >> 18);
+    return z;
+}
+
 __device__ __forceinline__ uint32_t mt_next(WaveMT &m)
 {
     if (m.p >= 624) {
         mt_twist_wave(m.x);
         m.p = 0;
+        m.wb = -1;
     }
-    uint32_t z = m.x[m.p++];
-    z ^= (z >> 11);
-    z ^= (z << 7) & 0x9d2c5680u;
-    z ^= (z << 15) & 0xefc60000u;
-    z ^= (z >> 18);
+    if (m.wb < 0 || m.p >= m.wb + 64) {
+        // refill: the next (up to) 64 words of this state, tempered per lane
+        const int k = m.p + (int)(threadIdx.x & 63);
+        m.win = k < 624 ? mt_temper(m.x[k]) : 0u;
+        m.wb = m.p;
+    }
+    const uint32_t z = (uint32_t)__builtin_amdgcn_readlane((int)m.win, m.p - m.wb);
+    m.p++;
     return z;
 }
 
@@ -353,6 +371,59 @@ __device__ __forceinline__ double block_sum(double v, double *red)
     double t = 0.0;
     for (int k = 0; k < nw; ++k) t += red[k];  // fixed order: deterministic
     __syncthreads();
+    return t;
+}
+
+// workgroup barrier that orders LDS only: waits for this wave's LDS
+// operations, not for its global loads and stores in flight (for loops whose
+// waves exchange data through LDS alone between two such barriers)
+__device__ __forceinline__ void lds_sync()
+{
+    __asm__ volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// FP64 wave sum through DPP row operations (quad permutes, row shifts, row
+// broadcasts; rows of 16 lanes), which avoid the LDS crossbar of a shuffle;
+// fixed order, every lane receives lane 63's total
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
+                                            (unsigned long long)(unsigned)lo));
+}
+
+__device__ __forceinline__ double wave_sum_dpp(double v)
+{
+    v += dpp_f64<0xb1>(v);    // quad_perm 1,0,3,2
+    v += dpp_f64<0x4e>(v);    // quad_perm 2,3,0,1
+    v += dpp_f64<0x114>(v);   // row_shr:4
+    v += dpp_f64<0x118>(v);   // row_shr:8   (lane 15 of each row: the row's sum)
+    v += dpp_f64<0x142>(v);   // row_bcast:15
+    v += dpp_f64<0x143>(v);   // row_bcast:31 (lane 63: the wave's sum)
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, 63);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
+                                            (unsigned long long)(unsigned)lo));
+}
+
+// block_sum with one barrier: `red` must not be written again before every
+// wave has passed another barrier (callers alternate two buffers between
+// calls separated by a barrier).  LDS-only barrier when `lds_only`.
+__device__ __forceinline__ double block_sum1(double v, double *red, bool lds_only)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    v = wave_sum_dpp(v);
+    if (lane == 0) red[w] = v;
+    if (lds_only)
+        lds_sync();
+    else
+        __syncthreads();
+    double t = 0.0;
+    for (int k = 0; k < nw; ++k) t += red[k];  // fixed order: deterministic
     return t;
 }
 

@@ -147,6 +147,9 @@ struct fgx_system {
     // cut at the Ewald rc (fgx_set_cell_list): mode 0 off, 1 automatic
     // capacity, >= 2 that many records per cell
     int cell_mode = 1;
+    // mixed AtomicTranslation sweeps without Ewald stage the partners in LDS
+    // when they fit (fgx_set_lds_partners): 1 on, 0 read them from device memory
+    int lds_partners = 1;
     int cell_nc = 0, cell_cap = 0;         // cells per axis, records per cell (allocated)
     int cell_cap_min = 0;                  // automatic capacity after an overflow
     int *d_cell_cnt = nullptr;             // nc^3 counts + overflow word

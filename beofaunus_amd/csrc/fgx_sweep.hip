@@ -107,6 +107,9 @@ struct Prepared {
 };
 
 static constexpr size_t AT_DYN_LDS_MAX = 110 * 1024;
+// dynamic LDS for the partner slice of a mixed AtomicTranslation chain
+// without Ewald (the sweep's static LDS is under 29 KiB of the CU's 160 KiB)
+static constexpr size_t AT_LDS_PART_MAX = 128 * 1024;
 // dynamic LDS for the Ewald state of one chain (the static LDS of the
 // mixed sweep leaves this much of the CU's 160 KiB); more k-vectors than
 // this holds keep S(k), dS(k), A_k and n in device memory instead
@@ -326,6 +329,9 @@ static int prepare_chain(fgx_system *s, ChainDev &C, hipStream_t st, double *du_
     C.cell_slot = nullptr;
     C.sk_dS = nullptr;
     C.sk_kid = nullptr;
+    C.ldsp = 0;
+    C.sp_j0 = 0;
+    C.sp_j1 = s->n;
     C.cell_type = 1;
     if (cubic && (dh_like || ew_lj)) {
         const double unit = s->box.len[0] / 4294967296.0;
@@ -529,6 +535,14 @@ int prepare_batch(std::vector<fgx_system *> &sys, const std::vector<int> &ncalls
         if (chains[c].cell_nc > 0) cells = true;
         FGX_CHECK(chains[c].lj_screen == chains[0].lj_screen, FGX_EUNSUPPORTED,
                   "batched chains must agree on a Lennard-Jones cut at the Ewald rc");
+        // mixed AtomicTranslation without Ewald: a chain whose partners fit
+        // keeps them in LDS for the launch
+        const size_t part = (size_t)s->n * LDS_PART_BYTES;
+        if (s->lds_partners && !s->has_ewald && s->pot.precision == FGX_MIXED &&
+            chains[c].mv.type == MOVE_AT && s->pot.ntypes2 <= LDS_TYPES2 && part <= AT_LDS_PART_MAX) {
+            chains[c].ldsp = 1;
+            dyn = std::max(dyn, part);
+        }
     }
     // pair terms cut at rc: the screening kernel (cell lists where present)
     if (!rc) cell = chains[0].lj_screen != 0;
@@ -973,6 +987,13 @@ extern "C" int fgx_set_cell_list(fgx_system *s, int mode)
     s->cell_mode = mode;
     s->cell_cap_min = 0;
     s->cell_valid = false;
+    return FGX_OK;
+}
+
+extern "C" int fgx_set_lds_partners(fgx_system *s, int on)
+{
+    FGX_CHECK(s && (on == 0 || on == 1), FGX_EARG, "fgx_set_lds_partners: on is 0 or 1");
+    s->lds_partners = on;
     return FGX_OK;
 }
 

@@ -104,6 +104,35 @@ struct ChainDev {
     // trial's dS(k) and the packed n-triples; S(k) is Q itself, A_k aks
     double2 *sk_dS;
     int *sk_kid;
+    // partners [sp_j0, sp_j1) staged in LDS for the launch (mixed
+    // AtomicTranslation without Ewald): packed record, FP64 position, charge
+    // and radius, FP32 radius and type, LDS_PART_BYTES per particle
+    int ldsp;
+    int sp_j0, sp_j1;
+};
+
+static constexpr int LDS_PART_BYTES = 16 + 5 * 8 + 4 + 4;
+
+// the LDS-resident partner arrays of a chain (see ChainDev::ldsp); LDS
+// address-space pointers, so accesses compile to LDS (not flat) instructions
+#define FGX_LDS __attribute__((address_space(3)))
+typedef int lds_v4i __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ int4 v4i_to_int4(lds_v4i v) { return make_int4(v.x, v.y, v.z, v.w); }
+__device__ __forceinline__ lds_v4i int4_to_v4i(int4 v) { return lds_v4i{v.x, v.y, v.z, v.w}; }
+struct LdsPart {
+    FGX_LDS lds_v4i *pk;
+    FGX_LDS double *x, *y, *z, *q, *r;
+    FGX_LDS float *rf;
+    FGX_LDS int *t;
+    int j0, ns;
+    __device__ void carve(FGX_LDS void *base, int n)
+    {
+        pk = (FGX_LDS lds_v4i *)base;
+        x = (FGX_LDS double *)(pk + n);
+        y = x + n; z = y + n; q = z + n; r = q + n;
+        rf = (FGX_LDS float *)(r + n);
+        t = (FGX_LDS int *)(rf + n);
+    }
 };
 
 // ------------------------------------------------------------------------
@@ -311,6 +340,18 @@ struct PartnerBatch {
             T[u] = NEED_T ? gload(C.t + row, tid) : 0;
         }
     }
+    // the same rows from the LDS-resident slice (rows past its end read zero)
+    __device__ __forceinline__ void load_lds(const LdsPart &L, int base, int nthr, int tid)
+    {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int row = base + u * nthr + tid;
+            const bool in = row < L.ns;
+            P[u] = v4i_to_int4(in ? L.pk[row] : lds_v4i{0, 0, 0, 0});
+            R[u] = HS ? (in ? L.rf[row] : 0.f) : 0.f;
+            T[u] = NEED_T ? (in ? L.t[row] : 0) : 0;
+        }
+    }
 };
 
 // true where the mixed AtomicTranslation loop has a fixed-unit form
@@ -322,14 +363,18 @@ __host__ __device__ constexpr bool at_fx_ok()
             (SR == SR_NONE || SR == SR_HS));
 }
 
-template <int ELEC, int SR, int U, bool FX, bool SCREEN_ONLY = false>
+template <int ELEC, int SR, int U, bool FX, bool SCREEN_ONLY = false, bool LP = false>
 __device__ __forceinline__ double pair_loop_mixed(const ChainDev &C, const float *__restrict__ tabf,
                                                   const double *__restrict__ tab, int i, int4 me,
                                                   float ar, int at, double aq64, double ar64,
                                                   double tnx, double tny, double tnz, double tox,
                                                   double toy, double toz, int *dq, int qcap,
-                                                  int &nq)
+                                                  int &nq, const LdsPart L = LdsPart{})
 {
+    // LP: partners are the LDS-resident slice L; rows and queue entries are
+    // slice-local (particle L.j0 + row)
+    const int nrow = LP ? L.ns : C.n;
+    const int iself = LP ? i - L.j0 : i;
     const int tid = threadIdx.x, nthr = blockDim.x, lane = tid & 63;
     const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const float aq = __int_as_float(me.w);
@@ -364,7 +409,7 @@ __device__ __forceinline__ double pair_loop_mixed(const ChainDev &C, const float
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int j = base + u * nthr + tid;
-            const bool ok = j < C.n && j != i;
+            const bool ok = j < nrow && j != iself;
             const int4 P = X.P[u];
             if constexpr (FX) {
                 float dx = (float)(int32_t)(nx - P.x), dy = (float)(int32_t)(ny - P.y),
@@ -453,6 +498,10 @@ __device__ __forceinline__ double pair_loop_mixed(const ChainDev &C, const float
                         const int k = cnt + __popcll(m & below);
                         if (k < qcap)
                             dq[k] = j;
+                        else if constexpr (LP)
+                            acc += du_pair64_v<ELEC, SR>(C, tab, tnx, tny, tnz, tox, toy, toz,
+                                                         aq64, ar64, at, L.x[j], L.y[j], L.z[j],
+                                                         L.q[j], L.r[j], L.t[j]);
                         else
                             acc += du_pair64<ELEC, SR>(C, tab, tnx, tny, tnz, tox, toy, toz,
                                                        aq64, ar64, at, j);
@@ -464,12 +513,20 @@ __device__ __forceinline__ double pair_loop_mixed(const ChainDev &C, const float
     };
     const int step = U * nthr;
     auto run = [&](auto mode) {
-        A.load(C, 0, nthr, tid);
-        for (int base = 0; base < C.n; base += 2 * step) {
-            B.load(C, base + step, nthr, tid);        // clamped: always in bounds
-            eval(mode, A, base);
-            A.load(C, base + 2 * step, nthr, tid);
-            if (base + step < C.n) eval(mode, B, base + step);
+        if constexpr (LP) {
+            // LDS latency needs no second batch in flight
+            for (int base = 0; base < nrow; base += step) {
+                A.load_lds(L, base, nthr, tid);
+                eval(mode, A, base);
+            }
+        } else {
+            A.load(C, 0, nthr, tid);
+            for (int base = 0; base < C.n; base += 2 * step) {
+                B.load(C, base + step, nthr, tid);        // clamped: always in bounds
+                eval(mode, A, base);
+                A.load(C, base + 2 * step, nthr, tid);
+                if (base + step < C.n) eval(mode, B, base + step);
+            }
         }
     };
     if constexpr (FX && ELEC == ELEC_EWALD && SCREEN_ONLY)
@@ -717,6 +774,18 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
         lcnt = kid + (EW ? C.K : 0);
     }
     __shared__ int lcell_ok;
+    // mixed sweep without Ewald: the partner slice is LDS-resident and the
+    // trial loop's barriers order LDS only (its waves exchange nothing else)
+    constexpr bool LPOK = !EW && MIX && !CELL;
+    const bool ldsp = LPOK && C.ldsp;
+    LdsPart L;
+    L.j0 = ldsp ? C.sp_j0 : 0;
+    L.ns = ldsp ? C.sp_j1 - C.sp_j0 : 0;
+    L.carve((FGX_LDS void *)dyn, L.ns);
+    // the whole chain in LDS: the moved particle's data comes from there too
+    const bool lds_all = ldsp && L.j0 == 0 && L.ns == C.n;
+    __shared__ double red2[2][AT_THREADS / 64];
+    int rpar = 0;
 
     for (int k = tid; k < 624; k += nthr) {
         mtg[k] = C.rng[k];
@@ -762,6 +831,17 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
     // the moved particle's data, loads issued together
     auto at_fetch = [&](TrialAT &t) {
         const int i = t.i;
+        if (LPOK && lds_all) {
+            t.ti = L.t[i];
+            t.ox = L.x[i]; t.oy = L.y[i]; t.oz = L.z[i];
+            t.pk = v4i_to_int4(L.pk[i]);
+            t.rf = L.rf[i];
+            t.q64 = L.q[i];
+            t.r64 = L.r[i];
+            t.wq = 0.0;
+            t.slot = 0;
+            return;
+        }
         t.ti = gload(C.t, i);
         t.ox = gload(C.x, i); t.oy = gload(C.y, i); t.oz = gload(C.z, i);
         t.pk = gload(C.pk, i);
@@ -782,6 +862,19 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
     const double *tab = lds_tab ? ltab : C.tab;
     const float *tabf = (lds_tab && MIX) ? ltabf : C.tabf;
     const int kcc1 = C.kcc + 1;
+    if constexpr (LPOK) {
+        if (ldsp) {
+            for (int k = tid; k < L.ns; k += nthr) {
+                const int j = L.j0 + k;
+                L.pk[k] = int4_to_v4i(gload(C.pk, j));
+                L.x[k] = C.x[j]; L.y[k] = C.y[j]; L.z[k] = C.z[j];
+                L.q[k] = gload(C.q, j);
+                L.r[k] = gload(C.r, j);
+                L.rf[k] = gload(C.rf, j);
+                L.t[k] = gload(C.t, j);
+            }
+        }
+    }
     if constexpr (EW && !SKG) {
         for (int k = tid; k < C.K; k += nthr) {
             S[k] = make_double2(C.Q[2 * k], C.Q[2 * k + 1]);
@@ -883,7 +976,10 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
                     tn.pf = 0;
                 }
             }
-            __syncthreads();
+            if (ldsp)
+                lds_sync();
+            else
+                __syncthreads();
             const int i = tr.i;
             // next trial in this call and in this launch
             const bool ahead = rep + 1 < C.repeat && used < budget;
@@ -905,10 +1001,7 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
             // data (wave 0; the reference's draw order is unchanged).  With a
             // cell list it overlaps the other waves' cell scan, otherwise the
             // reciprocal sum and the FP64 pass after the pair loop.
-            auto draw_ahead = [&]() {
-                TrialAT nxt;
-                at_draw(nxt);
-                at_fetch(nxt);
+            auto commit_ahead = [&](const TrialAT &nxt) {
                 if (tid == 0) {
                     tn.pf = 1;
                     tn.i = nxt.i; tn.ti = nxt.ti; tn.pk = nxt.pk; tn.rf = nxt.rf;
@@ -918,6 +1011,12 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
                     tn.u = nxt.u;
                     tn.ox = nxt.ox; tn.oy = nxt.oy; tn.oz = nxt.oz;
                 }
+            };
+            auto draw_ahead = [&]() {
+                TrialAT nxt;
+                at_draw(nxt);
+                at_fetch(nxt);
+                commit_ahead(nxt);
             };
             const bool use_cells = CELL && lcell_ok;
             if (use_cells && wave0 && ahead) draw_ahead();
@@ -936,8 +1035,21 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
             } else {
                 int nq = 0;
                 int *dq = dlist + wv * qcap;
+                bool done = false;
+                if constexpr (LPOK) {
+                    if (ldsp) {
+                        // two rows per thread and batch: the LDS loop then
+                        // needs no more registers than the device-memory one
+                        acc = pair_loop_mixed<ELEC, SR, 2, FX, false, true>(
+                            C, ltabf, ltab, i, tr.pk, tr.rf, tr.ti, tr.q64, tr.r64, tr.nx, tr.ny,
+                            tr.nz, tr.ox, tr.oy, tr.oz, dq, qcap, nq, L);
+                        done = true;
+                    }
+                }
                 // (the fixed-unit form is only launched with LDS tables)
-                if (use_cells)
+                if (done)
+                    ;
+                else if (use_cells)
                     acc = cell_loop<ELEC, SR>(C, ltab, lcheb, lcnt, i, tr.pk, tr.q64, tr.r64, tr.ti,
                                               tr.nx, tr.ny, tr.nz, tr.ox, tr.oy, tr.oz);
                 else if (FX || lds_tab)
@@ -951,7 +1063,12 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
                                                               nq);
                 if ((tid & 63) == 0) nqs[wv] = nq;
             }
-            if (EW || MIX) __syncthreads();   // etab and FP64 queues complete
+            if (EW || MIX) {   // etab and FP64 queues complete
+                if (ldsp)
+                    lds_sync();
+                else
+                    __syncthreads();
+            }
             if (!use_cells && wave0 && ahead) draw_ahead();
             // FP64 work of the mixed path goes to the last F wavefronts, the
             // reciprocal-space sum to the others, so the two run side by side
@@ -1012,11 +1129,25 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
                     int w = 0, k = e;
                     while (k >= nqs[w]) { k -= nqs[w]; ++w; }
                     const int j = dlist[w * qcap + k];
-                    acc += du_pair64_mixed<ELEC, SR>(C, tab, lcheb, tr.nx, tr.ny, tr.nz, tr.ox,
-                                                     tr.oy, tr.oz, tr.q64, tr.r64, tr.ti, j);
+                    if (LPOK && ldsp)
+                        acc += du_pair64_v<ELEC, SR>(C, tab, tr.nx, tr.ny, tr.nz, tr.ox, tr.oy,
+                                                     tr.oz, tr.q64, tr.r64, tr.ti, L.x[j], L.y[j],
+                                                     L.z[j], L.q[j], L.r[j], L.t[j]);
+                    else
+                        acc += du_pair64_mixed<ELEC, SR>(C, tab, lcheb, tr.nx, tr.ny, tr.nz,
+                                                         tr.ox, tr.oy, tr.oz, tr.q64, tr.r64,
+                                                         tr.ti, j);
                 }
             }
-            double du = block_sum(acc, red);
+            double du;
+            if constexpr (EW) {
+                du = block_sum(acc, red);
+            } else {
+                // one barrier: the two buffers alternate, and the next
+                // trial's first barrier separates their uses
+                du = block_sum1(acc, red2[rpar], ldsp);
+                rpar ^= 1;
+            }
             if (EW && C.surf_pref != 0.0) {
                 const double Mx = sst[0], My = sst[1], Mz = sst[2];
                 double dMx = wq * (tr.nx - tr.ox), dMy = wq * (tr.ny - tr.oy), dMz = wq * (tr.nz - tr.oz);
@@ -1027,7 +1158,8 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
             // ---- Movebase::metropolis (move.h:567-573): the uniform was drawn
             // with the trial (it is the next number of the same stream); every
             // thread takes the same decision from the same du
-            const bool accept = !(tr.u > exp(-du));
+            // (du <= 0: exp(-du) >= 1 > u, so the test accepts without exp)
+            const bool accept = du <= 0.0 || !(tr.u > exp(-du));
             if (tid == 0) tr.acc = accept ? 1 : 0;
             n_trials++;
             if (tid == 0 && C.du_rec && rec < C.cap) {
@@ -1044,6 +1176,13 @@ __global__ __launch_bounds__(AT_THREADS) void k_sweep_at(const ChainDev *__restr
                                               to_fixed(tr.ny, C.box.inv[1]),
                                               to_fixed(tr.nz, C.box.inv[2]), tr.pk.w);
                     C.pk_w[i] = np;
+                    if constexpr (LPOK) {
+                        const int il = i - L.j0;
+                        if (ldsp && il >= 0 && il < L.ns) {
+                            L.pk[il] = int4_to_v4i(np);
+                            L.x[il] = tr.nx; L.y[il] = tr.ny; L.z[il] = tr.nz;
+                        }
+                    }
                     if (CELL && lcell_ok) {
                         // the moved record changes in place, or moves to its
                         // new cell (the old cell's last record fills the hole)

@@ -129,6 +129,7 @@ def lib():
         "fgx_set_launch": (I, [P, I, I]),
         "fgx_set_energy_matrix": (I, [P, I]),
         "fgx_set_cell_list": (I, [P, I]),
+        "fgx_set_lds_partners": (I, [P, I]),
         "fgx_isobaric_propagate": (I, [P, D, D, I, C.POINTER(Stats), _dp, _ip, C.c_long]),
         "fgx_propagate_npt": (I, [P, D, D, I, I, C.POINTER(Stats), _dp, _ip, C.c_long]),
         "fgx_set_space": (I, [P, _dp]),
@@ -407,6 +408,13 @@ class System:
         """Cell list of the mixed AtomicTranslation sweep: 0 off, 1 automatic,
         n >= 2 records per cell."""
         _check(lib().fgx_set_cell_list(self.h, int(mode)))
+
+    def set_lds_partners(self, on):
+        """Mixed AtomicTranslation sweep without Ewald: keep the partners of a
+        chain of up to 2,048 particles in LDS (True, default) or read them from
+        device memory (False); the same pair arithmetic, summed in another
+        order."""
+        _check(lib().fgx_set_lds_partners(self.h, 1 if on else 0))
 
     def cell_list(self):
         """(cells per axis, records per cell) of the last sweep; (0, 0) when

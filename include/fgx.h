@@ -245,6 +245,13 @@ int fgx_set_energy_matrix(fgx_system *s, int on);
 int fgx_set_cell_list(fgx_system *s, int mode);
 /* cells per axis and records per cell of the last sweep (0, 0: no list) */
 int fgx_get_cell_list(const fgx_system *s, int *cells_per_axis, int *capacity);
+/* Mixed AtomicTranslation sweep without Ewald: a chain of up to 2,048
+ * particles keeps its partners (packed record, FP64 position, charge,
+ * radius, type) in LDS for the launch.  The same pair arithmetic as the
+ * device-memory loop; partners are visited in other batches, so dU agrees
+ * to FP64 rounding of the sum.  on = 1 (default) or 0 (partners read from
+ * device memory).  A schedule knob with no reference counterpart. */
+int fgx_set_lds_partners(fgx_system *s, int on);
 /* Autotuner: times `ncalls` Propagator::move() calls of a probe chain under
  * every schedule and keeps the fastest on the probe (whose state advances). */
 int fgx_autotune(fgx_system *probe, int ncalls, int *threads, int *unroll, double *best_seconds);
