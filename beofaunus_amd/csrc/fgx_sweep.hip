@@ -820,6 +820,25 @@ extern "C" int fgx_rng_get_state(fgx_system *s, uint32_t words[624], int *pos)
     return FGX_OK;
 }
 
+// the move-internal engine (Movebase::_slump, move.h:356-360)
+extern "C" int fgx_rng_set_internal(fgx_system *s, const uint32_t words[624], int pos)
+{
+    FGX_CHECK(s && words && pos >= 0 && pos <= 624, FGX_EARG, "fgx_rng_set_internal: bad arguments");
+    if (int rc = pull_device_state(s)) return rc;
+    std::memcpy(s->rng_internal, words, sizeof(uint32_t) * 624);
+    s->rng_internal[624] = (uint32_t)pos;
+    return FGX_OK;
+}
+
+extern "C" int fgx_rng_get_internal(fgx_system *s, uint32_t words[624], int *pos)
+{
+    FGX_CHECK(s && words && pos, FGX_EARG, "fgx_rng_get_internal: bad arguments");
+    if (int rc = pull_device_state(s)) return rc;
+    std::memcpy(words, s->rng_internal, sizeof(uint32_t) * 624);
+    *pos = (int)s->rng_internal[624];
+    return FGX_OK;
+}
+
 extern "C" int fgx_propagator_init(fgx_system *s)
 {
     FGX_CHECK(s, FGX_EARG, "fgx_propagator_init: null system");

@@ -377,6 +377,8 @@ struct fgx_temper {
     std::vector<int> rec_acc, rec_slot, rec_part;
     std::vector<double> rec_du, rec_ret;
     std::vector<long> rec_cycles;
+    // each rank's ParallelTempering::move return value in the last exchange
+    std::vector<double> last_ret;
 };
 
 static void temper_free(fgx_temper *t)
@@ -694,10 +696,22 @@ extern "C" int fgx_temper_exchange(fgx_temper *t, int *acc, double *du)
     if ((rc = pull_records(t, 1))) return rc;
     std::vector<double> drift(R, 0.0);
     if (t->cfg[0]->has_ewald && (rc = ewald_after_exchange(t, before, drift.data()))) return rc;
+    t->last_ret.assign(R, 0.0);
     for (int r = 0; r < R; ++r) {
         if (acc) acc[r] = t->rec_acc[r];
         if (du) du[r] = t->rec_du[r];
+        // Movebase::move: the accepted move's return value, then
+        // pot->update(acceptance) (move.h:538-546)
+        if (t->rec_acc[r]) t->last_ret[r] = t->rec_ret[r] + drift[r];
     }
+    return FGX_OK;
+}
+
+extern "C" int fgx_temper_returns(fgx_temper *t, double *ret)
+{
+    FGX_CHECK(t && ret, FGX_EARG, "fgx_temper_returns: bad arguments");
+    FGX_CHECK((int)t->last_ret.size() == t->R, FGX_ESTATE, "fgx_temper_returns: no exchange yet");
+    for (int r = 0; r < t->R; ++r) ret[r] = t->last_ret[r];
     return FGX_OK;
 }
 
@@ -809,6 +823,12 @@ extern "C" int fgx_temper_run(fgx_temper **tempers, int nt, int rounds, int call
     for (int k = 0; k < nt && !rc; ++k) {
         fgx_temper *t = tempers[k];
         if ((rc = pull_records(t, rounds))) break;
+        // the last round's exchange (no Ewald replicas here: no update term)
+        t->last_ret.assign(t->R, 0.0);
+        if (rounds > 0)
+            for (int r = 0; r < t->R; ++r)
+                if (t->rec_acc[(size_t)(rounds - 1) * t->R + r])
+                    t->last_ret[r] = t->rec_ret[(size_t)(rounds - 1) * t->R + r];
         std::vector<long> ti(8 * t->R);
         std::vector<double> td(8 * t->R);
         FGX_HIP(hipMemcpy(ti.data(), t->d_tot_i, sizeof(long) * ti.size(), hipMemcpyDeviceToHost));

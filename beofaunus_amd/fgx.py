@@ -121,6 +121,8 @@ def lib():
         "fgx_rng_seed": (I, [P, C.c_uint32]),
         "fgx_rng_set_state": (I, [P, C.POINTER(C.c_uint32), I]),
         "fgx_rng_get_state": (I, [P, C.POINTER(C.c_uint32), _ip]),
+        "fgx_rng_set_internal": (I, [P, C.POINTER(C.c_uint32), I]),
+        "fgx_rng_get_internal": (I, [P, C.POINTER(C.c_uint32), _ip]),
         "fgx_propagator_init": (I, [P]),
         "fgx_propagate": (I, [P, I, C.POINTER(Stats), _dp, _ip, C.c_long]),
         "fgx_propagate_batch": (I, [pp, I, I, C.POINTER(Stats)]),
@@ -156,6 +158,7 @@ def lib():
         "fgx_temper_records": (I, [P, _ip, _ip, _dp, _ip, _dp]),
         "fgx_temper_exchange": (I, [P, _ip, _dp]),
         "fgx_temper_slots": (I, [P, _ip]),
+        "fgx_temper_returns": (I, [P, _dp]),
         "fgx_temper_destroy": (I, [P]),
     }
     for name, (res, args) in sig.items():
@@ -447,6 +450,17 @@ class System:
         _check(lib().fgx_rng_get_state(self.h, w, C.byref(p)))
         return list(w), p.value
 
+    def rng_internal_state(self):
+        """the move-internal engine (Movebase::_slump): (624 words, position)"""
+        w = (C.c_uint32 * 624)()
+        p = C.c_int()
+        _check(lib().fgx_rng_get_internal(self.h, w, C.byref(p)))
+        return list(w), p.value
+
+    def rng_set_internal_state(self, words, pos):
+        w = (C.c_uint32 * 624)(*[int(v) & 0xFFFFFFFF for v in words])
+        _check(lib().fgx_rng_set_internal(self.h, w, int(pos)))
+
     def propagator_init(self):
         _check(lib().fgx_propagator_init(self.h))
 
@@ -530,6 +544,12 @@ class Temper:
         s = np.zeros(self.R, dtype=np.int32)
         _check(lib().fgx_temper_slots(self.h, _i(s)))
         return s
+
+    def returns(self):
+        """each rank's ParallelTempering::move return value in the last exchange"""
+        r = np.zeros(self.R)
+        _check(lib().fgx_temper_returns(self.h, _d(r)))
+        return r
 
     def close(self):
         if getattr(self, "h", None):

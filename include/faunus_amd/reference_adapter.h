@@ -23,6 +23,11 @@
  *       entirely on the device: reads the global and move-internal engines
  *       (slump.h:126, move.h:356-360), writes the coordinates and engines
  *       back, so the host state afterwards is what the CPU Propagator leaves.
+ *   Move::GPUParallelTempering<Tspace> -- Move::ParallelTempering (move.h:
+ *       6520-6772, the Propagator's "temper" key, move.h:7840) for replicas
+ *       that share one process and one GPU instead of one MPI rank each: one
+ *       exchange attempt on every rank in one device call, and the ranks'
+ *       GPUPropagators driven in lock-step up to each temper pick.
  *
  * Errors from libfgx become std::runtime_error, as the reference reports
  * configuration errors.
@@ -33,9 +38,12 @@
 #include <faunus/faunus.h>
 #include <faunus/ewald.h>
 
+#include <cmath>
+#include <map>
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../fgx.h"
@@ -186,6 +194,13 @@ public:
 
     fgx_system *handle() { return h; }
     bool hasPressure() const { return prm.has_pressure != 0; }
+    const AMD::GPUParams &params() const { return prm; }
+
+    /* hand device systems over between two ranks' terms: after a replica
+     * exchange the configuration a rank receives stays in the device system
+     * that held it, which the exchange has relabelled with the receiving
+     * rank's Bjerrum length and engines */
+    void swapHandle(GPUNonbonded &o) { std::swap(h, o.h); }
 
     static std::string device()
     {
@@ -474,6 +489,14 @@ public:
      * one), dp and run fraction (move.h:5455-5464) */
     int npt_index = -1, nregistered = 0;
     double npt_dp = 0, npt_prob = 1;
+    /* a ParallelTempering move ("temper", move.h:7840): its position in the
+     * pick list, its JSON entry, and the number of moves the Propagator
+     * picks among */
+    int temper_index = -1, nmoves = 0;
+    Tmjson temper_json;
+
+    Energy::GPUNonbonded<Tspace> &energy() { return gpu; }
+    Tspace &space() { return spc; }
 
     GPUPropagator(Tmjson &in, Energy::GPUNonbonded<Tspace> &e, Tspace &s) : gpu(e), spc(s)
     {
@@ -484,6 +507,13 @@ public:
         for (auto i = m.begin(); i != m.end(); ++i) {
             bool at = i.key() == "atomtranslate", tr = i.key() == "moltransrot";
             if (i.key() == "_jsonfile") continue;
+            if (i.key() == "temper") {
+                // registered as the reference registers it under MPI; the
+                // exchanges run through Move::GPUParallelTempering
+                temper_index = pos++;
+                temper_json = i.value();
+                continue;
+            }
             if (i.key() == "random") {
                 if (!i.value().is_object()) continue;
                 if (i.value().value("hardware", false))
@@ -523,6 +553,10 @@ public:
                                         v["dprot"] | 0.0, center));
             }
         }
+        nmoves = pos;
+        if (temper_index >= 0 && (npt_index >= 0 || nregistered != 1))
+            throw std::runtime_error("GPUPropagator: temper runs beside exactly one "
+                                     "atomtranslate or moltransrot move");
         pushGlobalEngine();
         AMD::check(fgx_propagator_init(gpu.handle()));
         if (reseed) {
@@ -560,9 +594,68 @@ public:
         is >> slump.eng;
     }
 
-    /* n Propagator::move() calls; returns the summed energy change */
+    /* the move-internal engine (Movebase::_slump) of the device run */
+    RandomTwister<> internalEngine()
+    {
+        uint32_t w[624];
+        int pos;
+        AMD::check(fgx_rng_get_internal(gpu.handle(), w, &pos));
+        std::ostringstream o;
+        for (int k = 0; k < 624; ++k) o << w[k] << " ";
+        o << pos;
+        std::istringstream is(o.str());
+        RandomTwister<> r;
+        is >> r.eng;
+        return r;
+    }
+
+    void setInternalEngine(RandomTwister<> &r)
+    {
+        std::ostringstream o;
+        o << r.eng;
+        std::istringstream is(o.str());
+        uint32_t w[624];
+        for (int k = 0; k < 624; ++k) { unsigned long v; is >> v; w[k] = (uint32_t)v; }
+        int pos;
+        is >> pos;
+        AMD::check(fgx_rng_set_internal(gpu.handle(), w, pos));
+    }
+
+    /* Propagator::move() calls before the next one that picks the temper
+     * move, at most m: replays the picks on a copy of the move engine with
+     * the draws each call of the device move takes from it (the pick,
+     * randomMolId over one molecule, Movebase::run; move.h:515-526,
+     * 575-581, 7887) */
+    int callsBeforeTemper(int m)
+    {
+        if (temper_index < 0) return m;
+        RandomTwister<> e = internalEngine();
+        for (int c = 0; c < m; ++c) {
+            if (e.range(0, nmoves - 1) == temper_index) return c;
+            (void)e.range(0, 0);
+            (void)e();
+        }
+        return m;
+    }
+
+    /* the Propagator's pick of the temper move, taken from the device move
+     * engine (the move's own draws follow in the exchange) */
+    void takeTemperPick()
+    {
+        RandomTwister<> e = internalEngine();
+        if (e.range(0, nmoves - 1) != temper_index)
+            throw std::runtime_error("GPUPropagator: the next pick is not the temper move");
+        setInternalEngine(e);
+    }
+
+    /* n Propagator::move() calls; returns the summed energy change.  With a
+     * temper move in the list the calls must end before its next pick
+     * (Move::GPUParallelTempering::propagate drives the ranks through it). */
     double move(int n = 1)
     {
+        if (temper_index >= 0 && callsBeforeTemper(n) < n)
+            throw std::runtime_error("GPUPropagator: a temper move is picked within these calls; "
+                                     "drive the ranks with GPUParallelTempering::propagate");
         if (npt_index < 0)
             AMD::check(fgx_propagate(gpu.handle(), n, &last, nullptr, nullptr, 0));
         else if (nregistered == 0)
@@ -592,6 +685,142 @@ public:
         }
         pullGlobalEngine();
         return last.utot;
+    }
+};
+
+/* Move::ParallelTempering (move.h:6520-6772) for replicas that share one
+ * process and one GPU: rank r is the r-th GPUPropagator (its Space, its
+ * GPUNonbonded term with its own Bjerrum length, its engines).  One move()
+ * is the temper move of every rank at once (what R MPI ranks do together):
+ * the run() draw of each rank's move engine, the partner direction from a
+ * controller engine shared by all ranks (MPIController::random), du =
+ * (U_r(x_p) - U_r(x_r)) + (U_p(x_r) - U_p(x_p)) (_energyChange) and the
+ * Metropolis draw of each rank's global engine, all in one device call
+ * (fgx_temper_exchange).  An accepted pair swaps configurations as the
+ * reference's particle transmission does: the device systems holding them
+ * change owners and the ranks' Spaces receive the new coordinates.  The
+ * reference needs the ranks' engines in lock-step (its temper moves meet
+ * over MPI); propagate() requires the same and throws otherwise.  Build
+ * every rank's Space before the ranks' GPUPropagators: each Space adds the
+ * input's atom list to Faunus's global atom table, which the device systems
+ * copy, and the ranks' Hamiltonians must agree in everything but lB. */
+template <class Tspace>
+class GPUParallelTempering {
+    std::vector<GPUPropagator<Tspace> *> rk;
+    std::vector<fgx_system *> cfg;   // device systems in creation order
+    std::vector<int> slot;           // slot[r]: device system rank r holds
+    fgx_temper *t = nullptr;
+
+public:
+    /* per exchange: the Propagator call it took place in, and per rank the
+     * swap flag, the pair's energy change and what the rank's move returned */
+    struct Record {
+        long call;
+        std::vector<int> acc;
+        std::vector<double> du, ret;
+    };
+    std::vector<Record> history;
+    std::map<std::string, Average<double>> accmap;   // acceptance per rank pair (_info)
+    long calls = 0;
+
+    /* ctl_seed seeds the controller engine (MPIController::random) */
+    GPUParallelTempering(const std::vector<GPUPropagator<Tspace> *> &ranks, uint32_t ctl_seed = 5489u)
+        : rk(ranks)
+    {
+        const int R = (int)rk.size();
+        if (R < 2) throw std::runtime_error("GPUParallelTempering: needs two or more ranks");
+        for (auto *p : rk) {
+            if (p->temper_index < 0)
+                throw std::runtime_error("GPUParallelTempering: a rank's Propagator has no temper move");
+            cfg.push_back(p->energy().handle());
+            slot.push_back((int)slot.size());
+        }
+        double prob = rk[0]->temper_json.value("prob", 1.0);   // runfraction (move.h:6602)
+        AMD::check(fgx_temper_create(cfg.data(), R, ctl_seed, prob, &t));
+    }
+    ~GPUParallelTempering() { if (t) fgx_temper_destroy(t); }
+    GPUParallelTempering(const GPUParallelTempering &) = delete;
+    GPUParallelTempering &operator=(const GPUParallelTempering &) = delete;
+
+    int size() const { return (int)rk.size(); }
+
+    /* one ParallelTempering::move on every rank; returns what each rank's
+     * Movebase::move returned */
+    std::vector<double> move()
+    {
+        const int R = size();
+        Record rec{calls, std::vector<int>(R), std::vector<double>(R), std::vector<double>(R)};
+        AMD::check(fgx_temper_exchange(t, rec.acc.data(), rec.du.data()));
+        AMD::check(fgx_temper_returns(t, rec.ret.data()));
+        std::vector<int> now(R);
+        AMD::check(fgx_temper_slots(t, now.data()));
+        // partners are neighbouring ranks sharing one finite energy change
+        // (findPartner, move.h:6616-6628); an accepted pair's device systems
+        // change owners with the configurations they hold
+        for (int r = 0; r + 1 < R; ++r) {
+            const int p = r + 1;
+            if (!(rec.du[r] < HUGE_VAL) || rec.du[r] != rec.du[p]) continue;
+            std::ostringstream id;
+            id << r << " <-> " << p;   // ParallelTempering::id (move.h:6738-6746)
+            accmap[id.str()] += rec.acc[r] ? 1.0 : 0.0;
+            if (rec.acc[r] && now[p] == slot[r] && now[r] == slot[p]) {
+                rk[r]->energy().swapHandle(rk[p]->energy());
+                download(r);
+                download(p);
+            }
+            ++r;
+        }
+        slot = now;
+        history.push_back(rec);
+        return rec.ret;
+    }
+
+    /* n Propagator::move() calls of every rank in lock-step: the device
+     * sweeps up to the next temper pick, then the exchange */
+    void propagate(int n)
+    {
+        int done = 0;
+        while (done < n) {
+            int k = rk[0]->callsBeforeTemper(n - done);
+            for (auto *p : rk)
+                if (p->callsBeforeTemper(n - done) != k)
+                    throw std::runtime_error("GPUParallelTempering: the ranks' move engines are "
+                                             "not in lock-step");
+            if (k > 0) {
+                for (auto *p : rk) p->move(k);
+                done += k;
+                calls += k;
+            }
+            if (done < n) {
+                for (auto *p : rk) p->takeTemperPick();
+                move();
+                for (auto *p : rk) p->pullGlobalEngine();
+                ++done;
+                ++calls;
+            }
+        }
+    }
+
+private:
+    /* the rank's Space after receiving a configuration (_acceptMove,
+     * move.h:6755-6762): p, trial and the mass centres */
+    void download(int r)
+    {
+        Tspace &spc = rk[r]->space();
+        fgx_system *h = rk[r]->energy().handle();
+        std::vector<double> xyz(3 * spc.p.size());
+        AMD::check(fgx_get_positions(h, 0, xyz.data()));
+        for (size_t i = 0; i < spc.p.size(); ++i) {
+            spc.p[i].x() = xyz[3 * i]; spc.p[i].y() = xyz[3 * i + 1]; spc.p[i].z() = xyz[3 * i + 2];
+            spc.trial[i] = spc.p[i];
+        }
+        int gi = 0;
+        for (auto g : spc.groupList()) {
+            double cm[3], ct[3];
+            AMD::check(fgx_get_group_cm(h, gi++, cm, ct));
+            g->cm = Point(cm[0], cm[1], cm[2]);
+            g->cm_trial = g->cm;
+        }
     }
 };
 

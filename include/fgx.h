@@ -172,6 +172,9 @@ int fgx_add_move(fgx_system *s, int type, int molid, double prob, int peratom, i
 int fgx_rng_seed(fgx_system *s, uint32_t seed);
 int fgx_rng_set_state(fgx_system *s, const uint32_t words[624], int pos);
 int fgx_rng_get_state(fgx_system *s, uint32_t words[624], int *pos);
+/* the move-internal engine (Movebase::_slump, move.h:356-360): full state */
+int fgx_rng_set_internal(fgx_system *s, const uint32_t words[624], int pos);
+int fgx_rng_get_internal(fgx_system *s, uint32_t words[624], int *pos);
 /* Propagator constructor: copies the global engine into the move engine
  * (move.h:7781) */
 int fgx_propagator_init(fgx_system *s);
@@ -362,6 +365,11 @@ int fgx_temper_records(fgx_temper *t, int *rounds, int *acc, double *du, int *sl
                        double *sweep_seconds);
 /* slot[r] = index of the configuration (replica handle) rank r holds */
 int fgx_temper_slots(fgx_temper *t, int *slot);
+/* what each rank's ParallelTempering::move returned in the last exchange
+ * (Movebase::move, move.h:515-555): after an accepted swap the rank's own
+ * unew - uold (useAlternativeReturnEnergy, move.h:6604) plus the Ewald
+ * update's energy change, else 0 */
+int fgx_temper_returns(fgx_temper *t, double *ret);
 int fgx_temper_destroy(fgx_temper *t);
 
 #ifdef __cplusplus

@@ -11,9 +11,12 @@
 // Built from the reference sources in place by oracle/Makefile (target
 // adapter), output in oracle/_ref/, run by tests/test_gpu_adapter.py.
 //
-// Usage: adapter_test <kind> <config.json> <statefile> <ncalls>
+// Usage: adapter_test <kind> <config.json> <statefile> <ncalls> [ranks]
 // (kinds ending in _npt: ExternalPressure and a Propagator holding every move
-// of the input, Isobaric among them)
+// of the input, Isobaric among them; dhhs_temper: `ranks` replicas of the
+// input at Bjerrum lengths lB (1 + r/4), each with a GPUPropagator of the
+// input's moves (a "temper" move among them), driven in lock-step by
+// Move::GPUParallelTempering -- printed for the test to replay on the oracle)
 
 #include <faunus/faunus.h>
 #include <faunus/ewald.h>
@@ -245,6 +248,58 @@ static int run_npt(Tmjson &in, const char *state, int ncalls, int elec, int sr, 
     return 0;
 }
 
+// D: replicas on one GPU, Move::GPUParallelTempering over their GPUPropagators
+static int run_temper(Tmjson &in, const char *state, int ncalls, int R)
+{
+    std::string eng0;   // the state file's engine: every rank starts from it
+    std::vector<std::unique_ptr<Tspace>> spc;
+    std::vector<std::unique_ptr<Energy::GPUNonbonded<Tspace>>> pot;
+    std::vector<std::unique_ptr<Move::GPUPropagator<Tspace>>> prop;
+    std::vector<Move::GPUPropagator<Tspace> *> ranks;
+    auto prm0 = AMD::GPUParams::fromJson(in, FGX_ELEC_DH, FGX_SR_HS, FGX_NB_PLAIN, FGX_FP64);
+    // every Space first: each one reads the input's atom list into the
+    // global atom table, which the device systems then copy
+    for (int r = 0; r < R; ++r) {
+        spc.emplace_back(new Tspace(in));
+        spc[r]->load(state, Tspace::RESIZE);
+        if (r == 0) eng0 = engine_text();
+    }
+    for (int r = 0; r < R; ++r) {
+        auto prm = prm0;
+        prm.lB = prm0.lB * (1.0 + 0.25 * r);
+        pot.emplace_back(new Energy::GPUNonbonded<Tspace>(prm));
+        std::istringstream is(eng0);
+        is >> slump.eng;
+        prop.emplace_back(new Move::GPUPropagator<Tspace>(in, *pot[r], *spc[r]));
+        ranks.push_back(prop[r].get());
+    }
+    Move::GPUParallelTempering<Tspace> pt(ranks, 11u);
+    pt.propagate(ncalls);
+    std::printf("{\n\"D_lB0\": %.17g,\n\"D_rng0\": \"%s\",\n", prm0.lB, eng0.c_str());
+    std::vector<double> calls, du, ret;
+    std::vector<int> acc;
+    for (auto &h : pt.history) {
+        calls.push_back((double)h.call);
+        for (int r = 0; r < R; ++r) {
+            acc.push_back(h.acc[r]);
+            du.push_back(h.du[r]);
+            ret.push_back(h.ret[r]);
+        }
+    }
+    pvec("D_calls", calls);
+    pivec("D_acc", acc);
+    pvec("D_du", du);
+    pvec("D_ret", ret);
+    std::vector<double> x;
+    for (int r = 0; r < R; ++r) {
+        auto c = coords(*spc[r]);
+        x.insert(x.end(), c.begin(), c.end());
+    }
+    pvec("D_xyz", x);
+    std::printf("\"D_swaps\": %d\n}\n", (int)pt.accmap.size());
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     std::cout.setstate(std::ios_base::failbit);
@@ -264,6 +319,8 @@ int main(int argc, char **argv)
     if (kind == "lj") return run<E_lj>(kind, in, argv[3], ncalls, FGX_ELEC_NONE, FGX_SR_LJ, FGX_NB_PLAIN);
     if (kind == "ewaldlj") return run<E_ewaldlj>(kind, in, argv[3], ncalls, FGX_ELEC_EWALD, FGX_SR_LJ, FGX_NB_VECTOR);
     if (kind == "ljcut") return run<E_ljcut>(kind, in, argv[3], ncalls, FGX_ELEC_NONE, FGX_SR_LJ, FGX_NB_PLAIN, true);
+    if (kind == "dhhs_temper")
+        return run_temper(in, argv[3], ncalls, argc > 5 ? std::atoi(argv[5]) : 4);
     if (kind == "dhhs_npt")
         return run_npt<E_dhhs>(in, argv[3], ncalls, FGX_ELEC_DH, FGX_SR_HS, FGX_NB_PLAIN);
     if (kind == "ewaldlj_npt")

@@ -1039,6 +1039,17 @@ void or_propagator_init(or_sys *s)
     s->propagator_started = 0;
 }
 
+int or_propagator_pick(or_sys *s, int nmoves)
+{   /* Propagator::move's pick (move.h:7887): an element of its nmoves moves */
+    return or_range(&s->internal, 0, nmoves - 1);
+}
+
+int or_propagator_peek(const or_sys *s, int nmoves)
+{   /* the next pick, the engine left as it is */
+    or_mt e = s->internal;
+    return or_range(&e, 0, nmoves - 1);
+}
+
 /* groups with molId == molid in Space::molTrack order (= group order) */
 static int find_molecules(or_sys *s, int molid, int *out)
 {

@@ -130,6 +130,10 @@ void or_rng_seed(or_sys *s, uint32_t seed);          /* global slump seed */
 int  or_rng_set_text(or_sys *s, const char *txt);     /* global slump state */
 /* Propagator construction copies the global engine into the move engine */
 void or_propagator_init(or_sys *s);
+/* the Propagator's move pick among nmoves moves from the system's move
+ * engine (move.h:7887); returns the index */
+int or_propagator_pick(or_sys *s, int nmoves);
+int or_propagator_peek(const or_sys *s, int nmoves);
 /* run `ncalls` Propagator::move() calls; records per-trial du and accept
  * flag (up to cap) and returns the number of trials run; *utot gets the sum
  * of returned energies and *uinit the initial system energy. */

@@ -109,6 +109,8 @@ def lib():
         L.or_rng_seed.argtypes = [P, C.c_uint32]
         L.or_rng_set_text.restype = I; L.or_rng_set_text.argtypes = [P, C.c_char_p]
         L.or_propagator_init.argtypes = [P]
+        L.or_propagator_pick.restype = I; L.or_propagator_pick.argtypes = [P, I]
+        L.or_propagator_peek.restype = I; L.or_propagator_peek.argtypes = [P, I]
         L.or_propagate.restype = C.c_long
         L.or_propagate.argtypes = [P, I, dp, ip, C.c_long, dp, dp]
         L.or_last_trials.restype = C.c_long
@@ -224,13 +226,13 @@ class OracleSystem:
                              self.ng, iptr(gf), iptr(gb), iptr(gs), iptr(gm))
 
     @classmethod
-    def from_fixture(cls, fx):
+    def from_fixture(cls, fx, lB=None):
         e = fx.get("energy") or fx.get("mc")
         p = fx["particles"]
         nb = fx["config"]["energy"]["nonbonded"]
         kappa = 1.0 / nb["debyelength"] if "debyelength" in nb else 0.0
         sys_ = cls(fx["kind"], fx["box"], p["xyz"], p["charge"], p["radius"], p["mw"], p["id"],
-                   fx["groups"], e["lB"], kappa=kappa,
+                   fx["groups"], e["lB"] if lB is None else lB, kappa=kappa,
                    type_sigma=e["type_sigma"], type_eps=e["type_eps"],
                    type_charge=e["type_charge"], type_radius=e["type_radius"],
                    type_dp=e["type_dp"], g2g_cut=nb.get("cutoff_g2g", 0.0),

@@ -110,3 +110,66 @@ def test_reference_isobaric_on_gpu_energy(name):
     a = np.array(r["A_xyz"])
     assert np.allclose(np.array(r["B_xyz"]), a, rtol=0, atol=1e-9)
     assert np.allclose(np.array(r["C_xyz"]), a, rtol=0, atol=1e-9)
+
+
+def test_parallel_tempering_propagators():
+    """Move::GPUParallelTempering over four ranks' GPUPropagators whose
+    moves are the input's atomtranslate and a "temper" move (prob 0.75): the
+    ranks are replicas of salt_dhhs at Bjerrum lengths lB (1 + r/4), driven
+    in lock-step through the Propagator's picks.  Replayed on the oracle (its
+    Propagator calls, or_pt_step for the temper picks, a controller engine
+    seeded alike): exchanges in the same calls, identical swap decisions,
+    pair energy changes and returned energies within 1e-9, each rank's final
+    coordinates within 1e-9 A."""
+    import torch
+    from oracle_py import OracleEngine, OracleSystem, lib as olib, pt_step
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    if not os.path.exists(BIN):
+        pytest.skip("oracle/_ref/adapter_test not built (needs the reference tree at build time)")
+    fx = load_fixture("salt_dhhs")
+    fx = dict(fx, config=json.loads(json.dumps(fx["config"])))
+    prob = 0.75
+    fx["config"]["moves"]["temper"] = {"prob": prob}
+    R, ncalls = 4, 24
+    print("[adapter] temper, %d ranks, %d calls" % (R, ncalls), flush=True)
+    with tempfile.TemporaryDirectory() as d:
+        c, s = write_inputs(fx, d)
+        out = subprocess.run([BIN, "dhhs_temper", c, s, str(ncalls), str(R)], capture_output=True,
+                             text=True, timeout=300)
+        assert out.returncode == 0, out.stderr[-2000:]
+        r = json.loads(out.stdout)
+    lB0 = r["D_lB0"]
+    ranks = []
+    for k in range(R):
+        o = OracleSystem.from_fixture(fx, lB=lB0 * (1.0 + 0.25 * k))
+        assert olib().or_rng_set_text(o.h, r["D_rng0"].encode()) == 0
+        olib().or_propagator_init(o.h)
+        ranks.append(o)
+    ctl = OracleEngine(11)
+    ex = []
+    for call in range(ncalls):
+        picks = {olib().or_propagator_peek(o.h, 2) for o in ranks}
+        assert len(picks) == 1
+        if picks.pop() == 0:   # "atomtranslate" sorts before "temper"
+            for o in ranks:
+                o.propagate(1)
+        else:
+            for o in ranks:
+                olib().or_propagator_pick(o.h, 2)
+            ex.append((call,) + pt_step(ranks, ctl, prob))
+    print("  %d exchanges, %d swaps" % (len(ex), sum(int(e[1].sum()) // 2 for e in ex)), flush=True)
+    assert len(ex) >= 3
+    assert [int(v) for v in r["D_calls"]] == [e[0] for e in ex]
+    acc = np.array(r["D_acc"]).reshape(-1, R)
+    du = farr(r["D_du"]).reshape(-1, R)
+    ret = farr(r["D_ret"]).reshape(-1, R)
+    for k, (call, oacc, odu, oret) in enumerate(ex):
+        assert np.array_equal(acc[k], oacc), (call, acc[k], oacc)
+        fin = np.isfinite(odu)
+        assert np.array_equal(np.isfinite(du[k]), fin)
+        assert np.allclose(du[k][fin], odu[fin], rtol=1e-9, atol=1e-9)
+        assert np.allclose(ret[k], oret, rtol=1e-9, atol=1e-9)
+    x = np.array(r["D_xyz"]).reshape(R, -1)
+    for k in range(R):
+        assert np.max(np.abs(x[k] - ranks[k].xyz(0).reshape(-1))) <= 1e-9, k
