@@ -150,6 +150,10 @@ struct fgx_system {
     // mixed AtomicTranslation sweeps without Ewald stage the partners in LDS
     // when they fit (fgx_set_lds_partners): 1 on, 0 read them from device memory
     int lds_partners = 1;
+    // where the AtomicTranslation sweep keeps the Ewald state (S(k), dS(k),
+    // A_k, n) during a launch (fgx_set_ewald_state): 0 LDS when it fits, 1
+    // device memory
+    int ewald_state_hbm = 0;
     int cell_nc = 0, cell_cap = 0;         // cells per axis, records per cell (allocated)
     int cell_cap_min = 0;                  // automatic capacity after an overflow
     int *d_cell_cnt = nullptr;             // nc^3 counts + overflow word

@@ -550,7 +550,7 @@ int prepare_batch(std::vector<fgx_system *> &sys, const std::vector<int> &ncalls
     // Ewald chains keep S(k), dS(k), A_k and n in device memory
     bool skg = false;
     for (int c = 0; c < nc && !rc; ++c)
-        skg = skg || (sys[c]->has_ewald && ewald_in_hbm(sys[c]->ew.K));
+        skg = skg || (sys[c]->has_ewald && (ewald_in_hbm(sys[c]->ew.K) || sys[c]->ewald_state_hbm));
     if (!rc && skg) {
         // dynamic LDS then holds only the cell counts
         dyn = 0;
@@ -1006,6 +1006,13 @@ extern "C" int fgx_set_cell_list(fgx_system *s, int mode)
     s->cell_mode = mode;
     s->cell_cap_min = 0;
     s->cell_valid = false;
+    return FGX_OK;
+}
+
+extern "C" int fgx_set_ewald_state(fgx_system *s, int hbm)
+{
+    FGX_CHECK(s && (hbm == 0 || hbm == 1), FGX_EARG, "fgx_set_ewald_state: hbm is 0 or 1");
+    s->ewald_state_hbm = hbm;
     return FGX_OK;
 }
 

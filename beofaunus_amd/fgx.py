@@ -132,6 +132,7 @@ def lib():
         "fgx_set_energy_matrix": (I, [P, I]),
         "fgx_set_cell_list": (I, [P, I]),
         "fgx_set_lds_partners": (I, [P, I]),
+        "fgx_set_ewald_state": (I, [P, I]),
         "fgx_isobaric_propagate": (I, [P, D, D, I, C.POINTER(Stats), _dp, _ip, C.c_long]),
         "fgx_propagate_npt": (I, [P, D, D, I, I, C.POINTER(Stats), _dp, _ip, C.c_long]),
         "fgx_set_space": (I, [P, _dp]),
@@ -411,6 +412,11 @@ class System:
         """Cell list of the mixed AtomicTranslation sweep: 0 off, 1 automatic,
         n >= 2 records per cell."""
         _check(lib().fgx_set_cell_list(self.h, int(mode)))
+
+    def set_ewald_state(self, hbm):
+        """Ewald sweep: S(k), dS(k), A_k and n in LDS when they fit (False,
+        default) or in device memory (True)."""
+        _check(lib().fgx_set_ewald_state(self.h, 1 if hbm else 0))
 
     def set_lds_partners(self, on):
         """Mixed AtomicTranslation sweep without Ewald: keep the partners of a

@@ -255,6 +255,12 @@ int fgx_get_cell_list(const fgx_system *s, int *cells_per_axis, int *capacity);
  * to FP64 rounding of the sum.  on = 1 (default) or 0 (partners read from
  * device memory).  A schedule knob with no reference counterpart. */
 int fgx_set_lds_partners(fgx_system *s, int on);
+/* Ewald AtomicTranslation sweep: keep S(k), the trial's dS(k), A_k and the
+ * k-vector triples in LDS for the launch when they fit (hbm = 0, default) or
+ * in device memory (hbm = 1; always so beyond the LDS capacity), which frees
+ * the LDS for a second workgroup per CU.  The same arithmetic either way.
+ * A schedule knob with no reference counterpart. */
+int fgx_set_ewald_state(fgx_system *s, int hbm);
 /* Autotuner: times `ncalls` Propagator::move() calls of a probe chain under
  * every schedule and keeps the fastest on the probe (whose state advances). */
 int fgx_autotune(fgx_system *probe, int ncalls, int *threads, int *unroll, double *best_seconds);

@@ -247,3 +247,35 @@ def test_i_external(gpu, name):
         assert ui.sum() == pytest.approx(s.g_external(0), rel=1e-12)
     finally:
         s.close()
+
+
+@pytest.mark.parametrize("name", ["ewald_tinfoil", "ewald_surface"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_ewald_state_in_device_memory(gpu, name, precision):
+    """The Ewald sweep with S(k), dS(k), A_k and n in device memory
+    (fgx_set_ewald_state) at k-vector counts that also fit LDS: FP64 against
+    the reference's trials (identical accepts, dU to 1e-10), mixed within
+    1e-5 kT of them."""
+    from fixture_systems import fgx_system_from_fixture
+    say("[ewald state in device memory] %s %s" % (name, precision))
+    fx = load_fixture(name)
+    mc = fx["mc"]
+    s = fgx_system_from_fixture(fx, precision=gpu.FP64 if precision == "fp64" else gpu.MIXED)
+    s.set_ewald_state(True)
+    s.rng_seed(fx["rng_seed"])
+    s.propagator_init()
+    st, du, acc = s.propagate(mc["ncalls"], record=100000)
+    ref_du = farr(mc["du"])
+    ref_acc = np.array(mc["acc"], dtype=np.int32)
+    fin = np.isfinite(ref_du)
+    if precision == "fp64":
+        assert np.array_equal(acc, ref_acc)
+        scale = max(1.0, np.max(np.abs(ref_du[fin])))
+        assert np.all(np.abs(du[fin] - ref_du[fin]) <= REL * np.maximum(np.abs(ref_du[fin]), scale))
+        assert np.allclose(s.positions().reshape(-1), farr(mc["final_xyz"]), rtol=0, atol=1e-9)
+    else:
+        mism = np.nonzero(acc != ref_acc)[0]
+        upto = mism[0] if len(mism) else len(acc)
+        f = fin[:upto] & np.isfinite(du[:upto])
+        assert upto > 0 and np.max(np.abs(du[:upto][f] - ref_du[:upto][f])) <= 1e-5
+    s.close()
