@@ -12,7 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "libfgx.so")
-SOURCES = ["fgx_sweep_at_ewald_mixed.hip", "fgx_sweep_at_dh.hip", "fgx_sweep_at_plain.hip",
+SOURCES = ["fgx_sweep_blk_ewald.hip", "fgx_sweep_blk_dh.hip",
+           "fgx_sweep_at_ewald_mixed.hip", "fgx_sweep_at_dh.hip", "fgx_sweep_at_plain.hip",
            "fgx_sweep_at_ewald64.hip", "fgx_sweep_tr.hip", "fgx_sweep.hip", "fgx_system.hip",
            "fgx_temper.hip", "fgx_server.hip"]
 

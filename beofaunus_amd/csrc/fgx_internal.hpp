@@ -162,6 +162,16 @@ struct fgx_system {
     int *d_cell_slot = nullptr;            // n: cell * cap + slot
     size_t cell_bytes = 0;
     bool cell_valid = false;               // cleared whenever the accepted state changes
+    // block AtomicTranslation sweep (fgx_set_block_sweep): 0 off, 1 where an
+    // instance exists; workgroups per chain (0: automatic)
+    int blk_mode = 0;
+    int blk_nsplit = 0;
+    int blk_parts = 0;                     // workgroups of the last block sweep (0: one-trial sweep)
+    double *d_xbuf = nullptr;              // exchange slots of the chain's parts
+    int *d_xcnt = nullptr;                 // arrival / exit counters (zero between launches)
+    int *d_xerr = nullptr;                 // launch error word (batch owner)
+    int2 *d_blkmap = nullptr;              // workgroup -> (chain, part) of the batch (owner)
+    int blkmap_cap = 0;
     // Energy::ExternalPressure term (1/A^3)
     bool press_on = false;
     double press = 0.0;

@@ -12,7 +12,7 @@
 #include <cstring>
 #include <vector>
 
-#include "fgx_sweep_kernels.hpp"
+#include "fgx_sweep_blk.hpp"
 
 namespace fgx {
 
@@ -444,7 +444,177 @@ struct Launch {
     hipStream_t st = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = FGX_OK;
+    int *d_xerr = nullptr;   // block sweep: the launch's error word
 };
+
+static int blk_instance(bool mix, const Pot &P, size_t *static_lds, int *block)
+{
+    return blk_instance_pair(mix, P, static_lds, block) ||
+           blk_instance_ewald(mix, P, static_lds, block);
+}
+
+// Automatic workgroups per chain for block sweeps: with at least half as many
+// chains as compute units every chain keeps one workgroup; otherwise the
+// units go, one at a time, to the chain with the most pair work per
+// workgroup (n^2 / parts), a part keeping at least 512 partner rows.
+static std::vector<int> blk_auto_parts(const std::vector<fgx_system *> &sys, int ncu)
+{
+    const int nc = (int)sys.size();
+    std::vector<int> parts(nc, 1);
+    int used = nc;
+    if (2 * nc >= ncu) return parts;
+    for (;;) {
+        int best = -1;
+        double bw = 0.0;
+        for (int c = 0; c < nc; ++c) {
+            const fgx_system *s = sys[c];
+            if (s->blk_nsplit > 0) continue;
+            const int cap = std::min(BLK_NSPLIT_MAX, std::max(1, s->n / 512));
+            if (parts[c] >= cap) continue;
+            const double w = (double)s->n * s->n / parts[c];
+            if (w > bw) { bw = w; best = c; }
+        }
+        if (best < 0 || used + 1 > ncu) break;
+        ++parts[best];
+        ++used;
+    }
+    return parts;
+}
+
+// Block-sweep plan of a batch: every chain must have an instance and fit,
+// else the batch keeps the one-trial sweep (an explicit split that cannot be
+// honoured is an error).  Sets the chains' parts, LDS layout and exchange
+// buffers, and the workgroup map: a chain's parts on one XCD (workgroups are
+// dealt to the 8 XCDs round-robin by index), the XCDs balanced by workgroups.
+static int plan_blk(std::vector<fgx_system *> &sys, std::vector<ChainDev> &chains_io, bool skg,
+                    BatchPlan &B)
+{
+    const int nc = (int)sys.size();
+    // changes go to a copy, kept only when the batch takes the block sweep
+    std::vector<ChainDev> chains = chains_io;
+    fgx_system *s0 = sys[0];
+    B.blk = false;
+    const bool mix = s0->pot.precision == FGX_MIXED;
+    size_t st_lds = 0;
+    int bsz = 0;
+    (void)skg;   // the block sweep lays out its own (per-part) Ewald state
+    if (!s0->blk_mode || s0->moves[0].type != MOVE_AT) return FGX_OK;
+    if (!blk_instance(mix, s0->pot, &st_lds, &bsz) || st_lds == 0) return FGX_OK;
+    for (int c = 0; c < nc; ++c) {
+        const fgx_system *s = sys[c];
+        if (!s->blk_mode || chains[c].cell_nc > 0 || (mix && !chains[c].fx) ||
+            s->pot.ntypes2 > LDS_TYPES2 || s->n < 1 || s->n >= (1 << 24))
+            return FGX_OK;
+    }
+    int dev = 0, ncu = 0;
+    FGX_HIP(hipGetDevice(&dev));
+    FGX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const size_t lds_cap = 160 * 1024;
+    std::vector<int> parts = blk_auto_parts(sys, ncu);
+    if (!B.allow_split) {
+        for (int c = 0; c < nc; ++c) {
+            FGX_CHECK(sys[c]->blk_nsplit <= 1, FGX_EUNSUPPORTED,
+                      "block sweep: this entry point runs each chain in one workgroup");
+            parts[c] = 1;
+        }
+    }
+    size_t dyn = 0;
+    bool split = false;
+    for (int c = 0; c < nc; ++c) {
+        fgx_system *s = sys[c];
+        int P = s->blk_nsplit > 0 ? s->blk_nsplit : parts[c];
+        FGX_CHECK(P <= BLK_NSPLIT_MAX && P <= s->n, FGX_EARG,
+                  "block sweep: more workgroups per chain than partners or than 32");
+        const int K = s->has_ewald ? s->ew.K : 0;
+        const int ks = (K + P - 1) / P;
+        const int kcc1 = s->has_ewald ? s->ew.kcc + 1 : 1;
+        const int ns = (s->n + P - 1) / P;
+        const BlkLds base = blk_lds(ks, bsz, kcc1, s->has_ewald ? 1 : 0, 0);
+        const BlkLds full = blk_lds(ks, bsz, kcc1, s->has_ewald ? 1 : 0, ns);
+        if (st_lds + base.total > lds_cap) {
+            FGX_CHECK(s->blk_nsplit == 0, FGX_EUNSUPPORTED,
+                      "block sweep: the Ewald k-vectors of one workgroup exceed its LDS");
+            return FGX_OK;
+        }
+        const bool ldsp = s->lds_partners && st_lds + full.total <= lds_cap;
+        chains[c].ldsp = ldsp ? 1 : 0;
+        chains[c].sp_j0 = 0;
+        chains[c].sp_j1 = s->n;
+        dyn = std::max(dyn, ldsp ? full.total : base.total);
+        parts[c] = P;
+        split = split || P > 1;
+        if (!s->d_xbuf) {
+            FGX_HIP(hipMalloc(&s->d_xbuf, sizeof(double) * 2 * BLK_NSPLIT_MAX * BLK_NV_MAX));
+            FGX_HIP(hipMalloc(&s->d_xcnt, sizeof(int) * 2));
+            FGX_HIP(hipMemset(s->d_xcnt, 0, sizeof(int) * 2));
+            s->device_bytes += sizeof(double) * 2 * BLK_NSPLIT_MAX * BLK_NV_MAX + sizeof(int) * 2;
+        }
+        chains[c].nsplit = P;
+        chains[c].xbuf = s->d_xbuf;
+        chains[c].xcnt = s->d_xcnt;
+    }
+    if (!s0->d_xerr) {
+        FGX_HIP(hipMalloc(&s0->d_xerr, sizeof(int)));
+        FGX_HIP(hipMemset(s0->d_xerr, 0, sizeof(int)));
+    }
+    for (int c = 0; c < nc; ++c) chains[c].xerr = s0->d_xerr;
+    const int nx = 8;
+    std::vector<std::vector<int2>> per(nx);
+    std::vector<int> order(nc);
+    for (int c = 0; c < nc; ++c) order[c] = c;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return (double)parts[a] * sys[a]->n > (double)parts[b] * sys[b]->n;
+    });
+    for (int c : order) {
+        int x = 0;
+        for (int y = 1; y < nx; ++y)
+            if (per[y].size() < per[x].size()) x = y;
+        for (int p = 0; p < parts[c]; ++p) per[x].push_back(make_int2(c, p));
+    }
+    size_t rows = 0;
+    for (int x = 0; x < nx; ++x) rows = std::max(rows, per[x].size());
+    const int ngrid = (int)rows * nx;
+    // parts of a chain wait for each other: all workgroups must be resident
+    // together (one 512-thread workgroup per compute unit)
+    FGX_CHECK(!split || ngrid <= ncu, FGX_EUNSUPPORTED,
+              "block sweep: chains split over more workgroups than the device holds at once");
+    std::vector<int2> map((size_t)ngrid, make_int2(-1, -1));
+    for (int x = 0; x < nx; ++x)
+        for (size_t r = 0; r < per[x].size(); ++r) map[r * nx + x] = per[x][r];
+    if (s0->blkmap_cap < ngrid) {
+        if (s0->d_blkmap) (void)hipFree(s0->d_blkmap);
+        s0->d_blkmap = nullptr;
+        s0->blkmap_cap = 0;
+        FGX_HIP(hipMalloc(&s0->d_blkmap, sizeof(int2) * ngrid));
+        s0->blkmap_cap = ngrid;
+    }
+    FGX_HIP(hipMemcpyAsync(s0->d_blkmap, map.data(), sizeof(int2) * ngrid, hipMemcpyHostToDevice,
+                           s0->stream));
+    for (int c = 0; c < nc; ++c) sys[c]->blk_parts = parts[c];
+    chains_io = chains;
+    B.blk = true;
+    B.d_map = s0->d_blkmap;
+    B.ngrid = ngrid;
+    B.nthr = BLK_THREADS;
+    B.dyn = dyn;
+    B.d_xerr = s0->d_xerr;
+    return FGX_OK;
+}
+
+// the launch error word of a block sweep: FGX_EDEVICE when a part gave up
+// waiting for the others; clears it and the chains' counters
+int blk_check(const std::vector<fgx_system *> &sys, int *d_xerr)
+{
+    if (!d_xerr) return FGX_OK;
+    int err = 0;
+    FGX_HIP(hipMemcpy(&err, d_xerr, sizeof(int), hipMemcpyDeviceToHost));
+    if (!err) return FGX_OK;
+    FGX_HIP(hipMemset(d_xerr, 0, sizeof(int)));
+    for (fgx_system *s : sys)
+        if (s->d_xcnt) FGX_HIP(hipMemset(s->d_xcnt, 0, sizeof(int) * 2));
+    set_error("block sweep: a workgroup of a split chain waited too long for the others");
+    return FGX_EDEVICE;
+}
 
 // Everything of a batched launch up to the kernel: per-chain state on the
 // device, cell lists and energy caches rebuilt where stale, the ChainDev
@@ -576,6 +746,13 @@ int prepare_batch(std::vector<fgx_system *> &sys, const std::vector<int> &ncalls
     }
     // the cell scan spreads 27 cells over waves 1..7 of a 512-thread block
     if (cells) nthr = AT_THREADS;
+    for (int c = 0; c < nc; ++c) {
+        chains[c].nsplit = 1;
+        chains[c].xbuf = nullptr;
+        chains[c].xcnt = nullptr;
+        chains[c].xerr = nullptr;
+        if (!rc) sys[c]->blk_parts = 0;
+    }
     if (!rc) {
         // cell lists that no longer match the accepted state
         std::vector<ChainDev> stale;
@@ -630,6 +807,9 @@ int prepare_batch(std::vector<fgx_system *> &sys, const std::vector<int> &ncalls
                     if (chains[c].emat) sys[c]->emat_valid = true;
         }
     }
+    BatchPlan BB;
+    BB.allow_split = B.allow_split;
+    if (!rc) rc = plan_blk(sys, chains, skg, BB);
     if (!rc) {
         ChainDev *d_chains = (ChainDev *)s0->d_batch;
         FGX_HIP(hipMemcpyAsync(d_chains, chains.data(), sizeof(ChainDev) * nc, hipMemcpyHostToDevice,
@@ -645,6 +825,14 @@ int prepare_batch(std::vector<fgx_system *> &sys, const std::vector<int> &ncalls
         B.skg = skg;
         B.pot = s0->pot;
         B.move_type = s0->moves[0].type;
+        B.blk = BB.blk;
+        if (BB.blk) {
+            B.d_map = BB.d_map;
+            B.ngrid = BB.ngrid;
+            B.nthr = BB.nthr;
+            B.dyn = BB.dyn;
+            B.d_xerr = BB.d_xerr;
+        }
     }
     return rc;
 }
@@ -653,6 +841,18 @@ int prepare_batch(std::vector<fgx_system *> &sys, const std::vector<int> &ncalls
 // capture records exactly this launch).
 int enqueue_batch(const BatchPlan &B, hipStream_t st)
 {
+    if (B.blk) {
+        const bool mix = B.pot.precision == FGX_MIXED;
+        int rc = B.pot.elec == ELEC_EWALD
+                     ? launch_blk_ewald(mix, B.pot, st, B.d_chains, B.d_map, B.ngrid, B.nthr, B.dyn)
+                     : launch_blk_pair(mix, B.pot, st, B.d_chains, B.d_map, B.ngrid, B.nthr, B.dyn);
+        if (rc) {
+            set_error("block sweep: no kernel for this Hamiltonian");
+            return rc;
+        }
+        FGX_HIP(hipGetLastError());
+        return FGX_OK;
+    }
     if (B.move_type == MOVE_AT)
         return B.pot.precision == FGX_MIXED
                    ? launch_at<true>(B.pot, st, B.d_chains, B.nchains, B.nthr, B.dyn, B.ncalls,
@@ -687,18 +887,20 @@ int mirror_trial(std::vector<fgx_system *> &sys, hipStream_t st)
 // systems overlap on the device.
 static int launch_chains(std::vector<fgx_system *> &sys, const std::vector<int> &ncalls,
                          const std::vector<long> *max_trials, double *du_rec, int *acc_rec,
-                         long cap, Launch &L)
+                         long cap, Launch &L, bool allow_split = true)
 {
     hipStream_t st = sys[0]->stream;
     L.sys = sys;
     L.st = st;
     BatchPlan B;
+    B.allow_split = allow_split;
     int rc = prepare_batch(sys, ncalls, max_trials, du_rec, acc_rec, cap, B);
     if (!rc) {
         FGX_HIP(hipEventCreate(&L.e0));
         FGX_HIP(hipEventCreate(&L.e1));
         FGX_HIP(hipEventRecord(L.e0, st));
         rc = enqueue_batch(B, st);
+        L.d_xerr = B.blk ? B.d_xerr : nullptr;
         FGX_HIP(hipEventRecord(L.e1, st));
         if (!rc) rc = mirror_trial(sys, st);
     }
@@ -720,6 +922,7 @@ static int finish_chains(Launch &L, double *seconds, std::vector<long> *stats_i,
         set_error(std::string("sweep kernel failed: ") + hipGetErrorString(err));
         rc = FGX_EHIP;
     }
+    if (!rc) rc = blk_check(L.sys, L.d_xerr);
     float ms = 0;
     if (!rc && L.e0 && L.e1) FGX_HIP(hipEventElapsedTime(&ms, L.e0, L.e1));
     if (seconds) *seconds = ms * 1e-3;
@@ -743,7 +946,7 @@ int batch_launch(std::vector<fgx_system *> &sys, const std::vector<int> &ncalls,
 {
     Launch *L = new Launch;
     *handle = L;
-    return launch_chains(sys, ncalls, &max_trials, nullptr, nullptr, 0, *L);
+    return launch_chains(sys, ncalls, &max_trials, nullptr, nullptr, 0, *L, false);
 }
 
 int batch_finish(void *handle, double *seconds, std::vector<long> *stats_i,
@@ -1020,6 +1223,22 @@ extern "C" int fgx_set_lds_partners(fgx_system *s, int on)
 {
     FGX_CHECK(s && (on == 0 || on == 1), FGX_EARG, "fgx_set_lds_partners: on is 0 or 1");
     s->lds_partners = on;
+    return FGX_OK;
+}
+
+extern "C" int fgx_set_block_sweep(fgx_system *s, int mode, int nsplit)
+{
+    FGX_CHECK(s && (mode == 0 || mode == 1) && nsplit >= 0 && nsplit <= BLK_NSPLIT_MAX, FGX_EARG,
+              "fgx_set_block_sweep: mode is 0 or 1, nsplit in 0..32");
+    s->blk_mode = mode;
+    s->blk_nsplit = nsplit;
+    return FGX_OK;
+}
+
+extern "C" int fgx_get_block_parts(const fgx_system *s, int *parts)
+{
+    FGX_CHECK(s && parts, FGX_EARG, "fgx_get_block_parts: bad arguments");
+    *parts = s->blk_parts;
     return FGX_OK;
 }
 

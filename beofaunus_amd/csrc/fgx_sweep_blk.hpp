@@ -1,0 +1,1098 @@
+// fgx_sweep_blk.hpp -- block AtomicTranslation sweep: the device-resident
+// Propagator::move (move.h:7877-7891) over AtomicTranslation trials
+// (move.h:818-901) evaluated B at a time.
+//
+// The random numbers of a trial (group, particle, three displacement
+// canonicals, the Metropolis uniform; move.h:825, group.h:132, move.h:843-845,
+// move.h:570) do not depend on earlier decisions, so B consecutive trials of a
+// Propagator call are drawn ahead and their energy changes evaluated against
+// one snapshot of the accepted configuration in a single pass over the
+// partners: every partner record is loaded once for B trials.  The trials are
+// then decided in the reference's order with exact corrections for what the
+// earlier trials of the block changed:
+//
+//   dU_t = P_t + sum_{j in block, j != i_t} [u(n_t, x_j) - u(o_t, x_j)]
+//               + 2 (2 pi / V) lB sum_{s < t accepted} sum_k A_k Re(dS_s* dS_t)
+//               + surface term with the current dipole,
+//
+// where P_t sums the partners outside the block at their snapshot positions
+// and x_j is the block particle's position just before trial t (its new
+// position if an earlier trial of the block moved it and was accepted).  A
+// trial whose particle an earlier accepted trial of the block moved, and the
+// trials after an accepted move that triggers the Ewald S(k) refresh
+// (ewald.h:626-643), are left for the next block, with their draws.  The
+// decisions are the sequential ones; energy changes differ from a one-trial
+// sweep in rounding only.
+//
+// A chain may span several workgroups ("parts"): part p owns the partner rows
+// [p n / P, (p + 1) n / P) and the k-vectors [p K / P, (p + 1) K / P).  Every
+// part replays the same RNG streams and takes the same decisions from the
+// same totals; per block the parts exchange their partial sums through
+// device memory (one counter barrier per block, bounded spin: a part that
+// waits too long sets the launch's error word and the chain stops).  All
+// parts write every accepted move, so each part reads its own writes.
+#pragma once
+
+#include "fgx_sweep_kernels.hpp"
+
+namespace fgx {
+
+static constexpr int BLK_NSPLIT_MAX = 32;    // workgroups per chain
+static constexpr int BLK_NV_MAX = 16;        // partial sums exchanged per block
+static constexpr int BLK_THREADS = 512;
+static constexpr int BLK_CB_PAIR = 2;        // trials per pass over the partners, mixed path
+static constexpr int BLK_CB_EWALD = 2;
+static constexpr long BLK_SPIN_MAX = 1L << 22;   // s_sleep rounds of a part waiting for the others
+
+// one trial of a block: its draws, the moved particle's data, the proposal
+struct BlkTrial {
+    int i, ti, collide, first;
+    int32_t fnx, fny, fnz, fox, foy, foz;   // fixed-point new / old position
+    int pkw;                                // FP32 charge bits of the packed record
+    float rf;                               // FP32 radius
+    double q64, r64, wq, u, lu;   // lu = -ln u
+    double t3[3];
+    double ox, oy, oz, nx, ny, nz;
+};
+
+// byte offsets of a block-sweep workgroup's dynamic LDS: Ewald S(k) slice
+// (double2), A_k (double), packed n (int); per-trial phase tables
+// [B][old, new][3 kcc1] (double2); the partner slice (LdsPart)
+struct BlkLds {
+    size_t s, a, kid, etab, part, total;
+};
+__host__ __device__ inline BlkLds blk_lds(int ks, int B, int kcc1, int ew, int ns)
+{
+    BlkLds o;
+    o.s = 0;
+    o.a = (size_t)16 * ks;
+    o.kid = (size_t)24 * ks;
+    o.etab = ((size_t)28 * ks + 15) / 16 * 16;
+    o.part = o.etab + (ew ? (size_t)B * 2 * 3 * kcc1 * 16 : 0);
+    o.total = o.part + (size_t)ns * LDS_PART_BYTES;
+    return o;
+}
+
+// trial data a pass over the partners needs, per trial of the block
+struct BlkFx {
+    int32_t nx, ny, nz, ox, oy, oz;
+    float aq, ar;
+    int ti;
+};
+
+// uniform_int_distribution<int>(0, erange - 1) from one word (Lemire's
+// method, as mt_range); ok = false when the method needs another word
+__device__ __forceinline__ uint32_t lemire_word(uint32_t word, uint32_t erange, bool &ok)
+{
+    const uint64_t prod = (uint64_t)word * (uint64_t)erange;
+    const uint32_t low = (uint32_t)prod;
+    if (low < erange) {
+        const uint32_t thr = (uint32_t)(0u - erange) % erange;
+        if (low < thr) ok = false;
+    }
+    return (uint32_t)(prod >> 32);
+}
+
+#pragma clang fp contract(off)
+// generate_canonical<double,53> from its two words (as mt_canonical)
+__device__ __forceinline__ double canonical_words(uint32_t w1, uint32_t w2)
+{
+    double sum = (double)w1;
+    sum += (double)w2 * 4294967296.0;
+    double r = sum / 18446744073709551616.0;
+    if (r >= 1.0) r = 0x1.fffffffffffffp-1;
+    return r;
+}
+#pragma clang fp contract(on)
+
+template <int ELEC, int SR, bool MIX, int B>
+__global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__restrict__ chains,
+                                                           const int2 *__restrict__ wgmap)
+{
+    constexpr bool EW = (ELEC == ELEC_EWALD);
+    constexpr int NG = EW ? B * (B - 1) / 2 : 0;
+    constexpr int NV = B + NG;
+    static_assert(NV <= BLK_NV_MAX, "block too large");
+    constexpr bool NEED_T = (SR == SR_LJ || ELEC == ELEC_EWALD);
+    // trials per pass over the partners (mixed path)
+    constexpr int CB = EW ? BLK_CB_EWALD : BLK_CB_PAIR;
+    const int2 wm = wgmap[blockIdx.x];
+    if (wm.x < 0) return;
+    const ChainDev &C = chains[wm.x];
+    const int part = wm.y, nsplit = C.nsplit;
+    const int tid = threadIdx.x, nthr = blockDim.x, lane = tid & 63, wv = tid >> 6;
+    const int nw = nthr >> 6;
+    const bool wave0 = tid < 64;
+
+    __shared__ uint32_t mtg[624], mti[624];
+    __shared__ BlkTrial R[B];
+    __shared__ int bidx[B];
+    __shared__ BlkFx F[B];
+    __shared__ double red[BLK_THREADS / 64][NV];
+    __shared__ double redp[BLK_THREADS / 64][B];   // partner-pass sums per wave and trial
+    __shared__ double tot[BLK_NV_MAX];
+    __shared__ double xc[B][B], yc[B][B];
+    __shared__ int dlist[MIX ? DCAP : 1];
+    __shared__ int nqs[BLK_THREADS / 64];
+    __shared__ double ltab[TAB_COUNT * LDS_TYPES2];
+    __shared__ float ltabf[MIX ? TAB_COUNT * LDS_TYPES2 : 1];
+    __shared__ int lgvec[AT_LDS_GROUPS];
+    __shared__ GroupDesc lgroups[AT_LDS_GROUPS];
+    __shared__ double ltdp[AT_LDS_TYPES], leff[AT_LDS_TYPES];
+    __shared__ double lcheb[EW && MIX ? EW_CHEB_MAX : 1];
+    __shared__ double2 rtab[EW ? REFRESH_CHUNK * 3 * REFRESH_KAX : 1];
+    __shared__ double rw[EW ? REFRESH_CHUNK : 1];
+    // accepted moves of the last block (index, new position), for the fetch
+    __shared__ int lmv_i[B];
+    __shared__ double lmv_x[B], lmv_y[B], lmv_z[B];
+    __shared__ int sgo, sabort, snlmv;
+    __shared__ int snd, saccm, srefresh;   // wave 0's decision of the block
+    __shared__ double sdu[B], swq[B];
+    extern __shared__ double2 dyn[];
+
+    // ---- this part's slices
+    const int j0 = (int)((long)part * C.n / nsplit);
+    const int j1 = (int)((long)(part + 1) * C.n / nsplit);
+    const int K = EW ? C.K : 0;
+    const int k0 = (int)((long)part * K / nsplit);
+    const int k1 = (int)((long)(part + 1) * K / nsplit);
+    const int ks = k1 - k0;
+    const int kcc1 = C.kcc + 1;
+    const bool ldsp = C.ldsp != 0;
+    const BlkLds lo = blk_lds(ks, B, kcc1, EW ? 1 : 0, ldsp ? j1 - j0 : 0);
+    char *dbase = (char *)dyn;
+    double2 *S = (double2 *)(dbase + lo.s);
+    double *Ak = (double *)(dbase + lo.a);
+    int *kid = (int *)(dbase + lo.kid);
+    double2 *etab = (double2 *)(dbase + lo.etab);
+    LdsPart L;
+    L.j0 = j0;
+    L.ns = ldsp ? j1 - j0 : 0;
+    L.carve((FGX_LDS void *)(dbase + lo.part), L.ns);
+
+    // ---- chain barrier across the parts (release / acquire at agent scope)
+    int epoch = 0;          // barriers passed (uniform)
+    int xpar = 0;           // exchange buffer in use (alternates per exchange)
+    auto chain_sync = [&]() {
+        __threadfence();
+        __syncthreads();
+        ++epoch;
+        if (tid == 0) {
+            __hip_atomic_fetch_add(C.xcnt, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            const int target = epoch * nsplit;
+            long spins = 0;
+            while (__hip_atomic_load(C.xcnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                __builtin_amdgcn_s_sleep(2);
+                if (++spins > BLK_SPIN_MAX || __hip_atomic_load(C.xerr, __ATOMIC_RELAXED,
+                                                                __HIP_MEMORY_SCOPE_AGENT)) {
+                    __hip_atomic_store(C.xerr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    sabort = 1;
+                    break;
+                }
+            }
+            __threadfence();
+        }
+        __syncthreads();
+    };
+    // tot[0..nv) of this part -> the chain's totals (parts summed in order)
+    auto exchange = [&](int nv) {
+        if (nsplit == 1) return;
+        double *xb = C.xbuf + (size_t)xpar * BLK_NSPLIT_MAX * BLK_NV_MAX;
+        if (tid < nv)
+            __hip_atomic_store(&xb[part * BLK_NV_MAX + tid], tot[tid], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        chain_sync();
+        if (tid < nv) {
+            double s = 0.0;
+            for (int p = 0; p < nsplit; ++p)
+                s += __hip_atomic_load(&xb[p * BLK_NV_MAX + tid], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            tot[tid] = s;
+        }
+        __syncthreads();
+        xpar ^= 1;
+    };
+
+    // ---- chain state into LDS
+    if (tid == 0) {
+        sabort = 0;
+        snlmv = 0;
+    }
+    for (int k = tid; k < 624; k += nthr) {
+        mtg[k] = C.rng[k];
+        mti[k] = C.rng[625 + k];
+    }
+    WaveMT G{mtg, (int)C.rng[624]}, I{mti, (int)C.rng[625 + 624]};
+    const bool lds_grp = C.ngvec <= AT_LDS_GROUPS && C.ng <= AT_LDS_GROUPS &&
+                         C.pot.ntypes <= AT_LDS_TYPES;
+    if (lds_grp) {
+        for (int k = tid; k < C.ngvec; k += nthr) lgvec[k] = C.gvec[k];
+        for (int k = tid; k < C.ng; k += nthr) lgroups[k] = C.groups[k];
+        for (int k = tid; k < C.pot.ntypes; k += nthr) {
+            ltdp[k] = C.type_dp[k];
+            leff[k] = EW ? C.eff[k] : 0.0;
+        }
+    }
+    if constexpr (EW && MIX)
+        for (int k = tid; k < C.ew_ncheb; k += nthr) lcheb[k] = C.ew_cheb[k];
+    // type tables (launched only with ntypes^2 <= LDS_TYPES2)
+    for (int k = tid; k < TAB_COUNT * C.pot.ntypes2; k += nthr) {
+        const int t = k / C.pot.ntypes2;
+        ltab[k] = C.tab[k];
+        if (MIX) ltabf[k] = C.tabf[k] * ((EW && t == TAB_S2) ? C.fx_s2scale : 1.f);
+    }
+    const double *tab = ltab;
+    const float *tabf = ltabf;
+    if (ldsp) {
+        for (int k = tid; k < L.ns; k += nthr) {
+            const int j = j0 + k;
+            L.pk[k] = int4_to_v4i(gload(C.pk, j));
+            L.x[k] = C.x[j]; L.y[k] = C.y[j]; L.z[k] = C.z[j];
+            L.q[k] = gload(C.q, j);
+            L.r[k] = gload(C.r, j);
+            L.rf[k] = gload(C.rf, j);
+            L.t[k] = gload(C.t, j);
+        }
+    }
+    if constexpr (EW) {
+        for (int k = tid; k < ks; k += nthr) {
+            const int kg = k0 + k;
+            S[k] = make_double2(C.Q[2 * kg], C.Q[2 * kg + 1]);
+            Ak[k] = C.aks[kg];
+            kid[k] = kpack(C.kn[3 * kg], C.kn[3 * kg + 1], C.kn[3 * kg + 2]);
+        }
+    }
+    // surface dipole M = sum q eff r over all particles (getSurfaceEnergy,
+    // ewald.h:278-292); every part sums it alike
+    double Mx = 0.0, My = 0.0, Mz = 0.0;
+    __shared__ double redm[BLK_THREADS / 64];
+    if (EW && C.surf_pref != 0.0) {
+        for (int j = tid; j < C.n; j += nthr) {
+            const double w = C.q[j] * C.eff[C.t[j]];
+            Mx += w * C.x[j]; My += w * C.y[j]; Mz += w * C.z[j];
+        }
+        Mx = block_sum(Mx, redm);
+        My = block_sum(My, redm);
+        Mz = block_sum(Mz, redm);
+    }
+    int ew_cnt = EW ? *C.ew_cnt : 0;
+    double utot = 0.0, dusum = 0.0;
+    long n_trials = 0, n_acc = 0, n_inf = 0, n_ref = 0;
+    const long long t_start = wall_clock64();
+    long rec = 0;
+    __syncthreads();
+    // every part has read the chain state before any part writes it back
+    if (nsplit > 1) chain_sync();
+
+    const double dirx = C.mv.dir[0], diry = C.mv.dir[1], dirz = C.mv.dir[2];
+    const long budget = C.max_trials > 0 ? C.max_trials : LONG_MAX;
+    long used = 0;
+    int calls_done = 0;
+    bool stop = false;
+    int resume_rep = (C.resume && C.resume[0]) ? C.resume[1] : -1;
+    int nr = 0;   // trials in the ring (uniform)
+    const int qcap = DCAP / nw;
+    // fixed-unit pair constants, kept in registers across the partner loop
+    const float fx_rc2 = C.fx_rc2, lj_s2f = C.lj_s2f, lj_e4f = C.lj_eps4f, fx_rs = C.fx_rs;
+    const float fx_c1 = C.fx_c1, exact = C.pot.exact_f;
+    const int ntypes = C.pot.ntypes, ntypes2 = C.pot.ntypes2;
+    const volatile BlkFx *Fv = F;
+    const volatile BlkTrial *Rv = R;
+
+    for (int call = 0; call < C.ncalls && !stop && !sabort; ++call) {
+        int rep0 = 0;
+        bool go;
+        if (resume_rep >= 0) {
+            go = true;
+            rep0 = resume_rep;
+            resume_rep = -1;
+        } else {
+            if (wave0) {
+                // ---- Propagator::move (move.h:7887) + Movebase::move prologue
+                (void)mt_range(I, 0, C.nmoves - 1);
+                (void)mt_range(I, 0, 0);
+                const bool run = mt_canonical(I) < C.mv.prob;
+                if (tid == 0) sgo = run ? 1 : 0;
+            }
+            __syncthreads();
+            go = sgo != 0;
+            __syncthreads();
+        }
+        if (!go) { ++calls_done; continue; }
+        int drawn = rep0;
+        nr = 0;
+        for (;;) {
+            // ---- fill the ring with the next trials of this call (wave 0)
+            int nfill = min(B - nr, C.repeat - drawn);
+            if ((long)nfill > budget - used) nfill = (int)(budget - used);
+            if (nr == 0 && nfill == 0) {
+                if (drawn < C.repeat) {   // trial budget of the launch spent
+                    stop = true;
+                    if (tid == 0 && part == 0 && C.resume) { C.resume[0] = 1; C.resume[1] = drawn; }
+                }
+                break;
+            }
+            const int nb = nr + nfill;
+            if (wave0) {
+                // draws: a trial takes ten words (two uniform_int, four
+                // canonicals) unless a uniform_int rejects; lane l draws trial
+                // l from its ten words of the current state, up to the first
+                // rejection, and the rest are drawn one by one
+                int done = 0;
+                if (nfill > 0 && G.p + 10 * nfill <= 624) {
+                    bool okl = true;
+                    int pi = 0;
+                    double a = 0.0, b = 0.0, c = 0.0, u = 0.0;
+                    if (lane < nfill) {
+                        const int p = G.p + 10 * lane;
+                        uint32_t w[10];
+#pragma unroll
+                        for (int k = 0; k < 10; ++k) w[k] = mt_temper(mtg[p + k]);
+                        const int gk = (int)lemire_word(w[0], (uint32_t)C.ngvec, okl);
+                        const int g = okl ? (lds_grp ? lgvec[gk] : C.gvec[gk]) : 0;
+                        const GroupDesc gd = lds_grp ? lgroups[g] : C.groups[g];
+                        pi = gd.front + (int)lemire_word(w[1], (uint32_t)(gd.back - gd.front) + 1u, okl);
+                        a = canonical_words(w[2], w[3]);
+                        b = canonical_words(w[4], w[5]);
+                        c = canonical_words(w[6], w[7]);
+                        u = canonical_words(w[8], w[9]);
+                    }
+                    const uint64_t bad = __ballot(lane < nfill && !okl);
+                    done = bad ? __ffsll((unsigned long long)bad) - 1 : nfill;
+                    if (lane < done) {
+                        BlkTrial &t = R[nr + lane];
+                        t.i = pi;
+                        t.t3[0] = a; t.t3[1] = b; t.t3[2] = c;
+                        t.u = u;
+                    }
+                    G.p += 10 * done;
+                    G.wb = -1;
+                }
+                for (int f = done; f < nfill; ++f) {
+                    const int gk = mt_range(G, 0, C.ngvec - 1);
+                    const int g = lds_grp ? lgvec[gk] : C.gvec[gk];
+                    const GroupDesc gd = lds_grp ? lgroups[g] : C.groups[g];
+                    const int i = gd.front + mt_range(G, 0, gd.back - gd.front);
+                    const double a = mt_canonical(G), b = mt_canonical(G), c = mt_canonical(G);
+                    const double u = mt_canonical(G);
+                    if (tid == 0) {
+                        BlkTrial &t = R[nr + f];
+                        t.i = i;
+                        t.t3[0] = a; t.t3[1] = b; t.t3[2] = c;
+                        t.u = u;
+                    }
+                }
+                wave_sync_lds();
+                // ---- the moved particles' data and proposals (one lane per trial)
+                if (lane < B) {
+                    if (lane < nb) {
+                        BlkTrial &t = R[lane];
+                        const int i = t.i;
+                        double ox = gload(C.x, i), oy = gload(C.y, i), oz = gload(C.z, i);
+                        // a move accepted in the last block may still be in flight
+                        for (int m = 0; m < snlmv; ++m)
+                            if (lmv_i[m] == i) { ox = lmv_x[m]; oy = lmv_y[m]; oz = lmv_z[m]; }
+                        const int ti = gload(C.t, i);
+                        t.ti = ti;
+                        t.q64 = gload(C.q, i);
+                        t.r64 = gload(C.r, i);
+                        t.rf = gload(C.rf, i);
+                        t.pkw = gload(C.pk, i).w;
+                        t.wq = EW ? t.q64 * (lds_grp ? leff[ti] : C.eff[ti]) : 0.0;
+                        // Metropolis (move.h:567-573) rejects when u > exp(-dU),
+                        // i.e. when dU > -ln u
+                        t.lu = -log(t.u);
+                        const double dp = lds_grp ? ltdp[ti] : C.type_dp[ti];
+                        double nx, ny, nz;
+                        {
+#pragma clang fp contract(off)
+                            double tx = dirx * dp, ty = diry * dp, tz = dirz * dp;
+                            tx *= t.t3[0] - 0.5;
+                            ty *= t.t3[1] - 0.5;
+                            tz *= t.t3[2] - 0.5;
+                            nx = ox + tx; ny = oy + ty; nz = oz + tz;
+                            boundary_d(C.box, nx, ny, nz);
+                        }
+                        t.ox = ox; t.oy = oy; t.oz = oz;
+                        t.nx = nx; t.ny = ny; t.nz = nz;
+                        t.collide = collision_d(C.box, nx, ny, nz) ? 1 : 0;
+                        t.fox = to_fixed(ox, C.box.inv[0]);
+                        t.foy = to_fixed(oy, C.box.inv[1]);
+                        t.foz = to_fixed(oz, C.box.inv[2]);
+                        t.fnx = to_fixed(nx, C.box.inv[0]);
+                        t.fny = to_fixed(ny, C.box.inv[1]);
+                        t.fnz = to_fixed(nz, C.box.inv[2]);
+                        int first = lane;
+                        for (int s2 = lane - 1; s2 >= 0; --s2)
+                            if (R[s2].i == i) first = s2;
+                        t.first = first;
+                        bidx[lane] = i;
+                        BlkFx f;
+                        f.nx = t.fnx; f.ny = t.fny; f.nz = t.fnz;
+                        f.ox = t.fox; f.oy = t.foy; f.oz = t.foz;
+                        const float aq = __int_as_float(t.pkw);
+                        f.aq = C.fx ? aq * C.fx_lbs : aq;
+                        f.ar = C.fx ? t.rf * fx_rs : t.rf;
+                        f.ti = ti;
+                        F[lane] = f;
+                    } else {
+                        bidx[lane] = -1;
+                    }
+                }
+            }
+            used += nfill;
+            drawn += nfill;
+            __syncthreads();
+            // ---- Ewald phase tables exp(i 2 pi n x / L), old and new, per trial
+            if constexpr (EW) {
+                const int per = 3 * kcc1, ntab = nb * 2 * per;
+                for (int e = tid; e < ntab; e += nthr) {
+                    const int b = e / (2 * per), rem = e % (2 * per);
+                    const int which = rem / per, r2 = rem % per;
+                    const int axis = r2 / kcc1, nn = r2 % kcc1;
+                    const BlkTrial &t = R[b];
+                    const double coord = which ? (axis == 0 ? t.nx : axis == 1 ? t.ny : t.nz)
+                                               : (axis == 0 ? t.ox : axis == 1 ? t.oy : t.oz);
+                    double sn, cs;
+                    sincospi(2.0 * (double)nn * coord * C.box.inv[axis], &sn, &cs);
+                    etab[(b * 2 + which) * per + r2] = make_double2(cs, sn);
+                }
+            }
+            // ---- pair part over this part's partners outside the block, CB
+            // trials per pass over the partners (a trial's data is read from
+            // LDS per partner, not held across the loop); each chunk's sums
+            // go to redp at once
+            double acc[B];
+#pragma unroll
+            for (int b = 0; b < B; ++b) acc[b] = 0.0;
+            int bi[B];
+#pragma unroll
+            for (int b = 0; b < B; ++b) bi[b] = bidx[b];
+            if constexpr (MIX) {
+                const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+                int *dq = dlist + wv * qcap;
+                int cnt = 0;
+                // mode 1: one LJ parameter pair and no strong-pair test beyond
+                // rc; 2: nothing beyond rc (CutShift LJ at rc)
+                const int mode = (EW && C.lj_screen) ? 2
+                                 : (EW && C.lj_uniform && !C.lj_check) ? 1 : 0;
+                auto pass = [&](auto M, auto UU, int bc) __attribute__((always_inline)) {
+                    constexpr int MODE = decltype(M)::value;
+                    constexpr int U = decltype(UU)::value;   // records per thread per step
+                    // this chunk's trial constants, wave-uniform (scalar registers)
+                    int32_t tnx[CB], tny[CB], tnz[CB], tox[CB], toy[CB], toz[CB], tti[CB];
+                    float taq[CB], tar[CB];
+#pragma unroll
+                    for (int c = 0; c < CB; ++c) {
+                        const int b = min(bc + c, B - 1);
+                        tnx[c] = __builtin_amdgcn_readfirstlane(F[b].nx);
+                        tny[c] = __builtin_amdgcn_readfirstlane(F[b].ny);
+                        tnz[c] = __builtin_amdgcn_readfirstlane(F[b].nz);
+                        tox[c] = __builtin_amdgcn_readfirstlane(F[b].ox);
+                        toy[c] = __builtin_amdgcn_readfirstlane(F[b].oy);
+                        toz[c] = __builtin_amdgcn_readfirstlane(F[b].oz);
+                        tti[c] = __builtin_amdgcn_readfirstlane(F[b].ti);
+                        taq[c] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(F[b].aq)));
+                        tar[c] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(F[b].ar)));
+                    }
+                    double ac[CB];
+#pragma unroll
+                    for (int c = 0; c < CB; ++c) ac[c] = 0.0;
+                    struct Rec {
+                        int4 P[U];
+                        float R[U];
+                        int T[U];
+                    };
+                    auto fetch = [&](Rec &X, int base) __attribute__((always_inline)) {
+#pragma unroll
+                        for (int u = 0; u < U; ++u) {
+                            const int j = base + u * nthr + tid;
+                            if (ldsp) {
+                                const int r = j - j0;
+                                const bool in = j < j1;
+                                X.P[u] = v4i_to_int4(in ? L.pk[r] : lds_v4i{0, 0, 0, 0});
+                                X.R[u] = (SR == SR_HS && in) ? L.rf[r] : 0.f;
+                                X.T[u] = (NEED_T && MODE == 0 && in) ? L.t[r] : 0;
+                            } else {
+                                // rows past n read the zero padding
+                                X.P[u] = gload(C.pk, j);
+                                X.R[u] = SR == SR_HS ? gload(C.rf, j) : 0.f;
+                                X.T[u] = (NEED_T && MODE == 0) ? gload(C.t, j) : 0;
+                            }
+                        }
+                    };
+                    auto eval = [&](const Rec &X, int base) __attribute__((always_inline)) {
+                        unsigned dm = 0;
+#pragma unroll
+                        for (int u = 0; u < U; ++u) {
+                            const int j = base + u * nthr + tid;
+                            bool ok = j < j1;
+#pragma unroll
+                            for (int b = 0; b < B; ++b) ok = ok && j != bi[b];
+                            const int4 P = X.P[u];
+                            const float bq = __int_as_float(P.w);
+                            const float rb = X.R[u] * fx_rs;
+#pragma unroll
+                            for (int c = 0; c < CB; ++c) {
+                                if (bc + c >= nb) break;
+                                float dx = (float)(int32_t)(tnx[c] - P.x), dy = (float)(int32_t)(tny[c] - P.y),
+                                      dz = (float)(int32_t)(tnz[c] - P.z);
+                                const float r2n = dx * dx + dy * dy + dz * dz;
+                                dx = (float)(int32_t)(tox[c] - P.x);
+                                dy = (float)(int32_t)(toy[c] - P.y);
+                                dz = (float)(int32_t)(toz[c] - P.z);
+                                const float r2o = dx * dx + dy * dy + dz * dz;
+                                float d;
+                                bool bad;
+                                if constexpr (EW) {
+                                    if constexpr (MODE == 2) {
+                                        bad = fminf(r2n, r2o) <= fx_rc2;
+                                        d = 0.f;
+                                    } else {
+                                        float s2 = lj_s2f, e4 = lj_e4f;
+                                        if constexpr (MODE == 0) {
+                                            const int k = tti[c] * ntypes + X.T[u];
+                                            s2 = tabf[TAB_S2 * ntypes2 + k];
+                                            e4 = tabf[TAB_EPS4 * ntypes2 + k];
+                                        }
+                                        float xn = s2 * __builtin_amdgcn_rcpf(r2n);
+                                        float xo = s2 * __builtin_amdgcn_rcpf(r2o);
+                                        xn = xn * xn * xn;
+                                        xo = xo * xo * xo;
+                                        const float vn = xn * xn - xn, vo = xo * xo - xo;
+                                        bad = fminf(r2n, r2o) <= fx_rc2;
+                                        if constexpr (MODE == 0)
+                                            bad = bad || !(fabsf(e4 * vn) <= exact) ||
+                                                  !(fabsf(e4 * vo) <= exact);
+                                        d = e4 * (vn - vo);
+                                    }
+                                } else {
+                                    // fixed-unit Coulomb / Debye-Hueckel (+ hard
+                                    // sphere); charge lB q / s, radius r c / s
+                                    const float aq = taq[c], ar = tar[c];
+                                    float un = 0.f, uo = 0.f;
+                                    const float rin = __builtin_amdgcn_rsqf(r2n);
+                                    const float rio = __builtin_amdgcn_rsqf(r2o);
+                                    if constexpr (ELEC == ELEC_COULOMB) {
+                                        un = aq * bq * rin;
+                                        uo = aq * bq * rio;
+                                    } else if constexpr (ELEC == ELEC_DH) {
+                                        un = aq * bq * rin * __builtin_amdgcn_exp2f(fx_c1 * (r2n * rin));
+                                        uo = aq * bq * rio * __builtin_amdgcn_exp2f(fx_c1 * (r2o * rio));
+                                    }
+                                    bad = !(fabsf(un) <= exact) || !(fabsf(uo) <= exact);
+                                    if constexpr (SR == SR_HS) {
+                                        const float m = ar + rb;
+                                        bad = bad || (r2n < m * m) || (r2o < m * m);
+                                    }
+                                    d = un - uo;
+                                }
+                                dm |= ((ok && bad) ? 1u : 0u) << (u * CB + c);
+                                ac[c] += (double)__int_as_float(__float_as_int(d) & ((ok && !bad) ? -1 : 0));
+                            }
+                        }
+                        if (__ballot(dm != 0)) {
+#pragma unroll
+                            for (int u = 0; u < U; ++u)
+#pragma unroll
+                                for (int c = 0; c < CB; ++c) {
+                                    const bool dd = (dm >> (u * CB + c)) & 1u;
+                                    const uint64_t m = __ballot(dd);
+                                    if (m) {
+                                        if (dd) {
+                                            const int j = base + u * nthr + tid;
+                                            const int k = cnt + __popcll(m & below);
+                                            if (k < qcap) {
+                                                dq[k] = ((bc + c) << 24) | j;
+                                            } else {
+                                                const BlkTrial &t = R[bc + c];
+                                                ac[c] += ldsp
+                                                    ? du_pair64_v<ELEC, SR>(C, tab, t.nx, t.ny, t.nz, t.ox, t.oy,
+                                                                            t.oz, t.q64, t.r64, t.ti, L.x[j - j0],
+                                                                            L.y[j - j0], L.z[j - j0], L.q[j - j0],
+                                                                            L.r[j - j0], L.t[j - j0])
+                                                    : du_pair64_mixed<ELEC, SR>(C, tab, lcheb, t.nx, t.ny, t.nz,
+                                                                                t.ox, t.oy, t.oz, t.q64, t.r64,
+                                                                                t.ti, j);
+                                            }
+                                        }
+                                        cnt += __popcll(m);
+                                    }
+                                }
+                        }
+                    };
+                    const int step = U * nthr;
+                    Rec A, Bq;
+                    fetch(A, j0);
+                    for (int base = j0; base < j1; base += 2 * step) {
+                        fetch(Bq, base + step);   // device memory: within the zero padding
+                        eval(A, base);
+                        fetch(A, base + 2 * step);
+                        if (base + step < j1) eval(Bq, base + step);
+                    }
+#pragma unroll
+                    for (int c = 0; c < CB; ++c) {
+                        const double sv = wave_sum_dpp(ac[c]);
+                        if (lane == 0 && bc + c < B) redp[wv][bc + c] = sv;
+                    }
+                };
+                // partners in LDS: one record per thread and step; in device
+                // memory: two, the next two in flight
+                auto pass_u = [&](auto M, int bc) __attribute__((always_inline)) {
+                    if (ldsp) pass(M, std::integral_constant<int, 1>{}, bc);
+                    else pass(M, std::integral_constant<int, 2>{}, bc);
+                };
+                for (int bc = 0; bc < nb; bc += CB) {
+                    if constexpr (EW) {
+                        if (mode == 2) pass_u(std::integral_constant<int, 2>{}, bc);
+                        else if (mode == 1) pass_u(std::integral_constant<int, 1>{}, bc);
+                        else pass_u(std::integral_constant<int, 0>{}, bc);
+                    } else {
+                        pass_u(std::integral_constant<int, 0>{}, bc);
+                    }
+                }
+                if (lane == 0) nqs[wv] = min(cnt, qcap);
+            } else {
+                // FP64: one trial per pass over the partners
+                for (int b = 0; b < nb; ++b) {
+                    const double tnx = Rv[b].nx, tny = Rv[b].ny, tnz = Rv[b].nz;
+                    const double tox = Rv[b].ox, toy = Rv[b].oy, toz = Rv[b].oz;
+                    const double tq = Rv[b].q64, tr = Rv[b].r64;
+                    const int tt = Rv[b].ti;
+                    double ac = 0.0;
+                    for (int j = j0 + tid; j < j1; j += nthr) {
+                        bool ok = true;
+#pragma unroll
+                        for (int b2 = 0; b2 < B; ++b2) ok = ok && j != bi[b2];
+                        if (!ok) continue;
+                        if (ldsp) {
+                            const int r = j - j0;
+                            ac += du_pair64_v<ELEC, SR>(C, tab, tnx, tny, tnz, tox, toy, toz, tq, tr, tt,
+                                                        L.x[r], L.y[r], L.z[r], L.q[r], L.r[r], L.t[r]);
+                        } else {
+                            ac += du_pair64_v<ELEC, SR>(C, tab, tnx, tny, tnz, tox, toy, toz, tq, tr, tt,
+                                                        C.x[j], C.y[j], C.z[j], C.q[j], C.r[j], C.t[j]);
+                        }
+                    }
+                    const double sv = wave_sum_dpp(ac);
+                    if (lane == 0) redp[wv][b] = sv;
+                }
+            }
+            // ---- corrections: the block's own particles at their position
+            // before (X) and after (Y) an accepted move, FP64, over all threads
+            for (int e = tid; e < 2 * nb * nb; e += nthr) {
+                const int e2 = e % (nb * nb);
+                const int s2 = e2 / nb, t = e2 % nb;
+                const bool yv = e >= nb * nb;
+                const BlkTrial &a = R[t], &o = R[s2];
+                double v = 0.0;
+                if (s2 != t && a.i != o.i && (!yv || s2 < t))
+                    v = du_pair64_v<ELEC, SR>(C, tab, a.nx, a.ny, a.nz, a.ox, a.oy, a.oz, a.q64, a.r64,
+                                              a.ti, yv ? o.nx : o.ox, yv ? o.ny : o.oy,
+                                              yv ? o.nz : o.oz, o.q64, o.r64, o.ti);
+                if (yv) yc[s2][t] = v;
+                else xc[s2][t] = v;
+            }
+            __syncthreads();   // FP64 queues, corrections and phase tables complete
+            // ---- FP64 pass of the mixed path over the queued pairs
+            if constexpr (MIX) {
+                int Qn = 0;
+                for (int w = 0; w < nw; ++w) Qn += nqs[w];
+                for (int e = tid; e < Qn; e += nthr) {
+                    int w = 0, k = e;
+                    while (k >= nqs[w]) { k -= nqs[w]; ++w; }
+                    const int ent = dlist[w * qcap + k];
+                    const int b = ent >> 24, j = ent & 0xffffff;
+                    const BlkTrial &t = R[b];
+                    const double v =
+                        ldsp ? du_pair64_v<ELEC, SR>(C, tab, t.nx, t.ny, t.nz, t.ox, t.oy, t.oz, t.q64,
+                                                     t.r64, t.ti, L.x[j - j0], L.y[j - j0], L.z[j - j0],
+                                                     L.q[j - j0], L.r[j - j0], L.t[j - j0])
+                             : du_pair64_mixed<ELEC, SR>(C, tab, lcheb, t.nx, t.ny, t.nz, t.ox, t.oy,
+                                                         t.oz, t.q64, t.r64, t.ti, j);
+#pragma unroll
+                    for (int bb = 0; bb < B; ++bb)
+                        if (bb == b) acc[bb] += v;
+                }
+            }
+            // ---- reciprocal part over this part's k-vectors: per trial
+            // A_k (2 Re(S* dS_t) + |dS_t|^2), and A_k Re(dS_s* dS_t) between
+            // trials (the accept update recomputes dS of the accepted trials)
+            double gacc[NG > 0 ? NG : 1];
+            if constexpr (EW) {
+                double rec_e[B];
+#pragma unroll
+                for (int b = 0; b < B; ++b) rec_e[b] = 0.0;
+#pragma unroll
+                for (int g = 0; g < NG; ++g) gacc[g] = 0.0;
+                const int per = 3 * kcc1;
+                for (int kl = tid; kl < ks; kl += nthr) {
+                    double dsr[B], dsi[B];
+#pragma unroll
+                    for (int b = 0; b < B; ++b) { dsr[b] = 0.0; dsi[b] = 0.0; }
+                    {
+                        int nx, ny, nz;
+                        kunpack(kid[kl], nx, ny, nz);
+                        const double2 sk = S[kl];
+                        const double a = Ak[kl];
+#pragma unroll
+                        for (int b = 0; b < B; ++b) {
+                            if (b < nb) {
+                                double er, ei, orr, oi;
+                                ew_phase(etab + (b * 2 + 1) * per, kcc1, nx, ny, nz, er, ei);
+                                ew_phase(etab + (b * 2) * per, kcc1, nx, ny, nz, orr, oi);
+                                const double wq = R[b].wq;
+                                const double dr = wq * (er - orr), di = wq * (ei - oi);
+                                dsr[b] = dr;
+                                dsi[b] = di;
+                                rec_e[b] += a * (2.0 * (sk.x * dr + sk.y * di) + (dr * dr + di * di));
+                            }
+                        }
+                        int g = 0;
+#pragma unroll
+                        for (int s2 = 0; s2 < B; ++s2)
+#pragma unroll
+                            for (int t = s2 + 1; t < B; ++t, ++g)
+                                gacc[g] += a * (dsr[s2] * dsr[t] + dsi[s2] * dsi[t]);
+                    }
+                }
+#pragma unroll
+                for (int b = 0; b < B; ++b) acc[b] += C.recip_pref * rec_e[b];
+            }
+            // ---- block reduction of the partial sums (fixed order)
+#pragma unroll
+            for (int v = 0; v < B; ++v) {
+                const double sv = wave_sum_dpp(acc[v]);
+                if (lane == 0) red[wv][v] = sv;
+            }
+#pragma unroll
+            for (int g = 0; g < NG; ++g) {
+                const double sv = wave_sum_dpp(gacc[g]);
+                if (lane == 0) red[wv][B + g] = sv;
+            }
+            __syncthreads();
+            if (tid < NV) {
+                double sv = 0.0;
+                for (int w = 0; w < nw; ++w) sv += red[w][tid];
+                if (tid < nb)
+                    for (int w = 0; w < nw; ++w) sv += redp[w][tid];
+                tot[tid] = sv;
+            }
+            exchange(NV);
+            if (sabort) break;
+            // ---- Metropolis decisions in order (wave 0; lane t holds trial
+            // t's energy change as a finite sum and counts of +inf, -inf and
+            // NaN terms, which add up as IEEE sums do)
+            if (wave0) {
+                wave_sync_lds();
+                const bool mine = lane < nb;
+                double fin = 0.0;
+                int npi = 0, nmi = 0, nna = 0;
+                auto add = [&](double v, int sgn) {
+                    if (v != v) nna += sgn;
+                    else if (v == __builtin_inf()) npi += sgn;
+                    else if (v == -__builtin_inf()) nmi += sgn;
+                    else fin += sgn > 0 ? v : -v;
+                };
+                int myi = -1;
+                double lu = 0.0;
+                int mycol = 0;
+                double Xs[B], Ys[B];
+                unsigned app = 0;   // earlier trials whose acceptance changes this one
+                if (mine) {
+                    myi = R[lane].i;
+                    lu = R[lane].lu;
+                    mycol = R[lane].collide;
+                    add(tot[lane], 1);
+                    for (int f = 0; f < nb; ++f)
+                        if (R[f].first == f && R[f].i != myi) add(xc[f][lane], 1);
+                }
+#pragma unroll
+                for (int s2 = 0; s2 < B; ++s2) {
+                    Xs[s2] = 0.0;
+                    Ys[s2] = 0.0;
+                    if (mine && s2 < lane && R[s2].i != myi) {
+                        app |= 1u << s2;
+                        Xs[s2] = xc[R[s2].first][lane];
+                        Ys[s2] = yc[s2][lane];
+                        if constexpr (EW)
+                            Ys[s2] += 2.0 * C.recip_pref * tot[B + s2 * (2 * B - s2 - 1) / 2 + (lane - s2 - 1)];
+                    }
+                }
+                int nd = nb;
+                unsigned accm = 0;
+                bool refresh = false, moved = false;
+                double mydu = 0.0;
+#pragma unroll
+                for (int t = 0; t < B; ++t) {
+                    if (t >= nb) break;
+                    // a trial whose particle an accepted trial of this block
+                    // moved waits for the next block
+                    if (__builtin_amdgcn_readlane((int)moved, t)) { nd = t; break; }
+                    double v = (nna > 0 || (npi > 0 && nmi > 0)) ? __builtin_nan("")
+                               : npi > 0 ? __builtin_inf()
+                               : nmi > 0 ? -__builtin_inf() : fin;
+                    const long long vb = __double_as_longlong(v);
+                    const int lo = __builtin_amdgcn_readlane((int)vb, t);
+                    const int hi = __builtin_amdgcn_readlane((int)(vb >> 32), t);
+                    double du = __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
+                                                                 (unsigned long long)(unsigned)lo));
+                    double dMx = 0.0, dMy = 0.0, dMz = 0.0;
+                    if (EW && C.surf_pref != 0.0) {
+                        const BlkTrial &T = R[t];
+                        dMx = T.wq * (T.nx - T.ox);
+                        dMy = T.wq * (T.ny - T.oy);
+                        dMz = T.wq * (T.nz - T.oz);
+                        const double Nx = Mx + dMx, Ny = My + dMy, Nz = Mz + dMz;
+                        du += C.surf_pref *
+                              ((Nx * Nx + (Ny * Ny + Nz * Nz)) - (Mx * Mx + (My * My + Mz * Mz)));
+                    }
+                    if (__builtin_amdgcn_readlane(mycol, t)) du = __builtin_inf();   // energy.h:3597-3607
+                    const long long lb = __double_as_longlong(lu);
+                    const double lut = __longlong_as_double(
+                        (long long)(((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(lb >> 32), t) << 32) |
+                                    (unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)lb, t)));
+                    const bool accept = du <= 0.0 || !(du > lut);
+                    if (lane == t) mydu = du;
+                    if (accept) {
+                        accm |= 1u << t;
+                        utot += du;
+                        dusum += du;
+                        Mx += dMx; My += dMy; Mz += dMz;
+                        if ((app >> t) & 1u) {
+                            add(Xs[t], -1);
+                            add(Ys[t], 1);
+                        }
+                        moved = moved || (myi == __builtin_amdgcn_readlane(myi, t));
+                        if (EW && ++ew_cnt > C.update_frequency - 1) {
+                            refresh = true;
+                            nd = t + 1;
+                            break;
+                        }
+                    } else if (du == __builtin_inf()) {
+                        n_inf++;
+                    }
+                }
+                n_trials += nd;
+                n_acc += __popc(accm);
+                if (lane == 0) { snd = nd; saccm = (int)accm; srefresh = refresh ? 1 : 0; }
+                if (lane < B) {
+                    sdu[lane] = mydu;
+                    swq[lane] = lane < nb ? R[lane].wq : 0.0;
+                }
+            }
+            __syncthreads();
+            const int nd = snd;
+            const unsigned accm = (unsigned)saccm;
+            const bool refresh = srefresh != 0;
+            // ---- accepted moves into the chain state
+            if (tid < nd) {
+                const BlkTrial &T = R[tid];
+                const bool a = (accm >> tid) & 1u;
+                if (a) {   // AtomicTranslation::_acceptMove (move.h:866-877)
+                    const int i = T.i;
+                    C.x[i] = T.nx; C.y[i] = T.ny; C.z[i] = T.nz;
+                    C.pk_w[i] = make_int4(T.fnx, T.fny, T.fnz, T.pkw);
+                    if (ldsp && i >= j0 && i < j1) {
+                        const int r = i - j0;
+                        L.pk[r] = int4_to_v4i(make_int4(T.fnx, T.fny, T.fnz, T.pkw));
+                        L.x[r] = T.nx; L.y[r] = T.ny; L.z[r] = T.nz;
+                    }
+                }
+                if (part == 0 && C.du_rec && rec + tid < C.cap) {
+                    C.du_rec[rec + tid] = sdu[tid];
+                    C.acc_rec[rec + tid] = a ? 1 : 0;
+                }
+            }
+            rec += nd;
+            if constexpr (EW) {
+                // S(k) += dS(k) of the accepted trials, in order (dS as the
+                // reciprocal pass formed it, from the same phase tables)
+                if (accm) {
+                    const int per = 3 * kcc1;
+                    for (int kl = tid; kl < ks; kl += nthr) {
+                        int nx, ny, nz;
+                        kunpack(kid[kl], nx, ny, nz);
+                        double2 sk = S[kl];
+#pragma unroll
+                        for (int b = 0; b < B; ++b)
+                            if (b < nd && ((accm >> b) & 1u)) {
+                                double er, ei, orr, oi;
+                                ew_phase(etab + (b * 2 + 1) * per, kcc1, nx, ny, nz, er, ei);
+                                ew_phase(etab + (b * 2) * per, kcc1, nx, ny, nz, orr, oi);
+                                const double wq = swq[b];
+                                sk.x = sk.x + wq * (er - orr);
+                                sk.y = sk.y + wq * (ei - oi);
+                            }
+                        S[kl] = sk;
+                    }
+                }
+            }
+            if (wave0) {
+                wave_sync_lds();   // the wave's reads of the ring are done
+                if (tid == 0) {
+                    int m = 0;
+                    for (int b = 0; b < nd; ++b)
+                        if ((accm >> b) & 1u) {
+                            lmv_i[m] = R[b].i;
+                            lmv_x[m] = R[b].nx; lmv_y[m] = R[b].ny; lmv_z[m] = R[b].nz;
+                            ++m;
+                        }
+                    snlmv = m;
+                    // undecided trials keep their draws for the next block
+                    for (int b = nd; b < nb; ++b) R[b - nd] = R[b];
+                }
+                wave_sync_lds();
+            }
+            nr = nb - nd;
+            if constexpr (EW) {
+                if (refresh) {
+                    // NonbondedEwald::update (ewald.h:626-643): S(k) from all
+                    // positions; the reciprocal-energy change enters utot
+                    __syncthreads();
+                    double eb = 0.0;
+                    for (int k = tid; k < ks; k += nthr)
+                        eb += Ak[k] * (S[k].x * S[k].x + S[k].y * S[k].y);
+                    for (int k = tid; k < ks; k += nthr) S[k] = make_double2(0.0, 0.0);
+                    const bool tabled = kcc1 <= REFRESH_KAX;
+                    const int per = 3 * kcc1;
+                    for (int jc = 0; jc < C.n; jc += REFRESH_CHUNK) {
+                        const int nj = min(REFRESH_CHUNK, C.n - jc);
+                        __syncthreads();
+                        if (tabled) {
+                            for (int e = tid; e < nj * per; e += nthr) {
+                                const int jj = e / per, rem = e % per;
+                                const int axis = rem / kcc1, nn = rem % kcc1;
+                                const int j = jc + jj;
+                                const double coord = axis == 0 ? C.x[j] : axis == 1 ? C.y[j] : C.z[j];
+                                double sn, cs;
+                                sincospi(2.0 * (double)nn * coord * C.box.inv[axis], &sn, &cs);
+                                rtab[jj * per + axis * kcc1 + nn] = make_double2(cs, sn);
+                            }
+                        }
+                        for (int jj = tid; jj < nj; jj += nthr)
+                            rw[jj] = C.q[jc + jj] * C.eff[C.t[jc + jj]];
+                        __syncthreads();
+                        for (int k = tid; k < ks; k += nthr) {
+                            int nx, ny, nz;
+                            kunpack(kid[k], nx, ny, nz);
+                            double sr = 0.0, si = 0.0;
+                            for (int jj = 0; jj < nj; ++jj) {
+                                double er, ei;
+                                if (tabled) {
+                                    ew_phase(&rtab[jj * per], kcc1, nx, ny, nz, er, ei);
+                                } else {
+                                    const int j = jc + jj;
+                                    const double ph = 2.0 * ((double)nx * C.x[j] * C.box.inv[0] +
+                                                             (double)ny * C.y[j] * C.box.inv[1] +
+                                                             (double)nz * C.z[j] * C.box.inv[2]);
+                                    sincospi(ph, &ei, &er);
+                                }
+                                sr += rw[jj] * er;
+                                si += rw[jj] * ei;
+                            }
+                            const double2 sk = S[k];
+                            S[k] = make_double2(sk.x + sr, sk.y + si);
+                        }
+                    }
+                    double ea = 0.0;
+                    for (int k = tid; k < ks; k += nthr)
+                        ea += Ak[k] * (S[k].x * S[k].x + S[k].y * S[k].y);
+                    eb = block_sum(eb, redm);
+                    ea = block_sum(ea, redm);
+                    if (tid == 0) { tot[0] = eb; tot[1] = ea; }
+                    __syncthreads();
+                    exchange(2);
+                    if (sabort) break;
+                    utot += C.recip_pref * (tot[1] - tot[0]);
+                    if (C.surf_pref != 0.0) {
+                        Mx = My = Mz = 0.0;
+                        for (int j = tid; j < C.n; j += nthr) {
+                            const double w = C.q[j] * C.eff[C.t[j]];
+                            Mx += w * C.x[j]; My += w * C.y[j]; Mz += w * C.z[j];
+                        }
+                        Mx = block_sum(Mx, redm);
+                        My = block_sum(My, redm);
+                        Mz = block_sum(Mz, redm);
+                    }
+                    ew_cnt = 0;
+                    n_ref++;
+                    __syncthreads();
+                }
+            }
+        }
+        if (!stop && !sabort) ++calls_done;
+    }
+    // ---- write back the chain state
+    if (nsplit > 1 && !sabort) chain_sync();
+    __syncthreads();
+    if (part == 0) {
+        if (!stop && tid == 0 && C.resume) C.resume[0] = 0;
+        for (int k = tid; k < 624; k += nthr) {
+            C.rng[k] = mtg[k];
+            C.rng[625 + k] = mti[k];
+        }
+        if (tid == 0) {
+            C.rng[624] = (uint32_t)G.p;
+            C.rng[625 + 624] = (uint32_t)I.p;
+            C.stats_i[0] = n_trials;
+            C.stats_i[1] = n_acc;
+            C.stats_i[2] = n_inf;
+            C.stats_i[3] = n_ref;
+            C.stats_i[4] = wall_clock64() - t_start;
+            C.stats_i[5] = calls_done;
+            C.stats_i[6] = 0;
+            C.stats_d[0] = utot;
+            C.stats_d[1] = dusum;
+            if (EW) *C.ew_cnt = ew_cnt;
+        }
+    }
+    if constexpr (EW) {
+        for (int k = tid; k < ks; k += nthr) {
+            C.Q[2 * (k0 + k)] = S[k].x;
+            C.Q[2 * (k0 + k) + 1] = S[k].y;
+        }
+    }
+    // the last part out resets the chain's counters for the next launch
+    if (nsplit > 1 && tid == 0) {
+        __threadfence();
+        const int d = __hip_atomic_fetch_add(C.xcnt + 1, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (d == nsplit - 1) {
+            __hip_atomic_store(C.xcnt, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(C.xcnt + 1, 0, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+template <int ELEC, int SR, bool MIX, int B>
+static void launch_blk_f(hipStream_t st, const ChainDev *d_chains, const int2 *d_map, int ngrid,
+                         int nthr, size_t dyn)
+{
+    if (dyn > 0)
+        (void)hipFuncSetAttribute((const void *)k_sweep_blk<ELEC, SR, MIX, B>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+    hipLaunchKernelGGL((k_sweep_blk<ELEC, SR, MIX, B>), dim3(ngrid), dim3(nthr), dyn, st, d_chains,
+                       d_map);
+}
+
+// static LDS of a block-sweep instance (for the host's dynamic-LDS budget)
+template <int ELEC, int SR, bool MIX, int B>
+static size_t blk_static_lds()
+{
+    hipFuncAttributes a{};
+    if (hipFuncGetAttributes(&a, (const void *)k_sweep_blk<ELEC, SR, MIX, B>) != hipSuccess) return 0;
+    return a.sharedSizeBytes;
+}
+
+// Block sweep launchers (fgx_sweep_blk_*.hip): FGX_EUNSUPPORTED without an
+// error message for a Hamiltonian they have no instance of.
+int launch_blk_pair(bool mix, const Pot &P, hipStream_t st, const ChainDev *d_chains,
+                    const int2 *d_map, int ngrid, int nthr, size_t dyn);
+int launch_blk_ewald(bool mix, const Pot &P, hipStream_t st, const ChainDev *d_chains,
+                     const int2 *d_map, int ngrid, int nthr, size_t dyn);
+// whether an instance exists for this Hamiltonian (1) and its static LDS and
+// trials per block
+int blk_instance_pair(bool mix, const Pot &P, size_t *static_lds, int *block);
+int blk_instance_ewald(bool mix, const Pot &P, size_t *static_lds, int *block);
+
+}  // namespace fgx

@@ -109,6 +109,13 @@ struct ChainDev {
     // and radius, FP32 radius and type, LDS_PART_BYTES per particle
     int ldsp;
     int sp_j0, sp_j1;
+    // block sweep (k_sweep_blk): workgroups sharing the chain, and their
+    // exchange slots (2 x BLK_NSPLIT_MAX x BLK_NV_MAX doubles), arrival and
+    // exit counters (zero between launches) and the launch's error word
+    int nsplit;
+    double *xbuf;
+    int *xcnt;
+    int *xerr;
 };
 
 static constexpr int LDS_PART_BYTES = 16 + 5 * 8 + 4 + 4;
@@ -2023,11 +2030,21 @@ struct BatchPlan {
     bool fx = false, cell = false;
     bool skg = false;   // Ewald S(k) in device memory
     Pot pot{};
+    // block sweep: chains may span several workgroups (set by the caller
+    // before prepare_batch: launches that may run beside other launches, as
+    // the job server's, must not wait on workgroups another launch holds)
+    bool allow_split = true;
+    // block sweep: workgroup map (chain, part) and grid size
+    bool blk = false;
+    const int2 *d_map = nullptr;
+    int ngrid = 0;
+    int *d_xerr = nullptr;
 };
 int prepare_batch(std::vector<fgx_system *> &sys, const std::vector<int> &ncalls,
                   const std::vector<long> *max_trials, double *du_rec, int *acc_rec, long cap,
                   BatchPlan &B);
 int enqueue_batch(const BatchPlan &B, hipStream_t st);
+int blk_check(const std::vector<fgx_system *> &sys, int *d_xerr);
 int mirror_trial(std::vector<fgx_system *> &sys, hipStream_t st);
 
 // Per-family launchers (fgx_sweep_at_*.hip, fgx_sweep_tr.hip).  Each returns

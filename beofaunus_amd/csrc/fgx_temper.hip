@@ -799,6 +799,7 @@ extern "C" int fgx_temper_run(fgx_temper **tempers, int nt, int rounds, int call
             set_error(std::string("fgx_temper_run: ") + hipGetErrorString(se));
             rc = FGX_EHIP;
         }
+        if (!rc && B.blk) rc = blk_check(sys, B.d_xerr);
     }
     float ms = 0.f;
     if (!rc) (void)hipEventElapsedTime(&ms, e0, e1);

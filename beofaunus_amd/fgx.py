@@ -132,6 +132,8 @@ def lib():
         "fgx_set_energy_matrix": (I, [P, I]),
         "fgx_set_cell_list": (I, [P, I]),
         "fgx_set_lds_partners": (I, [P, I]),
+        "fgx_set_block_sweep": (I, [P, I, I]),
+        "fgx_get_block_parts": (I, [P, C.POINTER(C.c_int)]),
         "fgx_set_ewald_state": (I, [P, I]),
         "fgx_isobaric_propagate": (I, [P, D, D, I, C.POINTER(Stats), _dp, _ip, C.c_long]),
         "fgx_propagate_npt": (I, [P, D, D, I, I, C.POINTER(Stats), _dp, _ip, C.c_long]),
@@ -424,6 +426,20 @@ class System:
         device memory (False); the same pair arithmetic, summed in another
         order."""
         _check(lib().fgx_set_lds_partners(self.h, 1 if on else 0))
+
+    def set_block_sweep(self, on, nsplit=0):
+        """Block AtomicTranslation sweep: B trials per pass over the partners,
+        decided in order with exact corrections (True), or one trial at a time
+        (False, default).  nsplit > 1 spreads a chain over that many
+        workgroups; 0 chooses automatically."""
+        _check(lib().fgx_set_block_sweep(self.h, 1 if on else 0, int(nsplit)))
+
+    def block_parts(self):
+        """workgroups per chain of the last device sweep if it was a block
+        sweep, 0 if it was the one-trial sweep"""
+        p = C.c_int()
+        _check(lib().fgx_get_block_parts(self.h, C.byref(p)))
+        return p.value
 
     def cell_list(self):
         """(cells per axis, records per cell) of the last sweep; (0, 0) when

@@ -261,6 +261,23 @@ int fgx_set_lds_partners(fgx_system *s, int on);
  * the LDS for a second workgroup per CU.  The same arithmetic either way.
  * A schedule knob with no reference counterpart. */
 int fgx_set_ewald_state(fgx_system *s, int hbm);
+/* Block AtomicTranslation sweep: B consecutive trials of a Propagator call
+ * (their draws do not depend on earlier decisions) are evaluated against one
+ * snapshot in a single pass over the partners and decided in order with
+ * exact corrections for the block's own moves; the accept sequence is the
+ * one-trial sweep's, energy changes agree to FP64 rounding.  mode 1 uses it
+ * where an instance exists (Debye-Hueckel / Coulomb with or without hard
+ * spheres, Lennard-Jones in FP64, Ewald + Lennard-Jones; no cell list, Ewald
+ * state in LDS), 0 keeps the one-trial sweep.  nsplit > 1 spreads one chain
+ * over that many workgroups (at most 32), which exchange partial sums once
+ * per block; 0 chooses automatically.  A launch whose chains cannot all be
+ * co-resident is refused (FGX_EUNSUPPORTED); a part that waits too long for
+ * the others stops the launch with FGX_EDEVICE.  A schedule knob with no
+ * reference counterpart. */
+int fgx_set_block_sweep(fgx_system *s, int mode, int nsplit);
+/* workgroups per chain of the last AtomicTranslation device sweep if it was a
+ * block sweep, 0 if it was the one-trial sweep */
+int fgx_get_block_parts(const fgx_system *s, int *parts);
 /* Autotuner: times `ncalls` Propagator::move() calls of a probe chain under
  * every schedule and keeps the fastest on the probe (whose state advances). */
 int fgx_autotune(fgx_system *probe, int ncalls, int *threads, int *unroll, double *best_seconds);
