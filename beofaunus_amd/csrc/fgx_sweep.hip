@@ -536,7 +536,7 @@ static int plan_blk(std::vector<fgx_system *> &sys, std::vector<ChainDev> &chain
                       "block sweep: the Ewald k-vectors of one workgroup exceed its LDS");
             return FGX_OK;
         }
-        const bool ldsp = s->lds_partners && st_lds + full.total <= lds_cap;
+        const bool ldsp = s->lds_partners && ns <= BLK_LDS_ROWS && st_lds + full.total <= lds_cap;
         chains[c].ldsp = ldsp ? 1 : 0;
         chains[c].sp_j0 = 0;
         chains[c].sp_j1 = s->n;

@@ -42,11 +42,13 @@ static constexpr int BLK_NV_MAX = 16;        // partial sums exchanged per block
 static constexpr int BLK_THREADS = 512;
 static constexpr int BLK_CB_PAIR = 2;        // trials per pass over the partners, mixed path
 static constexpr int BLK_CB_EWALD = 2;
+static constexpr int BLK_LDS_ROWS = 2048;    // partner rows of an LDS-resident slice, at most
 static constexpr long BLK_SPIN_MAX = 1L << 22;   // s_sleep rounds of a part waiting for the others
 
 // one trial of a block: its draws, the moved particle's data, the proposal
 struct BlkTrial {
     int i, ti, collide, first;
+    int fetched;                            // particle data read since the draw
     int32_t fnx, fny, fnz, fox, foy, foz;   // fixed-point new / old position
     int pkw;                                // FP32 charge bits of the packed record
     float rf;                               // FP32 radius
@@ -80,6 +82,16 @@ struct BlkFx {
     int ti;
 };
 
+// lane l's double, in every lane
+__device__ __forceinline__ double readlane_f64(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
+                                            (unsigned long long)(unsigned)lo));
+}
+
 // uniform_int_distribution<int>(0, erange - 1) from one word (Lemire's
 // method, as mt_range); ok = false when the method needs another word
 __device__ __forceinline__ uint32_t lemire_word(uint32_t word, uint32_t erange, bool &ok)
@@ -104,6 +116,74 @@ __device__ __forceinline__ double canonical_words(uint32_t w1, uint32_t w2)
     return r;
 }
 #pragma clang fp contract(on)
+
+// constants of the fixed-unit pair form, wave-uniform
+struct BlkFxConst {
+    float rc2, s2, e4, rs, c1, exact;
+    int ntypes, ntypes2;
+};
+
+// Fixed-unit pair term of one trial against one partner record (mixed path):
+// returns u(new) - u(old) in FP32; `bad` marks a pair whose value must come
+// from FP64 (inside the Ewald rc, strong, non-finite or near hard-sphere
+// contact).  Trial: fixed-point new / old position, type, charge lB q / s and
+// radius r c / s; partner: packed record, FP32 radius, type.  MODE (Ewald):
+// 0 type tables, 1 one LJ parameter pair, 2 nothing beyond rc.
+template <int ELEC, int SR, int MODE>
+__device__ __forceinline__ float blk_pair_fx(const BlkFxConst &K, const float *tabf, int32_t tnx,
+                                             int32_t tny, int32_t tnz, int32_t tox, int32_t toy,
+                                             int32_t toz, int tti, float taq, float tar, int4 P,
+                                             float Rb, int Tb, bool &bad)
+{
+    float dx = (float)(int32_t)(tnx - P.x), dy = (float)(int32_t)(tny - P.y),
+          dz = (float)(int32_t)(tnz - P.z);
+    const float r2n = dx * dx + dy * dy + dz * dz;
+    dx = (float)(int32_t)(tox - P.x);
+    dy = (float)(int32_t)(toy - P.y);
+    dz = (float)(int32_t)(toz - P.z);
+    const float r2o = dx * dx + dy * dy + dz * dz;
+    if constexpr (ELEC == ELEC_EWALD) {
+        if constexpr (MODE == 2) {
+            bad = fminf(r2n, r2o) <= K.rc2;
+            return 0.f;
+        } else {
+            float s2 = K.s2, e4 = K.e4;
+            if constexpr (MODE == 0) {
+                const int k = tti * K.ntypes + Tb;
+                s2 = tabf[TAB_S2 * K.ntypes2 + k];
+                e4 = tabf[TAB_EPS4 * K.ntypes2 + k];
+            }
+            float xn = s2 * __builtin_amdgcn_rcpf(r2n);
+            float xo = s2 * __builtin_amdgcn_rcpf(r2o);
+            xn = xn * xn * xn;
+            xo = xo * xo * xo;
+            const float vn = xn * xn - xn, vo = xo * xo - xo;
+            bad = fminf(r2n, r2o) <= K.rc2;
+            if constexpr (MODE == 0)
+                bad = bad || !(fabsf(e4 * vn) <= K.exact) || !(fabsf(e4 * vo) <= K.exact);
+            return e4 * (vn - vo);
+        }
+    } else {
+        // Coulomb / Debye-Hueckel (+ hard sphere)
+        const float bq = __int_as_float(P.w);
+        float un = 0.f, uo = 0.f;
+        const float rin = __builtin_amdgcn_rsqf(r2n);
+        const float rio = __builtin_amdgcn_rsqf(r2o);
+        if constexpr (ELEC == ELEC_COULOMB) {
+            un = taq * bq * rin;
+            uo = taq * bq * rio;
+        } else if constexpr (ELEC == ELEC_DH) {
+            un = taq * bq * rin * __builtin_amdgcn_exp2f(K.c1 * (r2n * rin));
+            uo = taq * bq * rio * __builtin_amdgcn_exp2f(K.c1 * (r2o * rio));
+        }
+        bad = !(fabsf(un) <= K.exact) || !(fabsf(uo) <= K.exact);
+        if constexpr (SR == SR_HS) {
+            const float m = tar + Rb * K.rs;
+            bad = bad || (r2n < m * m) || (r2o < m * m);
+        }
+        return un - uo;
+    }
+}
 
 template <int ELEC, int SR, bool MIX, int B>
 __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__restrict__ chains,
@@ -133,6 +213,8 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
     __shared__ double tot[BLK_NV_MAX];
     __shared__ double xc[B][B], yc[B][B];
     __shared__ int dlist[MIX ? DCAP : 1];
+    // block particles in the LDS-resident partner slice (bit per row)
+    __shared__ uint32_t exclb[BLK_LDS_ROWS / 32 + 2];
     __shared__ int nqs[BLK_THREADS / 64];
     __shared__ double ltab[TAB_COUNT * LDS_TYPES2];
     __shared__ float ltabf[MIX ? TAB_COUNT * LDS_TYPES2 : 1];
@@ -170,19 +252,20 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
     L.ns = ldsp ? j1 - j0 : 0;
     L.carve((FGX_LDS void *)(dbase + lo.part), L.ns);
 
-    // ---- chain barrier across the parts (release / acquire at agent scope)
+    // ---- chain barrier across the parts.  Wave 0 alone writes what the
+    // parts exchange, so one release fence of its thread 0 publishes it; the
+    // spin reads the counter relaxed and one acquire fence follows.
     int epoch = 0;          // barriers passed (uniform)
     int xpar = 0;           // exchange buffer in use (alternates per exchange)
     auto chain_sync = [&]() {
-        __threadfence();
-        __syncthreads();
         ++epoch;
         if (tid == 0) {
-            __hip_atomic_fetch_add(C.xcnt, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __hip_atomic_fetch_add(C.xcnt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             const int target = epoch * nsplit;
             long spins = 0;
-            while (__hip_atomic_load(C.xcnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-                __builtin_amdgcn_s_sleep(2);
+            while (__hip_atomic_load(C.xcnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                __builtin_amdgcn_s_sleep(1);
                 if (++spins > BLK_SPIN_MAX || __hip_atomic_load(C.xerr, __ATOMIC_RELAXED,
                                                                 __HIP_MEMORY_SCOPE_AGENT)) {
                     __hip_atomic_store(C.xerr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -190,11 +273,12 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
                     break;
                 }
             }
-            __threadfence();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
         __syncthreads();
     };
-    // tot[0..nv) of this part -> the chain's totals (parts summed in order)
+    // tot[0..nv) of this part -> the chain's totals (parts summed in order);
+    // tot is written by wave 0 (tid < nv)
     auto exchange = [&](int nv) {
         if (nsplit == 1) return;
         double *xb = C.xbuf + (size_t)xpar * BLK_NSPLIT_MAX * BLK_NV_MAX;
@@ -203,11 +287,11 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
                                __HIP_MEMORY_SCOPE_AGENT);
         chain_sync();
         if (tid < nv) {
-            double s = 0.0;
+            double sv = 0.0;
             for (int p = 0; p < nsplit; ++p)
-                s += __hip_atomic_load(&xb[p * BLK_NV_MAX + tid], __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-            tot[tid] = s;
+                sv += __hip_atomic_load(&xb[p * BLK_NV_MAX + tid], __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT);
+            tot[tid] = sv;
         }
         __syncthreads();
         xpar ^= 1;
@@ -218,6 +302,8 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
         sabort = 0;
         snlmv = 0;
     }
+    for (int k = tid; k < BLK_LDS_ROWS / 32 + 2; k += nthr) exclb[k] = 0u;
+    if (tid < B) bidx[tid] = -1;
     for (int k = tid; k < 624; k += nthr) {
         mtg[k] = C.rng[k];
         mti[k] = C.rng[625 + k];
@@ -293,11 +379,88 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
     int nr = 0;   // trials in the ring (uniform)
     const int qcap = DCAP / nw;
     // fixed-unit pair constants, kept in registers across the partner loop
-    const float fx_rc2 = C.fx_rc2, lj_s2f = C.lj_s2f, lj_e4f = C.lj_eps4f, fx_rs = C.fx_rs;
-    const float fx_c1 = C.fx_c1, exact = C.pot.exact_f;
-    const int ntypes = C.pot.ntypes, ntypes2 = C.pot.ntypes2;
-    const volatile BlkFx *Fv = F;
+    const BlkFxConst KF{C.fx_rc2, C.lj_s2f, C.lj_eps4f, C.fx_rs, C.fx_c1, C.pot.exact_f,
+                        C.pot.ntypes, C.pot.ntypes2};
+    const float fx_rs = C.fx_rs;
+    // waves per trial of the LDS-resident partner pass
+    const int wpt = nw >= B ? nw / B : 1;
     const volatile BlkTrial *Rv = R;
+
+    // draws of `cnt` trials into ring slots at.. (wave 0): a trial takes ten
+    // words (two uniform_int, four canonicals) unless a uniform_int rejects;
+    // lane l draws trial l from its ten words of the current state, up to
+    // the first rejection, and the rest are drawn one by one
+    auto draw = [&](int at, int cnt) {
+        while (cnt > 0) {
+            // trials whose ten words lie in the current state: in parallel
+            const int k = min(cnt, (624 - G.p) / 10);
+            int done = 0;
+            if (k > 0) {
+                bool okl = true;
+                int pi = 0;
+                double a = 0.0, b = 0.0, c = 0.0, u = 0.0;
+                if (lane < k) {
+                    const int p = G.p + 10 * lane;
+                    uint32_t w[10];
+#pragma unroll
+                    for (int q = 0; q < 10; ++q) w[q] = mt_temper(mtg[p + q]);
+                    const int gk = (int)lemire_word(w[0], (uint32_t)C.ngvec, okl);
+                    const int g = okl ? (lds_grp ? lgvec[gk] : C.gvec[gk]) : 0;
+                    const GroupDesc gd = lds_grp ? lgroups[g] : C.groups[g];
+                    pi = gd.front + (int)lemire_word(w[1], (uint32_t)(gd.back - gd.front) + 1u, okl);
+                    a = canonical_words(w[2], w[3]);
+                    b = canonical_words(w[4], w[5]);
+                    c = canonical_words(w[6], w[7]);
+                    u = canonical_words(w[8], w[9]);
+                }
+                const uint64_t bad = __ballot(lane < k && !okl);
+                done = bad ? __ffsll((unsigned long long)bad) - 1 : k;
+                if (lane < done) {
+                    BlkTrial &t = R[at + lane];
+                    t.i = pi;
+                    t.t3[0] = a; t.t3[1] = b; t.t3[2] = c;
+                    t.u = u;
+                    t.fetched = 0;
+                }
+                G.p += 10 * done;
+                G.wb = -1;
+                at += done;
+                cnt -= done;
+            }
+            if (cnt > 0) {
+                // one trial word by word: the state twists within its words,
+                // or one of its uniform_int draws needs a second word
+                const int gk = mt_range(G, 0, C.ngvec - 1);
+                const int g = lds_grp ? lgvec[gk] : C.gvec[gk];
+                const GroupDesc gd = lds_grp ? lgroups[g] : C.groups[g];
+                const int i = gd.front + mt_range(G, 0, gd.back - gd.front);
+                const double a = mt_canonical(G), b = mt_canonical(G), c = mt_canonical(G);
+                const double u = mt_canonical(G);
+                if (tid == 0) {
+                    BlkTrial &t = R[at];
+                    t.i = i;
+                    t.t3[0] = a; t.t3[1] = b; t.t3[2] = c;
+                    t.u = u;
+                    t.fetched = 0;
+                }
+                ++at;
+                --cnt;
+            }
+        }
+    };
+    // the moved particle's data from the accepted vector (one lane)
+    auto fetch_particle = [&](BlkTrial &t) {
+        const int i = t.i;
+        t.ox = gload(C.x, i); t.oy = gload(C.y, i); t.oz = gload(C.z, i);
+        const int ti = gload(C.t, i);
+        t.ti = ti;
+        t.q64 = gload(C.q, i);
+        t.r64 = gload(C.r, i);
+        t.rf = gload(C.rf, i);
+        t.pkw = gload(C.pk, i).w;
+        t.wq = EW ? t.q64 * (lds_grp ? leff[ti] : C.eff[ti]) : 0.0;
+        t.fetched = 1;
+    };
 
     for (int call = 0; call < C.ncalls && !stop && !sabort; ++call) {
         int rep0 = 0;
@@ -323,8 +486,7 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
         nr = 0;
         for (;;) {
             // ---- fill the ring with the next trials of this call (wave 0)
-            int nfill = min(B - nr, C.repeat - drawn);
-            if ((long)nfill > budget - used) nfill = (int)(budget - used);
+            const int nfill = nr >= B ? 0 : (int)min((long)min(B - nr, C.repeat - drawn), budget - used);
             if (nr == 0 && nfill == 0) {
                 if (drawn < C.repeat) {   // trial budget of the launch spent
                     stop = true;
@@ -332,73 +494,41 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
                 }
                 break;
             }
-            const int nb = nr + nfill;
+            if (wave0) draw(nr, nfill);
+            nr += nfill;
+            used += nfill;
+            drawn += nfill;
+            const int nb = min(nr, B);
             if (wave0) {
-                // draws: a trial takes ten words (two uniform_int, four
-                // canonicals) unless a uniform_int rejects; lane l draws trial
-                // l from its ten words of the current state, up to the first
-                // rejection, and the rest are drawn one by one
-                int done = 0;
-                if (nfill > 0 && G.p + 10 * nfill <= 624) {
-                    bool okl = true;
-                    int pi = 0;
-                    double a = 0.0, b = 0.0, c = 0.0, u = 0.0;
-                    if (lane < nfill) {
-                        const int p = G.p + 10 * lane;
-                        uint32_t w[10];
+                wave_sync_lds();
+                // ---- the moved particles' data (read once per draw),
+                // corrected for the moves the last block accepted
+                {
+                    const int nl = snlmv;
+                    const int li = lane < nl ? lmv_i[lane] : -1;
+                    if (lane < nr) {
+                        BlkTrial &t = R[lane];
+                        if (!t.fetched) fetch_particle(t);
+                        int hit = -1;
 #pragma unroll
-                        for (int k = 0; k < 10; ++k) w[k] = mt_temper(mtg[p + k]);
-                        const int gk = (int)lemire_word(w[0], (uint32_t)C.ngvec, okl);
-                        const int g = okl ? (lds_grp ? lgvec[gk] : C.gvec[gk]) : 0;
-                        const GroupDesc gd = lds_grp ? lgroups[g] : C.groups[g];
-                        pi = gd.front + (int)lemire_word(w[1], (uint32_t)(gd.back - gd.front) + 1u, okl);
-                        a = canonical_words(w[2], w[3]);
-                        b = canonical_words(w[4], w[5]);
-                        c = canonical_words(w[6], w[7]);
-                        u = canonical_words(w[8], w[9]);
-                    }
-                    const uint64_t bad = __ballot(lane < nfill && !okl);
-                    done = bad ? __ffsll((unsigned long long)bad) - 1 : nfill;
-                    if (lane < done) {
-                        BlkTrial &t = R[nr + lane];
-                        t.i = pi;
-                        t.t3[0] = a; t.t3[1] = b; t.t3[2] = c;
-                        t.u = u;
-                    }
-                    G.p += 10 * done;
-                    G.wb = -1;
-                }
-                for (int f = done; f < nfill; ++f) {
-                    const int gk = mt_range(G, 0, C.ngvec - 1);
-                    const int g = lds_grp ? lgvec[gk] : C.gvec[gk];
-                    const GroupDesc gd = lds_grp ? lgroups[g] : C.groups[g];
-                    const int i = gd.front + mt_range(G, 0, gd.back - gd.front);
-                    const double a = mt_canonical(G), b = mt_canonical(G), c = mt_canonical(G);
-                    const double u = mt_canonical(G);
-                    if (tid == 0) {
-                        BlkTrial &t = R[nr + f];
-                        t.i = i;
-                        t.t3[0] = a; t.t3[1] = b; t.t3[2] = c;
-                        t.u = u;
+                        for (int m = 0; m < B; ++m)
+                            if (m < nl && __builtin_amdgcn_readlane(li, m) == t.i) hit = m;
+                        if (hit >= 0) { t.ox = lmv_x[hit]; t.oy = lmv_y[hit]; t.oz = lmv_z[hit]; }
                     }
                 }
                 wave_sync_lds();
-                // ---- the moved particles' data and proposals (one lane per trial)
+                // the last block's particles leave the slice's exclusion bits
+                if (ldsp && lane < B) {
+                    const int io = bidx[lane];
+                    if (io >= j0 && io < j1) exclb[(io - j0) >> 5] = 0u;
+                }
+                wave_sync_lds();
+                // ---- proposals (one lane per trial of the block)
                 if (lane < B) {
                     if (lane < nb) {
                         BlkTrial &t = R[lane];
-                        const int i = t.i;
-                        double ox = gload(C.x, i), oy = gload(C.y, i), oz = gload(C.z, i);
-                        // a move accepted in the last block may still be in flight
-                        for (int m = 0; m < snlmv; ++m)
-                            if (lmv_i[m] == i) { ox = lmv_x[m]; oy = lmv_y[m]; oz = lmv_z[m]; }
-                        const int ti = gload(C.t, i);
-                        t.ti = ti;
-                        t.q64 = gload(C.q, i);
-                        t.r64 = gload(C.r, i);
-                        t.rf = gload(C.rf, i);
-                        t.pkw = gload(C.pk, i).w;
-                        t.wq = EW ? t.q64 * (lds_grp ? leff[ti] : C.eff[ti]) : 0.0;
+                        const int i = t.i, ti = t.ti;
+                        const double ox = t.ox, oy = t.oy, oz = t.oz;
                         // Metropolis (move.h:567-573) rejects when u > exp(-dU),
                         // i.e. when dU > -ln u
                         t.lu = -log(t.u);
@@ -413,7 +543,6 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
                             nx = ox + tx; ny = oy + ty; nz = oz + tz;
                             boundary_d(C.box, nx, ny, nz);
                         }
-                        t.ox = ox; t.oy = oy; t.oz = oz;
                         t.nx = nx; t.ny = ny; t.nz = nz;
                         t.collide = collision_d(C.box, nx, ny, nz) ? 1 : 0;
                         t.fox = to_fixed(ox, C.box.inv[0]);
@@ -423,10 +552,13 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
                         t.fny = to_fixed(ny, C.box.inv[1]);
                         t.fnz = to_fixed(nz, C.box.inv[2]);
                         int first = lane;
-                        for (int s2 = lane - 1; s2 >= 0; --s2)
-                            if (R[s2].i == i) first = s2;
+#pragma unroll
+                        for (int s2 = B - 1; s2 >= 0; --s2)
+                            if (s2 < lane && __builtin_amdgcn_readlane(i, s2) == i) first = s2;
                         t.first = first;
                         bidx[lane] = i;
+                        if (ldsp && i >= j0 && i < j1)
+                            atomicOr(&exclb[(i - j0) >> 5], 1u << ((i - j0) & 31));
                         BlkFx f;
                         f.nx = t.fnx; f.ny = t.fny; f.nz = t.fnz;
                         f.ox = t.fox; f.oy = t.foy; f.oz = t.foz;
@@ -440,8 +572,6 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
                     }
                 }
             }
-            used += nfill;
-            drawn += nfill;
             __syncthreads();
             // ---- Ewald phase tables exp(i 2 pi n x / L), old and new, per trial
             if constexpr (EW) {
@@ -507,18 +637,10 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
 #pragma unroll
                         for (int u = 0; u < U; ++u) {
                             const int j = base + u * nthr + tid;
-                            if (ldsp) {
-                                const int r = j - j0;
-                                const bool in = j < j1;
-                                X.P[u] = v4i_to_int4(in ? L.pk[r] : lds_v4i{0, 0, 0, 0});
-                                X.R[u] = (SR == SR_HS && in) ? L.rf[r] : 0.f;
-                                X.T[u] = (NEED_T && MODE == 0 && in) ? L.t[r] : 0;
-                            } else {
-                                // rows past n read the zero padding
-                                X.P[u] = gload(C.pk, j);
-                                X.R[u] = SR == SR_HS ? gload(C.rf, j) : 0.f;
-                                X.T[u] = (NEED_T && MODE == 0) ? gload(C.t, j) : 0;
-                            }
+                            // rows past n read the zero padding
+                            X.P[u] = gload(C.pk, j);
+                            X.R[u] = SR == SR_HS ? gload(C.rf, j) : 0.f;
+                            X.T[u] = (NEED_T && MODE == 0) ? gload(C.t, j) : 0;
                         }
                     };
                     auto eval = [&](const Rec &X, int base) __attribute__((always_inline)) {
@@ -530,63 +652,13 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
 #pragma unroll
                             for (int b = 0; b < B; ++b) ok = ok && j != bi[b];
                             const int4 P = X.P[u];
-                            const float bq = __int_as_float(P.w);
-                            const float rb = X.R[u] * fx_rs;
 #pragma unroll
                             for (int c = 0; c < CB; ++c) {
                                 if (bc + c >= nb) break;
-                                float dx = (float)(int32_t)(tnx[c] - P.x), dy = (float)(int32_t)(tny[c] - P.y),
-                                      dz = (float)(int32_t)(tnz[c] - P.z);
-                                const float r2n = dx * dx + dy * dy + dz * dz;
-                                dx = (float)(int32_t)(tox[c] - P.x);
-                                dy = (float)(int32_t)(toy[c] - P.y);
-                                dz = (float)(int32_t)(toz[c] - P.z);
-                                const float r2o = dx * dx + dy * dy + dz * dz;
-                                float d;
                                 bool bad;
-                                if constexpr (EW) {
-                                    if constexpr (MODE == 2) {
-                                        bad = fminf(r2n, r2o) <= fx_rc2;
-                                        d = 0.f;
-                                    } else {
-                                        float s2 = lj_s2f, e4 = lj_e4f;
-                                        if constexpr (MODE == 0) {
-                                            const int k = tti[c] * ntypes + X.T[u];
-                                            s2 = tabf[TAB_S2 * ntypes2 + k];
-                                            e4 = tabf[TAB_EPS4 * ntypes2 + k];
-                                        }
-                                        float xn = s2 * __builtin_amdgcn_rcpf(r2n);
-                                        float xo = s2 * __builtin_amdgcn_rcpf(r2o);
-                                        xn = xn * xn * xn;
-                                        xo = xo * xo * xo;
-                                        const float vn = xn * xn - xn, vo = xo * xo - xo;
-                                        bad = fminf(r2n, r2o) <= fx_rc2;
-                                        if constexpr (MODE == 0)
-                                            bad = bad || !(fabsf(e4 * vn) <= exact) ||
-                                                  !(fabsf(e4 * vo) <= exact);
-                                        d = e4 * (vn - vo);
-                                    }
-                                } else {
-                                    // fixed-unit Coulomb / Debye-Hueckel (+ hard
-                                    // sphere); charge lB q / s, radius r c / s
-                                    const float aq = taq[c], ar = tar[c];
-                                    float un = 0.f, uo = 0.f;
-                                    const float rin = __builtin_amdgcn_rsqf(r2n);
-                                    const float rio = __builtin_amdgcn_rsqf(r2o);
-                                    if constexpr (ELEC == ELEC_COULOMB) {
-                                        un = aq * bq * rin;
-                                        uo = aq * bq * rio;
-                                    } else if constexpr (ELEC == ELEC_DH) {
-                                        un = aq * bq * rin * __builtin_amdgcn_exp2f(fx_c1 * (r2n * rin));
-                                        uo = aq * bq * rio * __builtin_amdgcn_exp2f(fx_c1 * (r2o * rio));
-                                    }
-                                    bad = !(fabsf(un) <= exact) || !(fabsf(uo) <= exact);
-                                    if constexpr (SR == SR_HS) {
-                                        const float m = ar + rb;
-                                        bad = bad || (r2n < m * m) || (r2o < m * m);
-                                    }
-                                    d = un - uo;
-                                }
+                                const float d = blk_pair_fx<ELEC, SR, MODE>(
+                                    KF, tabf, tnx[c], tny[c], tnz[c], tox[c], toy[c], toz[c], tti[c],
+                                    taq[c], tar[c], P, X.R[u], X.T[u], bad);
                                 dm |= ((ok && bad) ? 1u : 0u) << (u * CB + c);
                                 ac[c] += (double)__int_as_float(__float_as_int(d) & ((ok && !bad) ? -1 : 0));
                             }
@@ -606,14 +678,9 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
                                                 dq[k] = ((bc + c) << 24) | j;
                                             } else {
                                                 const BlkTrial &t = R[bc + c];
-                                                ac[c] += ldsp
-                                                    ? du_pair64_v<ELEC, SR>(C, tab, t.nx, t.ny, t.nz, t.ox, t.oy,
-                                                                            t.oz, t.q64, t.r64, t.ti, L.x[j - j0],
-                                                                            L.y[j - j0], L.z[j - j0], L.q[j - j0],
-                                                                            L.r[j - j0], L.t[j - j0])
-                                                    : du_pair64_mixed<ELEC, SR>(C, tab, lcheb, t.nx, t.ny, t.nz,
-                                                                                t.ox, t.oy, t.oz, t.q64, t.r64,
-                                                                                t.ti, j);
+                                                ac[c] += du_pair64_mixed<ELEC, SR>(C, tab, lcheb, t.nx, t.ny,
+                                                                                   t.nz, t.ox, t.oy, t.oz,
+                                                                                   t.q64, t.r64, t.ti, j);
                                             }
                                         }
                                         cnt += __popcll(m);
@@ -636,46 +703,116 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
                         if (lane == 0 && bc + c < B) redp[wv][bc + c] = sv;
                     }
                 };
-                // partners in LDS: one record per thread and step; in device
-                // memory: two, the next two in flight
-                auto pass_u = [&](auto M, int bc) __attribute__((always_inline)) {
-                    if (ldsp) pass(M, std::integral_constant<int, 1>{}, bc);
-                    else pass(M, std::integral_constant<int, 2>{}, bc);
+                // partners in LDS: wave w takes trial w / wpt over rows
+                // (w % wpt) 64 + lane (mod 64 wpt) of the slice, block particles
+                // excluded by their bit
+                auto lds_pass = [&](auto M) __attribute__((always_inline)) {
+                    constexpr int MODE = decltype(M)::value;
+                    const int sub = wv % wpt;
+                    for (int b = wv / wpt; b < nb; b += nw / wpt) {
+                    const int32_t tnx = __builtin_amdgcn_readfirstlane(F[b].nx);
+                    const int32_t tny = __builtin_amdgcn_readfirstlane(F[b].ny);
+                    const int32_t tnz = __builtin_amdgcn_readfirstlane(F[b].nz);
+                    const int32_t tox = __builtin_amdgcn_readfirstlane(F[b].ox);
+                    const int32_t toy = __builtin_amdgcn_readfirstlane(F[b].oy);
+                    const int32_t toz = __builtin_amdgcn_readfirstlane(F[b].oz);
+                    const int tti = __builtin_amdgcn_readfirstlane(F[b].ti);
+                    const float taq = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(F[b].aq)));
+                    const float tar = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(F[b].ar)));
+                    const int ns = j1 - j0;
+                    double ac = 0.0;
+                    for (int r0 = sub * 64; r0 < ns; r0 += 64 * wpt) {
+                        const int r = r0 + lane;
+                        const bool in = r < ns;
+                        const int4 P = v4i_to_int4(in ? L.pk[r] : lds_v4i{0, 0, 0, 0});
+                        const float Rb = (SR == SR_HS && in) ? L.rf[r] : 0.f;
+                        const int Tb = (NEED_T && MODE == 0 && in) ? L.t[r] : 0;
+                        const bool ok = in && !((exclb[r >> 5] >> (r & 31)) & 1u);
+                        bool bad;
+                        const float d = blk_pair_fx<ELEC, SR, MODE>(KF, tabf, tnx, tny, tnz, tox, toy, toz,
+                                                                    tti, taq, tar, P, Rb, Tb, bad);
+                        ac += (double)__int_as_float(__float_as_int(d) & ((ok && !bad) ? -1 : 0));
+                        const bool dd = ok && bad;
+                        const uint64_t m = __ballot(dd);
+                        if (m) {
+                            if (dd) {
+                                const int k = cnt + __popcll(m & below);
+                                if (k < qcap) {
+                                    dq[k] = (b << 24) | (j0 + r);
+                                } else {
+                                    const BlkTrial &t = R[b];
+                                    ac += du_pair64_v<ELEC, SR>(C, tab, t.nx, t.ny, t.nz, t.ox, t.oy, t.oz,
+                                                                t.q64, t.r64, t.ti, L.x[r], L.y[r], L.z[r],
+                                                                L.q[r], L.r[r], L.t[r]);
+                                }
+                            }
+                            cnt += __popcll(m);
+                        }
+                    }
+                    const double sv = wave_sum_dpp(ac);
+                    if (lane == 0) redp[sub][b] = sv;
+                    }
                 };
-                for (int bc = 0; bc < nb; bc += CB) {
+                if (ldsp) {
                     if constexpr (EW) {
-                        if (mode == 2) pass_u(std::integral_constant<int, 2>{}, bc);
-                        else if (mode == 1) pass_u(std::integral_constant<int, 1>{}, bc);
-                        else pass_u(std::integral_constant<int, 0>{}, bc);
+                        if (mode == 2) lds_pass(std::integral_constant<int, 2>{});
+                        else if (mode == 1) lds_pass(std::integral_constant<int, 1>{});
+                        else lds_pass(std::integral_constant<int, 0>{});
                     } else {
-                        pass_u(std::integral_constant<int, 0>{}, bc);
+                        lds_pass(std::integral_constant<int, 0>{});
+                    }
+                } else {
+                    // partners in device memory: CB trials per pass, two
+                    // records per thread and step, the next two in flight
+                    for (int bc = 0; bc < nb; bc += CB) {
+                        if constexpr (EW) {
+                            if (mode == 2) pass(std::integral_constant<int, 2>{}, std::integral_constant<int, 2>{}, bc);
+                            else if (mode == 1) pass(std::integral_constant<int, 1>{}, std::integral_constant<int, 2>{}, bc);
+                            else pass(std::integral_constant<int, 0>{}, std::integral_constant<int, 2>{}, bc);
+                        } else {
+                            pass(std::integral_constant<int, 0>{}, std::integral_constant<int, 2>{}, bc);
+                        }
                     }
                 }
                 if (lane == 0) nqs[wv] = min(cnt, qcap);
             } else {
-                // FP64: one trial per pass over the partners
-                for (int b = 0; b < nb; ++b) {
-                    const double tnx = Rv[b].nx, tny = Rv[b].ny, tnz = Rv[b].nz;
-                    const double tox = Rv[b].ox, toy = Rv[b].oy, toz = Rv[b].oz;
-                    const double tq = Rv[b].q64, tr = Rv[b].r64;
-                    const int tt = Rv[b].ti;
-                    double ac = 0.0;
-                    for (int j = j0 + tid; j < j1; j += nthr) {
-                        bool ok = true;
-#pragma unroll
-                        for (int b2 = 0; b2 < B; ++b2) ok = ok && j != bi[b2];
-                        if (!ok) continue;
-                        if (ldsp) {
-                            const int r = j - j0;
+                if (ldsp) {
+                    // partners in LDS: wave w takes trials w / wpt, as above
+                    const int sub = wv % wpt;
+                    for (int b = wv / wpt; b < nb; b += nw / wpt) {
+                        const double tnx = R[b].nx, tny = R[b].ny, tnz = R[b].nz;
+                        const double tox = R[b].ox, toy = R[b].oy, toz = R[b].oz;
+                        const double tq = R[b].q64, tr = R[b].r64;
+                        const int tt = R[b].ti;
+                        const int ns = j1 - j0;
+                        double ac = 0.0;
+                        for (int r = sub * 64 + lane; r < ns; r += 64 * wpt) {
+                            if ((exclb[r >> 5] >> (r & 31)) & 1u) continue;
                             ac += du_pair64_v<ELEC, SR>(C, tab, tnx, tny, tnz, tox, toy, toz, tq, tr, tt,
                                                         L.x[r], L.y[r], L.z[r], L.q[r], L.r[r], L.t[r]);
-                        } else {
+                        }
+                        const double sv = wave_sum_dpp(ac);
+                        if (lane == 0) redp[sub][b] = sv;
+                    }
+                } else {
+                    // partners in device memory: one trial per pass
+                    for (int b = 0; b < nb; ++b) {
+                        const double tnx = Rv[b].nx, tny = Rv[b].ny, tnz = Rv[b].nz;
+                        const double tox = Rv[b].ox, toy = Rv[b].oy, toz = Rv[b].oz;
+                        const double tq = Rv[b].q64, tr = Rv[b].r64;
+                        const int tt = Rv[b].ti;
+                        double ac = 0.0;
+                        for (int j = j0 + tid; j < j1; j += nthr) {
+                            bool ok = true;
+#pragma unroll
+                            for (int b2 = 0; b2 < B; ++b2) ok = ok && j != bi[b2];
+                            if (!ok) continue;
                             ac += du_pair64_v<ELEC, SR>(C, tab, tnx, tny, tnz, tox, toy, toz, tq, tr, tt,
                                                         C.x[j], C.y[j], C.z[j], C.q[j], C.r[j], C.t[j]);
                         }
+                        const double sv = wave_sum_dpp(ac);
+                        if (lane == 0) redp[wv][b] = sv;
                     }
-                    const double sv = wave_sum_dpp(ac);
-                    if (lane == 0) redp[wv][b] = sv;
                 }
             }
             // ---- corrections: the block's own particles at their position
@@ -775,7 +912,7 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
                 double sv = 0.0;
                 for (int w = 0; w < nw; ++w) sv += red[w][tid];
                 if (tid < nb)
-                    for (int w = 0; w < nw; ++w) sv += redp[w][tid];
+                    for (int w = 0; w < (ldsp ? wpt : nw); ++w) sv += redp[w][tid];
                 tot[tid] = sv;
             }
             exchange(NV);
@@ -786,91 +923,175 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
             if (wave0) {
                 wave_sync_lds();
                 const bool mine = lane < nb;
-                double fin = 0.0;
-                int npi = 0, nmi = 0, nna = 0;
-                auto add = [&](double v, int sgn) {
-                    if (v != v) nna += sgn;
-                    else if (v == __builtin_inf()) npi += sgn;
-                    else if (v == -__builtin_inf()) nmi += sgn;
-                    else fin += sgn > 0 ? v : -v;
-                };
-                int myi = -1;
-                double lu = 0.0;
-                int mycol = 0;
-                double Xs[B], Ys[B];
-                unsigned app = 0;   // earlier trials whose acceptance changes this one
-                if (mine) {
-                    myi = R[lane].i;
-                    lu = R[lane].lu;
-                    mycol = R[lane].collide;
-                    add(tot[lane], 1);
-                    for (int f = 0; f < nb; ++f)
-                        if (R[f].first == f && R[f].i != myi) add(xc[f][lane], 1);
-                }
+                // every load first (lane-parallel), then the sequential part on
+                // per-lane registers (lane t: trial t) and a few scalars
+                const int myi = mine ? R[lane].i : -1;
+                const int myf = mine ? R[lane].first : 0;
+                const double lu = mine ? R[lane].lu : 0.0;
+                const int mycol = mine ? R[lane].collide : 0;
+                const double base = mine ? tot[lane] : 0.0;
+                // per earlier trial s: X (its particle's snapshot term), Y (after
+                // its accepted move), and whether it moves this lane's particle
+                // (same) or changes this lane's energy (app)
+                unsigned same = 0, app = 0, firsts = 0;
+                double xt[B], Xs[B], Ys[B];
 #pragma unroll
                 for (int s2 = 0; s2 < B; ++s2) {
-                    Xs[s2] = 0.0;
-                    Ys[s2] = 0.0;
-                    if (mine && s2 < lane && R[s2].i != myi) {
-                        app |= 1u << s2;
-                        Xs[s2] = xc[R[s2].first][lane];
-                        Ys[s2] = yc[s2][lane];
-                        if constexpr (EW)
-                            Ys[s2] += 2.0 * C.recip_pref * tot[B + s2 * (2 * B - s2 - 1) / 2 + (lane - s2 - 1)];
+                    const int is = __builtin_amdgcn_readlane(myi, s2);
+                    const int fs = __builtin_amdgcn_readlane(myf, s2);
+                    const bool use = mine && s2 < nb;
+                    const bool other = use && is != myi;
+                    same |= (use && is == myi ? 1u : 0u) << s2;
+                    app |= (other && s2 < lane ? 1u : 0u) << s2;
+                    firsts |= (other && fs == s2 ? 1u : 0u) << s2;
+                    xt[s2] = use ? xc[s2][lane] : 0.0;
+                    Xs[s2] = use ? xc[fs][lane] : 0.0;
+                    Ys[s2] = use ? yc[s2][lane] : 0.0;
+                    if constexpr (EW) {
+                        const bool g = mine && s2 < lane;
+                        const int gi = g ? B + s2 * (2 * B - s2 - 1) / 2 + (lane - s2 - 1) : 0;
+                        const double gv = g ? tot[gi] : 0.0;
+                        Ys[s2] += 2.0 * C.recip_pref * gv;
                     }
+                }
+                bool odd = base != base || base == -__builtin_inf();   // NaN or -inf somewhere
+#pragma unroll
+                for (int s2 = 0; s2 < B; ++s2) {
+                    const double x1 = ((firsts >> s2) & 1u) ? xt[s2] : 0.0;
+                    const double x2 = ((app >> s2) & 1u) ? Xs[s2] : 0.0;
+                    const double x3 = ((app >> s2) & 1u) ? Ys[s2] : 0.0;
+                    odd = odd || x1 != x1 || x1 == -__builtin_inf() || x2 != x2 ||
+                          x2 == -__builtin_inf() || x3 != x3 || x3 == -__builtin_inf();
                 }
                 int nd = nb;
                 unsigned accm = 0;
-                bool refresh = false, moved = false;
+                bool refresh = false;
                 double mydu = 0.0;
+                int moved = 0;   // this lane's particle was moved by an accepted trial
+                if (!__ballot(odd) && !(EW && C.surf_pref != 0.0)) {
+                    // lane t's energy change: a finite sum and a count of +inf
+                    // terms (a collision counts as one: energy.h:3597-3607),
+                    // and what an accepted earlier trial s changes in them
+                    auto fpart = [](double v) { return v == __builtin_inf() ? 0.0 : v; };
+                    auto ipart = [](double v) { return v == __builtin_inf() ? 1 : 0; };
+                    double fin0 = fpart(base);
+                    int npi0 = (mycol ? 1 : 0) + ipart(base);
+                    double dfin[B];
+                    int dinf[B];
 #pragma unroll
-                for (int t = 0; t < B; ++t) {
-                    if (t >= nb) break;
+                    for (int s2 = 0; s2 < B; ++s2) {
+                        if ((firsts >> s2) & 1u) {
+                            fin0 += fpart(xt[s2]);
+                            npi0 += ipart(xt[s2]);
+                        }
+                        const bool ap = (app >> s2) & 1u;
+                        dfin[s2] = ap ? fpart(Ys[s2]) - fpart(Xs[s2]) : 0.0;
+                        dinf[s2] = ap ? ipart(Ys[s2]) - ipart(Xs[s2]) : 0;
+                    }
+                    // The decisions as a fixed point: from a guess of the
+                    // accepted set, every lane forms its energy change and
+                    // Metropolis decision (move.h:567-573) at once; trial t
+                    // depends on trials s < t only, so after k rounds the first
+                    // k decisions are the sequential ones, and a set that
+                    // reproduces itself is the sequential answer.
+                    unsigned am = 0;
+                    for (int it = 0; it <= B; ++it) {
+                        double f = fin0;
+                        int ni = npi0;
+#pragma unroll
+                        for (int s2 = 0; s2 < B; ++s2)
+                            if ((am >> s2) & 1u) { f += dfin[s2]; ni += dinf[s2]; }
+                        mydu = ni > 0 ? __builtin_inf() : f;
+                        const bool acc = mine && (mydu <= 0.0 || !(mydu > lu));
+                        const unsigned nm = (unsigned)__ballot(acc);
+                        if (nm == am) break;
+                        am = nm;
+                    }
                     // a trial whose particle an accepted trial of this block
                     // moved waits for the next block
-                    if (__builtin_amdgcn_readlane((int)moved, t)) { nd = t; break; }
-                    double v = (nna > 0 || (npi > 0 && nmi > 0)) ? __builtin_nan("")
-                               : npi > 0 ? __builtin_inf()
-                               : nmi > 0 ? -__builtin_inf() : fin;
-                    const long long vb = __double_as_longlong(v);
-                    const int lo = __builtin_amdgcn_readlane((int)vb, t);
-                    const int hi = __builtin_amdgcn_readlane((int)(vb >> 32), t);
-                    double du = __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
-                                                                 (unsigned long long)(unsigned)lo));
-                    double dMx = 0.0, dMy = 0.0, dMz = 0.0;
-                    if (EW && C.surf_pref != 0.0) {
-                        const BlkTrial &T = R[t];
-                        dMx = T.wq * (T.nx - T.ox);
-                        dMy = T.wq * (T.ny - T.oy);
-                        dMz = T.wq * (T.nz - T.oz);
-                        const double Nx = Mx + dMx, Ny = My + dMy, Nz = Mz + dMz;
-                        du += C.surf_pref *
-                              ((Nx * Nx + (Ny * Ny + Nz * Nz)) - (Mx * Mx + (My * My + Mz * Mz)));
-                    }
-                    if (__builtin_amdgcn_readlane(mycol, t)) du = __builtin_inf();   // energy.h:3597-3607
-                    const long long lb = __double_as_longlong(lu);
-                    const double lut = __longlong_as_double(
-                        (long long)(((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(lb >> 32), t) << 32) |
-                                    (unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)lb, t)));
-                    const bool accept = du <= 0.0 || !(du > lut);
-                    if (lane == t) mydu = du;
-                    if (accept) {
-                        accm |= 1u << t;
-                        utot += du;
-                        dusum += du;
-                        Mx += dMx; My += dMy; Mz += dMz;
-                        if ((app >> t) & 1u) {
-                            add(Xs[t], -1);
-                            add(Ys[t], 1);
-                        }
-                        moved = moved || (myi == __builtin_amdgcn_readlane(myi, t));
-                        if (EW && ++ew_cnt > C.update_frequency - 1) {
+                    const unsigned below_me = (1u << lane) - 1u;
+                    const unsigned cut = (unsigned)__ballot(mine && (same & am & below_me) != 0u);
+                    if (cut) nd = __ffs(cut) - 1;
+                    accm = am & ((nd >= 32 ? 0u : (1u << nd)) - 1u);
+                    if constexpr (EW) {
+                        // NonbondedEwald::update: the accepted move that takes
+                        // the counter past update_frequency - 1 ends the block
+                        const int need = C.update_frequency - ew_cnt;   // accepts until the refresh
+                        if (need >= 1 && __popc(accm) >= need) {
+                            unsigned m2 = accm;
+                            for (int k = 1; k < need; ++k) m2 &= m2 - 1u;
+                            const int tr = __ffs(m2) - 1;
+                            nd = tr + 1;
+                            accm &= (2u << tr) - 1u;
                             refresh = true;
-                            nd = t + 1;
-                            break;
                         }
-                    } else if (du == __builtin_inf()) {
-                        n_inf++;
+                        ew_cnt += __popc(accm);
+                    }
+#pragma unroll
+                    for (int t = 0; t < B; ++t)
+                        if ((accm >> t) & 1u) {
+                            const double d = readlane_f64(mydu, t);
+                            utot += d;
+                            dusum += d;
+                        }
+                    n_inf += __popcll(__ballot(lane < nd && !((accm >> lane) & 1u) &&
+                                               mydu == __builtin_inf()));
+                } else {
+                    // the same with counts of +inf, -inf and NaN terms, which
+                    // add up as IEEE sums do
+                    double fin = 0.0;
+                    int npi = 0, nmi = 0, nna = 0;
+                    auto add = [&](double v, int sgn) {
+                        const bool isn = v != v, isp = v == __builtin_inf(),
+                                   ism = v == -__builtin_inf();
+                        nna += isn ? sgn : 0;
+                        npi += isp ? sgn : 0;
+                        nmi += ism ? sgn : 0;
+                        fin += (isn || isp || ism) ? 0.0 : (sgn > 0 ? v : -v);
+                    };
+                    add(base, 1);
+#pragma unroll
+                    for (int s2 = 0; s2 < B; ++s2)
+                        if ((firsts >> s2) & 1u) add(xt[s2], 1);
+#pragma unroll
+                    for (int t = 0; t < B; ++t) {
+                        if (t >= nb) break;
+                        if (__builtin_amdgcn_readlane(moved, t)) { nd = t; break; }
+                        const double v = (nna > 0 || (npi > 0 && nmi > 0)) ? __builtin_nan("")
+                                         : npi > 0 ? __builtin_inf()
+                                         : nmi > 0 ? -__builtin_inf() : fin;
+                        double du = readlane_f64(v, t);
+                        double dMx = 0.0, dMy = 0.0, dMz = 0.0;
+                        if (EW && C.surf_pref != 0.0) {
+                            const BlkTrial &T = R[t];
+                            dMx = T.wq * (T.nx - T.ox);
+                            dMy = T.wq * (T.ny - T.oy);
+                            dMz = T.wq * (T.nz - T.oz);
+                            const double Nx = Mx + dMx, Ny = My + dMy, Nz = Mz + dMz;
+                            du += C.surf_pref *
+                                  ((Nx * Nx + (Ny * Ny + Nz * Nz)) - (Mx * Mx + (My * My + Mz * Mz)));
+                        }
+                        if (__builtin_amdgcn_readlane(mycol, t)) du = __builtin_inf();   // energy.h:3597-3607
+                        const bool accept = du <= 0.0 || !(du > readlane_f64(lu, t));
+                        if (lane == t) mydu = du;
+                        if (accept) {
+                            accm |= 1u << t;
+                            utot += du;
+                            dusum += du;
+                            if (EW) { Mx += dMx; My += dMy; Mz += dMz; }
+                            if ((app >> t) & 1u) {
+                                add(Xs[t], -1);
+                                add(Ys[t], 1);
+                            }
+                            moved |= (int)((same >> t) & 1u);
+                            if (EW && ++ew_cnt > C.update_frequency - 1) {
+                                refresh = true;
+                                nd = t + 1;
+                                break;
+                            }
+                        } else if (du == __builtin_inf()) {
+                            n_inf++;
+                        }
                     }
                 }
                 n_trials += nd;
@@ -940,11 +1161,11 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
                         }
                     snlmv = m;
                     // undecided trials keep their draws for the next block
-                    for (int b = nd; b < nb; ++b) R[b - nd] = R[b];
+                    for (int b = nd; b < nr; ++b) R[b - nd] = R[b];
                 }
                 wave_sync_lds();
             }
-            nr = nb - nd;
+            nr -= nd;
             if constexpr (EW) {
                 if (refresh) {
                     // NonbondedEwald::update (ewald.h:626-643): S(k) from all

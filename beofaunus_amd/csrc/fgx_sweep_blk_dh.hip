@@ -1,12 +1,12 @@
 // fgx_sweep_blk_dh.hip -- block AtomicTranslation sweep instances for
 // Debye-Hueckel (potentials.h:3303-3375) and Coulomb (potentials.h:2515-2568)
 // with or without HardSphere (potentials.h:455-475), and LennardJonesLB
-// alone in FP64: eight trials per block.
+// alone in FP64: sixteen trials per block.
 #include "fgx_sweep_blk.hpp"
 
 namespace fgx {
 
-static constexpr int BLK_B_PAIR = 8;
+static constexpr int BLK_B_PAIR = 16;
 
 #define FGX_BLK_CASES(X)                                  \
     X(ELEC_DH, SR_NONE, true) X(ELEC_DH, SR_NONE, false)  \
