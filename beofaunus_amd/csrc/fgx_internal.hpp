@@ -163,8 +163,10 @@ struct fgx_system {
     size_t cell_bytes = 0;
     bool cell_valid = false;               // cleared whenever the accepted state changes
     // block AtomicTranslation sweep (fgx_set_block_sweep): 0 off, 1 where an
-    // instance exists; workgroups per chain (0: automatic)
-    int blk_mode = 0;
+    // instance exists, 2 automatic (1, except batches of unsplit mixed Ewald
+    // chains with partners in device memory); workgroups per chain (0:
+    // automatic)
+    int blk_mode = 2;
     int blk_nsplit = 0;
     int blk_parts = 0;                     // workgroups of the last block sweep (0: one-trial sweep)
     double *d_xbuf = nullptr;              // exchange slots of the chain's parts

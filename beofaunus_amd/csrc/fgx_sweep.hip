@@ -578,6 +578,13 @@ static int plan_blk(std::vector<fgx_system *> &sys, std::vector<ChainDev> &chain
     // together (one 512-thread workgroup per compute unit)
     FGX_CHECK(!split || ngrid <= ncu, FGX_EUNSUPPORTED,
               "block sweep: chains split over more workgroups than the device holds at once");
+    // automatic mode: unsplit mixed Ewald chains whose partners stay in
+    // device memory run faster one trial at a time (the partner loop there
+    // is the whole cost, and its one-trial form is the better tuned)
+    bool keep = false;
+    for (int c = 0; c < nc; ++c)
+        keep = keep || sys[c]->blk_mode == 1 || parts[c] > 1 || chains[c].ldsp;
+    if (!keep && s0->pot.elec == ELEC_EWALD && mix) return FGX_OK;
     std::vector<int2> map((size_t)ngrid, make_int2(-1, -1));
     for (int x = 0; x < nx; ++x)
         for (size_t r = 0; r < per[x].size(); ++r) map[r * nx + x] = per[x][r];
@@ -1228,8 +1235,8 @@ extern "C" int fgx_set_lds_partners(fgx_system *s, int on)
 
 extern "C" int fgx_set_block_sweep(fgx_system *s, int mode, int nsplit)
 {
-    FGX_CHECK(s && (mode == 0 || mode == 1) && nsplit >= 0 && nsplit <= BLK_NSPLIT_MAX, FGX_EARG,
-              "fgx_set_block_sweep: mode is 0 or 1, nsplit in 0..32");
+    FGX_CHECK(s && mode >= 0 && mode <= 2 && nsplit >= 0 && nsplit <= BLK_NSPLIT_MAX, FGX_EARG,
+              "fgx_set_block_sweep: mode is 0, 1 or 2, nsplit in 0..32");
     s->blk_mode = mode;
     s->blk_nsplit = nsplit;
     return FGX_OK;

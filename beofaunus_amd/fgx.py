@@ -429,10 +429,13 @@ class System:
 
     def set_block_sweep(self, on, nsplit=0):
         """Block AtomicTranslation sweep: B trials per pass over the partners,
-        decided in order with exact corrections (True), or one trial at a time
-        (False, default).  nsplit > 1 spreads a chain over that many
-        workgroups; 0 chooses automatically."""
-        _check(lib().fgx_set_block_sweep(self.h, 1 if on else 0, int(nsplit)))
+        decided in order with exact corrections (True), one trial at a time
+        (False), or "auto" (default: block sweeps except for batches of
+        unsplit mixed Ewald chains with partners in device memory).  nsplit
+        > 1 spreads a chain over that many workgroups; 0 chooses
+        automatically."""
+        mode = 2 if on == "auto" else (1 if on else 0)
+        _check(lib().fgx_set_block_sweep(self.h, mode, int(nsplit)))
 
     def block_parts(self):
         """workgroups per chain of the last device sweep if it was a block

@@ -267,8 +267,10 @@ int fgx_set_ewald_state(fgx_system *s, int hbm);
  * exact corrections for the block's own moves; the accept sequence is the
  * one-trial sweep's, energy changes agree to FP64 rounding.  mode 1 uses it
  * where an instance exists (Debye-Hueckel / Coulomb with or without hard
- * spheres, Lennard-Jones in FP64, Ewald + Lennard-Jones; no cell list, Ewald
- * state in LDS), 0 keeps the one-trial sweep.  nsplit > 1 spreads one chain
+ * spheres, Lennard-Jones in FP64, Ewald + Lennard-Jones; no cell list), 0
+ * keeps the one-trial sweep, 2 (default) is 1 except for batches of mixed
+ * Ewald chains that all keep one workgroup with partners in device memory,
+ * which run one trial at a time.  nsplit > 1 spreads one chain
  * over that many workgroups (at most 32), which exchange partial sums once
  * per block; 0 chooses automatically.  A launch whose chains cannot all be
  * co-resident is refused (FGX_EUNSUPPORTED); a part that waits too long for
