@@ -174,6 +174,8 @@ struct fgx_system {
     int *d_xerr = nullptr;                 // launch error word (batch owner)
     int2 *d_blkmap = nullptr;              // workgroup -> (chain, part) of the batch (owner)
     int blkmap_cap = 0;
+    void *d_mirror = nullptr;              // trial-mirror copy jobs of the batch (owner)
+    int mirror_cap = 0;
     // Energy::ExternalPressure term (1/A^3)
     bool press_on = false;
     double press = 0.0;

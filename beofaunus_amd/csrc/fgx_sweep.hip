@@ -871,19 +871,59 @@ int enqueue_batch(const BatchPlan &B, hipStream_t st)
 }
 
 // Trial vector and trial S(k) mirror the accepted state after device sweeps
-// (undo/accept leave p == trial between moves).
+// (undo/accept leave p == trial between moves): one copy launch for the
+// whole batch, each system a row of the grid.
+struct MirrorJob {
+    const double *sx, *sy, *sz, *sq;
+    double *dx, *dy, *dz, *dq;
+    int n, nq;
+};
+
+__global__ __launch_bounds__(256) void k_mirror(const MirrorJob *__restrict__ jobs)
+{
+    const MirrorJob J = jobs[blockIdx.y];
+    const int stride = gridDim.x * blockDim.x;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < J.n; i += stride) {
+        J.dx[i] = J.sx[i];
+        J.dy[i] = J.sy[i];
+        J.dz[i] = J.sz[i];
+    }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < J.nq; i += stride) J.dq[i] = J.sq[i];
+}
+
 int mirror_trial(std::vector<fgx_system *> &sys, hipStream_t st)
 {
-    for (fgx_system *s : sys) {
-        if (s->has_ewald)
-            FGX_HIP(hipMemcpyAsync(s->ew.Qt, s->ew.Q, sizeof(double) * 2 * s->ew.K,
-                                   hipMemcpyDeviceToDevice, st));
-        for (int k = 0; k < 3; ++k) {
-            double *src = k == 0 ? s->v[0].x : k == 1 ? s->v[0].y : s->v[0].z;
-            double *dst = k == 0 ? s->v[1].x : k == 1 ? s->v[1].y : s->v[1].z;
-            FGX_HIP(hipMemcpyAsync(dst, src, sizeof(double) * s->n, hipMemcpyDeviceToDevice, st));
-        }
+    const int nc = (int)sys.size();
+    if (nc == 0) return FGX_OK;
+    fgx_system *s0 = sys[0];
+    std::vector<MirrorJob> jobs(nc);
+    int maxn = 1;
+    for (int c = 0; c < nc; ++c) {
+        fgx_system *s = sys[c];
+        MirrorJob &J = jobs[c];
+        J.sx = s->v[0].x; J.sy = s->v[0].y; J.sz = s->v[0].z;
+        J.dx = s->v[1].x; J.dy = s->v[1].y; J.dz = s->v[1].z;
+        J.n = s->n;
+        J.sq = s->has_ewald ? s->ew.Q : nullptr;
+        J.dq = s->has_ewald ? s->ew.Qt : nullptr;
+        J.nq = s->has_ewald ? 2 * s->ew.K : 0;
+        maxn = std::max(maxn, std::max(J.n, J.nq));
     }
+    if (s0->mirror_cap < nc) {
+        if (s0->d_mirror) (void)hipFree(s0->d_mirror);
+        s0->d_mirror = nullptr;
+        s0->mirror_cap = 0;
+        FGX_HIP(hipMalloc(&s0->d_mirror, sizeof(MirrorJob) * nc));
+        s0->mirror_cap = nc;
+    }
+    MirrorJob *d_jobs = (MirrorJob *)s0->d_mirror;
+    FGX_HIP(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(MirrorJob) * nc, hipMemcpyHostToDevice, st));
+    const int gx = std::min(64, (maxn + 255) / 256);
+    for (int c0 = 0; c0 < nc; c0 += 65535) {
+        const int rows = std::min(65535, nc - c0);
+        hipLaunchKernelGGL(k_mirror, dim3(gx, rows), dim3(256), 0, st, d_jobs + c0);
+    }
+    FGX_HIP(hipGetLastError());
     return FGX_OK;
 }
 

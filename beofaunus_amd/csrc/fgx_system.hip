@@ -968,7 +968,7 @@ extern "C" int fgx_destroy(fgx_system *s)
                     s->d_gvec, s->d_batch,
                     s->d_bstat_i, s->d_bstat_d, s->d_resume, s->d_cell_cnt, s->d_cell_rec,
                     s->d_cell_idx, s->d_cell_slot, s->d_stage, s->d_xbuf, s->d_xcnt,
-                    s->d_xerr, s->d_blkmap};
+                    s->d_xerr, s->d_blkmap, s->d_mirror};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (s->h_out) (void)hipHostFree(s->h_out);
