@@ -38,10 +38,12 @@
 namespace fgx {
 
 static constexpr int BLK_NSPLIT_MAX = 32;    // workgroups per chain
-static constexpr int BLK_NV_MAX = 16;        // partial sums exchanged per block
+static constexpr int BLK_NV_MAX = 40;        // partial sums exchanged per block
 static constexpr int BLK_THREADS = 512;
-static constexpr int BLK_CB_PAIR = 2;        // trials per pass over the partners, mixed path
-static constexpr int BLK_CB_EWALD = 2;
+// partners in device memory (mixed path): trials per pass over the partners
+// and records per thread and step (the next step's records in flight)
+static constexpr int BLK_CB_PAIR = 2, BLK_U_PAIR = 2;
+static constexpr int BLK_CB_EWALD = 2, BLK_U_EWALD = 2;
 static constexpr int BLK_LDS_ROWS = 2048;    // partner rows of an LDS-resident slice, at most
 static constexpr long BLK_SPIN_MAX = 1L << 22;   // s_sleep rounds of a part waiting for the others
 
@@ -59,18 +61,22 @@ struct BlkTrial {
 
 // byte offsets of a block-sweep workgroup's dynamic LDS: Ewald S(k) slice
 // (double2), A_k (double), packed n (int); per-trial phase tables
-// [B][old, new][3 kcc1] (double2); the partner slice (LdsPart)
+// [B][old, new][3 kcc1] (double2); a bit per partner row of the slice (the
+// block's own particles, excluded from the pass); the LDS-resident partner
+// slice (LdsPart, ns rows; 0 when the partners stay in device memory)
 struct BlkLds {
-    size_t s, a, kid, etab, part, total;
+    size_t s, a, kid, etab, excl, part, total;
 };
-__host__ __device__ inline BlkLds blk_lds(int ks, int B, int kcc1, int ew, int ns)
+__host__ __device__ inline int blk_excl_words(int rows) { return (rows + 31) / 32 + 2; }
+__host__ __device__ inline BlkLds blk_lds(int ks, int B, int kcc1, int ew, int rows, int ns)
 {
     BlkLds o;
     o.s = 0;
     o.a = (size_t)16 * ks;
     o.kid = (size_t)24 * ks;
     o.etab = ((size_t)28 * ks + 15) / 16 * 16;
-    o.part = o.etab + (ew ? (size_t)B * 2 * 3 * kcc1 * 16 : 0);
+    o.excl = o.etab + (ew ? (size_t)B * 2 * 3 * kcc1 * 16 : 0);
+    o.part = (o.excl + 4 * (size_t)blk_excl_words(rows) + 15) / 16 * 16;
     o.total = o.part + (size_t)ns * LDS_PART_BYTES;
     return o;
 }
@@ -213,8 +219,6 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
     __shared__ double tot[BLK_NV_MAX];
     __shared__ double xc[B][B], yc[B][B];
     __shared__ int dlist[MIX ? DCAP : 1];
-    // block particles in the LDS-resident partner slice (bit per row)
-    __shared__ uint32_t exclb[BLK_LDS_ROWS / 32 + 2];
     __shared__ int nqs[BLK_THREADS / 64];
     __shared__ double ltab[TAB_COUNT * LDS_TYPES2];
     __shared__ float ltabf[MIX ? TAB_COUNT * LDS_TYPES2 : 1];
@@ -241,12 +245,13 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
     const int ks = k1 - k0;
     const int kcc1 = C.kcc + 1;
     const bool ldsp = C.ldsp != 0;
-    const BlkLds lo = blk_lds(ks, B, kcc1, EW ? 1 : 0, ldsp ? j1 - j0 : 0);
+    const BlkLds lo = blk_lds(ks, B, kcc1, EW ? 1 : 0, j1 - j0, ldsp ? j1 - j0 : 0);
     char *dbase = (char *)dyn;
     double2 *S = (double2 *)(dbase + lo.s);
     double *Ak = (double *)(dbase + lo.a);
     int *kid = (int *)(dbase + lo.kid);
     double2 *etab = (double2 *)(dbase + lo.etab);
+    uint32_t *exclb = (uint32_t *)(dbase + lo.excl);   // block particles in the slice
     LdsPart L;
     L.j0 = j0;
     L.ns = ldsp ? j1 - j0 : 0;
@@ -302,7 +307,7 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
         sabort = 0;
         snlmv = 0;
     }
-    for (int k = tid; k < BLK_LDS_ROWS / 32 + 2; k += nthr) exclb[k] = 0u;
+    for (int k = tid; k < blk_excl_words(j1 - j0); k += nthr) exclb[k] = 0u;
     if (tid < B) bidx[tid] = -1;
     for (int k = tid; k < 624; k += nthr) {
         mtg[k] = C.rng[k];
@@ -518,7 +523,7 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
                 }
                 wave_sync_lds();
                 // the last block's particles leave the slice's exclusion bits
-                if (ldsp && lane < B) {
+                if (lane < B) {
                     const int io = bidx[lane];
                     if (io >= j0 && io < j1) exclb[(io - j0) >> 5] = 0u;
                 }
@@ -557,7 +562,7 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
                             if (s2 < lane && __builtin_amdgcn_readlane(i, s2) == i) first = s2;
                         t.first = first;
                         bidx[lane] = i;
-                        if (ldsp && i >= j0 && i < j1)
+                        if (i >= j0 && i < j1)
                             atomicOr(&exclb[(i - j0) >> 5], 1u << ((i - j0) & 31));
                         BlkFx f;
                         f.nx = t.fnx; f.ny = t.fny; f.nz = t.fnz;
@@ -595,9 +600,8 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
             double acc[B];
 #pragma unroll
             for (int b = 0; b < B; ++b) acc[b] = 0.0;
-            int bi[B];
-#pragma unroll
-            for (int b = 0; b < B; ++b) bi[b] = bidx[b];
+            // a partner row of the slice that is one of the block's particles
+            auto excluded = [&](int j) { return (exclb[(j - j0) >> 5] >> ((j - j0) & 31)) & 1u; };
             if constexpr (MIX) {
                 const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
                 int *dq = dlist + wv * qcap;
@@ -643,14 +647,18 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
                             X.T[u] = (NEED_T && MODE == 0) ? gload(C.t, j) : 0;
                         }
                     };
+                    // one LJ parameter pair beyond rc (|u| < 1e-2 kT per pair): a
+                    // step's terms are summed in FP32 and widened once
+                    constexpr bool BATCH32 = EW && MODE == 1;
+                    float bs[CB];
+#pragma unroll
+                    for (int c = 0; c < CB; ++c) bs[c] = 0.f;
                     auto eval = [&](const Rec &X, int base) __attribute__((always_inline)) {
                         unsigned dm = 0;
 #pragma unroll
                         for (int u = 0; u < U; ++u) {
                             const int j = base + u * nthr + tid;
-                            bool ok = j < j1;
-#pragma unroll
-                            for (int b = 0; b < B; ++b) ok = ok && j != bi[b];
+                            const bool ok = j < j1 && !excluded(j);
                             const int4 P = X.P[u];
 #pragma unroll
                             for (int c = 0; c < CB; ++c) {
@@ -660,8 +668,14 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
                                     KF, tabf, tnx[c], tny[c], tnz[c], tox[c], toy[c], toz[c], tti[c],
                                     taq[c], tar[c], P, X.R[u], X.T[u], bad);
                                 dm |= ((ok && bad) ? 1u : 0u) << (u * CB + c);
-                                ac[c] += (double)__int_as_float(__float_as_int(d) & ((ok && !bad) ? -1 : 0));
+                                const float v = __int_as_float(__float_as_int(d) & ((ok && !bad) ? -1 : 0));
+                                if constexpr (BATCH32) bs[c] += v;
+                                else ac[c] += (double)v;
                             }
+                        }
+                        if constexpr (BATCH32) {
+#pragma unroll
+                            for (int c = 0; c < CB; ++c) { ac[c] += (double)bs[c]; bs[c] = 0.f; }
                         }
                         if (__ballot(dm != 0)) {
 #pragma unroll
@@ -762,15 +776,16 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
                         lds_pass(std::integral_constant<int, 0>{});
                     }
                 } else {
-                    // partners in device memory: CB trials per pass, two
-                    // records per thread and step, the next two in flight
+                    // partners in device memory: CB trials per pass, UM records
+                    // per thread and step, the next step's in flight
+                    constexpr int UM = EW ? BLK_U_EWALD : BLK_U_PAIR;
                     for (int bc = 0; bc < nb; bc += CB) {
                         if constexpr (EW) {
-                            if (mode == 2) pass(std::integral_constant<int, 2>{}, std::integral_constant<int, 2>{}, bc);
-                            else if (mode == 1) pass(std::integral_constant<int, 1>{}, std::integral_constant<int, 2>{}, bc);
-                            else pass(std::integral_constant<int, 0>{}, std::integral_constant<int, 2>{}, bc);
+                            if (mode == 2) pass(std::integral_constant<int, 2>{}, std::integral_constant<int, UM>{}, bc);
+                            else if (mode == 1) pass(std::integral_constant<int, 1>{}, std::integral_constant<int, UM>{}, bc);
+                            else pass(std::integral_constant<int, 0>{}, std::integral_constant<int, UM>{}, bc);
                         } else {
-                            pass(std::integral_constant<int, 0>{}, std::integral_constant<int, 2>{}, bc);
+                            pass(std::integral_constant<int, 0>{}, std::integral_constant<int, UM>{}, bc);
                         }
                     }
                 }
@@ -803,10 +818,7 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
                         const int tt = Rv[b].ti;
                         double ac = 0.0;
                         for (int j = j0 + tid; j < j1; j += nthr) {
-                            bool ok = true;
-#pragma unroll
-                            for (int b2 = 0; b2 < B; ++b2) ok = ok && j != bi[b2];
-                            if (!ok) continue;
+                            if (excluded(j)) continue;
                             ac += du_pair64_v<ELEC, SR>(C, tab, tnx, tny, tnz, tox, toy, toz, tq, tr, tt,
                                                         C.x[j], C.y[j], C.z[j], C.q[j], C.r[j], C.t[j]);
                         }

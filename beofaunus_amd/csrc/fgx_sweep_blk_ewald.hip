@@ -1,7 +1,6 @@
 // fgx_sweep_blk_ewald.hip -- block AtomicTranslation sweep instances for
 // NonbondedEwald<LennardJonesLB> (ewald.h:229-760, potentials.h:2003-2116):
-// four trials per block (their dS(k) stay in registers between the
-// reciprocal sum and the accept update).
+// four trials per block.
 #include "fgx_sweep_blk.hpp"
 
 namespace fgx {
