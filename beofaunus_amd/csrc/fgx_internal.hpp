@@ -168,6 +168,10 @@ struct fgx_system {
     // automatic)
     int blk_mode = 2;
     int blk_nsplit = 0;
+    // XCD that holds all of the chain's workgroups (-1: chosen per launch);
+    // set by callers that run several block-sweep launches at once and keep
+    // every XCD's split workgroups within its compute units
+    int blk_xcd = -1;
     int blk_parts = 0;                     // workgroups of the last block sweep (0: one-trial sweep)
     double *d_xbuf = nullptr;              // exchange slots of the chain's parts
     int *d_xcnt = nullptr;                 // arrival / exit counters (zero between launches)
@@ -196,9 +200,13 @@ int pull_device_state(fgx_system *s);
 // asynchronous batched sweeps (fgx_sweep.hip), used by the job server:
 // chain c runs up to ncalls[c] Propagator::move() calls and at most
 // max_trials[c] trials (> 0); a call cut short resumes in the next launch.
-// Statistics slot 5 holds the calls completed.
+// Statistics slot 5 holds the calls completed.  Block sweeps take each
+// chain's blk_nsplit workgroups (on XCD blk_xcd when set).
 int batch_launch(std::vector<fgx_system *> &sys, const std::vector<int> &ncalls,
                  const std::vector<long> &max_trials, void **handle);
+// Workgroups of a job-server chain's block sweeps: fixed by the chain alone
+// (one per 512 partner rows, at most cap); 0 when it cannot run them.
+int blk_job_parts(const fgx_system *s, int cap);
 int batch_finish(void *handle, double *seconds, std::vector<long> *stats_i,
                  std::vector<double> *stats_d);
 // the device sweeps' checks of a system and its registered move (the ones a

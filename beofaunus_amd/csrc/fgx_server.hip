@@ -12,10 +12,12 @@
 // measured trial rate, so every launch lasts about one tick whatever the
 // job sizes: a large job's sweep spans several ticks (the kernel resumes an
 // unfinished Propagator::move() call), a small job runs several sweeps per
-// tick, and arrivals join at the next tick (continuous batching).  A job's result does not
-// depend on the other jobs it shares launches with: chains are independent
-// workgroups and every reduction has a fixed order.  A job the device
-// sweeps cannot run fails at admission, alone.
+// tick, and arrivals join at the next tick (continuous batching).  A large
+// job whose chain runs the block AtomicTranslation sweep is spread over
+// several workgroups of one XCD, as many as its size alone fixes.  A job's
+// result does not depend on the other jobs it shares launches with: its
+// workgroups and launch shape are its own, and every reduction has a fixed
+// order.  A job the device sweeps cannot run fails at admission, alone.
 //
 // The reference runs one simulation per process (src/examples/bulk.cpp) and
 // has no counterpart to this component.
@@ -23,7 +25,6 @@
 
 #include <algorithm>
 #include <chrono>
-#include <cmath>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -69,6 +70,11 @@ struct Job {
     double chain_sum = 0;          // device seconds over all launches
     double trial_max = 0;          // slowest launch, seconds per trial
     double trial_est = 0;          // current estimate, seconds per trial
+    // block-sweep workgroups of the job's launches, fixed at admission (0:
+    // one-trial sweep); a job with more than one runs only in ticks whose
+    // XCD packing has room for it, and ran holds the last such tick
+    int parts = 0;
+    long ran = -1;
     fgx_stats st{};
     std::vector<double> final_xyz;
     int error = FGX_OK;
@@ -103,9 +109,14 @@ long kind_of(const JobSpec &j)
 
 }  // namespace
 
+// most workgroups one job's block sweeps take: room on its XCD (32 compute
+// units on MI355X) for the single-workgroup jobs of a tick beside it
+constexpr int SERVER_PARTS_MAX = 24;
+
 struct fgx_server {
     size_t budget = 0, resident = 0, peak_resident = 0;
     int max_active = 0;
+    int cu_per_xcd = 0;
     double tick = 0.05;
     int next_id = 1;
     std::deque<int> queue;
@@ -127,8 +138,13 @@ static int admit(fgx_server *sv, Job &J)
     if (!rc) rc = fgx_rng_seed(s, j.seed);
     if (!rc) rc = fgx_propagator_init(s);
     // one fixed schedule for every job, so a result does not depend on what
-    // else is resident
+    // else is resident: the launch shape, and the block sweep's workgroups
+    // from the job alone (jobs without a block sweep keep the one-trial one)
     if (!rc) rc = fgx_set_launch(s, 512, 4);
+    if (!rc) {
+        J.parts = blk_job_parts(s, std::min(SERVER_PARTS_MAX, sv->cu_per_xcd));
+        rc = fgx_set_block_sweep(s, J.parts ? 1 : 0, std::max(1, J.parts));
+    }
     // a job the device sweeps cannot run fails here, alone, instead of
     // failing the shared launch of its kind
     if (!rc) rc = check_chain(s);
@@ -142,8 +158,8 @@ static int admit(fgx_server *sv, Job &J)
     J.t_admit = now_s();
     sv->resident += J.bytes;
     sv->peak_resident = std::max(sv->peak_resident, sv->resident);
-    // first trial-time guess: one CU at 1e9 pair evaluations per second
-    J.trial_est = 2.0 * (double)j.n / 1e9 + 2e-6;
+    // first trial-time guess: 1e9 pair evaluations per second per CU
+    J.trial_est = 2.0 * (double)j.n / (1e9 * std::max(1, J.parts)) + 2e-6;
     return FGX_OK;
 }
 
@@ -175,6 +191,13 @@ extern "C" int fgx_server_create(size_t hbm_budget, int max_active, double tick_
     sv->budget = hbm_budget;
     sv->max_active = max_active;
     sv->tick = tick_seconds;
+    // compute units of one XCD (eight per device): the room one job's split
+    // block sweeps may take
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        ncu = 0;
+    sv->cu_per_xcd = std::max(1, ncu / 8);
     *out = sv;
     return FGX_OK;
 }
@@ -258,6 +281,7 @@ extern "C" int fgx_server_step(fgx_server *sv, int *n_admitted, int *n_finished)
     }
     // ---- one tick: a batched launch per kernel kind, all in flight together
     std::map<long, std::vector<Job *>> groups;
+    std::vector<Job *> runnable;
     for (auto &kv : sv->jobs) {
         Job &J = *kv.second;
         if (J.state != 1) continue;
@@ -266,7 +290,41 @@ extern "C" int fgx_server_step(fgx_server *sv, int *n_admitted, int *n_finished)
             ++fin;
             continue;
         }
-        groups[kind_of(J.spec)].push_back(&J);
+        runnable.push_back(&J);
+    }
+    // Jobs of more than one workgroup wait on each other's workgroups, so
+    // every XCD's share of them must fit its compute units at once beside
+    // the other launches of the tick (whose workgroups end by themselves):
+    // they are packed onto XCDs, the longest-waiting first, and a job that
+    // does not fit waits for a later tick.  Launches group jobs by kernel
+    // kind and by workgroups per chain.
+    {
+        const int nx = 8, cux = sv->cu_per_xcd;
+        int narrow = 0;
+        std::vector<Job *> wide;
+        for (Job *J : runnable) {
+            J->sys->blk_xcd = -1;
+            if (J->parts > 1) wide.push_back(J);
+            else ++narrow;
+        }
+        std::stable_sort(wide.begin(), wide.end(),
+                         [](const Job *a, const Job *b) { return a->ran < b->ran; });
+        const int reserve = std::min((narrow + nx - 1) / nx, std::max(0, cux - SERVER_PARTS_MAX));
+        std::vector<int> room(nx, cux - reserve);
+        for (Job *J : wide) {
+            int x = 0;
+            for (int y = 1; y < nx; ++y)
+                if (room[y] > room[x]) x = y;
+            if (room[x] < J->parts) continue;
+            room[x] -= J->parts;
+            J->sys->blk_xcd = x;
+            J->ran = sv->ticks;
+        }
+        for (Job *J : runnable) {
+            if (J->parts > 1 && J->sys->blk_xcd < 0) continue;
+            groups[kind_of(J->spec) * 4 + (J->parts > 1 ? 2 : 0) + (J->parts > 0 ? 1 : 0)]
+                .push_back(J);
+        }
     }
     struct InFlight {
         std::vector<Job *> jobs;
@@ -370,6 +428,7 @@ extern "C" int fgx_server_result(fgx_server *sv, int id, fgx_job_result *r, doub
     r->mean_sweep_seconds = J.done ? J.chain_sum / J.done : 0.0;
     r->max_sweep_seconds = J.done ? J.trial_max * ((double)J.st.trials / J.done) : 0.0;
     r->device_bytes = J.bytes;
+    r->workgroups = J.parts;
     if (xyz) {
         FGX_CHECK(!J.final_xyz.empty(), FGX_ESTATE,
                   "fgx_server_result: positions were not kept for this job");

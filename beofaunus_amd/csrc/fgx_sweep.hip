@@ -229,6 +229,30 @@ int check_chain(const fgx_system *s)
     return chain_limits(s, gvec);
 }
 
+// CutShift LJ: the fixed-unit Ewald form takes it only at the Ewald rc,
+// where every pair term vanishes beyond rc (the loop then only screens)
+static bool pot_lj_at_rc(const fgx_system *s)
+{
+    return s->pot.lj_rc2 > 0.0 && s->pot.lj_rc2 == s->pot.ew_rc2;
+}
+
+static bool pot_ew_lj(const fgx_system *s)
+{
+    return s->pot.elec == ELEC_EWALD && s->pot.sr == SR_LJ && s->pot.ntypes2 <= LDS_TYPES2 &&
+           (s->pot.lj_rc2 == 0.0 || pot_lj_at_rc(s));
+}
+
+// fixed-unit pair form (one fixed-point unit L / 2^32 on all axes): cubic
+// boxes with Debye-Hueckel / Coulomb / plain hard spheres, or Ewald + LJ
+static bool fixed_unit_form(const fgx_system *s)
+{
+    const bool cubic = s->box.len[0] == s->box.len[1] && s->box.len[0] == s->box.len[2];
+    const bool dh_like = (s->pot.sr == SR_NONE || s->pot.sr == SR_HS) &&
+                         (s->pot.elec == ELEC_NONE || s->pot.elec == ELEC_COULOMB ||
+                          s->pot.elec == ELEC_DH);
+    return cubic && (dh_like || pot_ew_lj(s));
+}
+
 static int prepare_chain(fgx_system *s, ChainDev &C, hipStream_t st, double *du_rec, int *acc_rec,
                          long cap)
 {
@@ -312,15 +336,8 @@ static int prepare_chain(fgx_system *s, ChainDev &C, hipStream_t st, double *du_
     C.lj_uniform = 0;
     C.lj_check = 1;
     C.lj_s2f = C.lj_eps4f = 0.f;
-    const bool cubic = s->box.len[0] == s->box.len[1] && s->box.len[0] == s->box.len[2];
-    const bool dh_like = (s->pot.sr == SR_NONE || s->pot.sr == SR_HS) &&
-                         (s->pot.elec == ELEC_NONE || s->pot.elec == ELEC_COULOMB ||
-                          s->pot.elec == ELEC_DH);
-    // CutShift LJ: the fixed-unit Ewald form takes it only at the Ewald rc,
-    // where every pair term vanishes beyond rc (the loop then only screens)
-    const bool lj_at_rc = s->pot.lj_rc2 > 0.0 && s->pot.lj_rc2 == s->pot.ew_rc2;
-    const bool ew_lj = s->pot.elec == ELEC_EWALD && s->pot.sr == SR_LJ &&
-                       s->pot.ntypes2 <= LDS_TYPES2 && (s->pot.lj_rc2 == 0.0 || lj_at_rc);
+    const bool lj_at_rc = pot_lj_at_rc(s);
+    const bool ew_lj = pot_ew_lj(s);
     C.lj_screen = 0;
     C.cell_nc = C.cell_cap = 0;
     C.cell_cnt = nullptr;
@@ -333,7 +350,7 @@ static int prepare_chain(fgx_system *s, ChainDev &C, hipStream_t st, double *du_
     C.sp_j0 = 0;
     C.sp_j1 = s->n;
     C.cell_type = 1;
-    if (cubic && (dh_like || ew_lj)) {
+    if (fixed_unit_form(s)) {
         const double unit = s->box.len[0] / 4294967296.0;
         C.fx = 1;
         C.fx_lbs = (float)(s->pot.lB / unit);
@@ -481,6 +498,27 @@ static std::vector<int> blk_auto_parts(const std::vector<fgx_system *> &sys, int
     return parts;
 }
 
+// Workgroups of a job-server chain's block sweeps, fixed by the chain alone
+// so that its result does not depend on the jobs beside it: one per 512
+// partner rows, at most cap; 0 when the chain cannot run the block sweep (no
+// instance, a cell list, no fixed-unit form on the mixed path, or LDS).
+int blk_job_parts(const fgx_system *s, int cap)
+{
+    const bool mix = s->pot.precision == FGX_MIXED;
+    size_t st_lds = 0;
+    int bsz = 0;
+    if (!s->blk_mode || s->moves.empty() || s->moves[0].type != MOVE_AT) return 0;
+    if (!blk_instance(mix, s->pot, &st_lds, &bsz) || st_lds == 0) return 0;
+    if (s->pot.ntypes2 > LDS_TYPES2 || s->n < 1 || s->n >= (1 << 24)) return 0;
+    if (mix && !fixed_unit_form(s)) return 0;
+    if (mix && pot_lj_at_rc(s) && pot_ew_lj(s) && s->cell_mode != 0) return 0;
+    const int P = std::max(1, std::min({cap, BLK_NSPLIT_MAX, s->n / 512}));
+    const int K = s->has_ewald ? s->ew.K : 0;
+    const BlkLds base = blk_lds((K + P - 1) / P, bsz, s->has_ewald ? s->ew.kcc + 1 : 1,
+                                s->has_ewald ? 1 : 0, (s->n + P - 1) / P, 0);
+    return st_lds + base.total <= 160 * 1024 ? P : 0;
+}
+
 // Block-sweep plan of a batch: every chain must have an instance and fit,
 // else the batch keeps the one-trial sweep (an explicit split that cannot be
 // honoured is an error).  Sets the chains' parts, LDS layout and exchange
@@ -511,13 +549,6 @@ static int plan_blk(std::vector<fgx_system *> &sys, std::vector<ChainDev> &chain
     FGX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     const size_t lds_cap = 160 * 1024;
     std::vector<int> parts = blk_auto_parts(sys, ncu);
-    if (!B.allow_split) {
-        for (int c = 0; c < nc; ++c) {
-            FGX_CHECK(sys[c]->blk_nsplit <= 1, FGX_EUNSUPPORTED,
-                      "block sweep: this entry point runs each chain in one workgroup");
-            parts[c] = 1;
-        }
-    }
     size_t dyn = 0;
     bool split = false;
     for (int c = 0; c < nc; ++c) {
@@ -529,8 +560,8 @@ static int plan_blk(std::vector<fgx_system *> &sys, std::vector<ChainDev> &chain
         const int ks = (K + P - 1) / P;
         const int kcc1 = s->has_ewald ? s->ew.kcc + 1 : 1;
         const int ns = (s->n + P - 1) / P;
-        const BlkLds base = blk_lds(ks, bsz, kcc1, s->has_ewald ? 1 : 0, 0);
-        const BlkLds full = blk_lds(ks, bsz, kcc1, s->has_ewald ? 1 : 0, ns);
+        const BlkLds base = blk_lds(ks, bsz, kcc1, s->has_ewald ? 1 : 0, ns, 0);
+        const BlkLds full = blk_lds(ks, bsz, kcc1, s->has_ewald ? 1 : 0, ns, ns);
         if (st_lds + base.total > lds_cap) {
             FGX_CHECK(s->blk_nsplit == 0, FGX_EUNSUPPORTED,
                       "block sweep: the Ewald k-vectors of one workgroup exceed its LDS");
@@ -562,13 +593,19 @@ static int plan_blk(std::vector<fgx_system *> &sys, std::vector<ChainDev> &chain
     std::vector<std::vector<int2>> per(nx);
     std::vector<int> order(nc);
     for (int c = 0; c < nc; ++c) order[c] = c;
+    // chains with a given XCD first, so that their parts take the lowest rows
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        const bool fa = sys[a]->blk_xcd >= 0, fb = sys[b]->blk_xcd >= 0;
+        if (fa != fb) return fa;
         return (double)parts[a] * sys[a]->n > (double)parts[b] * sys[b]->n;
     });
     for (int c : order) {
         int x = 0;
-        for (int y = 1; y < nx; ++y)
-            if (per[y].size() < per[x].size()) x = y;
+        if (sys[c]->blk_xcd >= 0)
+            x = sys[c]->blk_xcd % nx;
+        else
+            for (int y = 1; y < nx; ++y)
+                if (per[y].size() < per[x].size()) x = y;
         for (int p = 0; p < parts[c]; ++p) per[x].push_back(make_int2(c, p));
     }
     size_t rows = 0;
@@ -815,7 +852,6 @@ int prepare_batch(std::vector<fgx_system *> &sys, const std::vector<int> &ncalls
         }
     }
     BatchPlan BB;
-    BB.allow_split = B.allow_split;
     if (!rc) rc = plan_blk(sys, chains, skg, BB);
     if (!rc) {
         ChainDev *d_chains = (ChainDev *)s0->d_batch;
@@ -934,13 +970,12 @@ int mirror_trial(std::vector<fgx_system *> &sys, hipStream_t st)
 // systems overlap on the device.
 static int launch_chains(std::vector<fgx_system *> &sys, const std::vector<int> &ncalls,
                          const std::vector<long> *max_trials, double *du_rec, int *acc_rec,
-                         long cap, Launch &L, bool allow_split = true)
+                         long cap, Launch &L)
 {
     hipStream_t st = sys[0]->stream;
     L.sys = sys;
     L.st = st;
     BatchPlan B;
-    B.allow_split = allow_split;
     int rc = prepare_batch(sys, ncalls, max_trials, du_rec, acc_rec, cap, B);
     if (!rc) {
         FGX_HIP(hipEventCreate(&L.e0));
@@ -993,7 +1028,7 @@ int batch_launch(std::vector<fgx_system *> &sys, const std::vector<int> &ncalls,
 {
     Launch *L = new Launch;
     *handle = L;
-    return launch_chains(sys, ncalls, &max_trials, nullptr, nullptr, 0, *L, false);
+    return launch_chains(sys, ncalls, &max_trials, nullptr, nullptr, 0, *L);
 }
 
 int batch_finish(void *handle, double *seconds, std::vector<long> *stats_i,

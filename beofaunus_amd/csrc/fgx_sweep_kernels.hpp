@@ -2030,10 +2030,6 @@ struct BatchPlan {
     bool fx = false, cell = false;
     bool skg = false;   // Ewald S(k) in device memory
     Pot pot{};
-    // block sweep: chains may span several workgroups (set by the caller
-    // before prepare_batch: launches that may run beside other launches, as
-    // the job server's, must not wait on workgroups another launch holds)
-    bool allow_split = true;
     // block sweep: workgroup map (chain, part) and grid size
     bool blk = false;
     const int2 *d_map = nullptr;

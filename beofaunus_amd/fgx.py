@@ -70,7 +70,7 @@ class JobResult(C.Structure):
         ("status", C.c_int), ("sweeps", C.c_int), ("trials", C.c_long), ("accepted", C.c_long),
         ("utot", C.c_double), ("queue_seconds", C.c_double), ("latency_seconds", C.c_double),
         ("mean_sweep_seconds", C.c_double), ("max_sweep_seconds", C.c_double),
-        ("device_bytes", C.c_size_t),
+        ("device_bytes", C.c_size_t), ("workgroups", C.c_int),
     ]
 
     def as_dict(self):

@@ -308,7 +308,12 @@ int fgx_chain_seconds(const fgx_system *s, double *seconds);
  * served on one device.  Jobs are admitted in arrival order while their
  * device footprint fits the HBM budget, advance together in batched
  * launches (one per kernel kind, concurrent streams), and release their
- * memory when done.  No counterpart in the reference (one simulation per
+ * memory when done.  A job whose chain runs the block AtomicTranslation
+ * sweep takes one workgroup per 512 particles, at most 24 (fixed at
+ * admission, so that its result does not depend on the jobs beside it); the
+ * jobs of more than one workgroup of a tick are packed onto XCDs so that the
+ * workgroups of each are resident together, and one that does not fit waits
+ * for a later tick.  No counterpart in the reference (one simulation per
  * process there).
  * ------------------------------------------------------------------------ */
 typedef struct fgx_server fgx_server;
@@ -337,6 +342,7 @@ typedef struct {
     double latency_seconds;                /* submission to completion */
     double mean_sweep_seconds, max_sweep_seconds;   /* device time per sweep */
     size_t device_bytes;
+    int workgroups;                        /* per launch of its block sweeps (0: one-trial sweep) */
 } fgx_job_result;
 int fgx_server_create(size_t hbm_budget, int max_active, double tick_seconds, fgx_server **out);
 int fgx_server_submit(fgx_server *sv, const fgx_job *job, int *id);

@@ -143,6 +143,50 @@ def test_rigid_molecule_job_alone_equals_in_mix(gpu):
     assert abs(ra["utot"] - rm["utot"]) <= 1e-12 * max(1.0, abs(ra["utot"]))
 
 
+def test_split_jobs_alone_equal_in_mix(gpu):
+    """Jobs large enough for several workgroups (block sweeps spread over
+    one XCD) give the same result alone and in a mix whose split jobs do not
+    all fit the device at once (some wait for a later tick), beside
+    single-workgroup jobs; each job's workgroup count is its own."""
+    from beofaunus_amd import workloads
+    say("[server] split jobs alone vs in a mix that overfills the XCDs")
+    rho = 0.01
+    a = workloads.salt_dh(n=6000, L=(6000 / rho) ** (1 / 3), seed=81, lattice=True)
+    e = workloads.ewald_electrolyte(n=3000, L=(3000 / rho) ** (1 / 3), seed=82)
+    big = [workloads.salt_dh(n=12300, L=(12300 / rho) ** (1 / 3), seed=90 + k, lattice=True)
+           for k in range(9)]
+    small = [workloads.salt_dh(n=500, L=(500 / rho) ** (1 / 3), seed=83, lattice=True),
+             workloads.ewald_electrolyte(n=600, L=(600 / rho) ** (1 / 3), seed=84)]
+    alone = {}
+    for w, sd, sw in ((a, 5, 2), (e, 6, 1)):
+        sv = gpu.Server(1 << 33, tick_seconds=0.01)
+        j = sv.submit(w, seed=sd, sweeps=sw, keep_positions=True)
+        alone[sd] = drain(sv)[j]
+        sv.close()
+        say("  alone N=%d: %d workgroups, trials %d, utot %.12f"
+            % (len(w["charge"]), alone[sd]["workgroups"], alone[sd]["trials"], alone[sd]["utot"]))
+    assert alone[5]["workgroups"] == 11 and alone[6]["workgroups"] == 5
+    sv = gpu.Server(1 << 34, tick_seconds=0.01)
+    ids = {}
+    for k, w in enumerate(big):
+        ids[100 + k] = sv.submit(w, seed=100 + k, sweeps=1)
+    ids[5] = sv.submit(a, seed=5, sweeps=2, keep_positions=True)
+    for k, w in enumerate(small):
+        ids[200 + k] = sv.submit(w, seed=200 + k, sweeps=3)
+    ids[6] = sv.submit(e, seed=6, sweeps=1, keep_positions=True)
+    out = drain(sv)
+    sv.close()
+    assert all(r["status"] == 0 for r in out.values())
+    assert all(out[ids[100 + k]]["workgroups"] == 24 for k in range(len(big)))
+    for sd in (5, 6):
+        r, al = out[ids[sd]], alone[sd]
+        say("  in mix: %d workgroups, trials %d, utot %.12f" % (r["workgroups"], r["trials"], r["utot"]))
+        assert r["workgroups"] == al["workgroups"]
+        assert r["trials"] == al["trials"] and r["accepted"] == al["accepted"]
+        assert np.array_equal(r["xyz"], al["xyz"])
+        assert abs(r["utot"] - al["utot"]) <= 1e-12 * max(1.0, abs(al["utot"]))
+
+
 def test_budget_admission(gpu):
     """A budget that holds about one job at a time: every job completes, the
     peak resident footprint stays within the budget, and later jobs wait."""
@@ -247,6 +291,7 @@ def test_stream_sizes_big_jobs(gpu):
     for sd, w in ((61, big_dh), (62, big_ew)):
         r, a = out[ids[sd]], alone[sd]
         assert r["trials"] == a["trials"] == calls
+        assert r["workgroups"] == a["workgroups"] == 24
         assert r["accepted"] == a["accepted"]
         assert np.array_equal(r["xyz"], a["xyz"])
         o = _oracle_of(w)
