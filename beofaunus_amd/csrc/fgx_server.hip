@@ -109,8 +109,9 @@ long kind_of(const JobSpec &j)
 
 }  // namespace
 
-// most workgroups one job's block sweeps take: room on its XCD (32 compute
-// units on MI355X) for the single-workgroup jobs of a tick beside it
+// most workgroups one job's block sweeps take (of an XCD's 32 compute units
+// on MI355X): more parts per chain gain little against the exchange each
+// block pays between them
 constexpr int SERVER_PARTS_MAX = 24;
 
 struct fgx_server {
@@ -279,7 +280,7 @@ extern "C" int fgx_server_step(fgx_server *sv, int *n_admitted, int *n_finished)
         ++adm;
         ++active;
     }
-    // ---- one tick: a batched launch per kernel kind, all in flight together
+    // ---- one tick: a batched launch per kernel kind
     std::map<long, std::vector<Job *>> groups;
     std::vector<Job *> runnable;
     for (auto &kv : sv->jobs) {
@@ -292,26 +293,26 @@ extern "C" int fgx_server_step(fgx_server *sv, int *n_admitted, int *n_finished)
         }
         runnable.push_back(&J);
     }
-    // Jobs of more than one workgroup wait on each other's workgroups, so
-    // every XCD's share of them must fit its compute units at once beside
-    // the other launches of the tick (whose workgroups end by themselves):
-    // they are packed onto XCDs, the longest-waiting first, and a job that
-    // does not fit waits for a later tick.  Launches group jobs by kernel
-    // kind and by workgroups per chain.
+    // Jobs of more than one workgroup wait on each other's workgroups: they
+    // run in one launch of one kernel kind per tick (the longest-waiting
+    // job's), packed onto XCDs, the longest-waiting first, at most one XCD's
+    // compute units per XCD; a job that does not fit waits for a later tick.
+    // Launches group jobs by kernel kind and by workgroups per chain.
     {
         const int nx = 8, cux = sv->cu_per_xcd;
-        int narrow = 0;
         std::vector<Job *> wide;
         for (Job *J : runnable) {
             J->sys->blk_xcd = -1;
             if (J->parts > 1) wide.push_back(J);
-            else ++narrow;
         }
         std::stable_sort(wide.begin(), wide.end(),
                          [](const Job *a, const Job *b) { return a->ran < b->ran; });
-        const int reserve = std::min((narrow + nx - 1) / nx, std::max(0, cux - SERVER_PARTS_MAX));
-        std::vector<int> room(nx, cux - reserve);
+        std::vector<int> room(nx, cux);
+        long wide_kind = -1;
         for (Job *J : wide) {
+            const long k = kind_of(J->spec);
+            if (wide_kind >= 0 && k != wide_kind) continue;
+            wide_kind = k;
             int x = 0;
             for (int y = 1; y < nx; ++y)
                 if (room[y] > room[x]) x = y;
@@ -333,64 +334,72 @@ extern "C" int fgx_server_step(fgx_server *sv, int *n_admitted, int *n_finished)
         void *h = nullptr;
         int rc = FGX_OK;
     };
-    std::vector<InFlight> fl;
-    for (auto &g : groups) {
-        InFlight f;
-        f.jobs = g.second;
-        std::vector<fgx_system *> sys;
-        for (Job *J : f.jobs) {
-            f.ncalls.push_back(J->spec.sweeps - J->done);
-            const double b = sv->tick / std::max(J->trial_est, 1e-8);
-            f.budget.push_back(std::max(64L, (long)std::min(b, 1e12)));
-            sys.push_back(J->sys);
-        }
-        f.rc = batch_launch(sys, f.ncalls, f.budget, &f.h);
-        if (f.rc && f.jobs.size() > 1) {
-            // a shared launch that could not be prepared: run its jobs one
-            // by one, so only the one that fails is retired
-            (void)batch_finish(f.h, nullptr, nullptr, nullptr);
-            for (size_t c = 0; c < f.jobs.size(); ++c) {
-                InFlight g;
-                g.jobs = {f.jobs[c]};
-                g.ncalls = {f.ncalls[c]};
-                g.budget = {f.budget[c]};
-                std::vector<fgx_system *> one = {sys[c]};
-                g.rc = batch_launch(one, g.ncalls, g.budget, &g.h);
-                fl.push_back(std::move(g));
-            }
-            continue;
-        }
-        fl.push_back(std::move(f));
-    }
+    // The split jobs' launch runs alone, so that all its workgroups are
+    // resident together whatever XCDs the hardware deals them to; then the
+    // other launches, side by side.
     int rc_all = FGX_OK;
-    for (auto &f : fl) {
-        double secs = 0;
-        std::vector<long> si;
-        std::vector<double> sd;
-        int rc = batch_finish(f.h, &secs, f.rc ? nullptr : &si, &sd);
-        if (f.rc) rc = f.rc;
-        for (size_t c = 0; c < f.jobs.size(); ++c) {
-            Job &J = *f.jobs[c];
-            if (rc) {
-                retire(sv, J, rc);
-                ++fin;
+    for (int round = 0; round < 2; ++round) {
+        std::vector<InFlight> fl;
+        for (auto &g : groups) {
+            if (((g.first & 2) != 0) != (round == 0)) continue;
+            InFlight f;
+            f.jobs = g.second;
+            std::vector<fgx_system *> sys;
+            for (Job *J : f.jobs) {
+                f.ncalls.push_back(J->spec.sweeps - J->done);
+                const double b = sv->tick / std::max(J->trial_est, 1e-8);
+                f.budget.push_back(std::max(64L, (long)std::min(b, 1e12)));
+                sys.push_back(J->sys);
+            }
+            f.rc = batch_launch(sys, f.ncalls, f.budget, &f.h);
+            if (f.rc && f.jobs.size() > 1) {
+                // a shared launch that could not be prepared: run its jobs
+                // one by one, so only the one that fails is retired (of split
+                // jobs only the first this tick: they do not run side by side)
+                (void)batch_finish(f.h, nullptr, nullptr, nullptr);
+                const size_t nrun = round == 0 ? 1 : f.jobs.size();
+                for (size_t c = 0; c < nrun; ++c) {
+                    InFlight g1;
+                    g1.jobs = {f.jobs[c]};
+                    g1.ncalls = {f.ncalls[c]};
+                    g1.budget = {f.budget[c]};
+                    std::vector<fgx_system *> one = {sys[c]};
+                    g1.rc = batch_launch(one, g1.ncalls, g1.budget, &g1.h);
+                    fl.push_back(std::move(g1));
+                }
                 continue;
             }
-            collect_stats(J.sys, &si[8 * c], &sd[8 * c], &J.st, secs);
-            const long tr = si[8 * c];
-            J.chain_sum += J.sys->chain_seconds;
-            if (tr > 0) {
-                J.trial_est = J.sys->chain_seconds / tr;
-                J.trial_max = std::max(J.trial_max, J.trial_est);
-            }
-            J.done += (int)si[8 * c + 5];
-            if (J.done >= J.spec.sweeps) {
-                retire(sv, J, FGX_OK);
-                ++fin;
-            }
+            fl.push_back(std::move(f));
         }
-        // a job's own failure is in its result; a runtime error is everyone's
-        if (rc == FGX_EHIP && !rc_all) rc_all = rc;
+        for (auto &f : fl) {
+            double secs = 0;
+            std::vector<long> si;
+            std::vector<double> sd;
+            int rc = batch_finish(f.h, &secs, f.rc ? nullptr : &si, &sd);
+            if (f.rc) rc = f.rc;
+            for (size_t c = 0; c < f.jobs.size(); ++c) {
+                Job &J = *f.jobs[c];
+                if (rc) {
+                    retire(sv, J, rc);
+                    ++fin;
+                    continue;
+                }
+                collect_stats(J.sys, &si[8 * c], &sd[8 * c], &J.st, secs);
+                const long tr = si[8 * c];
+                J.chain_sum += J.sys->chain_seconds;
+                if (tr > 0) {
+                    J.trial_est = J.sys->chain_seconds / tr;
+                    J.trial_max = std::max(J.trial_max, J.trial_est);
+                }
+                J.done += (int)si[8 * c + 5];
+                if (J.done >= J.spec.sweeps) {
+                    retire(sv, J, FGX_OK);
+                    ++fin;
+                }
+            }
+            // a job's own failure is in its result; a runtime error is everyone's
+            if (rc == FGX_EHIP && !rc_all) rc_all = rc;
+        }
     }
     ++sv->ticks;
     if (n_admitted) *n_admitted = adm;
