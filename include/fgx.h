@@ -274,8 +274,10 @@ int fgx_set_ewald_state(fgx_system *s, int hbm);
  * over that many workgroups (at most 32), which exchange partial sums once
  * per block; 0 chooses automatically.  A launch whose chains cannot all be
  * co-resident is refused (FGX_EUNSUPPORTED); a part that waits too long for
- * the others stops the launch with FGX_EDEVICE.  A schedule knob with no
- * reference counterpart. */
+ * the others stops the launch with FGX_EDEVICE.  A split launch should not
+ * run beside other kernels that hold compute units for long (the job server
+ * runs its split jobs' launch alone).  A schedule knob with no reference
+ * counterpart. */
 int fgx_set_block_sweep(fgx_system *s, int mode, int nsplit);
 /* workgroups per chain of the last AtomicTranslation device sweep if it was a
  * block sweep, 0 if it was the one-trial sweep */
