@@ -1021,7 +1021,7 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
                     }
                     // a trial whose particle an accepted trial of this block
                     // moved waits for the next block
-                    const unsigned below_me = (1u << lane) - 1u;
+                    const unsigned below_me = lane < 32 ? (1u << lane) - 1u : ~0u;
                     const unsigned cut = (unsigned)__ballot(mine && (same & am & below_me) != 0u);
                     if (cut) nd = __ffs(cut) - 1;
                     accm = am & ((nd >= 32 ? 0u : (1u << nd)) - 1u);
