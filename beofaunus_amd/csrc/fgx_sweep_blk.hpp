@@ -282,8 +282,8 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
         }
         __syncthreads();
     };
-    // tot[0..nv) of this part -> the chain's totals (parts summed in order);
-    // tot is written by wave 0 (tid < nv)
+    // tot[0..nv) of this part -> the chain's totals (parts summed in a fixed
+    // order); tot is written by wave 0 (tid < nv) before the exchange
     auto exchange = [&](int nv) {
         if (nsplit == 1) return;
         double *xb = C.xbuf + (size_t)xpar * BLK_NSPLIT_MAX * BLK_NV_MAX;
@@ -291,12 +291,16 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
             __hip_atomic_store(&xb[part * BLK_NV_MAX + tid], tot[tid], __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
         chain_sync();
-        if (tid < nv) {
-            double sv = 0.0;
-            for (int p = 0; p < nsplit; ++p)
-                sv += __hip_atomic_load(&xb[p * BLK_NV_MAX + tid], __ATOMIC_RELAXED,
-                                        __HIP_MEMORY_SCOPE_AGENT);
-            tot[tid] = sv;
+        // value v summed by one wavefront, lane p loading part p's: every
+        // load in flight at once, the parts added in a fixed (DPP) order
+        for (int v = tid >> 6; v < nv; v += nthr >> 6) {
+            const int ln = tid & 63;
+            const double x = ln < nsplit ? __hip_atomic_load(&xb[ln * BLK_NV_MAX + v],
+                                                             __ATOMIC_RELAXED,
+                                                             __HIP_MEMORY_SCOPE_AGENT)
+                                         : 0.0;
+            const double sv = wave_sum_dpp(x);
+            if (ln == 0) tot[v] = sv;
         }
         __syncthreads();
         xpar ^= 1;
@@ -947,6 +951,9 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
                 // (same) or changes this lane's energy (app)
                 unsigned same = 0, app = 0, firsts = 0;
                 double xt[B], Xs[B], Ys[B];
+                // every lane loads (lanes past the block read column 0) and
+                // selects, so the loads need no branches
+                const int sl = mine ? lane : 0;
 #pragma unroll
                 for (int s2 = 0; s2 < B; ++s2) {
                     const int is = __builtin_amdgcn_readlane(myi, s2);
@@ -956,25 +963,28 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
                     same |= (use && is == myi ? 1u : 0u) << s2;
                     app |= (other && s2 < lane ? 1u : 0u) << s2;
                     firsts |= (other && fs == s2 ? 1u : 0u) << s2;
-                    xt[s2] = use ? xc[s2][lane] : 0.0;
-                    Xs[s2] = use ? xc[fs][lane] : 0.0;
-                    Ys[s2] = use ? yc[s2][lane] : 0.0;
+                    const double xv = xc[s2][sl], xf = xc[fs][sl], yv = yc[s2][sl];
+                    xt[s2] = use ? xv : 0.0;
+                    Xs[s2] = use ? xf : 0.0;
+                    Ys[s2] = use ? yv : 0.0;
                     if constexpr (EW) {
                         const bool g = mine && s2 < lane;
                         const int gi = g ? B + s2 * (2 * B - s2 - 1) / 2 + (lane - s2 - 1) : 0;
-                        const double gv = g ? tot[gi] : 0.0;
-                        Ys[s2] += 2.0 * C.recip_pref * gv;
+                        const double gv = tot[gi];
+                        Ys[s2] += g ? 2.0 * C.recip_pref * gv : 0.0;
                     }
                 }
-                bool odd = base != base || base == -__builtin_inf();   // NaN or -inf somewhere
+                // a NaN or -inf among this lane's terms makes their IEEE sum NaN
+                // or -inf (+inf + -inf is NaN); an overflow to -inf only sends
+                // the block down the sequential path
+                double probe = base;
 #pragma unroll
                 for (int s2 = 0; s2 < B; ++s2) {
                     const double x1 = ((firsts >> s2) & 1u) ? xt[s2] : 0.0;
-                    const double x2 = ((app >> s2) & 1u) ? Xs[s2] : 0.0;
-                    const double x3 = ((app >> s2) & 1u) ? Ys[s2] : 0.0;
-                    odd = odd || x1 != x1 || x1 == -__builtin_inf() || x2 != x2 ||
-                          x2 == -__builtin_inf() || x3 != x3 || x3 == -__builtin_inf();
+                    const bool ap = (app >> s2) & 1u;
+                    probe += x1 + ((ap ? Xs[s2] : 0.0) + (ap ? Ys[s2] : 0.0));
                 }
+                const bool odd = probe != probe || probe == -__builtin_inf();
                 int nd = nb;
                 unsigned accm = 0;
                 bool refresh = false;
@@ -992,10 +1002,9 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
                     int dinf[B];
 #pragma unroll
                     for (int s2 = 0; s2 < B; ++s2) {
-                        if ((firsts >> s2) & 1u) {
-                            fin0 += fpart(xt[s2]);
-                            npi0 += ipart(xt[s2]);
-                        }
+                        const bool fi = (firsts >> s2) & 1u;
+                        fin0 += fi ? fpart(xt[s2]) : 0.0;
+                        npi0 += fi ? ipart(xt[s2]) : 0;
                         const bool ap = (app >> s2) & 1u;
                         dfin[s2] = ap ? fpart(Ys[s2]) - fpart(Xs[s2]) : 0.0;
                         dinf[s2] = ap ? ipart(Ys[s2]) - ipart(Xs[s2]) : 0;
