@@ -187,6 +187,42 @@ def test_split_jobs_alone_equal_in_mix(gpu):
         assert abs(r["utot"] - al["utot"]) <= 1e-12 * max(1.0, abs(al["utot"]))
 
 
+def test_served_stream_split_jobs_complete(gpu):
+    """A stream like the served-stream benchmark's (64 jobs, N 500-50,000,
+    half Ewald, half Debye-Hueckel, mixed, 2 sweeps each, arriving while
+    earlier ones run): every job completes, every job from N = 1,024 up runs
+    split, and the split jobs' launches (run alone each tick) never time out
+    waiting for a workgroup."""
+    import time
+    from beofaunus_amd import workloads
+    say("[server] 64-job stream with split jobs")
+    jobs = workloads.serve_jobs(njobs=64, seed=5)
+    sv = gpu.Server(256 << 20, tick_seconds=0.05)
+    ids, out, k = {}, {}, 0
+    t0 = time.perf_counter()
+    while len(out) < len(jobs):
+        # four arrivals per tick until all are in
+        for _ in range(4):
+            if k < len(jobs):
+                ids[sv.submit(jobs[k]["w"], seed=1000 + k, sweeps=2)] = k
+                k += 1
+        inf = sv.info()
+        if inf["active"] or inf["queued"]:
+            sv.step()
+        for j in sv.poll():
+            out[j] = sv.result(j)
+        assert time.perf_counter() - t0 < 240, "stream did not finish"
+    sv.close()
+    bad = [(jobs[ids[j]]["n"], r["status"]) for j, r in out.items() if r["status"] != 0]
+    say("  %d jobs, %d failed, %d split" % (len(out), len(bad),
+                                           sum(1 for r in out.values() if r["workgroups"] > 1)))
+    assert not bad, bad
+    for j, r in out.items():
+        n = jobs[ids[j]]["n"]
+        assert r["sweeps"] == 2 and r["trials"] == 2 * n
+        assert r["workgroups"] == min(24, max(1, n // 512)), (n, r["workgroups"])
+
+
 def test_budget_admission(gpu):
     """A budget that holds about one job at a time: every job completes, the
     peak resident footprint stays within the budget, and later jobs wait."""
