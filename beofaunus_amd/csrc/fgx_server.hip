@@ -358,6 +358,7 @@ extern "C" int fgx_server_step(fgx_server *sv, int *n_admitted, int *n_finished)
                 // jobs only the first this tick: they do not run side by side)
                 (void)batch_finish(f.h, nullptr, nullptr, nullptr);
                 const size_t nrun = round == 0 ? 1 : f.jobs.size();
+                for (size_t c = nrun; c < f.jobs.size(); ++c) f.jobs[c]->ran = -1;   // first next tick
                 for (size_t c = 0; c < nrun; ++c) {
                     InFlight g1;
                     g1.jobs = {f.jobs[c]};
