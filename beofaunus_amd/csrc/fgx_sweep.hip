@@ -472,28 +472,71 @@ static int blk_instance(bool mix, const Pot &P, size_t *static_lds, int *block)
 
 // Automatic workgroups per chain for block sweeps: with at least half as many
 // chains as compute units every chain keeps one workgroup; otherwise the
-// units go, one at a time, to the chain with the most pair work per
-// workgroup (n^2 / parts), a part keeping at least 512 partner rows.
+// units go, one at a time, to the chain whose sweep takes longest, while that
+// shortens it, a part keeping at least 512 partner rows.  A sweep of n trials
+// in P workgroups takes about n (F + (n + 2K) / P) partner-row times: the
+// partner pass (and the k-vectors, about two rows each) shrinks with P, the
+// per-trial share of the block's draws, reduction, exchange and decisions, F,
+// does not.  On MI355X, fitted to Debye-Hueckel chains of 1,252-14,980
+// particles, F is about 2,700 rows on the mixed path and 400 on the FP64
+// path (whose rows cost four times as much).  The parts of a chain share one
+// XCD, so the result is fitted to the XCDs' compute units at the end.
+static constexpr double BLK_FIXED_ROWS_MIXED = 2700.0, BLK_FIXED_ROWS_FP64 = 400.0;
+
 static std::vector<int> blk_auto_parts(const std::vector<fgx_system *> &sys, int ncu)
 {
     const int nc = (int)sys.size();
     std::vector<int> parts(nc, 1);
-    int used = nc;
     if (2 * nc >= ncu) return parts;
-    for (;;) {
-        int best = -1;
-        double bw = 0.0;
+    auto sweep_time = [&](int c, int P) {
+        const fgx_system *s = sys[c];
+        const double rows = (double)s->n + 2.0 * (s->has_ewald ? s->ew.K : 0);
+        const double F = s->pot.precision == FGX_MIXED ? BLK_FIXED_ROWS_MIXED : BLK_FIXED_ROWS_FP64;
+        return (double)s->n * (F + rows / P);
+    };
+    int used = nc;
+    while (used < ncu) {
+        int worst = -1;
+        double tw = -1.0;
         for (int c = 0; c < nc; ++c) {
-            const fgx_system *s = sys[c];
-            if (s->blk_nsplit > 0) continue;
-            const int cap = std::min(BLK_NSPLIT_MAX, std::max(1, s->n / 512));
-            if (parts[c] >= cap) continue;
-            const double w = (double)s->n * s->n / parts[c];
-            if (w > bw) { bw = w; best = c; }
+            if (sys[c]->blk_nsplit > 0) continue;
+            const double t = sweep_time(c, parts[c]);
+            if (t > tw) { tw = t; worst = c; }
         }
-        if (best < 0 || used + 1 > ncu) break;
-        ++parts[best];
+        if (worst < 0) break;
+        const int cap = std::min(BLK_NSPLIT_MAX, std::max(1, sys[worst]->n / 512));
+        if (parts[worst] >= cap) break;   // the slowest chain cannot get faster
+        ++parts[worst];
         ++used;
+    }
+    // fit the XCD packing of plan_blk (largest first onto the least-loaded
+    // XCD): while an XCD holds more parts than it has compute units, the
+    // largest split chain there gives one back
+    const int nx = 8, cux = std::max(1, ncu / nx);
+    for (int guard = 0; guard < 64 * nc; ++guard) {
+        std::vector<int> order(nc), xcd(nc), load(nx, 0);
+        for (int c = 0; c < nc; ++c) order[c] = c;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            return (double)parts[a] * sys[a]->n > (double)parts[b] * sys[b]->n;
+        });
+        for (int c : order) {
+            int x = 0;
+            for (int y = 1; y < nx; ++y)
+                if (load[y] < load[x]) x = y;
+            xcd[c] = x;
+            load[x] += sys[c]->blk_nsplit > 0 ? sys[c]->blk_nsplit : parts[c];
+        }
+        int over = -1;
+        for (int x = 0; x < nx; ++x)
+            if (load[x] > cux && (over < 0 || load[x] > load[over])) over = x;
+        if (over < 0) break;
+        int big = -1;
+        for (int c = 0; c < nc; ++c)
+            if (xcd[c] == over && sys[c]->blk_nsplit <= 0 && parts[c] > 1 &&
+                (big < 0 || parts[c] > parts[big]))
+                big = c;
+        if (big < 0) break;   // explicit splits alone overfill it: plan_blk refuses
+        --parts[big];
     }
     return parts;
 }
