@@ -1172,17 +1172,28 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
             }
             if (wave0) {
                 wave_sync_lds();   // the wave's reads of the ring are done
-                if (tid == 0) {
-                    int m = 0;
-                    for (int b = 0; b < nd; ++b)
-                        if ((accm >> b) & 1u) {
-                            lmv_i[m] = R[b].i;
-                            lmv_x[m] = R[b].nx; lmv_y[m] = R[b].ny; lmv_z[m] = R[b].nz;
-                            ++m;
-                        }
-                    snlmv = m;
-                    // undecided trials keep their draws for the next block
-                    for (int b = nd; b < nr; ++b) R[b - nd] = R[b];
+                // the accepted moves, in order, one lane each (accm holds
+                // trials below nd only)
+                if (lane < nd && ((accm >> lane) & 1u)) {
+                    const int m = __popc(accm & ((1u << lane) - 1u));
+                    lmv_i[m] = R[lane].i;
+                    lmv_x[m] = R[lane].nx; lmv_y[m] = R[lane].ny; lmv_z[m] = R[lane].nz;
+                }
+                if (lane == 0) snlmv = __popc(accm);
+                // undecided trials keep their draws for the next block: read
+                // by every lane first, written after the wave's reads are done
+                constexpr int NW = (int)(sizeof(BlkTrial) / 4);
+                static_assert(sizeof(BlkTrial) % 4 == 0, "ring records copy as words");
+                const bool keep = lane >= nd && lane < nr;
+                uint32_t kt[NW];
+                const uint32_t *src = (const uint32_t *)&R[keep ? lane : 0];
+#pragma unroll
+                for (int w = 0; w < NW; ++w) kt[w] = src[w];
+                wave_sync_lds();
+                if (keep) {
+                    uint32_t *dst = (uint32_t *)&R[lane - nd];
+#pragma unroll
+                    for (int w = 0; w < NW; ++w) dst[w] = kt[w];
                 }
                 wave_sync_lds();
             }
