@@ -949,31 +949,33 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
                 // per earlier trial s: X (its particle's snapshot term), Y (after
                 // its accepted move), and whether it moves this lane's particle
                 // (same) or changes this lane's energy (app)
-                unsigned same = 0, app = 0, firsts = 0;
+                // the masks as integer bit sets (bit s: earlier trial s), the
+                // terms loaded by every lane (lanes past the block read column
+                // 0) and used only under the masks, so nothing here branches
+                unsigned eqm = 0u, ffm = 0u;
                 double xt[B], Xs[B], Ys[B];
-                // every lane loads (lanes past the block read column 0) and
-                // selects, so the loads need no branches
                 const int sl = mine ? lane : 0;
+                const unsigned ltm = lane < 32 ? (1u << lane) - 1u : ~0u;
 #pragma unroll
                 for (int s2 = 0; s2 < B; ++s2) {
                     const int is = __builtin_amdgcn_readlane(myi, s2);
                     const int fs = __builtin_amdgcn_readlane(myf, s2);
-                    const bool use = mine && s2 < nb;
-                    const bool other = use && is != myi;
-                    same |= (use && is == myi ? 1u : 0u) << s2;
-                    app |= (other && s2 < lane ? 1u : 0u) << s2;
-                    firsts |= (other && fs == s2 ? 1u : 0u) << s2;
-                    const double xv = xc[s2][sl], xf = xc[fs][sl], yv = yc[s2][sl];
-                    xt[s2] = use ? xv : 0.0;
-                    Xs[s2] = use ? xf : 0.0;
-                    Ys[s2] = use ? yv : 0.0;
+                    eqm |= is == myi ? 1u << s2 : 0u;
+                    ffm |= fs == s2 ? 1u << s2 : 0u;
+                    xt[s2] = xc[s2][sl];
+                    Xs[s2] = xc[fs][sl];
+                    Ys[s2] = yc[s2][sl];
                     if constexpr (EW) {
-                        const bool g = mine && s2 < lane;
+                        const bool g = mine && ((ltm >> s2) & 1u);
                         const int gi = g ? B + s2 * (2 * B - s2 - 1) / 2 + (lane - s2 - 1) : 0;
                         const double gv = tot[gi];
                         Ys[s2] += g ? 2.0 * C.recip_pref * gv : 0.0;
                     }
                 }
+                const unsigned usem = mine ? (nb >= 32 ? ~0u : (1u << nb) - 1u) : 0u;
+                const unsigned same = eqm & usem;          // trials of this lane's particle
+                const unsigned app = ~eqm & usem & ltm;     // earlier trials of other particles
+                const unsigned firsts = ~eqm & usem & ffm;  // ... each particle's first one
                 // a NaN or -inf among this lane's terms makes their IEEE sum NaN
                 // or -inf (+inf + -inf is NaN); an overflow to -inf only sends
                 // the block down the sequential path
