@@ -238,3 +238,29 @@ def test_block_sweep_batch_equals_one_trial_sweep(gpu, kind):
         assert p1 == p2 and np.array_equal(np.asarray(w1), np.asarray(w2))
     for s in systems + ref:
         s.close()
+
+
+def test_block_sweep_refuses_splits_that_cannot_be_resident(gpu):
+    """A batch whose explicit splits cannot all be resident at once (nine
+    chains of 32 workgroups each on 8 XCDs of 32 compute units) is refused
+    before anything runs; the same chains then run with one workgroup each."""
+    from beofaunus_amd import workloads
+    fgx = gpu
+    systems = []
+    for k in range(9):
+        w = workloads.salt_dh(1000, L=60.0 * 2.0 ** (1 / 3), seed=40 + k, lattice=True)
+        s = workloads.make_system(w, precision=fgx.MIXED)
+        s.rng_seed(k + 1)
+        s.propagator_init()
+        s.set_block_sweep(True, 32)
+        systems.append(s)
+    say("[block] nine chains of 32 workgroups")
+    with pytest.raises(fgx.FgxError):
+        fgx.propagate_batch(systems, 1)
+    for s in systems:
+        s.set_block_sweep(True, 1)
+    st = fgx.propagate_batch(systems, 1)
+    assert st["trials"] == 9 * 1000
+    assert all(s.block_parts() == 1 for s in systems)
+    for s in systems:
+        s.close()
