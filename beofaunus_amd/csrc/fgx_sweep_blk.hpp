@@ -98,6 +98,31 @@ __device__ __forceinline__ double readlane_f64(double v, int l)
                                             (unsigned long long)(unsigned)lo));
 }
 
+// sum over the four lanes of a quad in a fixed order; every lane of the
+// quad receives the same bits (IEEE addition commutes)
+__device__ __forceinline__ double quad_sum(double v)
+{
+    v += dpp_f64<0xb1>(v);          // quad_perm 1,0,3,2
+    return v + dpp_f64<0x4e>(v);    // quad_perm 2,3,0,1
+}
+
+__device__ __forceinline__ int quad_isum(int v)
+{
+    v += __builtin_amdgcn_update_dpp(0, v, 0xb1, 0xf, 0xf, false);
+    return v + __builtin_amdgcn_update_dpp(0, v, 0x4e, 0xf, 0xf, false);
+}
+
+// bit 4t of a wave mask -> bit t (t < 16)
+__device__ __forceinline__ unsigned quad_bits(uint64_t x)
+{
+    x &= 0x1111111111111111ull;
+    x = (x | (x >> 3)) & 0x0303030303030303ull;
+    x = (x | (x >> 6)) & 0x000f000f000f000full;
+    x = (x | (x >> 12)) & 0x000000ff000000ffull;
+    x = (x | (x >> 24)) & 0xffffull;
+    return (unsigned)x;
+}
+
 // uniform_int_distribution<int>(0, erange - 1) from one word (Lemire's
 // method, as mt_range); ok = false when the method needs another word
 __device__ __forceinline__ uint32_t lemire_word(uint32_t word, uint32_t erange, bool &ok)
@@ -199,6 +224,7 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
     constexpr int NG = EW ? B * (B - 1) / 2 : 0;
     constexpr int NV = B + NG;
     static_assert(NV <= BLK_NV_MAX, "block too large");
+    static_assert(B <= 16, "the decisions hold up to four lanes per trial");
     constexpr bool NEED_T = (SR == SR_LJ || ELEC == ELEC_EWALD);
     // trials per pass over the partners (mixed path)
     constexpr int CB = EW ? BLK_CB_EWALD : BLK_CB_PAIR;
@@ -941,196 +967,443 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
             }
             exchange(NV);
             if (sabort) break;
-            // ---- Metropolis decisions in order (wave 0; lane t holds trial
-            // t's energy change as a finite sum and counts of +inf, -inf and
-            // NaN terms, which add up as IEEE sums do)
-            if (wave0) {
-                wave_sync_lds();
-                const bool mine = lane < nb;
-                // every load first (lane-parallel), then the sequential part on
-                // per-lane registers (lane t: trial t) and a few scalars
-                const int myi = mine ? R[lane].i : -1;
-                const int myf = mine ? R[lane].first : 0;
-                const double lu = mine ? R[lane].lu : 0.0;
-                const int mycol = mine ? R[lane].collide : 0;
-                const double base = mine ? tot[lane] : 0.0;
-                // per earlier trial s: X (its particle's snapshot term), Y (after
-                // its accepted move), and whether it moves this lane's particle
-                // (same) or changes this lane's energy (app)
-                // the masks as integer bit sets (bit s: earlier trial s), the
-                // terms loaded by every lane (lanes past the block read column
-                // 0) and used only under the masks, so nothing here branches
-                unsigned eqm = 0u, ffm = 0u;
-                double xt[B], Xs[B], Ys[B];
-                const int sl = mine ? lane : 0;
-                const unsigned ltm = lane < 32 ? (1u << lane) - 1u : ~0u;
-#pragma unroll
-                for (int s2 = 0; s2 < B; ++s2) {
-                    const int is = __builtin_amdgcn_readlane(myi, s2);
-                    const int fs = __builtin_amdgcn_readlane(myf, s2);
-                    eqm |= is == myi ? 1u << s2 : 0u;
-                    ffm |= fs == s2 ? 1u << s2 : 0u;
-                    xt[s2] = xc[s2][sl];
-                    Xs[s2] = xc[fs][sl];
-                    Ys[s2] = yc[s2][sl];
-                    if constexpr (EW) {
-                        const bool g = mine && ((ltm >> s2) & 1u);
-                        const int gi = g ? B + s2 * (2 * B - s2 - 1) / 2 + (lane - s2 - 1) : 0;
-                        const double gv = tot[gi];
-                        Ys[s2] += g ? 2.0 * C.recip_pref * gv : 0.0;
-                    }
-                }
-                const unsigned usem = mine ? (nb >= 32 ? ~0u : (1u << nb) - 1u) : 0u;
-                const unsigned same = eqm & usem;          // trials of this lane's particle
-                const unsigned app = ~eqm & usem & ltm;     // earlier trials of other particles
-                const unsigned firsts = ~eqm & usem & ffm;  // ... each particle's first one
-                // a NaN or -inf among this lane's terms makes their IEEE sum NaN
-                // or -inf (+inf + -inf is NaN); an overflow to -inf only sends
-                // the block down the sequential path
-                double probe = base;
-#pragma unroll
-                for (int s2 = 0; s2 < B; ++s2) {
-                    const double x1 = ((firsts >> s2) & 1u) ? xt[s2] : 0.0;
-                    const bool ap = (app >> s2) & 1u;
-                    probe += x1 + ((ap ? Xs[s2] : 0.0) + (ap ? Ys[s2] : 0.0));
-                }
-                const bool odd = probe != probe || probe == -__builtin_inf();
-                int nd = nb;
-                unsigned accm = 0;
-                bool refresh = false;
-                double mydu = 0.0;
-                int moved = 0;   // this lane's particle was moved by an accepted trial
-                if (!__ballot(odd) && !(EW && C.surf_pref != 0.0)) {
-                    // lane t's energy change: a finite sum and a count of +inf
-                    // terms (a collision counts as one: energy.h:3597-3607),
-                    // and what an accepted earlier trial s changes in them
-                    auto fpart = [](double v) { return v == __builtin_inf() ? 0.0 : v; };
-                    auto ipart = [](double v) { return v == __builtin_inf() ? 1 : 0; };
-                    double fin0 = fpart(base);
-                    int npi0 = (mycol ? 1 : 0) + ipart(base);
-                    double dfin[B];
-                    int dinf[B];
-#pragma unroll
+            if constexpr (B >= 8) {
+                // ---- Metropolis decisions in order (wave 0).  Blocks of 8 or more
+                // trials: the fast form puts trial t on lanes 4t..4t+3, lane 4t+g
+                // holding the earlier trials s = g, g+4, ... of it; the partial
+                // sums meet by quad DPP.
+                // A trial's energy change there is a finite sum and a count of
+                // +inf terms.  A block with a NaN or -inf term, or with the
+                // surface term, takes the sequential form (one lane per trial,
+                // counts of +inf, -inf and NaN terms, which add up as IEEE sums do).
+                if (wave0) {
+                    wave_sync_lds();
+                    constexpr int LPT = 4;                      // lanes per trial
+                    constexpr int SPL = (B + LPT - 1) / LPT;    // earlier trials per lane
+                    const int qt = lane / LPT, qg = lane % LPT;
+                    const bool qmine = qt < nb;
+                    const int qs = qmine ? qt : 0;
+                    const int myi = qmine ? R[qs].i : -1;
+                    const int myf = qmine ? R[qs].first : 0;
+                    const double lu = qmine ? R[qs].lu : 0.0;
+                    const int mycol = qmine ? R[qs].collide : 0;
+                    const double base = qmine ? tot[qs] : 0.0;
+                    // bit s: earlier trial s moves this trial's particle (eqm), or
+                    // is the first trial of its particle in the block (ffm)
+                    unsigned eqm = 0u, ffm = 0u;
+    #pragma unroll
                     for (int s2 = 0; s2 < B; ++s2) {
-                        const bool fi = (firsts >> s2) & 1u;
-                        fin0 += fi ? fpart(xt[s2]) : 0.0;
-                        npi0 += fi ? ipart(xt[s2]) : 0;
-                        const bool ap = (app >> s2) & 1u;
-                        dfin[s2] = ap ? fpart(Ys[s2]) - fpart(Xs[s2]) : 0.0;
-                        dinf[s2] = ap ? ipart(Ys[s2]) - ipart(Xs[s2]) : 0;
+                        const int is = __builtin_amdgcn_readlane(myi, LPT * s2);
+                        const int fs = __builtin_amdgcn_readlane(myf, LPT * s2);
+                        eqm |= is == myi ? 1u << s2 : 0u;
+                        ffm |= fs == s2 ? 1u << s2 : 0u;
                     }
-                    // The decisions as a fixed point: from a guess of the
-                    // accepted set, every lane forms its energy change and
-                    // Metropolis decision (move.h:567-573) at once; trial t
-                    // depends on trials s < t only, so after k rounds the first
-                    // k decisions are the sequential ones, and a set that
-                    // reproduces itself is the sequential answer.
-                    unsigned am = 0;
-                    for (int it = 0; it <= B; ++it) {
-                        double f = fin0;
-                        int ni = npi0;
-#pragma unroll
-                        for (int s2 = 0; s2 < B; ++s2)
-                            if ((am >> s2) & 1u) { f += dfin[s2]; ni += dinf[s2]; }
-                        mydu = ni > 0 ? __builtin_inf() : f;
-                        const bool acc = mine && (mydu <= 0.0 || !(mydu > lu));
-                        const unsigned nm = (unsigned)__ballot(acc);
-                        if (nm == am) break;
-                        am = nm;
+                    const unsigned usem = qmine ? (nb >= 32 ? ~0u : (1u << nb) - 1u) : 0u;
+                    const unsigned ltm = (1u << qt) - 1u;      // qt < 16
+                    const unsigned same = eqm & usem;          // trials of this trial's particle
+                    const unsigned app = ~eqm & usem & ltm;     // earlier trials of other particles
+                    const unsigned firsts = ~eqm & usem & ffm;  // ... each particle's first one
+                    // this lane's earlier trials s: the first-trial term, X (the
+                    // snapshot term of s's particle) and Y (after s's move); bit k
+                    // of myfi / myap: s = qg + LPT k is in firsts / app
+                    double xt[SPL], Xs[SPL], Ys[SPL];
+                    unsigned myfi = 0u, myap = 0u;
+    #pragma unroll
+                    for (int k = 0; k < SPL; ++k) {
+                        const int s2 = qg + LPT * k;
+                        const bool in = s2 < B;
+                        const int sc = in ? s2 : 0;
+                        const int fs = s2 < nb ? R[sc].first : 0;
+                        xt[k] = xc[sc][qs];
+                        Xs[k] = xc[fs][qs];
+                        Ys[k] = yc[sc][qs];
+                        if constexpr (EW) {
+                            const bool gg = qmine && in && s2 < qt;
+                            const int gi = gg ? B + s2 * (2 * B - s2 - 1) / 2 + (qt - s2 - 1) : 0;
+                            const double gv = tot[gi];
+                            Ys[k] += gg ? 2.0 * C.recip_pref * gv : 0.0;
+                        }
+                        myfi |= (in && ((firsts >> sc) & 1u)) ? 1u << k : 0u;
+                        myap |= (in && ((app >> sc) & 1u)) ? 1u << k : 0u;
                     }
-                    // a trial whose particle an accepted trial of this block
-                    // moved waits for the next block
-                    const unsigned below_me = lane < 32 ? (1u << lane) - 1u : ~0u;
-                    const unsigned cut = (unsigned)__ballot(mine && (same & am & below_me) != 0u);
-                    if (cut) nd = __ffs(cut) - 1;
-                    accm = am & ((nd >= 32 ? 0u : (1u << nd)) - 1u);
-                    if constexpr (EW) {
-                        // NonbondedEwald::update: the accepted move that takes
-                        // the counter past update_frequency - 1 ends the block
-                        const int need = C.update_frequency - ew_cnt;   // accepts until the refresh
-                        if (need >= 1 && __popc(accm) >= need) {
-                            unsigned m2 = accm;
-                            for (int k = 1; k < need; ++k) m2 &= m2 - 1u;
-                            const int tr = __ffs(m2) - 1;
-                            nd = tr + 1;
-                            accm &= (2u << tr) - 1u;
-                            refresh = true;
-                        }
-                        ew_cnt += __popc(accm);
+                    // a NaN or -inf among a trial's terms makes their IEEE sum NaN
+                    // or -inf (+inf + -inf is NaN); an overflow to -inf only sends
+                    // the block down the sequential path
+                    double pp = 0.0;
+    #pragma unroll
+                    for (int k = 0; k < SPL; ++k) {
+                        const double x1 = ((myfi >> k) & 1u) ? xt[k] : 0.0;
+                        const bool ap = (myap >> k) & 1u;
+                        pp += x1 + ((ap ? Xs[k] : 0.0) + (ap ? Ys[k] : 0.0));
                     }
-#pragma unroll
-                    for (int t = 0; t < B; ++t)
-                        if ((accm >> t) & 1u) {
-                            const double d = readlane_f64(mydu, t);
-                            utot += d;
-                            dusum += d;
+                    const double probe = base + quad_sum(pp);
+                    const bool odd = probe != probe || probe == -__builtin_inf();
+                    int nd = nb;
+                    unsigned accm = 0;
+                    bool refresh = false;
+                    if (!__ballot(odd) && !(EW && C.surf_pref != 0.0)) {
+                        // a collision counts as one +inf term (energy.h:3597-3607)
+                        auto fpart = [](double v) { return v == __builtin_inf() ? 0.0 : v; };
+                        auto ipart = [](double v) { return v == __builtin_inf() ? 1 : 0; };
+                        double pf = 0.0;
+                        int pi = 0;
+                        double dfin[SPL];
+                        int dinf[SPL];
+    #pragma unroll
+                        for (int k = 0; k < SPL; ++k) {
+                            const bool fi = (myfi >> k) & 1u;
+                            pf += fi ? fpart(xt[k]) : 0.0;
+                            pi += fi ? ipart(xt[k]) : 0;
+                            const bool ap = (myap >> k) & 1u;
+                            dfin[k] = ap ? fpart(Ys[k]) - fpart(Xs[k]) : 0.0;
+                            dinf[k] = ap ? ipart(Ys[k]) - ipart(Xs[k]) : 0;
                         }
-                    n_inf += __popcll(__ballot(lane < nd && !((accm >> lane) & 1u) &&
-                                               mydu == __builtin_inf()));
-                } else {
-                    // the same with counts of +inf, -inf and NaN terms, which
-                    // add up as IEEE sums do
-                    double fin = 0.0;
-                    int npi = 0, nmi = 0, nna = 0;
-                    auto add = [&](double v, int sgn) {
-                        const bool isn = v != v, isp = v == __builtin_inf(),
-                                   ism = v == -__builtin_inf();
-                        nna += isn ? sgn : 0;
-                        npi += isp ? sgn : 0;
-                        nmi += ism ? sgn : 0;
-                        fin += (isn || isp || ism) ? 0.0 : (sgn > 0 ? v : -v);
-                    };
-                    add(base, 1);
-#pragma unroll
-                    for (int s2 = 0; s2 < B; ++s2)
-                        if ((firsts >> s2) & 1u) add(xt[s2], 1);
-#pragma unroll
-                    for (int t = 0; t < B; ++t) {
-                        if (t >= nb) break;
-                        if (__builtin_amdgcn_readlane(moved, t)) { nd = t; break; }
-                        const double v = (nna > 0 || (npi > 0 && nmi > 0)) ? __builtin_nan("")
-                                         : npi > 0 ? __builtin_inf()
-                                         : nmi > 0 ? -__builtin_inf() : fin;
-                        double du = readlane_f64(v, t);
-                        double dMx = 0.0, dMy = 0.0, dMz = 0.0;
-                        if (EW && C.surf_pref != 0.0) {
-                            const BlkTrial &T = R[t];
-                            dMx = T.wq * (T.nx - T.ox);
-                            dMy = T.wq * (T.ny - T.oy);
-                            dMz = T.wq * (T.nz - T.oz);
-                            const double Nx = Mx + dMx, Ny = My + dMy, Nz = Mz + dMz;
-                            du += C.surf_pref *
-                                  ((Nx * Nx + (Ny * Ny + Nz * Nz)) - (Mx * Mx + (My * My + Mz * Mz)));
-                        }
-                        if (__builtin_amdgcn_readlane(mycol, t)) du = __builtin_inf();   // energy.h:3597-3607
-                        const bool accept = du <= 0.0 || !(du > readlane_f64(lu, t));
-                        if (lane == t) mydu = du;
-                        if (accept) {
-                            accm |= 1u << t;
-                            utot += du;
-                            dusum += du;
-                            if (EW) { Mx += dMx; My += dMy; Mz += dMz; }
-                            if ((app >> t) & 1u) {
-                                add(Xs[t], -1);
-                                add(Ys[t], 1);
+                        const double fin0 = fpart(base) + quad_sum(pf);
+                        const int npi0 = (mycol ? 1 : 0) + ipart(base) + quad_isum(pi);
+                        // The decisions as a fixed point: from a guess of the
+                        // accepted set, every trial forms its energy change and
+                        // Metropolis decision (move.h:567-573) at once; trial t
+                        // depends on trials s < t only, so after k rounds the first
+                        // k decisions are the sequential ones, and a set that
+                        // reproduces itself is the sequential answer.
+                        unsigned am = 0;
+                        double mydu = 0.0;
+                        for (int it = 0; it <= B; ++it) {
+                            double f = 0.0;
+                            int ni = 0;
+    #pragma unroll
+                            for (int k = 0; k < SPL; ++k) {
+                                const bool on = (am >> (qg + LPT * k)) & 1u;
+                                f += on ? dfin[k] : 0.0;
+                                ni += on ? dinf[k] : 0;
                             }
-                            moved |= (int)((same >> t) & 1u);
-                            if (EW && ++ew_cnt > C.update_frequency - 1) {
+                            f = quad_sum(f);
+                            ni = npi0 + quad_isum(ni);
+                            mydu = ni > 0 ? __builtin_inf() : fin0 + f;
+                            const bool acc = qmine && (mydu <= 0.0 || !(mydu > lu));
+                            const unsigned nm = quad_bits(__ballot(acc && qg == 0));
+                            if (nm == am) break;
+                            am = nm;
+                        }
+                        // a trial whose particle an accepted trial of this block
+                        // moved waits for the next block
+                        const unsigned cut = quad_bits(__ballot(qg == 0 && qmine && (same & am & ltm) != 0u));
+                        if (cut) nd = __ffs(cut) - 1;
+                        accm = am & ((nd >= 32 ? 0u : (1u << nd)) - 1u);
+                        if constexpr (EW) {
+                            // NonbondedEwald::update: the accepted move that takes
+                            // the counter past update_frequency - 1 ends the block
+                            const int need = C.update_frequency - ew_cnt;   // accepts until the refresh
+                            if (need >= 1 && __popc(accm) >= need) {
+                                unsigned m2 = accm;
+                                for (int k = 1; k < need; ++k) m2 &= m2 - 1u;
+                                const int tr = __ffs(m2) - 1;
+                                nd = tr + 1;
+                                accm &= (2u << tr) - 1u;
                                 refresh = true;
-                                nd = t + 1;
-                                break;
                             }
-                        } else if (du == __builtin_inf()) {
-                            n_inf++;
+                            ew_cnt += __popc(accm);
+                        }
+    #pragma unroll
+                        for (int t = 0; t < B; ++t)
+                            if ((accm >> t) & 1u) {
+                                const double d = readlane_f64(mydu, LPT * t);
+                                utot += d;
+                                dusum += d;
+                            }
+                        n_inf += __popcll(__ballot(qg == 0 && qt < nd && !((accm >> qt) & 1u) &&
+                                                   mydu == __builtin_inf()));
+                        if (qg == 0 && qt < B) sdu[qt] = mydu;
+                    } else {
+                        // one lane per trial; the masks again, and earlier trial
+                        // s's terms read from LDS when used
+                        const bool mine = lane < nb;
+                        const int sl = mine ? lane : 0;
+                        const int si = mine ? R[sl].i : -1;
+                        const int sf = mine ? R[sl].first : 0;
+                        const double slu = mine ? R[sl].lu : 0.0;
+                        const int scol = mine ? R[sl].collide : 0;
+                        unsigned seq = 0u, sff = 0u;
+    #pragma unroll
+                        for (int s2 = 0; s2 < B; ++s2) {
+                            const int is = __builtin_amdgcn_readlane(si, s2);
+                            const int fs = __builtin_amdgcn_readlane(sf, s2);
+                            seq |= is == si ? 1u << s2 : 0u;
+                            sff |= fs == s2 ? 1u << s2 : 0u;
+                        }
+                        const unsigned suse = mine ? (nb >= 32 ? ~0u : (1u << nb) - 1u) : 0u;
+                        const unsigned slt = lane < 32 ? (1u << lane) - 1u : ~0u;
+                        const unsigned ssame = seq & suse;
+                        const unsigned sapp = ~seq & suse & slt;
+                        const unsigned sfirsts = ~seq & suse & sff;
+                        auto Xof = [&](int s2) { return xc[__builtin_amdgcn_readlane(sf, s2)][sl]; };
+                        auto Yof = [&](int s2) {
+                            double y = yc[s2][sl];
+                            if constexpr (EW) {
+                                const bool gg = mine && ((slt >> s2) & 1u);
+                                const int gi = gg ? B + s2 * (2 * B - s2 - 1) / 2 + (lane - s2 - 1) : 0;
+                                const double gv = tot[gi];
+                                y += gg ? 2.0 * C.recip_pref * gv : 0.0;
+                            }
+                            return y;
+                        };
+                        double mydu = 0.0;
+                        int moved = 0;   // this lane's particle was moved by an accepted trial
+                        double fin = 0.0;
+                        int npi = 0, nmi = 0, nna = 0;
+                        auto add = [&](double v, int sgn) {
+                            const bool isn = v != v, isp = v == __builtin_inf(),
+                                       ism = v == -__builtin_inf();
+                            nna += isn ? sgn : 0;
+                            npi += isp ? sgn : 0;
+                            nmi += ism ? sgn : 0;
+                            fin += (isn || isp || ism) ? 0.0 : (sgn > 0 ? v : -v);
+                        };
+                        add(mine ? tot[sl] : 0.0, 1);
+    #pragma unroll
+                        for (int s2 = 0; s2 < B; ++s2)
+                            if ((sfirsts >> s2) & 1u) add(xc[s2][sl], 1);
+    #pragma unroll
+                        for (int t = 0; t < B; ++t) {
+                            if (t >= nb) break;
+                            if (__builtin_amdgcn_readlane(moved, t)) { nd = t; break; }
+                            const double v = (nna > 0 || (npi > 0 && nmi > 0)) ? __builtin_nan("")
+                                             : npi > 0 ? __builtin_inf()
+                                             : nmi > 0 ? -__builtin_inf() : fin;
+                            double du = readlane_f64(v, t);
+                            double dMx = 0.0, dMy = 0.0, dMz = 0.0;
+                            if (EW && C.surf_pref != 0.0) {
+                                const BlkTrial &T = R[t];
+                                dMx = T.wq * (T.nx - T.ox);
+                                dMy = T.wq * (T.ny - T.oy);
+                                dMz = T.wq * (T.nz - T.oz);
+                                const double Nx = Mx + dMx, Ny = My + dMy, Nz = Mz + dMz;
+                                du += C.surf_pref *
+                                      ((Nx * Nx + (Ny * Ny + Nz * Nz)) - (Mx * Mx + (My * My + Mz * Mz)));
+                            }
+                            if (__builtin_amdgcn_readlane(scol, t)) du = __builtin_inf();   // energy.h:3597-3607
+                            const bool accept = du <= 0.0 || !(du > readlane_f64(slu, t));
+                            if (lane == t) mydu = du;
+                            if (accept) {
+                                accm |= 1u << t;
+                                utot += du;
+                                dusum += du;
+                                if (EW) { Mx += dMx; My += dMy; Mz += dMz; }
+                                if ((sapp >> t) & 1u) {
+                                    add(Xof(t), -1);
+                                    add(Yof(t), 1);
+                                }
+                                moved |= (int)((ssame >> t) & 1u);
+                                if (EW && ++ew_cnt > C.update_frequency - 1) {
+                                    refresh = true;
+                                    nd = t + 1;
+                                    break;
+                                }
+                            } else if (du == __builtin_inf()) {
+                                n_inf++;
+                            }
+                        }
+                        if (lane < B) sdu[lane] = mydu;
+                    }
+                    n_trials += nd;
+                    n_acc += __popc(accm);
+                    if (lane == 0) { snd = nd; saccm = (int)accm; srefresh = refresh ? 1 : 0; }
+                    if (lane < B) swq[lane] = lane < nb ? R[lane].wq : 0.0;
+                }
+            } else {
+                // ---- Metropolis decisions in order (wave 0), smaller blocks: lane
+                // t holds trial t's energy change as a finite sum and counts of
+                // +inf, -inf and NaN terms, which add up as IEEE sums do (the
+                // four-lane form spills registers in the Ewald instance and runs
+                // slower there)
+                if (wave0) {
+                    wave_sync_lds();
+                    const bool mine = lane < nb;
+                    // every load first (lane-parallel), then the sequential part on
+                    // per-lane registers (lane t: trial t) and a few scalars
+                    const int myi = mine ? R[lane].i : -1;
+                    const int myf = mine ? R[lane].first : 0;
+                    const double lu = mine ? R[lane].lu : 0.0;
+                    const int mycol = mine ? R[lane].collide : 0;
+                    const double base = mine ? tot[lane] : 0.0;
+                    // per earlier trial s: X (its particle's snapshot term), Y (after
+                    // its accepted move), and whether it moves this lane's particle
+                    // (same) or changes this lane's energy (app)
+                    // the masks as integer bit sets (bit s: earlier trial s), the
+                    // terms loaded by every lane (lanes past the block read column
+                    // 0) and used only under the masks, so nothing here branches
+                    unsigned eqm = 0u, ffm = 0u;
+                    double xt[B], Xs[B], Ys[B];
+                    const int sl = mine ? lane : 0;
+                    const unsigned ltm = lane < 32 ? (1u << lane) - 1u : ~0u;
+    #pragma unroll
+                    for (int s2 = 0; s2 < B; ++s2) {
+                        const int is = __builtin_amdgcn_readlane(myi, s2);
+                        const int fs = __builtin_amdgcn_readlane(myf, s2);
+                        eqm |= is == myi ? 1u << s2 : 0u;
+                        ffm |= fs == s2 ? 1u << s2 : 0u;
+                        xt[s2] = xc[s2][sl];
+                        Xs[s2] = xc[fs][sl];
+                        Ys[s2] = yc[s2][sl];
+                        if constexpr (EW) {
+                            const bool g = mine && ((ltm >> s2) & 1u);
+                            const int gi = g ? B + s2 * (2 * B - s2 - 1) / 2 + (lane - s2 - 1) : 0;
+                            const double gv = tot[gi];
+                            Ys[s2] += g ? 2.0 * C.recip_pref * gv : 0.0;
                         }
                     }
-                }
-                n_trials += nd;
-                n_acc += __popc(accm);
-                if (lane == 0) { snd = nd; saccm = (int)accm; srefresh = refresh ? 1 : 0; }
-                if (lane < B) {
-                    sdu[lane] = mydu;
-                    swq[lane] = lane < nb ? R[lane].wq : 0.0;
+                    const unsigned usem = mine ? (nb >= 32 ? ~0u : (1u << nb) - 1u) : 0u;
+                    const unsigned same = eqm & usem;          // trials of this lane's particle
+                    const unsigned app = ~eqm & usem & ltm;     // earlier trials of other particles
+                    const unsigned firsts = ~eqm & usem & ffm;  // ... each particle's first one
+                    // a NaN or -inf among this lane's terms makes their IEEE sum NaN
+                    // or -inf (+inf + -inf is NaN); an overflow to -inf only sends
+                    // the block down the sequential path
+                    double probe = base;
+    #pragma unroll
+                    for (int s2 = 0; s2 < B; ++s2) {
+                        const double x1 = ((firsts >> s2) & 1u) ? xt[s2] : 0.0;
+                        const bool ap = (app >> s2) & 1u;
+                        probe += x1 + ((ap ? Xs[s2] : 0.0) + (ap ? Ys[s2] : 0.0));
+                    }
+                    const bool odd = probe != probe || probe == -__builtin_inf();
+                    int nd = nb;
+                    unsigned accm = 0;
+                    bool refresh = false;
+                    double mydu = 0.0;
+                    int moved = 0;   // this lane's particle was moved by an accepted trial
+                    if (!__ballot(odd) && !(EW && C.surf_pref != 0.0)) {
+                        // lane t's energy change: a finite sum and a count of +inf
+                        // terms (a collision counts as one: energy.h:3597-3607),
+                        // and what an accepted earlier trial s changes in them
+                        auto fpart = [](double v) { return v == __builtin_inf() ? 0.0 : v; };
+                        auto ipart = [](double v) { return v == __builtin_inf() ? 1 : 0; };
+                        double fin0 = fpart(base);
+                        int npi0 = (mycol ? 1 : 0) + ipart(base);
+                        double dfin[B];
+                        int dinf[B];
+    #pragma unroll
+                        for (int s2 = 0; s2 < B; ++s2) {
+                            const bool fi = (firsts >> s2) & 1u;
+                            fin0 += fi ? fpart(xt[s2]) : 0.0;
+                            npi0 += fi ? ipart(xt[s2]) : 0;
+                            const bool ap = (app >> s2) & 1u;
+                            dfin[s2] = ap ? fpart(Ys[s2]) - fpart(Xs[s2]) : 0.0;
+                            dinf[s2] = ap ? ipart(Ys[s2]) - ipart(Xs[s2]) : 0;
+                        }
+                        // The decisions as a fixed point: from a guess of the
+                        // accepted set, every lane forms its energy change and
+                        // Metropolis decision (move.h:567-573) at once; trial t
+                        // depends on trials s < t only, so after k rounds the first
+                        // k decisions are the sequential ones, and a set that
+                        // reproduces itself is the sequential answer.
+                        unsigned am = 0;
+                        for (int it = 0; it <= B; ++it) {
+                            double f = fin0;
+                            int ni = npi0;
+    #pragma unroll
+                            for (int s2 = 0; s2 < B; ++s2)
+                                if ((am >> s2) & 1u) { f += dfin[s2]; ni += dinf[s2]; }
+                            mydu = ni > 0 ? __builtin_inf() : f;
+                            const bool acc = mine && (mydu <= 0.0 || !(mydu > lu));
+                            const unsigned nm = (unsigned)__ballot(acc);
+                            if (nm == am) break;
+                            am = nm;
+                        }
+                        // a trial whose particle an accepted trial of this block
+                        // moved waits for the next block
+                        const unsigned below_me = lane < 32 ? (1u << lane) - 1u : ~0u;
+                        const unsigned cut = (unsigned)__ballot(mine && (same & am & below_me) != 0u);
+                        if (cut) nd = __ffs(cut) - 1;
+                        accm = am & ((nd >= 32 ? 0u : (1u << nd)) - 1u);
+                        if constexpr (EW) {
+                            // NonbondedEwald::update: the accepted move that takes
+                            // the counter past update_frequency - 1 ends the block
+                            const int need = C.update_frequency - ew_cnt;   // accepts until the refresh
+                            if (need >= 1 && __popc(accm) >= need) {
+                                unsigned m2 = accm;
+                                for (int k = 1; k < need; ++k) m2 &= m2 - 1u;
+                                const int tr = __ffs(m2) - 1;
+                                nd = tr + 1;
+                                accm &= (2u << tr) - 1u;
+                                refresh = true;
+                            }
+                            ew_cnt += __popc(accm);
+                        }
+    #pragma unroll
+                        for (int t = 0; t < B; ++t)
+                            if ((accm >> t) & 1u) {
+                                const double d = readlane_f64(mydu, t);
+                                utot += d;
+                                dusum += d;
+                            }
+                        n_inf += __popcll(__ballot(lane < nd && !((accm >> lane) & 1u) &&
+                                                   mydu == __builtin_inf()));
+                    } else {
+                        // the same with counts of +inf, -inf and NaN terms, which
+                        // add up as IEEE sums do
+                        double fin = 0.0;
+                        int npi = 0, nmi = 0, nna = 0;
+                        auto add = [&](double v, int sgn) {
+                            const bool isn = v != v, isp = v == __builtin_inf(),
+                                       ism = v == -__builtin_inf();
+                            nna += isn ? sgn : 0;
+                            npi += isp ? sgn : 0;
+                            nmi += ism ? sgn : 0;
+                            fin += (isn || isp || ism) ? 0.0 : (sgn > 0 ? v : -v);
+                        };
+                        add(base, 1);
+    #pragma unroll
+                        for (int s2 = 0; s2 < B; ++s2)
+                            if ((firsts >> s2) & 1u) add(xt[s2], 1);
+    #pragma unroll
+                        for (int t = 0; t < B; ++t) {
+                            if (t >= nb) break;
+                            if (__builtin_amdgcn_readlane(moved, t)) { nd = t; break; }
+                            const double v = (nna > 0 || (npi > 0 && nmi > 0)) ? __builtin_nan("")
+                                             : npi > 0 ? __builtin_inf()
+                                             : nmi > 0 ? -__builtin_inf() : fin;
+                            double du = readlane_f64(v, t);
+                            double dMx = 0.0, dMy = 0.0, dMz = 0.0;
+                            if (EW && C.surf_pref != 0.0) {
+                                const BlkTrial &T = R[t];
+                                dMx = T.wq * (T.nx - T.ox);
+                                dMy = T.wq * (T.ny - T.oy);
+                                dMz = T.wq * (T.nz - T.oz);
+                                const double Nx = Mx + dMx, Ny = My + dMy, Nz = Mz + dMz;
+                                du += C.surf_pref *
+                                      ((Nx * Nx + (Ny * Ny + Nz * Nz)) - (Mx * Mx + (My * My + Mz * Mz)));
+                            }
+                            if (__builtin_amdgcn_readlane(mycol, t)) du = __builtin_inf();   // energy.h:3597-3607
+                            const bool accept = du <= 0.0 || !(du > readlane_f64(lu, t));
+                            if (lane == t) mydu = du;
+                            if (accept) {
+                                accm |= 1u << t;
+                                utot += du;
+                                dusum += du;
+                                if (EW) { Mx += dMx; My += dMy; Mz += dMz; }
+                                if ((app >> t) & 1u) {
+                                    add(Xs[t], -1);
+                                    add(Ys[t], 1);
+                                }
+                                moved |= (int)((same >> t) & 1u);
+                                if (EW && ++ew_cnt > C.update_frequency - 1) {
+                                    refresh = true;
+                                    nd = t + 1;
+                                    break;
+                                }
+                            } else if (du == __builtin_inf()) {
+                                n_inf++;
+                            }
+                        }
+                    }
+                    n_trials += nd;
+                    n_acc += __popc(accm);
+                    if (lane == 0) { snd = nd; saccm = (int)accm; srefresh = refresh ? 1 : 0; }
+                    if (lane < B) {
+                        sdu[lane] = mydu;
+                        swq[lane] = lane < nb ? R[lane].wq : 0.0;
+                    }
                 }
             }
             __syncthreads();
