@@ -237,7 +237,10 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
     const bool wave0 = tid < 64;
 
     __shared__ uint32_t mtg[624], mti[624];
-    __shared__ BlkTrial R[B];
+    // the ring: the block's trials and up to B more drawn ahead (the draw
+    // sequence does not depend on the decisions)
+    __shared__ BlkTrial R[2 * B];
+    __shared__ int sgp;   // position of the trial generator between the drawing waves
     __shared__ int bidx[B];
     __shared__ BlkFx F[B];
     __shared__ double red[BLK_THREADS / 64][NV];
@@ -344,6 +347,7 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
         mti[k] = C.rng[625 + k];
     }
     WaveMT G{mtg, (int)C.rng[624]}, I{mti, (int)C.rng[625 + 624]};
+    if (tid == 0) sgp = (int)C.rng[624];
     const bool lds_grp = C.ngvec <= AT_LDS_GROUPS && C.ng <= AT_LDS_GROUPS &&
                          C.pot.ntypes <= AT_LDS_TYPES;
     if (lds_grp) {
@@ -421,11 +425,14 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
     const int wpt = nw >= B ? nw / B : 1;
     const volatile BlkTrial *Rv = R;
 
-    // draws of `cnt` trials into ring slots at.. (wave 0): a trial takes ten
-    // words (two uniform_int, four canonicals) unless a uniform_int rejects;
-    // lane l draws trial l from its ten words of the current state, up to
-    // the first rejection, and the rest are drawn one by one
+    // draws of `cnt` trials into ring slots at.. (one wave: wave 0 at a
+    // block's start, wave 1 during the decisions): a trial takes ten words
+    // (two uniform_int, four canonicals) unless a uniform_int rejects; lane
+    // l draws trial l from its ten words of the current state, up to the
+    // first rejection, and the rest are drawn one by one
     auto draw = [&](int at, int cnt) {
+        G.p = sgp;
+        G.wb = -1;
         while (cnt > 0) {
             // trials whose ten words lie in the current state: in parallel
             const int k = min(cnt, (624 - G.p) / 10);
@@ -471,7 +478,7 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
                 const int i = gd.front + mt_range(G, 0, gd.back - gd.front);
                 const double a = mt_canonical(G), b = mt_canonical(G), c = mt_canonical(G);
                 const double u = mt_canonical(G);
-                if (tid == 0) {
+                if (lane == 0) {
                     BlkTrial &t = R[at];
                     t.i = i;
                     t.t3[0] = a; t.t3[1] = b; t.t3[2] = c;
@@ -482,6 +489,8 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
                 --cnt;
             }
         }
+        wave_sync_lds();
+        if (lane == 0) sgp = G.p;
     };
     // the moved particle's data from the accepted vector (one lane)
     auto fetch_particle = [&](BlkTrial &t) {
@@ -520,7 +529,8 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
         int drawn = rep0;
         nr = 0;
         for (;;) {
-            // ---- fill the ring with the next trials of this call (wave 0)
+            // ---- fill the ring with the next trials of this call (wave 0;
+            // usually drawn ahead already)
             const int nfill = nr >= B ? 0 : (int)min((long)min(B - nr, C.repeat - drawn), budget - used);
             if (nr == 0 && nfill == 0) {
                 if (drawn < C.repeat) {   // trial budget of the launch spent
@@ -529,7 +539,7 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
                 }
                 break;
             }
-            if (wave0) draw(nr, nfill);
+            if (wave0 && nfill > 0) draw(nr, nfill);
             nr += nfill;
             used += nfill;
             drawn += nfill;
@@ -967,6 +977,9 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
             }
             exchange(NV);
             if (sabort) break;
+            // trials drawn ahead by wave 1 while wave 0 decides: the ring's
+            // free slots, within this call and the launch's trial budget
+            const int kpre = nw >= 2 ? (int)min((long)min(2 * B - nr, C.repeat - drawn), budget - used) : 0;
             if constexpr (B >= 8) {
                 // ---- Metropolis decisions in order (wave 0).  Blocks of 8 or more
                 // trials: the fast form puts trial t on lanes 4t..4t+3, lane 4t+g
@@ -1406,7 +1419,11 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
                     }
                 }
             }
+            if (wv == 1 && kpre > 0) draw(nr, kpre);
             __syncthreads();
+            nr += kpre;
+            used += kpre;
+            drawn += kpre;
             const int nd = snd;
             const unsigned accm = (unsigned)saccm;
             const bool refresh = srefresh != 0;
@@ -1569,7 +1586,7 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
             C.rng[625 + k] = mti[k];
         }
         if (tid == 0) {
-            C.rng[624] = (uint32_t)G.p;
+            C.rng[624] = (uint32_t)sgp;
             C.rng[625 + 624] = (uint32_t)I.p;
             C.stats_i[0] = n_trials;
             C.stats_i[1] = n_acc;
