@@ -1419,7 +1419,15 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
                     }
                 }
             }
-            if (wv == 1 && kpre > 0) draw(nr, kpre);
+            if (wv == 1 && kpre > 0) {
+                // with their particles' data: the accepted vector holds every
+                // move before this block's (written blocks ago, barriers
+                // between); this block's moves are corrected for at the next
+                // block's start, as for every trial in the ring
+                draw(nr, kpre);
+                wave_sync_lds();
+                if (lane < kpre) fetch_particle(R[nr + lane]);
+            }
             __syncthreads();
             nr += kpre;
             used += kpre;
