@@ -868,17 +868,32 @@ __global__ __launch_bounds__(BLK_THREADS) void k_sweep_blk(const ChainDev *__res
                 }
             }
             // ---- corrections: the block's own particles at their position
-            // before (X) and after (Y) an accepted move, FP64, over all threads
+            // before (X) and after (Y) an accepted move, over all threads
             for (int e = tid; e < 2 * nb * nb; e += nthr) {
                 const int e2 = e % (nb * nb);
                 const int s2 = e2 / nb, t = e2 % nb;
                 const bool yv = e >= nb * nb;
                 const BlkTrial &a = R[t], &o = R[s2];
                 double v = 0.0;
-                if (s2 != t && a.i != o.i && (!yv || s2 < t))
-                    v = du_pair64_v<ELEC, SR>(C, tab, a.nx, a.ny, a.nz, a.ox, a.oy, a.oz, a.q64, a.r64,
-                                              a.ti, yv ? o.nx : o.ox, yv ? o.ny : o.oy,
-                                              yv ? o.nz : o.oz, o.q64, o.r64, o.ti);
+                if (s2 != t && a.i != o.i && (!yv || s2 < t)) {
+                    // mixed path with Coulomb or Debye-Hueckel (+ hard spheres):
+                    // the partner pass's fixed-unit FP32 form, FP64 for the
+                    // pairs it marks
+                    bool bad = true;
+                    if constexpr (MIX && !EW && SR != SR_LJ) {
+                        const BlkFx &ft = F[t], &fs = F[s2];
+                        const int4 P = make_int4(yv ? fs.nx : fs.ox, yv ? fs.ny : fs.oy,
+                                                 yv ? fs.nz : fs.oz, o.pkw);
+                        const float d = blk_pair_fx<ELEC, SR, 0>(KF, tabf, ft.nx, ft.ny, ft.nz, ft.ox,
+                                                                 ft.oy, ft.oz, ft.ti, ft.aq, ft.ar, P,
+                                                                 o.rf, o.ti, bad);
+                        v = (double)d;
+                    }
+                    if (bad)
+                        v = du_pair64_v<ELEC, SR>(C, tab, a.nx, a.ny, a.nz, a.ox, a.oy, a.oz, a.q64,
+                                                  a.r64, a.ti, yv ? o.nx : o.ox, yv ? o.ny : o.oy,
+                                                  yv ? o.nz : o.oz, o.q64, o.r64, o.ti);
+                }
                 if (yv) yc[s2][t] = v;
                 else xc[s2][t] = v;
             }
